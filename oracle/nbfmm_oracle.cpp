@@ -1,0 +1,446 @@
+// nbfmm CPU oracle -- TEST INFRASTRUCTURE ONLY.
+//
+// A plain C++ restatement of the emfomy/nbfmm force-evaluation path
+// (nbfmm::Solver::solve) and its Stormer-Verlet update, used by the tests as
+// the parity checker and by bench.py as the timed CPU baseline.  Nothing in the
+// product library links, loads or calls this file.
+//
+// Every stage follows the reference source expression by expression; each
+// function names the reference file:line it restates.  Where the reference has
+// a defect (head tail never written when N % 1024 == 0, the zero/scatter race in
+// p2mAssigning, no clamp of the cell index) the oracle implements the intended
+// semantics that the reference's own unit tests pin.
+//
+// Layouts are the reference's:
+//   position float2[N] (x,y interleaved), weight float[N], index int2[N],
+//   perm int[N], head int[B*B+1],
+//   cell cubes [levels][B][B] where level-l cell (X,Y) lives at
+//   l*B*B + (Y<<l)*B + (X<<l)                      (m2m.cu:35, m2l.cu:41).
+//
+// Arithmetic: compiled with -ffp-contract=off so every float operation rounds
+// exactly as written (the reference is built without fast-math,
+// CMakeLists.txt:18).  Templated on the real type; the float instantiation is
+// the parity oracle, the double one serves finite-difference checks.
+//
+// Parity status of each function (which check pins it) is listed in the
+// header comment of each function.
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+// include/nbfmm/core/kernel_function.hpp:29-40.  The reference hard-codes
+// epsilon = 0.1 (:35); here it is a parameter.
+//   d = s - t;  r = sqrt(d.d);  tmp = m / (r*r*r + eps);  a = d*tmp
+// Pinned by: tests/test_oracle.py closed-form two-body values and the
+// exact-regime checks against an independently written fp64 direct sum.
+template <class T>
+inline void kernel_accum(T tx, T ty, T sx, T sy, T m, T eps, T& ax, T& ay) {
+  const T dx = sx - tx;
+  const T dy = sy - ty;
+  const T r = std::sqrt(dx * dx + dy * dy);
+  const T tmp = m / (r * r * r + eps);
+  ax += dx * tmp;
+  ay += dy * tmp;
+}
+
+inline int cube_index(int B, int l, int X, int Y) { return l * B * B + (Y << l) * B + (X << l); }
+
+void set_threads(int nthreads) {
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+  (void)nthreads;
+#endif
+}
+
+// source/nbfmm/core/solver/predo.cu:26-39 (computeParticleIndex) with the cell
+// size of predo.cu:112-113.  The reference does not clamp (a particle at
+// x_max gets index B); the oracle clamps into [0, B-1] and counts the clamps,
+// the same contract the HIP library implements (out-of-box flag).
+// Pinned by: bit-exact comparison with numpy float32 (x - lo) / cell, floor.
+template <class T>
+int compute_index(int n, int B, const T* pos, T xmin, T ymin, T cw, T ch, int* index) {
+  int clamped = 0;
+  for (int i = 0; i < n; ++i) {
+    const T vx = std::floor((pos[2 * i] - xmin) / cw);
+    const T vy = std::floor((pos[2 * i + 1] - ymin) / ch);
+    int ix, iy;
+    bool bad = false;
+    if (!(vx >= T(0))) { ix = 0; bad = true; }
+    else if (vx >= T(B)) { ix = B - 1; bad = true; }
+    else ix = (int)vx;
+    if (!(vy >= T(0))) { iy = 0; bad = true; }
+    else if (vy >= T(B)) { iy = B - 1; bad = true; }
+    else iy = (int)vy;
+    index[2 * i] = ix;
+    index[2 * i + 1] = iy;
+    clamped += bad ? 1 : 0;
+  }
+  return clamped;
+}
+
+// predo.cu:105-135: index, perm = stable sort of (iy, ix) row-major
+// (thrust::sort_by_key with operator< of include/nbfmm/utility/operator.hpp:107-109),
+// head = first sorted slot of each cell with head[B*B] = N (intended
+// semantics of extractHead, predo.cu:49-74), gather of position and weight
+// (permuteInputVector, predo.cu:86-100).
+// The stable sort is a counting sort on key = iy*B + ix: equal keys keep
+// their input order.
+// Pinned by: numpy argsort(kind="stable"), bincount/cumsum head and the
+// predo checks of check/core/solver/predo.cu:50-66.
+template <class T>
+int predo(int L, int n, const T* pos0, const T* w0, const T limits[4], T* pos, T* w, int* index,
+          int* perm, int* head) {
+  const int B = 1 << (L + 1);
+  const int C = B * B;
+  const T cw = (limits[2] - limits[0]) / T(B);
+  const T ch = (limits[3] - limits[1]) / T(B);
+  std::vector<int> idx0(2 * (size_t)std::max(n, 1));
+  const int clamped = compute_index<T>(n, B, pos0, limits[0], limits[1], cw, ch, idx0.data());
+  std::vector<int> count(C + 1, 0);
+  for (int i = 0; i < n; ++i) count[idx0[2 * i] + idx0[2 * i + 1] * B]++;
+  head[0] = 0;
+  for (int c = 0; c < C; ++c) head[c + 1] = head[c] + count[c];
+  std::vector<int> fill(head, head + C);
+  for (int i = 0; i < n; ++i) {
+    const int key = idx0[2 * i] + idx0[2 * i + 1] * B;
+    const int p = fill[key]++;
+    perm[p] = i;
+  }
+  for (int p = 0; p < n; ++p) {
+    const int i = perm[p];
+    pos[2 * p] = pos0[2 * i];
+    pos[2 * p + 1] = pos0[2 * i + 1];
+    w[p] = w0[i];
+    index[2 * p] = idx0[2 * i];
+    index[2 * p + 1] = idx0[2 * i + 1];
+  }
+  return clamped;
+}
+
+// source/nbfmm/core/solver/p2p.cu:30-69 (p2pNaive): for each sorted particle,
+// sum the kernel over the particles of the 3x3 neighbour cells, skipping
+// itself; cells in row-major order, particles in sorted order.
+// Pinned by: the all-pairs |dix|<=1 && |diy|<=1 formula of
+// check/core/solver/p2p.cu:34-41 (restated in numpy, fp64), and L=0 == direct sum.
+template <class T>
+void p2p(int L, int n, const T* pos, const T* w, const int* index, const int* head, T eps, T* eff) {
+  const int B = 1 << (L + 1);
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int i = 0; i < n; ++i) {
+    const int px = index[2 * i], py = index[2 * i + 1];
+    const T tx = pos[2 * i], ty = pos[2 * i + 1];
+    T ax = T(0), ay = T(0);
+    for (int j = py - 1; j <= py + 1; ++j) {
+      for (int k = px - 1; k <= px + 1; ++k) {
+        if (k >= 0 && k < B && j >= 0 && j < B) {
+          const int c = k + j * B;
+          for (int s = head[c]; s < head[c + 1]; ++s) {
+            if (s != i) kernel_accum<T>(tx, ty, pos[2 * s], pos[2 * s + 1], w[s], eps, ax, ay);
+          }
+        }
+      }
+    }
+    eff[2 * i] = ax;
+    eff[2 * i + 1] = ay;
+  }
+}
+
+// source/nbfmm/core/solver/p2m.cu:24-110, intended semantics (every leaf is
+// zeroed, then occupied leaves get centre of mass = sum(m*x)/sum(m) and mass
+// sum(m); the reference's zero/scatter race p2m.cu:62-73 is not reproduced).
+// Sums run in sorted order.
+// Pinned by: check/core/solver/p2m.cu:36-46 restated in numpy (fp64).
+template <class T>
+void p2m(int L, int n, const T* pos, const T* w, const int* head, T* cell_pos, T* cell_w) {
+  (void)n;
+  if (L <= 0) return;
+  const int B = 1 << (L + 1);
+  for (int c = 0; c < B * B; ++c) {
+    T sx = T(0), sy = T(0), sw = T(0);
+    for (int s = head[c]; s < head[c + 1]; ++s) {
+      sx += pos[2 * s] * w[s];
+      sy += pos[2 * s + 1] * w[s];
+      sw += w[s];
+    }
+    if (head[c] != head[c + 1]) {
+      cell_pos[2 * c] = sx / sw;
+      cell_pos[2 * c + 1] = sy / sw;
+    } else {
+      cell_pos[2 * c] = T(0);
+      cell_pos[2 * c + 1] = T(0);
+    }
+    cell_w[c] = sw;
+  }
+}
+
+// source/nbfmm/core/solver/m2m.cu:22-74: levels 1..L-1, each parent from its
+// four children (0,0),(1,0),(0,1),(1,1) in that order; mass 0 gives COM (0,0).
+// Pinned by: check/core/solver/m2m.cu:32-48 restated in numpy, mass
+// conservation and root COM == global COM.
+template <class T>
+void m2m(int L, T* cell_pos, T* cell_w) {
+  if (L <= 1) return;
+  const int B = 1 << (L + 1);
+  for (int l = 1; l < L; ++l) {
+    const int D = B >> l;
+    for (int Y = 0; Y < D; ++Y) {
+      for (int X = 0; X < D; ++X) {
+        const int p = cube_index(B, l, X, Y);
+        const int c00 = cube_index(B, l - 1, 2 * X, 2 * Y);
+        const int c10 = cube_index(B, l - 1, 2 * X + 1, 2 * Y);
+        const int c01 = cube_index(B, l - 1, 2 * X, 2 * Y + 1);
+        const int c11 = cube_index(B, l - 1, 2 * X + 1, 2 * Y + 1);
+        const T px = cell_pos[2 * c00] * cell_w[c00] + cell_pos[2 * c10] * cell_w[c10] +
+                     cell_pos[2 * c01] * cell_w[c01] + cell_pos[2 * c11] * cell_w[c11];
+        const T py = cell_pos[2 * c00 + 1] * cell_w[c00] + cell_pos[2 * c10 + 1] * cell_w[c10] +
+                     cell_pos[2 * c01 + 1] * cell_w[c01] + cell_pos[2 * c11 + 1] * cell_w[c11];
+        const T pw = cell_w[c00] + cell_w[c10] + cell_w[c01] + cell_w[c11];
+        if (pw == T(0)) {
+          cell_pos[2 * p] = T(0);
+          cell_pos[2 * p + 1] = T(0);
+        } else {
+          cell_pos[2 * p] = px / pw;
+          cell_pos[2 * p + 1] = py / pw;
+        }
+        cell_w[p] = pw;
+      }
+    }
+  }
+}
+
+// source/nbfmm/core/solver/m2l.cu:24-81: at every level l = 0..L-1, each
+// target cell sums the kernel, evaluated at its own COM, over the children of
+// its parent's 3x3 neighbourhood (window [p-2, p+4) per axis, clipped) that
+// are not adjacent to it (Chebyshev distance > 1); y outer, x inner.
+// Pinned by: the O(cells^2) rule of check/core/solver/m2l.cu:34-50 restated
+// in numpy, and partition exactness (each pair counted exactly once).
+template <class T>
+void m2l(int L, T eps, const T* cell_pos, const T* cell_w, T* cell_eff) {
+  if (L <= 0) return;
+  const int B = 1 << (L + 1);
+  for (int l = 0; l < L; ++l) {
+    const int D = B >> l;
+#pragma omp parallel for schedule(static)
+    for (int ty = 0; ty < D; ++ty) {
+      for (int tx = 0; tx < D; ++tx) {
+        const int px = tx & ~1, py = ty & ~1;
+        const int t = cube_index(B, l, tx, ty);
+        const T x0 = cell_pos[2 * t], y0 = cell_pos[2 * t + 1];
+        T ax = T(0), ay = T(0);
+        for (int y = py - 2; y < py + 4; ++y) {
+          if (y < 0 || y >= D) continue;
+          for (int x = px - 2; x < px + 4; ++x) {
+            if (x < 0 || x >= D) continue;
+            if (std::abs(x - tx) > 1 || std::abs(y - ty) > 1) {
+              const int s = cube_index(B, l, x, y);
+              kernel_accum<T>(x0, y0, cell_pos[2 * s], cell_pos[2 * s + 1], cell_w[s], eps, ax, ay);
+            }
+          }
+        }
+        cell_eff[2 * t] = ax;
+        cell_eff[2 * t + 1] = ay;
+      }
+    }
+  }
+}
+
+// source/nbfmm/core/solver/l2l.cu:21-56: for l = L-2 down to 0, every cell
+// adds its parent's (already accumulated) effect.
+// Pinned by: check/core/solver/l2l.cu:26-37 (push-down form) restated in numpy.
+template <class T>
+void l2l(int L, T* cell_eff) {
+  if (L <= 1) return;
+  const int B = 1 << (L + 1);
+  for (int l = L - 2; l >= 0; --l) {
+    const int D = B >> l;
+    for (int Y = 0; Y < D; ++Y) {
+      for (int X = 0; X < D; ++X) {
+        const int c = cube_index(B, l, X, Y);
+        const int p = cube_index(B, l + 1, X >> 1, Y >> 1);
+        cell_eff[2 * c] = cell_eff[2 * c] + cell_eff[2 * p];
+        cell_eff[2 * c + 1] = cell_eff[2 * c + 1] + cell_eff[2 * p + 1];
+      }
+    }
+  }
+}
+
+// source/nbfmm/core/solver/l2p.cu:23-48: effect[i] += leaf cell effect.
+// Pinned by: check/core/solver/l2p.cu:34-36 restated in numpy.
+template <class T>
+void l2p(int L, int n, const int* index, const T* cell_eff, T* eff) {
+  if (L <= 0) return;
+  const int B = 1 << (L + 1);
+  for (int i = 0; i < n; ++i) {
+    const int c = index[2 * i] + index[2 * i + 1] * B;
+    eff[2 * i] = eff[2 * i] + cell_eff[2 * c];
+    eff[2 * i + 1] = eff[2 * i + 1] + cell_eff[2 * c + 1];
+  }
+}
+
+// source/nbfmm/core/solver/postdo.cu:22-45: eff0[perm[i]] = eff[i].
+// Pinned by: check/core/solver/postdo.cu:35-37 (exact).
+template <class T>
+void postdo(int n, const int* perm, const T* eff, T* eff0) {
+  for (int i = 0; i < n; ++i) {
+    eff0[2 * perm[i]] = eff[2 * i];
+    eff0[2 * perm[i] + 1] = eff[2 * i + 1];
+  }
+}
+
+int cube_levels(int L) { return L < 1 ? 1 : L; }
+
+// source/nbfmm/core/solver/solve.cu:11-27: predo, p2p, p2m, m2m, m2l, l2l,
+// l2p, postdo.
+// Pinned by: partition exactness, momentum conservation, the exact regime
+// (<=1 particle per cell at every level == direct sum) and L=0 == direct sum.
+template <class T>
+int solve(int L, int n, const T limits[4], T eps, const T* pos0, const T* w0, T* eff0) {
+  const int B = 1 << (L + 1);
+  const size_t C = (size_t)B * B;
+  const size_t nn = (size_t)std::max(n, 1);
+  std::vector<T> pos(2 * nn), w(nn), eff(2 * nn);
+  std::vector<int> index(2 * nn), perm(nn), head(C + 1);
+  const size_t cube = C * cube_levels(L);
+  std::vector<T> cpos(2 * cube, T(0)), cw(cube, T(0)), ceff(2 * cube, T(0));
+  const int clamped = predo<T>(L, n, pos0, w0, limits, pos.data(), w.data(), index.data(), perm.data(), head.data());
+  p2p<T>(L, n, pos.data(), w.data(), index.data(), head.data(), eps, eff.data());
+  p2m<T>(L, n, pos.data(), w.data(), head.data(), cpos.data(), cw.data());
+  m2m<T>(L, cpos.data(), cw.data());
+  m2l<T>(L, eps, cpos.data(), cw.data(), ceff.data());
+  l2l<T>(L, ceff.data());
+  l2p<T>(L, n, index.data(), ceff.data(), eff.data());
+  postdo<T>(n, perm.data(), eff.data(), eff0);
+  return clamped;
+}
+
+// source/nbfmm/display/stars/update.cu:26-42 (updateDevice):
+//   x_new = 2*x - x_prev + a*tick*tick;  x_prev = x
+// evaluated left to right per component.  tick already includes sqrt(G)
+// (source/nbfmm/display/stars.cu:31).
+// Pinned by: the constant-acceleration closed form x_n = x_0 + n*v*dt + n^2*a*dt^2/2
+// (fp64) and a bit-exact numpy float32 restatement.
+template <class T>
+void update(int n, T tick, T* xcur, T* xpre, const T* acc) {
+  for (int i = 0; i < 2 * n; ++i) {
+    const T x = xcur[i];
+    xcur[i] = T(2) * x - xpre[i] + acc[i] * tick * tick;
+    xpre[i] = x;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int orc_version(void) { return 1; }
+
+int orc_max_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+void orc_set_threads(int nthreads) { set_threads(nthreads); }
+
+void orc_cell_size_f32(const float limits[4], int B, float out[2]) {
+  out[0] = (limits[2] - limits[0]) / (float)B;
+  out[1] = (limits[3] - limits[1]) / (float)B;
+}
+
+int orc_compute_index_f32(int L, int n, const float* pos, const float limits[4], int* index) {
+  const int B = 1 << (L + 1);
+  const float cw = (limits[2] - limits[0]) / (float)B;
+  const float ch = (limits[3] - limits[1]) / (float)B;
+  return compute_index<float>(n, B, pos, limits[0], limits[1], cw, ch, index);
+}
+
+int orc_predo_f32(int L, int n, const float* pos0, const float* w0, const float limits[4], float* pos, float* w,
+                  int* index, int* perm, int* head) {
+  return predo<float>(L, n, pos0, w0, limits, pos, w, index, perm, head);
+}
+
+void orc_p2p_f32(int L, int n, const float* pos, const float* w, const int* index, const int* head, float eps,
+                 float* eff) {
+  p2p<float>(L, n, pos, w, index, head, eps, eff);
+}
+
+void orc_p2m_f32(int L, int n, const float* pos, const float* w, const int* head, float* cell_pos, float* cell_w) {
+  p2m<float>(L, n, pos, w, head, cell_pos, cell_w);
+}
+
+void orc_m2m_f32(int L, float* cell_pos, float* cell_w) { m2m<float>(L, cell_pos, cell_w); }
+
+void orc_m2l_f32(int L, float eps, const float* cell_pos, const float* cell_w, float* cell_eff) {
+  m2l<float>(L, eps, cell_pos, cell_w, cell_eff);
+}
+
+void orc_l2l_f32(int L, float* cell_eff) { l2l<float>(L, cell_eff); }
+
+void orc_l2p_f32(int L, int n, const int* index, const float* cell_eff, float* eff) {
+  l2p<float>(L, n, index, cell_eff, eff);
+}
+
+void orc_postdo_f32(int n, const int* perm, const float* eff, float* eff0) { postdo<float>(n, perm, eff, eff0); }
+
+int orc_solve_f32(int L, int n, const float limits[4], float eps, const float* pos0, const float* w0, float* eff0) {
+  return solve<float>(L, n, limits, eps, pos0, w0, eff0);
+}
+
+int orc_solve_f64(int L, int n, const double limits[4], double eps, const double* pos0, const double* w0,
+                  double* eff0) {
+  return solve<double>(L, n, limits, eps, pos0, w0, eff0);
+}
+
+void orc_update_f32(int n, float tick, float* xcur, float* xpre, const float* acc) {
+  update<float>(n, tick, xcur, xpre, acc);
+}
+
+void orc_update_f64(int n, double tick, double* xcur, double* xpre, const double* acc) {
+  update<double>(n, tick, xcur, xpre, acc);
+}
+
+// Stars::update without prune (source/nbfmm/display/stars/update.cu:47-55):
+// solve at the current positions, then the Verlet update; `steps` times.
+int orc_run_f32(int L, int n, const float limits[4], float eps, float tick, int steps, float* xcur, float* xpre,
+                const float* w, float* acc) {
+  int clamped = 0;
+  for (int s = 0; s < steps; ++s) {
+    clamped += solve<float>(L, n, limits, eps, xcur, w, acc);
+    update<float>(n, tick, xcur, xpre, acc);
+  }
+  return clamped;
+}
+
+// fp64 O(N^2) direct sum with the same kernel form and epsilon
+// (main/demo.cu:59-66 is the reference's only end-to-end comparison).
+// Evaluates the targets listed in `targets` (all particles when targets is
+// null), writing eff[2*t] for the t-th listed target.
+void orc_direct_f64(int n, const float* pos, const float* w, double eps, int ntargets, const int* targets,
+                    double* eff) {
+  const int nt = targets ? ntargets : n;
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int t = 0; t < nt; ++t) {
+    const int i = targets ? targets[t] : t;
+    const double tx = pos[2 * i], ty = pos[2 * i + 1];
+    double ax = 0.0, ay = 0.0;
+    for (int j = 0; j < n; ++j) {
+      if (j == i) continue;
+      kernel_accum<double>(tx, ty, (double)pos[2 * j], (double)pos[2 * j + 1], (double)w[j], eps, ax, ay);
+    }
+    eff[2 * t] = ax;
+    eff[2 * t + 1] = ay;
+  }
+}
+
+}  // extern "C"

@@ -1,0 +1,211 @@
+"""ctypes front end of the CPU oracle (test infrastructure only).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module, and only as the checker or the timed CPU baseline.  The product
+library (nbfmm_amd) never loads it.
+
+All arrays use the reference layouts: position float32[N, 2], weight
+float32[N], index int32[N, 2], perm int32[N], head int32[B*B+1] and cell cubes
+[levels][B][B] (see nbfmm_oracle.cpp).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+_lib = None
+
+_f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+def build() -> str:
+    """Compile the oracle with its Makefile (g++ only; no GPU needed)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        c_int, c_float, c_double = ctypes.c_int, ctypes.c_float, ctypes.c_double
+        L.orc_max_threads.restype = c_int
+        L.orc_set_threads.argtypes = [c_int]
+        L.orc_compute_index_f32.argtypes = [c_int, c_int, _f32p, _f32p, _i32p]
+        L.orc_compute_index_f32.restype = c_int
+        L.orc_predo_f32.argtypes = [c_int, c_int, _f32p, _f32p, _f32p, _f32p, _f32p, _i32p, _i32p, _i32p]
+        L.orc_predo_f32.restype = c_int
+        L.orc_p2p_f32.argtypes = [c_int, c_int, _f32p, _f32p, _i32p, _i32p, c_float, _f32p]
+        L.orc_p2m_f32.argtypes = [c_int, c_int, _f32p, _f32p, _i32p, _f32p, _f32p]
+        L.orc_m2m_f32.argtypes = [c_int, _f32p, _f32p]
+        L.orc_m2l_f32.argtypes = [c_int, c_float, _f32p, _f32p, _f32p]
+        L.orc_l2l_f32.argtypes = [c_int, _f32p]
+        L.orc_l2p_f32.argtypes = [c_int, c_int, _i32p, _f32p, _f32p]
+        L.orc_postdo_f32.argtypes = [c_int, _i32p, _f32p, _f32p]
+        L.orc_solve_f32.argtypes = [c_int, c_int, _f32p, c_float, _f32p, _f32p, _f32p]
+        L.orc_solve_f32.restype = c_int
+        L.orc_solve_f64.argtypes = [c_int, c_int, _f64p, c_double, _f64p, _f64p, _f64p]
+        L.orc_solve_f64.restype = c_int
+        L.orc_update_f32.argtypes = [c_int, c_float, _f32p, _f32p, _f32p]
+        L.orc_update_f64.argtypes = [c_int, c_double, _f64p, _f64p, _f64p]
+        L.orc_run_f32.argtypes = [c_int, c_int, _f32p, c_float, c_float, c_int, _f32p, _f32p, _f32p, _f32p]
+        L.orc_run_f32.restype = c_int
+        L.orc_direct_f64.argtypes = [c_int, _f32p, _f32p, c_double, c_int, ctypes.c_void_p, _f64p]
+        _lib = L
+    return _lib
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def base_dim(L: int) -> int:
+    return 1 << (L + 1)
+
+
+def cube_levels(L: int) -> int:
+    return max(L, 1)
+
+
+def set_threads(n: int) -> None:
+    lib().orc_set_threads(int(n))
+
+
+def max_threads() -> int:
+    return int(lib().orc_max_threads())
+
+
+def compute_index(L, pos, limits):
+    pos = _f32(pos).reshape(-1, 2)
+    index = np.zeros((pos.shape[0], 2), np.int32)
+    clamped = lib().orc_compute_index_f32(L, pos.shape[0], pos, _f32(limits), index)
+    return index, clamped
+
+
+def predo(L, pos0, w0, limits):
+    pos0 = _f32(pos0).reshape(-1, 2)
+    w0 = _f32(w0)
+    n = pos0.shape[0]
+    B = base_dim(L)
+    pos = np.zeros_like(pos0)
+    w = np.zeros_like(w0)
+    index = np.zeros((n, 2), np.int32)
+    perm = np.zeros(n, np.int32)
+    head = np.zeros(B * B + 1, np.int32)
+    clamped = lib().orc_predo_f32(L, n, pos0, w0, _f32(limits), pos, w, index, perm, head)
+    return dict(position=pos, weight=w, index=index, perm=perm, head=head, clamped=clamped)
+
+
+def p2p(L, pos, w, index, head, eps):
+    pos = _f32(pos).reshape(-1, 2)
+    eff = np.zeros_like(pos)
+    lib().orc_p2p_f32(L, pos.shape[0], pos, _f32(w), _i32(index), _i32(head), float(eps), eff)
+    return eff
+
+
+def p2m(L, pos, w, head):
+    B = base_dim(L)
+    cube = B * B * cube_levels(L)
+    cpos = np.zeros((cube, 2), np.float32)
+    cw = np.zeros(cube, np.float32)
+    pos = _f32(pos).reshape(-1, 2)
+    lib().orc_p2m_f32(L, pos.shape[0], pos, _f32(w), _i32(head), cpos, cw)
+    return cpos, cw
+
+
+def m2m(L, cell_pos, cell_w):
+    cpos = _f32(cell_pos).copy()
+    cw = _f32(cell_w).copy()
+    lib().orc_m2m_f32(L, cpos, cw)
+    return cpos, cw
+
+
+def m2l(L, eps, cell_pos, cell_w):
+    cpos = _f32(cell_pos)
+    ceff = np.zeros_like(cpos)
+    lib().orc_m2l_f32(L, float(eps), cpos, _f32(cell_w), ceff)
+    return ceff
+
+
+def l2l(L, cell_eff):
+    ceff = _f32(cell_eff).copy()
+    lib().orc_l2l_f32(L, ceff)
+    return ceff
+
+
+def l2p(L, index, cell_eff, eff):
+    out = _f32(eff).copy()
+    lib().orc_l2p_f32(L, out.shape[0], _i32(index), _f32(cell_eff), out)
+    return out
+
+
+def postdo(perm, eff):
+    eff = _f32(eff).reshape(-1, 2)
+    out = np.zeros_like(eff)
+    lib().orc_postdo_f32(eff.shape[0], _i32(perm), eff, out)
+    return out
+
+
+def solve(L, pos0, w0, limits, eps):
+    pos0 = _f32(pos0).reshape(-1, 2)
+    eff = np.zeros_like(pos0)
+    clamped = lib().orc_solve_f32(L, pos0.shape[0], _f32(limits), float(eps), pos0, _f32(w0), eff)
+    return eff, clamped
+
+
+def solve_f64(L, pos0, w0, limits, eps):
+    pos0 = np.ascontiguousarray(pos0, np.float64).reshape(-1, 2)
+    eff = np.zeros_like(pos0)
+    lib().orc_solve_f64(L, pos0.shape[0], np.ascontiguousarray(limits, np.float64), float(eps), pos0,
+                        np.ascontiguousarray(w0, np.float64), eff)
+    return eff
+
+
+def update(tick, xcur, xpre, acc):
+    xcur = _f32(xcur).copy()
+    xpre = _f32(xpre).copy()
+    lib().orc_update_f32(xcur.shape[0], float(tick), xcur, xpre, _f32(acc))
+    return xcur, xpre
+
+
+def update_f64(tick, xcur, xpre, acc):
+    xcur = np.ascontiguousarray(xcur, np.float64).copy()
+    xpre = np.ascontiguousarray(xpre, np.float64).copy()
+    lib().orc_update_f64(xcur.shape[0], float(tick), xcur, xpre, np.ascontiguousarray(acc, np.float64))
+    return xcur, xpre
+
+
+def run(L, limits, eps, tick, steps, xcur, xpre, w):
+    """Stars::update without prune, `steps` times: returns (xcur, xpre, acc, clamped)."""
+    xcur = _f32(xcur).copy()
+    xpre = _f32(xpre).copy()
+    acc = np.zeros_like(xcur)
+    clamped = lib().orc_run_f32(L, xcur.shape[0], _f32(limits), float(eps), float(tick), int(steps), xcur, xpre,
+                                _f32(w), acc)
+    return xcur, xpre, acc, clamped
+
+
+def direct_f64(pos, w, eps, targets=None):
+    pos = _f32(pos).reshape(-1, 2)
+    n = pos.shape[0]
+    if targets is None:
+        out = np.zeros((n, 2), np.float64)
+        lib().orc_direct_f64(n, pos, _f32(w), float(eps), n, None, out)
+    else:
+        t = _i32(targets)
+        out = np.zeros((t.shape[0], 2), np.float64)
+        lib().orc_direct_f64(n, pos, _f32(w), float(eps), t.shape[0], t.ctypes.data_as(ctypes.c_void_p), out)
+    return out

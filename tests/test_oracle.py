@@ -1,0 +1,282 @@
+"""Pins the CPU oracle (oracle/nbfmm_oracle.cpp) before anything is checked against it.
+
+Each stage is compared with the float64 restatement of the reference's own
+unit-test formula (tests/npfmm.py, frozen into tests/golden/*.npz), and the
+whole solve with closed forms and invariants that fix it independently:
+L=0 equals the direct sum, the exact regime (at most one particle per cell on
+every level) equals the direct sum, momentum is conserved, mass is conserved
+up the tree, and the Verlet update matches a constant-acceleration closed form.
+CPU only.
+"""
+import numpy as np
+import pytest
+
+import npfmm
+from oracle import pyoracle as orc
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def max_rel_to_rms(a, b):
+    a = np.asarray(a, np.float64).reshape(-1, 2)
+    b = np.asarray(b, np.float64).reshape(-1, 2)
+    rms = np.sqrt((b ** 2).sum(1).mean())
+    return np.abs(a - b).max() / max(rms, 1e-300)
+
+
+@pytest.fixture(scope="module")
+def stage(golden):
+    return golden("stage_L3_N250.npz")
+
+
+def test_predo_matches_fixture(stage):
+    L = int(stage["L"])
+    out = orc.predo(L, stage["position"], stage["weight"], stage["limits"])
+    assert out["clamped"] == 0
+    np.testing.assert_array_equal(out["index"], stage["predo_index"])
+    np.testing.assert_array_equal(out["perm"], stage["predo_perm"])
+    np.testing.assert_array_equal(out["head"], stage["predo_head"])
+    np.testing.assert_array_equal(out["position"], stage["predo_position"])
+    np.testing.assert_array_equal(out["weight"], stage["predo_weight"])
+    # check/core/solver/predo.cu:50-66: every particle lies in its cell box and
+    # head ranges agree with index
+    B = npfmm.base_dim(L)
+    cw, ch = npfmm.cell_size_f32(stage["limits"], B)
+    lim = stage["limits"]
+    p, ix = out["position"], out["index"]
+    assert np.all(p[:, 0] >= ix[:, 0] * cw + lim[0]) and np.all(p[:, 0] < (ix[:, 0] + 1) * cw + lim[0])
+    assert np.all(p[:, 1] >= ix[:, 1] * ch + lim[1]) and np.all(p[:, 1] < (ix[:, 1] + 1) * ch + lim[1])
+    h = out["head"]
+    for c in range(B * B):
+        assert np.all(ix[h[c]:h[c + 1], 0] == c % B) and np.all(ix[h[c]:h[c + 1], 1] == c // B)
+
+
+def test_p2p_matches_fixture(stage):
+    L = int(stage["L"])
+    eff = orc.p2p(L, stage["position"], stage["weight"], stage["index"], stage["head"], float(stage["eps"]))
+    np.testing.assert_allclose(eff, stage["p2p_effect"], rtol=1e-5, atol=1e-4)
+    assert rel_l2(eff, stage["p2p_effect"]) < 1e-6
+
+
+def test_p2m_matches_fixture(stage):
+    L = int(stage["L"])
+    B = npfmm.base_dim(L)
+    cpos, cw = orc.p2m(L, stage["position"], stage["weight"], stage["head"])
+    np.testing.assert_allclose(cpos[:B * B], stage["p2m_cell_position"], rtol=1e-5, atol=1e-4)
+    np.testing.assert_allclose(cw[:B * B], stage["p2m_cell_weight"], rtol=1e-5, atol=1e-4)
+
+
+def test_m2m_matches_fixture(stage):
+    L = int(stage["L"])
+    B = npfmm.base_dim(L)
+    cpos = np.zeros_like(stage["cube_position"])
+    cw = np.zeros_like(stage["cube_weight"])
+    cpos[:B * B] = stage["cube_position"][:B * B]
+    cw[:B * B] = stage["cube_weight"][:B * B]
+    cpos, cw = orc.m2m(L, cpos, cw)
+    for l in range(L):
+        s = 1 << l
+        got_p = cpos.reshape(L, B, B, 2)[l, ::s, ::s]
+        got_w = cw.reshape(L, B, B)[l, ::s, ::s]
+        exp_p = stage["m2m_cube_position"].reshape(L, B, B, 2)[l, ::s, ::s]
+        exp_w = stage["m2m_cube_weight"].reshape(L, B, B)[l, ::s, ::s]
+        np.testing.assert_allclose(got_p, exp_p, rtol=1e-5, atol=1e-4)
+        np.testing.assert_allclose(got_w, exp_w, rtol=1e-5, atol=1e-4)
+
+
+def test_m2l_matches_fixture(stage):
+    L = int(stage["L"])
+    B = npfmm.base_dim(L)
+    ceff = orc.m2l(L, float(stage["eps"]), stage["cube_position"], stage["cube_weight"])
+    for l in range(L):
+        s = 1 << l
+        got = ceff.reshape(L, B, B, 2)[l, ::s, ::s]
+        exp = stage["m2l_cube_effect"].reshape(L, B, B, 2)[l, ::s, ::s]
+        np.testing.assert_allclose(got, exp, rtol=1e-5, atol=1e-4)
+
+
+def test_l2l_matches_fixture(stage):
+    L = int(stage["L"])
+    B = npfmm.base_dim(L)
+    ceff = orc.l2l(L, stage["cube_position"])
+    np.testing.assert_allclose(ceff[:B * B], stage["l2l_leaf_effect"], rtol=1e-5, atol=1e-4)
+
+
+def test_l2p_matches_fixture(stage):
+    L = int(stage["L"])
+    eff = orc.l2p(L, stage["index"], stage["cube_position"], stage["position"])
+    np.testing.assert_allclose(eff, stage["l2p_effect"], rtol=1e-6, atol=1e-5)
+
+
+def test_postdo_matches_fixture(stage):
+    out = orc.postdo(stage["perm"], stage["position"])
+    np.testing.assert_array_equal(out, stage["postdo_effect_origin"])
+
+
+def test_solve_matches_fixture(golden):
+    g = golden("solve_disk_L4_N2048.npz")
+    eff, clamped = orc.solve(int(g["L"]), g["position"], g["weight"], g["limits"], float(g["eps"]))
+    assert clamped == 0
+    assert rel_l2(eff, g["fmm_effect"]) < 1e-5
+    assert max_rel_to_rms(eff, g["fmm_effect"]) < 1e-4
+    # the monopole-only scheme's own error against the direct sum (about 0.16
+    # at this shape): reported, loosely bounded
+    err = rel_l2(eff, g["direct_effect"])
+    assert 0.01 < err < 0.5
+
+
+@pytest.mark.parametrize("L,n", [(0, 300), (1, 513), (2, 1024), (3, 700), (5, 2048)])
+def test_solve_matches_numpy_fmm_random(L, n):
+    rng = np.random.default_rng(100 + L)
+    limits = np.array([-2.0, -1.0, 3.0, 2.5], np.float32)
+    pos = np.stack([rng.uniform(-2, 3, n), rng.uniform(-1, 2.5, n)], 1).astype(np.float32)
+    w = rng.exponential(1.0, n).astype(np.float32)
+    eps = 1e-2
+    eff, _ = orc.solve(L, pos, w, limits, eps)
+    ref = npfmm.fmm_numpy(L, pos, w, limits, eps)
+    assert rel_l2(eff, ref) < 1e-5
+    assert max_rel_to_rms(eff, ref) < 1e-4
+
+
+def test_level0_is_direct_sum():
+    rng = np.random.default_rng(3)
+    n = 400
+    pos = rng.uniform(-1, 1, (n, 2)).astype(np.float32)
+    w = rng.uniform(0.5, 1.5, n).astype(np.float32)
+    lim = np.array([-1.1, -1.1, 1.1, 1.1], np.float32)
+    eff64 = orc.solve_f64(0, pos.astype(np.float64), w.astype(np.float64), lim.astype(np.float64), 1e-3)
+    d = orc.direct_f64(pos, w, 1e-3)
+    assert rel_l2(eff64, d) < 1e-13
+    eff, _ = orc.solve(0, pos, w, lim, 1e-3)
+    assert rel_l2(eff, d) < 1e-5
+
+
+@pytest.mark.parametrize("L", [1, 2, 3, 4, 6])
+def test_exact_regime_equals_direct(L):
+    """At most one particle per cell on every level: COMs are the particles
+    themselves, so the FMM must reproduce every pair of the direct sum exactly
+    once (a missing or doubled interaction shows here)."""
+    rng = np.random.default_rng(50 + L)
+    lim = np.array([-1.0, -1.0, 1.0, 1.0], np.float64)
+    # one particle in each top-level (4x4) cell, placed away from cell edges
+    top = 2.0 / 4
+    cells = [(i, j) for i in range(4) for j in range(4)]
+    pos = np.array([[lim[0] + (i + rng.uniform(0.2, 0.8)) * top, lim[1] + (j + rng.uniform(0.2, 0.8)) * top]
+                    for i, j in cells])
+    w = rng.uniform(0.5, 1.5, len(cells))
+    eff = orc.solve_f64(L, pos, w, lim, 1e-3)
+    d = orc.direct_f64(pos.astype(np.float32), w.astype(np.float32), 1e-3)
+    # the fp64 oracle is fed the float32-rounded positions the direct sum sees
+    eff32pos = orc.solve_f64(L, pos.astype(np.float32).astype(np.float64), w.astype(np.float32).astype(np.float64),
+                             lim, 1e-3)
+    assert rel_l2(eff32pos, d) < 1e-12
+    assert rel_l2(eff, npfmm.direct64(pos, w, 1e-3)) < 1e-12
+
+
+def test_one_particle_per_leaf_is_not_exact():
+    """The converse of the exact regime: one particle per leaf only is not
+    exact for L >= 2 (coarse cells hold several particles)."""
+    rng = np.random.default_rng(9)
+    L = 2
+    B = npfmm.base_dim(L)
+    lim = np.array([0.0, 0.0, 1.0, 1.0])
+    pos = np.array([[(i + rng.uniform(0.2, 0.8)) / B, (j + rng.uniform(0.2, 0.8)) / B] for i in range(B) for j in range(B)])
+    w = np.ones(B * B)
+    eff = orc.solve_f64(L, pos, w, lim, 1e-3)
+    assert rel_l2(eff, npfmm.direct64(pos, w, 1e-3)) > 1e-3
+
+
+@pytest.mark.parametrize("L", [2, 4, 5])
+def test_momentum_conserved(L):
+    rng = np.random.default_rng(11 + L)
+    n = 3000
+    pos = rng.normal(0, 0.4, (n, 2)).clip(-0.99, 0.99)
+    w = rng.uniform(0.5, 1.5, n)
+    lim = np.array([-1.0, -1.0, 1.0, 1.0])
+    eff = orc.solve_f64(L, pos, w, lim, 1e-3)
+    p = (w[:, None] * eff).sum(0)
+    scale = (w[:, None] * np.abs(eff)).sum(0)
+    assert np.all(np.abs(p) < 1e-12 * scale)
+
+
+def test_mass_conserved_up_the_tree():
+    rng = np.random.default_rng(12)
+    L, n = 5, 5000
+    B = npfmm.base_dim(L)
+    lim = np.array([-1, -1, 1, 1], np.float32)
+    pos = rng.uniform(-1, 1, (n, 2)).astype(np.float32)
+    w = rng.uniform(0.5, 1.5, n).astype(np.float32)
+    pre = orc.predo(L, pos, w, lim)
+    cpos, cw = orc.p2m(L, pre["position"], pre["weight"], pre["head"])
+    cpos, cw = orc.m2m(L, cpos, cw)
+    total = w.astype(np.float64).sum()
+    com = (pos.astype(np.float64) * w[:, None]).sum(0) / total
+    for l in range(L):
+        s = 1 << l
+        lw = cw.reshape(L, B, B)[l, ::s, ::s].astype(np.float64)
+        lp = cpos.reshape(L, B, B, 2)[l, ::s, ::s].astype(np.float64)
+        assert abs(lw.sum() - total) < 1e-4 * total
+        assert np.allclose((lp * lw[..., None]).sum((0, 1)) / lw.sum(), com, atol=1e-5)
+
+
+def test_compute_index_bitexact_and_clamp():
+    rng = np.random.default_rng(5)
+    L = 8
+    lim = np.array([-4, -4, 4, 4], np.float32)
+    B = npfmm.base_dim(L)
+    cw, _ = npfmm.cell_size_f32(lim, B)
+    # points on and next to cell edges, where rounding decides the cell
+    edges = (lim[0] + np.arange(B, dtype=np.float32) * cw).astype(np.float32)
+    xs = np.concatenate([edges, np.nextafter(edges, np.float32(-10)), np.nextafter(edges, np.float32(10)),
+                         rng.uniform(-4, 4, 5000).astype(np.float32)])
+    xs = xs[(xs >= lim[0]) & (xs < lim[2])]
+    pos = np.stack([xs, xs[::-1]], 1).astype(np.float32)
+    idx, clamped = orc.compute_index(L, pos, lim)
+    ref = npfmm.compute_index_f32(pos, lim, L)
+    inbox = (ref >= 0).all(1) & (ref < B).all(1)
+    np.testing.assert_array_equal(idx[inbox], ref[inbox])
+    assert clamped == int((~inbox).sum())
+    # out-of-box and NaN inputs are clamped and counted
+    bad = np.array([[4.0, 0.0], [-4.5, 0.0], [0.0, np.nan], [np.inf, 1.0]], np.float32)
+    idx, clamped = orc.compute_index(L, bad, lim)
+    assert clamped == 4
+    assert idx.min() >= 0 and idx.max() <= B - 1
+    assert tuple(idx[0]) == (B - 1, B // 2)
+
+
+def test_two_body_closed_form():
+    eps = 1e-3
+    pos = np.array([[0.0, 0.0], [0.3, 0.4]], np.float32)
+    w = np.array([2.0, 3.0], np.float32)
+    lim = np.array([-1, -1, 1, 1], np.float32)
+    eff, _ = orc.solve(3, pos, w, lim, eps)
+    r = 0.5
+    f = 1.0 / (r ** 3 + eps)
+    np.testing.assert_allclose(eff[0], 3.0 * f * np.array([0.3, 0.4]), rtol=1e-6)
+    np.testing.assert_allclose(eff[1], -2.0 * f * np.array([0.3, 0.4]), rtol=1e-6)
+
+
+def test_verlet_matches_fixture(golden):
+    g = golden("verlet.npz")
+    cur, pre = orc.update(float(g["tick"]), g["position_cur"], g["position_pre"], g["acceleration"])
+    np.testing.assert_array_equal(cur, g["expect_cur"])
+    np.testing.assert_array_equal(pre, g["expect_pre"])
+
+
+def test_verlet_constant_acceleration_closed_form():
+    dt = 1e-2
+    x0 = np.array([[0.1, -0.2]])
+    v = np.array([[0.5, 0.25]])
+    a = np.array([[-1.0, 2.0]])
+    # x_{-1} chosen so the scheme is exact for constant acceleration
+    xcur = x0.copy()
+    xpre = x0 - v * dt + 0.5 * a * dt * dt
+    for _ in range(50):
+        xcur, xpre = orc.update_f64(dt, xcur, xpre, a)
+    n = 50
+    exact = x0 + n * v * dt + 0.5 * (n * dt) ** 2 * a
+    np.testing.assert_allclose(xcur, exact, rtol=0, atol=1e-13)
