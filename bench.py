@@ -1,0 +1,182 @@
+"""Benchmark of the nbfmm force-evaluation + Verlet step on one MI355X.
+
+A step is one nbfmm::Solver::solve plus the Verlet update (Stars::update
+without prune) over the whole system.  Default workload: the two-disk
+collision, N = 2^20 particles, 512x512 leaves (num_level 8) over [-4,4]^2,
+eps 1e-3, dt 1e-4, seeded synthetic initial conditions.
+
+Prints ONE JSON line with the throughput in particle-steps/s, the P2P
+kernel's roofline figures (HIP events around the P2P launches of an
+instrumented pass over the same steps) and the CPU oracle timed on a bounded
+sample of the same workload as a baseline.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP32_PEAK_TFLOPS = 157.3   # MI355X published FP32 vector peak
+HBM_PEAK_GBS = 8000.0
+P2P_FLOPS_PER_PAIR = 14    # sub 2, r^2 3, sqrt 1, r^3 2, +eps 1, div 1, scale 2, accumulate 2
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", default="two_disk_collision")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU baseline sample length")
+    ap.add_argument("--no-accuracy", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; the multi-GPU sharding is a design "
+                                   "(spatial decomposition with a halo exchange), not built",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from nbfmm_amd import Solver
+    from nbfmm_amd.workloads import by_name
+
+    wl = by_name(args.workload)
+    n = wl.n
+    log(f"[bench] workload {wl.name} N={n} num_level={wl.num_level} steps={args.steps} warmup={args.warmup}")
+    dev = torch.device("cuda:0")
+    s = Solver(wl.num_level, n, wl.limits, wl.eps)
+    x = torch.as_tensor(wl.position, device=dev).contiguous()
+    p = torch.as_tensor(wl.position_pre, device=dev).contiguous()
+    w = torch.as_tensor(wl.weight, device=dev).contiguous()
+    a = torch.zeros_like(x)
+    x0, p0 = x.clone(), p.clone()
+    stream = torch.cuda.Stream()
+
+    # pair count at the initial state (the P2P work figure)
+    with torch.cuda.stream(stream):
+        s.solve(n, x, w, a)
+    stream.synchronize()
+    pairs0, occ0 = s.pair_count(n)
+
+    # warmup + timed region: hipGraph replay of the step
+    with torch.cuda.stream(stream):
+        s.run(n, wl.tick, args.warmup, x, p, w, a, use_graph=True)
+    stream.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    with torch.cuda.stream(stream):
+        s.run(n, wl.tick, args.steps, x, p, w, a, use_graph=True)
+    stream.synchronize()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    dt_s = t1 - t0
+    value = n * args.steps / dt_s
+    pairs1, occ1 = s.pair_count(n)
+    flags = s.flags()
+    log(f"[bench] graph: {dt_s * 1e3 / args.steps:.4f} ms/step, {value:.4e} particle-steps/s, flags={flags}")
+
+    # instrumented pass over the same number of steps from the same state:
+    # HIP events around each P2P launch on the launch stream
+    x.copy_(x0)
+    p.copy_(p0)
+    torch.cuda.synchronize()
+    s.profile(True)
+    s.p2p_time(reset=True)
+    te0 = time.perf_counter()
+    with torch.cuda.stream(stream):
+        s.run(n, wl.tick, args.steps, x, p, w, a, use_graph=False)
+    stream.synchronize()
+    te1 = time.perf_counter()
+    p2p_ms, launches = s.p2p_time(reset=True)
+    s.profile(False)
+    p2p_avg_ms = p2p_ms / max(launches, 1)
+    pairs = 0.5 * (pairs0 + pairs1)
+    achieved = P2P_FLOPS_PER_PAIR * pairs / (p2p_avg_ms * 1e-3) / 1e12
+    log(f"[bench] eager: {(te1 - te0) * 1e3 / args.steps:.4f} ms/step; P2P avg {p2p_avg_ms * 1e3:.1f} us over "
+        f"{launches} launches; pairs/launch {pairs:.4e}; {achieved:.2f} TFLOP/s")
+
+    accuracy = None
+    cpu = None
+    if not args.no_accuracy or not args.no_cpu:
+        from oracle import pyoracle as orc
+        lim = np.array(wl.limits, np.float32)
+    if not args.no_accuracy:
+        # force parity (GPU vs oracle) and the FMM error against the fp64 direct sum
+        ta = time.perf_counter()
+        with torch.cuda.stream(stream):
+            s.solve(n, x0, w, a)
+        stream.synchronize()
+        g = a.cpu().numpy().astype(np.float64)
+        ref, _ = orc.solve(wl.num_level, wl.position, wl.weight, lim, wl.eps)
+        rng = np.random.default_rng(0)
+        tg = np.sort(rng.choice(n, size=min(n, 1024), replace=False))
+        direct = orc.direct_f64(wl.position, wl.weight, wl.eps, tg)
+        accuracy = {
+            "gpu_vs_oracle_rel_l2": float(np.linalg.norm(g - ref) / np.linalg.norm(ref)),
+            "gpu_vs_direct_fp64_rel_l2": float(np.linalg.norm(g[tg] - direct) / np.linalg.norm(direct)),
+            "oracle_vs_direct_fp64_rel_l2": float(np.linalg.norm(ref[tg] - direct) / np.linalg.norm(direct)),
+            "direct_targets": int(tg.size),
+        }
+        log(f"[bench] accuracy {accuracy} ({time.perf_counter() - ta:.1f} s)")
+
+    if not args.no_cpu:
+        # CPU oracle (C++/OpenMP restatement of the reference algorithm) on a
+        # bounded sample: whole steps of the same workload, all host threads.
+        threads = orc.max_threads()
+        tc0 = time.perf_counter()
+        orc.run(wl.num_level, lim, wl.eps, wl.tick, 1, wl.position, wl.position_pre, wl.weight)
+        one = time.perf_counter() - tc0
+        ksteps = int(max(1, min(5, args.cpu_seconds // max(one, 1e-3))))
+        tc0 = time.perf_counter()
+        orc.run(wl.num_level, lim, wl.eps, wl.tick, ksteps, wl.position, wl.position_pre, wl.weight)
+        tcpu = time.perf_counter() - tc0
+        cpu = {"value": n * ksteps / tcpu, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+               "sample": f"{ksteps} full steps of {wl.name} N={n} (after 1 untimed step), OpenMP, "
+                         f"{tcpu:.1f} s"}
+        log(f"[bench] cpu baseline {cpu}")
+
+    out = {
+        "metric": "particle-steps/s at N=2^20; batched+adjoint vs plain, force rel err <=1e-4",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt_s * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": wl.name, "n": n, "num_level": wl.num_level, "leaf_grid": 1 << (wl.num_level + 1),
+                   "limits": list(wl.limits), "eps": wl.eps, "dt": wl.dt, "G": wl.G, "mode": "plain, hipGraph",
+                   "parallelism": "single GPU"},
+        "roofline": {"kernel": "k_p2p<kP2PStep>", "bound": "valu-fp32", "achieved": achieved,
+                     "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
+                     "traffic": None, "avg_launch_us": p2p_avg_ms * 1e3, "launches": launches,
+                     "pairs_per_launch": pairs, "flops_per_pair": P2P_FLOPS_PER_PAIR,
+                     "occupied_leaves": int(occ0)},
+        "cpu_baseline": cpu,
+        "accuracy": accuracy,
+    }
+    s.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

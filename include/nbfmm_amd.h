@@ -1,0 +1,153 @@
+/*
+ * nbfmm_amd: C ABI of the MI355X-native nbfmm force-evaluation and
+ * time-step path.
+ *
+ * It replaces the C++ class nbfmm::Solver (include/nbfmm/core/solver.hpp:22-209)
+ * and the Verlet step of nbfmm::Stars::update (source/nbfmm/display/stars/update.cu:26-55).
+ * Each entry point below names the reference interface it stands for.
+ *
+ * Conventions
+ *  - Every function returns an int status: NBFMM_OK (0) or one of the codes
+ *    below.  Nothing aborts; nbfmm_last_error() gives a message for the
+ *    calling thread.
+ *  - Array arguments named d_* are device pointers (hipMalloc'd or torch
+ *    tensors).  Particle layouts are the reference's: positions and effects
+ *    are float2[N] (x,y interleaved, passed as float*), weights float[N].
+ *  - `stream` is a hipStream_t passed as void* (NULL = the null stream).
+ *    Solver entry points only enqueue work on it: no host synchronisation, no
+ *    allocation, so a call can be captured into a hipGraph.
+ *  - A solver is not thread-safe: use one per stream/thread, as the
+ *    reference's Solver owns its scratch buffers.
+ *  - num_level keeps the reference meaning: the leaf grid is
+ *    base_dim x base_dim with base_dim = 2^(num_level+1)
+ *    (source/nbfmm/core/solver.cu:19), 0 <= num_level <= 8.
+ */
+#ifndef NBFMM_AMD_H_
+#define NBFMM_AMD_H_
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NBFMM_OK 0
+#define NBFMM_ERR_INVALID_ARGUMENT 1 /* bad size, level, limits, epsilon or null pointer */
+#define NBFMM_ERR_OUT_OF_MEMORY 2
+#define NBFMM_ERR_HIP 3              /* a HIP runtime call failed */
+#define NBFMM_ERR_CAPACITY 4         /* num_particle exceeds the solver's capacity */
+#define NBFMM_ERR_NO_DEVICE 5
+
+/* Status flags raised by the device (read with nbfmm_solver_flags). */
+#define NBFMM_FLAG_OUT_OF_BOX 1u /* a particle was outside the limits or NaN; its cell index was clamped */
+
+/* Buffer ids for nbfmm_buffer_read / nbfmm_buffer_write.  They name the
+ * reference Solver members that NBFMM_CHECK makes public
+ * (include/nbfmm/core/solver.hpp:24-76) and use their layouts:
+ *   POSITION float2[N] sorted, EFFECT float2[N] sorted, WEIGHT float[N] sorted,
+ *   INDEX int2[N] sorted cell indices, PERM int[N], HEAD int[base_dim^2+1],
+ *   CELL_POSITION float2 / CELL_EFFECT float2 / CELL_WEIGHT float cubes
+ *   [levels][base_dim][base_dim] with levels = max(num_level, 1); level-l
+ *   cell (X,Y) sits at l*base_dim^2 + (Y<<l)*base_dim + (X<<l)
+ *   (source/nbfmm/core/solver/m2m.cu:35).  Cube slots that no cell of that
+ *   level uses read as zero. */
+#define NBFMM_BUF_POSITION 0
+#define NBFMM_BUF_EFFECT 1
+#define NBFMM_BUF_WEIGHT 2
+#define NBFMM_BUF_INDEX 3
+#define NBFMM_BUF_PERM 4
+#define NBFMM_BUF_HEAD 5
+#define NBFMM_BUF_CELL_POSITION 6
+#define NBFMM_BUF_CELL_EFFECT 7
+#define NBFMM_BUF_CELL_WEIGHT 8
+
+typedef struct nbfmm_solver nbfmm_solver;
+
+/* Version of this ABI (major*10000 + minor*100 + patch). */
+int nbfmm_version(void);
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* nbfmm_last_error(void);
+
+/* nbfmm::Solver::Solver(num_level, max_num_particle, position_limits)
+ * (include/nbfmm/core/solver.hpp:87, source/nbfmm/core/solver.cu:14-46).
+ * position_limits = {x_min, y_min, x_max, y_max}.  epsilon is the kernel
+ * softening that the reference hard-codes to 0.1
+ * (include/nbfmm/core/kernel_function.hpp:35); it must be > 0.  Unlike the
+ * reference, max_num_particle is not limited to 65535 (solver.cu:26).
+ * Allocates all scratch on the current HIP device. */
+int nbfmm_solver_create(int num_level, int max_num_particle, const float position_limits[4], float epsilon,
+                        nbfmm_solver** out);
+
+/* nbfmm::Solver::~Solver (source/nbfmm/core/solver.cu:49-61). */
+int nbfmm_solver_destroy(nbfmm_solver* solver);
+
+/* Sizes the solver was created with. */
+int nbfmm_solver_info(const nbfmm_solver* solver, int* num_level, int* base_dim, int* max_num_particle);
+
+/* nbfmm::Solver::solve (include/nbfmm/core/solver.hpp:102-103,
+ * source/nbfmm/core/solver/solve.cu:11-27): effects of all particles in the
+ * caller's original order.  d_position_origin float2[N], d_weight_origin
+ * float[N], d_effect_origin float2[N] (output). */
+int nbfmm_solve(nbfmm_solver* solver, int num_particle, const float* d_position_origin,
+                const float* d_weight_origin, float* d_effect_origin, void* stream);
+
+/* The eight stages of nbfmm::Solver (include/nbfmm/core/solver.hpp:111-207),
+ * with the same pre/post conditions on the solver's buffers.  solve() is
+ * their composition; the stages exist so each one can be checked in
+ * isolation, as the reference's unit tests do (check/core/solver/ sources). */
+int nbfmm_predo(nbfmm_solver* solver, int num_particle, const float* d_position_origin,
+                const float* d_weight_origin, void* stream);                                 /* predo.cu:105-135 */
+int nbfmm_p2p(nbfmm_solver* solver, int num_particle, void* stream);                        /* p2p.cu:166-176 */
+int nbfmm_p2m(nbfmm_solver* solver, int num_particle, void* stream);                        /* p2m.cu:79-110 */
+int nbfmm_m2m(nbfmm_solver* solver, void* stream);                                          /* m2m.cu:59-74 */
+int nbfmm_m2l(nbfmm_solver* solver, void* stream);                                          /* m2l.cu:65-81 */
+int nbfmm_l2l(nbfmm_solver* solver, void* stream);                                          /* l2l.cu:41-56 */
+int nbfmm_l2p(nbfmm_solver* solver, int num_particle, void* stream);                        /* l2p.cu:40-48 */
+int nbfmm_postdo(nbfmm_solver* solver, int num_particle, float* d_effect_origin, void* stream); /* postdo.cu:38-45 */
+
+/* Access to the solver's buffers in the reference layouts (the NBFMM_CHECK
+ * public members, include/nbfmm/core/solver.hpp:24-28).  `count` is the
+ * number of particles for the per-particle buffers and is ignored for HEAD
+ * and the cubes.  nbfmm_buffer_bytes gives the byte size for that count.
+ * The copies are device-to-device on `stream`. */
+int nbfmm_buffer_bytes(const nbfmm_solver* solver, int buffer_id, int count, size_t* bytes);
+int nbfmm_buffer_read(nbfmm_solver* solver, int buffer_id, int count, void* d_dst, void* stream);
+int nbfmm_buffer_write(nbfmm_solver* solver, int buffer_id, int count, const void* d_src, void* stream);
+
+/* Synchronises the device and returns the accumulated NBFMM_FLAG_* bits of
+ * this solver; clear = 1 resets them. */
+int nbfmm_solver_flags(nbfmm_solver* solver, unsigned* flags, int clear);
+
+/* updateDevice (source/nbfmm/display/stars/update.cu:26-42):
+ *   x_cur' = 2*x_cur - x_pre + acc*tick*tick,  x_pre' = x_cur
+ * for num_star float2 entries.  tick already includes sqrt(G)
+ * (source/nbfmm/display/stars.cu:31). */
+int nbfmm_update(int num_star, float tick, float* d_position_cur, float* d_position_pre,
+                 const float* d_acceleration, void* stream);
+
+/* nbfmm::Stars::update without prune (source/nbfmm/display/stars/update.cu:47-55):
+ * solve at d_position_cur into d_acceleration, then the Verlet update.
+ * The L2P, un-permute and Verlet stages run fused in one kernel. */
+int nbfmm_step(nbfmm_solver* solver, int num_particle, float tick, float* d_position_cur,
+               float* d_position_pre, const float* d_weight, float* d_acceleration, void* stream);
+
+/* num_steps calls of nbfmm_step.  use_graph = 1 captures one step into a
+ * hipGraph on the first call and replays it (stream must not be NULL). */
+int nbfmm_run(nbfmm_solver* solver, int num_particle, float tick, int num_steps, float* d_position_cur,
+              float* d_position_pre, const float* d_weight, float* d_acceleration, int use_graph, void* stream);
+
+/* Stage timing: when enabled, solve/step record HIP events around the P2P
+ * kernel on `stream`; nbfmm_p2p_time synchronises and returns the summed
+ * milliseconds and the number of timed launches since the last reset.
+ * Pair counts for roofline figures come from nbfmm_pair_count (synchronises;
+ * ordered pairs of the P2P stage for the current head, self pairs excluded). */
+int nbfmm_profile_enable(nbfmm_solver* solver, int enable);
+int nbfmm_p2p_time(nbfmm_solver* solver, double* total_ms, int* launches, int reset);
+int nbfmm_pair_count(nbfmm_solver* solver, int num_particle, double* pairs, int* occupied_cells);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NBFMM_AMD_H_ */

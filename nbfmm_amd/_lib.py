@@ -1,0 +1,75 @@
+"""ctypes binding of libnbfmm_amd.so (include/nbfmm_amd.h)."""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "libnbfmm_amd.so")
+_lib = None
+
+c_int, c_float, c_void_p, c_size_t, c_double, c_uint = (ctypes.c_int, ctypes.c_float, ctypes.c_void_p,
+                                                        ctypes.c_size_t, ctypes.c_double, ctypes.c_uint)
+P = c_void_p
+
+# name -> argument types (every function returns int unless listed in _RESTYPE)
+SIGNATURES = {
+    "nbfmm_version": [],
+    "nbfmm_last_error": [],
+    "nbfmm_solver_create": [c_int, c_int, ctypes.POINTER(c_float), c_float, ctypes.POINTER(c_void_p)],
+    "nbfmm_solver_destroy": [P],
+    "nbfmm_solver_info": [P, ctypes.POINTER(c_int), ctypes.POINTER(c_int), ctypes.POINTER(c_int)],
+    "nbfmm_solve": [P, c_int, P, P, P, P],
+    "nbfmm_predo": [P, c_int, P, P, P],
+    "nbfmm_p2p": [P, c_int, P],
+    "nbfmm_p2m": [P, c_int, P],
+    "nbfmm_m2m": [P, P],
+    "nbfmm_m2l": [P, P],
+    "nbfmm_l2l": [P, P],
+    "nbfmm_l2p": [P, c_int, P],
+    "nbfmm_postdo": [P, c_int, P, P],
+    "nbfmm_buffer_bytes": [P, c_int, c_int, ctypes.POINTER(c_size_t)],
+    "nbfmm_buffer_read": [P, c_int, c_int, P, P],
+    "nbfmm_buffer_write": [P, c_int, c_int, P, P],
+    "nbfmm_solver_flags": [P, ctypes.POINTER(c_uint), c_int],
+    "nbfmm_update": [c_int, c_float, P, P, P, P],
+    "nbfmm_step": [P, c_int, c_float, P, P, P, P, P],
+    "nbfmm_run": [P, c_int, c_float, c_int, P, P, P, P, c_int, P],
+    "nbfmm_profile_enable": [P, c_int],
+    "nbfmm_p2p_time": [P, ctypes.POINTER(c_double), ctypes.POINTER(c_int), c_int],
+    "nbfmm_pair_count": [P, c_int, ctypes.POINTER(c_double), ctypes.POINTER(c_int)],
+}
+_RESTYPE = {"nbfmm_last_error": ctypes.c_char_p}
+
+
+class NbfmmError(RuntimeError):
+    CODES = {1: "invalid argument", 2: "out of memory", 3: "HIP error", 4: "capacity exceeded", 5: "no device"}
+
+    def __init__(self, code: int, msg: str):
+        self.code = code
+        super().__init__(f"nbfmm error {code} ({self.CODES.get(code, '?')}): {msg}")
+
+
+def lib_path() -> str:
+    return _LIB_PATH
+
+
+def lib():
+    """Load the HIP library; raises if it was not built (no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            raise ImportError(f"{_LIB_PATH} is missing: build it with `python -m nbfmm_amd.build`")
+        L = ctypes.CDLL(_LIB_PATH)
+        for name, args in SIGNATURES.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = _RESTYPE.get(name, c_int)
+        _lib = L
+    return _lib
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        msg = lib().nbfmm_last_error().decode(errors="replace")
+        raise NbfmmError(rc, msg)

@@ -1,0 +1,117 @@
+// Shared device-side definitions of the nbfmm_amd kernels (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace nbfmm_amd {
+
+constexpr int kWave = 64;
+constexpr int kMaxLevel = 8;          // base_dim <= 512 (source/nbfmm/core/solver.cu:25)
+constexpr int kSortThreads = 256;     // 4 waves per sort tile
+constexpr int kSortWaveItems = 8;     // 64-element chunks per wave
+constexpr int kSortTile = kSortThreads * kSortWaveItems;  // 2048 elements
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanChunk = kScanThreads * kScanItems;     // 2048 cells
+constexpr int kP2PThreads = 256;      // 4 waves, one leaf cell each
+
+// P2P output modes.
+enum P2PMode : int {
+  kP2POnly = 0,     // sorted effect = P2P sum (stage p2p)
+  kP2PFinal = 1,    // + leaf local (L2P), scattered to original order (postdo)
+  kP2PStep = 2,     // as kP2PFinal, then the Verlet update of that particle
+};
+
+// Everything one system's kernels need: sizes, box and the device arrays.
+// Passed by value to every kernel.
+struct SysView {
+  int n;            // particles
+  int L;            // reference num_level
+  int B;            // base_dim = 2^(L+1)
+  int lgB;          // L + 1
+  float xmin, ymin; // lower box corner
+  float cw, ch;     // leaf cell size, computed on the host as (max-min)/B in float
+  float eps;        // kernel softening
+  // caller arrays (original particle order)
+  const float2* pos0;
+  const float* w0;
+  // particle scratch (capacity max_n)
+  uint32_t* key0;   // unsorted cell keys iy*B + ix
+  uint32_t* keyA;   // radix pass-1 keys
+  uint32_t* valA;   // radix pass-1 values (original indices)
+  uint32_t* key;    // sorted cell keys
+  float4* part;     // sorted particles: x, y, m, original index (bits)
+  float2* eff;      // sorted effects (stage API)
+  // cell scratch
+  uint32_t* count;  // B*B leaf histogram, kept zero between solves
+  int* head;        // B*B+1
+  int* occ;         // occupied leaves in row-major order
+  int* nocc;        // number of occupied leaves
+  uint32_t* part_cnt;  // per scan block: particles, then offsets
+  uint32_t* part_occ;  // per scan block: occupied leaves, then offsets
+  uint32_t* hist;   // radix tile histograms [B][ntiles]
+  uint32_t* dtotal; // radix digit totals [B]
+  float4* mpole;    // per level, dense D_l x D_l: COM x, COM y, mass, 0
+  float2* local;    // per level, dense D_l x D_l: M2L effect (L2L accumulates)
+  int lvl_off[kMaxLevel + 1];  // start of level l in mpole/local; lvl_off[L] = total
+  uint32_t* flags;  // NBFMM_FLAG_* bits
+};
+
+// include/nbfmm/core/kernel_function.hpp:29-40 on the GPU:
+//   d = s - t; r = |d|; a += d * m / (r^3 + eps)
+// r comes from the hardware square root and 1/(r^3+eps) from the hardware
+// reciprocal (v_sqrt_f32, v_rcp_f32), not the IEEE-rounded sequences: the
+// result differs from the reference's correctly rounded sqrt and division by
+// about one ulp per pair.  A pair with d = 0 (a particle with itself)
+// contributes exactly 0 because eps > 0.
+__device__ __forceinline__ void kernel_accum(float tx, float ty, float sx, float sy, float m, float eps,
+                                             float& ax, float& ay) {
+  const float dx = sx - tx;
+  const float dy = sy - ty;
+  const float r2 = __builtin_fmaf(dx, dx, dy * dy);
+  const float r = __builtin_amdgcn_sqrtf(r2);
+  const float den = __builtin_fmaf(r2, r, eps);
+  const float f = m * __builtin_amdgcn_rcpf(den);
+  ax = __builtin_fmaf(dx, f, ax);
+  ay = __builtin_fmaf(dy, f, ay);
+}
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
+
+// Inclusive prefix sum over the 64 lanes of a wave.
+__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    const uint32_t o = __shfl_up(v, off, kWave);
+    if (lane >= off) v += o;
+  }
+  return v;
+}
+
+// Exclusive prefix sum over a block of NT threads; returns the block total
+// in *total.  Uses sh[NT/64] of shared scratch.  All threads must call it.
+template <int NT>
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* sh, uint32_t* total) {
+  constexpr int NW = NT / kWave;
+  const int lane = lane_id();
+  const int wave = threadIdx.x / kWave;
+  const uint32_t inc = wave_inclusive_scan(v);
+  if (lane == kWave - 1) sh[wave] = inc;
+  __syncthreads();
+  uint32_t wave_off = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < NW; ++w) {
+    const uint32_t s = sh[w];
+    if (w < wave) wave_off += s;
+    tot += s;
+  }
+  __syncthreads();
+  *total = tot;
+  return wave_off + inc - v;
+}
+
+inline int level_dim(int B, int l) { return B >> l; }
+
+}  // namespace nbfmm_amd

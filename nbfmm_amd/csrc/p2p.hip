@@ -1,0 +1,240 @@
+// Near field (P2P) with the fused L2P, un-permute and Verlet epilogue, plus
+// the per-particle stage kernels of the buffer API.
+//
+// Reference: source/nbfmm/core/solver/p2p.cu:30-176 (p2pNaive / p2pBlock),
+// l2p.cu:23-48, postdo.cu:22-45, source/nbfmm/display/stars/update.cu:26-42.
+//
+// P2P schedule: one wavefront per occupied leaf cell (grid-stride over the
+// occupied-leaf list).  The sources of a leaf are three contiguous ranges of
+// the sorted particles, one per neighbour row [head[x-1 + yB], head[x+1 + yB + 1])
+// (the row-range idea of p2p.cu:105-118).  For a chunk of nt <= 64 targets
+// the wave is split into K = 64/G copies of G = next_pow2(nt) lanes: lane
+// (t, k) accumulates target t against sources k, k+K, k+2K, ... and the K
+// partial sums are combined by a butterfly over the copies.  Sources are
+// read as one float4 {x, y, m, index} per pair; neighbouring leaves reuse
+// them from L1/L2.
+#include "common.hpp"
+#include "launch.hpp"
+#include "../../include/nbfmm_amd.h"
+
+namespace nbfmm_amd {
+
+static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
+
+template <int MODE>
+__global__ __launch_bounds__(kP2PThreads) void k_p2p(SysView v, StepArgs sa) {
+  const int lane = lane_id();
+  const int wave = threadIdx.x / kWave;
+  const int nocc = *v.nocc;
+  const int stride = gridDim.x * (kP2PThreads / kWave);
+  const int B = v.B;
+  const float eps = v.eps;
+  for (int j = blockIdx.x * (kP2PThreads / kWave) + wave; j < nocc; j += stride) {
+    const int c = v.occ[j];
+    const int cx = c & (B - 1), cy = c >> v.lgB;
+    const int t0 = v.head[c], t1 = v.head[c + 1];
+    const int xl = cx > 0 ? cx - 1 : 0;
+    const int xr = cx < B - 1 ? cx + 1 : B - 1;
+    int rs[3], re[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      const int y = cy - 1 + r;
+      if (y >= 0 && y < B) {
+        rs[r] = v.head[xl + y * B];
+        re[r] = v.head[xr + y * B + 1];
+      } else {
+        rs[r] = 0;
+        re[r] = 0;
+      }
+    }
+    float2 loc = make_float2(0.0f, 0.0f);
+    if (MODE != kP2POnly && v.L > 0) loc = v.local[c];
+    for (int tb = t0; tb < t1; tb += kWave) {
+      const int nt = min(kWave, t1 - tb);
+      const int lgG = nt <= 1 ? 0 : 32 - __clz(nt - 1);
+      const int G = 1 << lgG;
+      const int K = kWave >> lgG;
+      const int t = lane & (G - 1);
+      const int k = lane >> lgG;
+      const bool valid = t < nt;
+      const float4 pt = v.part[tb + (valid ? t : 0)];
+      float ax = 0.0f, ay = 0.0f;
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        for (int s = rs[r] + k; s < re[r]; s += K) {
+          const float4 ps = v.part[s];
+          kernel_accum(pt.x, pt.y, ps.x, ps.y, ps.z, eps, ax, ay);
+        }
+      }
+      for (int off = G; off < kWave; off <<= 1) {
+        ax += __shfl_xor(ax, off, kWave);
+        ay += __shfl_xor(ay, off, kWave);
+      }
+      if (k == 0 && valid) {
+        const int i = tb + t;
+        if (MODE == kP2POnly) {
+          v.eff[i] = make_float2(ax, ay);
+        } else {
+#pragma clang fp contract(off)
+          // l2p.cu:34: effect += cell effect; postdo.cu:32: scatter by perm
+          const float ex = ax + loc.x;
+          const float ey = ay + loc.y;
+          const uint32_t o = __float_as_uint(pt.w);
+          sa.acc0[o] = make_float2(ex, ey);
+          if (MODE == kP2PStep) {
+            // update.cu:39-41
+            const float2 x = sa.xcur[o];
+            const float2 xp = sa.xpre[o];
+            float2 nx;
+            nx.x = 2.0f * x.x - xp.x + ex * sa.tick * sa.tick;
+            nx.y = 2.0f * x.y - xp.y + ey * sa.tick * sa.tick;
+            sa.xcur[o] = nx;
+            sa.xpre[o] = x;
+          }
+        }
+      }
+    }
+  }
+}
+
+// L2P stage (l2p.cu:23-35): effect[i] += leaf effect of its cell.
+__global__ __launch_bounds__(256) void k_l2p(SysView v) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= v.n) return;
+  const float2 c = v.local[v.key[i]];
+  float2 e = v.eff[i];
+  e.x = e.x + c.x;
+  e.y = e.y + c.y;
+  v.eff[i] = e;
+}
+
+// postdo (postdo.cu:22-33): effect_origin[perm[i]] = effect[i].
+__global__ __launch_bounds__(256) void k_postdo(SysView v, float2* eff0) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= v.n) return;
+  eff0[__float_as_uint(v.part[i].w)] = v.eff[i];
+}
+
+// updateDevice (update.cu:26-42).
+__global__ __launch_bounds__(256) void k_update(int n, float tick, float2* xcur, float2* xpre, const float2* acc) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float2 x = xcur[i];
+  const float2 xp = xpre[i];
+  const float2 a = acc[i];
+  float2 nx;
+  nx.x = 2.0f * x.x - xp.x + a.x * tick * tick;
+  nx.y = 2.0f * x.y - xp.y + a.y * tick * tick;
+  xcur[i] = nx;
+  xpre[i] = x;
+}
+
+// Particle buffers in the reference layouts.
+__global__ __launch_bounds__(256) void k_part_read(SysView v, int which, int count, void* dst) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  switch (which) {
+    case NBFMM_BUF_POSITION: { const float4 p = v.part[i]; ((float2*)dst)[i] = make_float2(p.x, p.y); break; }
+    case NBFMM_BUF_WEIGHT: ((float*)dst)[i] = v.part[i].z; break;
+    case NBFMM_BUF_PERM: ((int*)dst)[i] = (int)__float_as_uint(v.part[i].w); break;
+    case NBFMM_BUF_EFFECT: ((float2*)dst)[i] = v.eff[i]; break;
+    case NBFMM_BUF_INDEX: {
+      const uint32_t k = v.key[i];
+      ((int2*)dst)[i] = make_int2((int)(k & (uint32_t)(v.B - 1)), (int)(k >> v.lgB));
+      break;
+    }
+    default: break;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_part_write(SysView v, int which, int count, const void* src) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  switch (which) {
+    case NBFMM_BUF_POSITION: { const float2 p = ((const float2*)src)[i]; v.part[i].x = p.x; v.part[i].y = p.y; break; }
+    case NBFMM_BUF_WEIGHT: v.part[i].z = ((const float*)src)[i]; break;
+    case NBFMM_BUF_PERM: v.part[i].w = __uint_as_float((uint32_t)((const int*)src)[i]); break;
+    case NBFMM_BUF_EFFECT: v.eff[i] = ((const float2*)src)[i]; break;
+    case NBFMM_BUF_INDEX: {
+      const int2 x = ((const int2*)src)[i];
+      v.key[i] = (uint32_t)(x.y * v.B + x.x);
+      break;
+    }
+    default: break;
+  }
+}
+
+// Ordered P2P pairs for the current head (self pairs excluded): the work
+// figure of the P2P roofline.
+__global__ __launch_bounds__(256) void k_pair_count(SysView v, unsigned long long* out) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long pairs = 0;
+  if (c < v.B * v.B) {
+    const int B = v.B;
+    const int cx = c & (B - 1), cy = c >> v.lgB;
+    const int nt = v.head[c + 1] - v.head[c];
+    if (nt > 0) {
+      const int xl = cx > 0 ? cx - 1 : 0;
+      const int xr = cx < B - 1 ? cx + 1 : B - 1;
+      long long src = 0;
+      for (int y = cy - 1; y <= cy + 1; ++y) {
+        if (y < 0 || y >= B) continue;
+        src += v.head[xr + y * B + 1] - v.head[xl + y * B];
+      }
+      pairs = (unsigned long long)nt * (unsigned long long)(src - 1);
+    }
+  }
+  // wave reduction then one atomic per wave
+  for (int off = 32; off > 0; off >>= 1) pairs += __shfl_xor(pairs, off, kWave);
+  if (lane_id() == 0 && pairs) atomicAdd(out, pairs);
+}
+
+hipError_t launch_p2p(const SysView& v, int mode, const StepArgs& a, int grid_blocks, hipStream_t s) {
+  if (v.n <= 0) return hipSuccess;
+  const dim3 g(grid_blocks), b(kP2PThreads);
+  switch (mode) {
+    case kP2POnly: hipLaunchKernelGGL(k_p2p<kP2POnly>, g, b, 0, s, v, a); break;
+    case kP2PFinal: hipLaunchKernelGGL(k_p2p<kP2PFinal>, g, b, 0, s, v, a); break;
+    default: hipLaunchKernelGGL(k_p2p<kP2PStep>, g, b, 0, s, v, a); break;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_l2p(const SysView& v, hipStream_t s) {
+  if (v.L <= 0 || v.n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_l2p, dim3(cdiv(v.n, 256)), dim3(256), 0, s, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_postdo(const SysView& v, float2* eff0, hipStream_t s) {
+  if (v.n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_postdo, dim3(cdiv(v.n, 256)), dim3(256), 0, s, v, eff0);
+  return hipGetLastError();
+}
+
+hipError_t launch_update(int n, float tick, float2* xcur, float2* xpre, const float2* acc, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_update, dim3(cdiv(n, 256)), dim3(256), 0, s, n, tick, xcur, xpre, acc);
+  return hipGetLastError();
+}
+
+hipError_t launch_part_read(const SysView& v, int which, int count, void* dst, hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_part_read, dim3(cdiv(count, 256)), dim3(256), 0, s, v, which, count, dst);
+  return hipGetLastError();
+}
+
+hipError_t launch_part_write(const SysView& v, int which, int count, const void* src, hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_part_write, dim3(cdiv(count, 256)), dim3(256), 0, s, v, which, count, src);
+  return hipGetLastError();
+}
+
+hipError_t launch_pair_count(const SysView& v, unsigned long long* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_pair_count, dim3(cdiv(v.B * v.B, 256)), dim3(256), 0, s, v, out);
+  return hipGetLastError();
+}
+
+}  // namespace nbfmm_amd

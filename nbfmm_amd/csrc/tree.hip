@@ -1,0 +1,208 @@
+// Quadtree passes of the monopole FMM: P2M, M2M, M2L, L2L.
+//
+// Reference: source/nbfmm/core/solver/{p2m,m2m,m2l,l2l}.cu.  The reference
+// keeps every level in a sparse B x B x L cube; here each level l is a dense
+// D_l x D_l array (D_l = B >> l), so every pass is a coalesced stream
+// (multipoles float4 {COM x, COM y, mass, 0}, locals float2).  The reference
+// cube layout is produced only by the buffer API (k_cube_read/k_cube_write).
+//
+// All tree arithmetic uses IEEE-rounded division and square root with FMA
+// contraction off, in the same expression order as the reference, so these
+// passes reproduce the CPU oracle bit for bit given the same inputs.
+#include "common.hpp"
+#include "launch.hpp"
+
+namespace nbfmm_amd {
+
+static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
+
+// kernel_function.hpp:29-40 with correctly rounded sqrt and division.
+__device__ __forceinline__ void kernel_accum_ieee(float tx, float ty, float sx, float sy, float m, float eps,
+                                                  float& ax, float& ay) {
+#pragma clang fp contract(off)
+  const float dx = sx - tx;
+  const float dy = sy - ty;
+  const float r = sqrtf(dx * dx + dy * dy);
+  const float tmp = m / (r * r * r + eps);
+  ax += dx * tmp;
+  ay += dy * tmp;
+}
+
+// P2M (p2m.cu:24-110, intended semantics): every leaf gets the centre of mass
+// sum(m*x)/sum(m) and mass sum(m) of its particles, empty leaves (0,0) and 0.
+// One thread per leaf, particles summed in sorted order.
+__global__ __launch_bounds__(256) void k_p2m(SysView v) {
+#pragma clang fp contract(off)
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= v.B * v.B) return;
+  const int a = v.head[c], b = v.head[c + 1];
+  float sx = 0.0f, sy = 0.0f, sw = 0.0f;
+  for (int s = a; s < b; ++s) {
+    const float4 p = v.part[s];
+    sx += p.x * p.z;
+    sy += p.y * p.z;
+    sw += p.z;
+  }
+  float4 out = make_float4(0.0f, 0.0f, sw, 0.0f);
+  if (b != a) {
+    out.x = sx / sw;
+    out.y = sy / sw;
+  }
+  v.mpole[c] = out;
+}
+
+// M2M (m2m.cu:22-54): parent COM = (sum of child COM*mass) / (sum of masses),
+// children (0,0),(1,0),(0,1),(1,1) in that order; a massless parent gets
+// COM (0,0).  One thread per parent cell of level l.
+__global__ __launch_bounds__(256) void k_m2m(SysView v, int l) {
+#pragma clang fp contract(off)
+  const int D = v.B >> l;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= D * D) return;
+  const int X = c % D, Y = c / D;
+  const int Dc = D * 2;
+  const float4* ch = v.mpole + v.lvl_off[l - 1];
+  const float4 c00 = ch[(2 * Y) * Dc + 2 * X];
+  const float4 c10 = ch[(2 * Y) * Dc + 2 * X + 1];
+  const float4 c01 = ch[(2 * Y + 1) * Dc + 2 * X];
+  const float4 c11 = ch[(2 * Y + 1) * Dc + 2 * X + 1];
+  const float px = c00.x * c00.z + c10.x * c10.z + c01.x * c01.z + c11.x * c11.z;
+  const float py = c00.y * c00.z + c10.y * c10.z + c01.y * c01.z + c11.y * c11.z;
+  const float pw = c00.z + c10.z + c01.z + c11.z;
+  float4 out = make_float4(0.0f, 0.0f, pw, 0.0f);
+  if (pw != 0.0f) {
+    out.x = px / pw;
+    out.y = py / pw;
+  }
+  v.mpole[v.lvl_off[l] + c] = out;
+}
+
+// M2L (m2l.cu:24-60), all levels in one launch: each target cell sums the
+// kernel at its own COM over the children of its parent's 3x3 neighbourhood
+// (window [p-2, p+4) per axis) that are not adjacent to it; y outer, x inner.
+__global__ __launch_bounds__(256) void k_m2l(SysView v) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= v.lvl_off[v.L]) return;
+  int l = 0;
+  while (t >= v.lvl_off[l + 1]) ++l;
+  const int D = v.B >> l;
+  const int c = t - v.lvl_off[l];
+  const int tx = c % D, ty = c / D;
+  const float4* mp = v.mpole + v.lvl_off[l];
+  const float4 tc = mp[c];
+  const int px = tx & ~1, py = ty & ~1;
+  float ax = 0.0f, ay = 0.0f;
+  for (int y = py - 2; y < py + 4; ++y) {
+    if (y < 0 || y >= D) continue;
+    for (int x = px - 2; x < px + 4; ++x) {
+      if (x < 0 || x >= D) continue;
+      if (abs(x - tx) > 1 || abs(y - ty) > 1) {
+        const float4 sc = mp[y * D + x];
+        kernel_accum_ieee(tc.x, tc.y, sc.x, sc.y, sc.z, v.eps, ax, ay);
+      }
+    }
+  }
+  v.local[t] = make_float2(ax, ay);
+}
+
+// L2L (l2l.cu:21-36): level l cell += its parent's accumulated effect.
+__global__ __launch_bounds__(256) void k_l2l(SysView v, int l) {
+#pragma clang fp contract(off)
+  const int D = v.B >> l;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= D * D) return;
+  const int X = c % D, Y = c / D;
+  const float2 p = v.local[v.lvl_off[l + 1] + (Y >> 1) * (D >> 1) + (X >> 1)];
+  float2 e = v.local[v.lvl_off[l] + c];
+  e.x = e.x + p.x;
+  e.y = e.y + p.y;
+  v.local[v.lvl_off[l] + c] = e;
+}
+
+// Reference cube [levels][B][B] <-> dense levels.
+__global__ __launch_bounds__(256) void k_cube_read(SysView v, int which, void* dst, int levels) {
+  const int B = v.B;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= levels * B * B) return;
+  const int l = i / (B * B);
+  const int rem = i - l * B * B;
+  const int row = rem / B, col = rem % B;
+  const int s = 1 << l;
+  const bool slot = l < v.L && (row % s) == 0 && (col % s) == 0;
+  const int D = B >> l;
+  const int d = slot ? v.lvl_off[l] + (row / s) * D + (col / s) : 0;
+  if (which == 0) {
+    const float4 m = slot ? v.mpole[d] : make_float4(0, 0, 0, 0);
+    ((float2*)dst)[i] = make_float2(m.x, m.y);
+  } else if (which == 1) {
+    ((float*)dst)[i] = slot ? v.mpole[d].z : 0.0f;
+  } else {
+    ((float2*)dst)[i] = slot ? v.local[d] : make_float2(0, 0);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_cube_write(SysView v, int which, const void* src, int levels) {
+  const int B = v.B;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= levels * B * B) return;
+  const int l = i / (B * B);
+  const int rem = i - l * B * B;
+  const int row = rem / B, col = rem % B;
+  const int s = 1 << l;
+  if (!(l < v.L && (row % s) == 0 && (col % s) == 0)) return;
+  const int D = B >> l;
+  const int d = v.lvl_off[l] + (row / s) * D + (col / s);
+  if (which == 0) {
+    const float2 p = ((const float2*)src)[i];
+    v.mpole[d].x = p.x;
+    v.mpole[d].y = p.y;
+  } else if (which == 1) {
+    v.mpole[d].z = ((const float*)src)[i];
+  } else {
+    v.local[d] = ((const float2*)src)[i];
+  }
+}
+
+hipError_t launch_p2m(const SysView& v, hipStream_t s) {
+  if (v.L <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_p2m, dim3(cdiv(v.B * v.B, 256)), dim3(256), 0, s, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_m2m(const SysView& v, hipStream_t s) {
+  for (int l = 1; l < v.L; ++l) {
+    const int D = v.B >> l;
+    hipLaunchKernelGGL(k_m2m, dim3(cdiv(D * D, 256)), dim3(256), 0, s, v, l);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_m2l(const SysView& v, hipStream_t s) {
+  if (v.L <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_m2l, dim3(cdiv(v.lvl_off[v.L], 256)), dim3(256), 0, s, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_l2l(const SysView& v, hipStream_t s) {
+  for (int l = v.L - 2; l >= 0; --l) {
+    const int D = v.B >> l;
+    hipLaunchKernelGGL(k_l2l, dim3(cdiv(D * D, 256)), dim3(256), 0, s, v, l);
+  }
+  return hipGetLastError();
+}
+
+static int cube_levels(int L) { return L < 1 ? 1 : L; }
+
+hipError_t launch_cube_read(const SysView& v, int which, void* dst, hipStream_t s) {
+  const int levels = cube_levels(v.L);
+  hipLaunchKernelGGL(k_cube_read, dim3(cdiv(levels * v.B * v.B, 256)), dim3(256), 0, s, v, which, dst, levels);
+  return hipGetLastError();
+}
+
+hipError_t launch_cube_write(const SysView& v, int which, const void* src, hipStream_t s) {
+  const int levels = cube_levels(v.L);
+  hipLaunchKernelGGL(k_cube_write, dim3(cdiv(levels * v.B * v.B, 256)), dim3(256), 0, s, v, which, src, levels);
+  return hipGetLastError();
+}
+
+}  // namespace nbfmm_amd

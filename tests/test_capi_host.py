@@ -1,0 +1,73 @@
+"""CPU checks of the C ABI library: it loads, exports every function that
+include/nbfmm_amd.h declares, and rejects bad arguments before touching a
+GPU.  No compute calls are made here."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from conftest import ROOT
+
+HEADER = os.path.join(ROOT, "include", "nbfmm_amd.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(nbfmm_\w+)\s*\(", text)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    from nbfmm_amd import _lib
+    if not os.path.exists(_lib.lib_path()):
+        pytest.fail("libnbfmm_amd.so is not built (python -m nbfmm_amd.build)")
+    return _lib.lib()
+
+
+def test_exports_every_declared_symbol(lib):
+    names = declared_functions()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(lib, n), n
+
+
+def test_binding_table_matches_header():
+    from nbfmm_amd import _lib
+    assert sorted(_lib.SIGNATURES) == declared_functions()
+
+
+def test_version(lib):
+    assert lib.nbfmm_version() == 10000
+
+
+def _create(lib, L, n, lim, eps):
+    arr = (ctypes.c_float * 4)(*lim)
+    h = ctypes.c_void_p()
+    rc = lib.nbfmm_solver_create(L, n, arr, eps, ctypes.byref(h))
+    return rc, h
+
+
+@pytest.mark.parametrize("L,n,lim,eps", [
+    (9, 100, (0, 0, 1, 1), 0.1),      # level beyond 8 (base_dim 1024, solver.cu:25)
+    (-1, 100, (0, 0, 1, 1), 0.1),
+    (3, 0, (0, 0, 1, 1), 0.1),        # no capacity
+    (3, 100, (1, 0, 0, 1), 0.1),      # x_min >= x_max (solver.cu:27)
+    (3, 100, (0, 1, 1, 1), 0.1),
+    (3, 100, (0, 0, 1, 1), 0.0),      # epsilon must be > 0
+    (3, 100, (0, 0, 1, 1), float("nan")),
+])
+def test_create_rejects_bad_arguments(lib, L, n, lim, eps):
+    rc, h = _create(lib, L, n, lim, eps)
+    assert rc == 1
+    assert not h.value
+    assert lib.nbfmm_last_error()
+
+
+def test_null_arguments_rejected(lib):
+    assert lib.nbfmm_solver_destroy(None) == 1
+    assert lib.nbfmm_solve(None, 10, None, None, None, None) == 1
+    assert lib.nbfmm_update(-1, 0.1, None, None, None, None) == 1
+    assert lib.nbfmm_update(5, 0.1, None, None, None, None) == 1
+    assert lib.nbfmm_solver_create(3, 10, None, 0.1, None) == 1
