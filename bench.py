@@ -141,7 +141,7 @@ def main():
         tc0 = time.perf_counter()
         orc.run(wl.num_level, lim, wl.eps, wl.tick, 1, wl.position, wl.position_pre, wl.weight)
         one = time.perf_counter() - tc0
-        ksteps = int(max(1, min(5, args.cpu_seconds // max(one, 1e-3))))
+        ksteps = int(max(1, min(400, args.cpu_seconds // max(one, 1e-3))))
         tc0 = time.perf_counter()
         orc.run(wl.num_level, lim, wl.eps, wl.tick, ksteps, wl.position, wl.position_pre, wl.weight)
         tcpu = time.perf_counter() - tc0

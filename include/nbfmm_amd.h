@@ -146,6 +146,36 @@ int nbfmm_profile_enable(nbfmm_solver* solver, int enable);
 int nbfmm_p2p_time(nbfmm_solver* solver, double* total_ms, int* launches, int reset);
 int nbfmm_pair_count(nbfmm_solver* solver, int num_particle, double* pairs, int* occupied_cells);
 
+/* ---------------------------------------------------------------- batches
+ * Many independent systems with their own particle counts, levels, boxes and
+ * softenings, advanced together: each stage of the step is one launch over
+ * all systems, and nbfmm_batch_run replays the whole step as one hipGraph.
+ * No reference counterpart: the reference solves one system per Solver
+ * (include/nbfmm/core/solver.hpp:22); a batch of S systems gives, system by
+ * system, the same bits as S separate solvers.
+ * Particle arrays are the systems' arrays concatenated in system order;
+ * nbfmm_batch_offsets gives the S+1 offsets (in particles). */
+typedef struct nbfmm_batch nbfmm_batch;
+
+int nbfmm_batch_create(int num_systems, const int* num_particle, const int* num_level,
+                       const float* position_limits /* 4 per system */, const float* epsilon,
+                       nbfmm_batch** out);
+int nbfmm_batch_destroy(nbfmm_batch* batch);
+int nbfmm_batch_info(const nbfmm_batch* batch, int* num_systems, int* total_particles);
+int nbfmm_batch_offsets(const nbfmm_batch* batch, int* part_off /* num_systems + 1 */);
+/* Verlet step sizes (dt*sqrt(G)) per system, host array; synchronises. */
+int nbfmm_batch_set_ticks(nbfmm_batch* batch, const float* ticks);
+/* solve() of every system. */
+int nbfmm_batch_solve(nbfmm_batch* batch, const float* d_position_origin, const float* d_weight_origin,
+                      float* d_effect_origin, void* stream);
+/* Stars::update without prune for every system. */
+int nbfmm_batch_step(nbfmm_batch* batch, float* d_position_cur, float* d_position_pre, const float* d_weight,
+                     float* d_acceleration, void* stream);
+int nbfmm_batch_run(nbfmm_batch* batch, int num_steps, float* d_position_cur, float* d_position_pre,
+                    const float* d_weight, float* d_acceleration, int use_graph, void* stream);
+/* Synchronises; flags[s] = NBFMM_FLAG_* bits of system s. */
+int nbfmm_batch_flags(nbfmm_batch* batch, unsigned* flags, int clear);
+
 #ifdef __cplusplus
 }
 #endif

@@ -38,6 +38,15 @@ SIGNATURES = {
     "nbfmm_profile_enable": [P, c_int],
     "nbfmm_p2p_time": [P, ctypes.POINTER(c_double), ctypes.POINTER(c_int), c_int],
     "nbfmm_pair_count": [P, c_int, ctypes.POINTER(c_double), ctypes.POINTER(c_int)],
+    "nbfmm_batch_create": [c_int, P, P, P, P, ctypes.POINTER(c_void_p)],
+    "nbfmm_batch_destroy": [P],
+    "nbfmm_batch_info": [P, ctypes.POINTER(c_int), ctypes.POINTER(c_int)],
+    "nbfmm_batch_offsets": [P, P],
+    "nbfmm_batch_set_ticks": [P, P],
+    "nbfmm_batch_solve": [P, P, P, P, P],
+    "nbfmm_batch_step": [P, P, P, P, P, P],
+    "nbfmm_batch_run": [P, c_int, P, P, P, P, c_int, P],
+    "nbfmm_batch_flags": [P, P, c_int],
 }
 _RESTYPE = {"nbfmm_last_error": ctypes.c_char_p}
 

@@ -1,0 +1,76 @@
+"""Many independent systems advanced together (C ABI nbfmm_batch_*).
+
+No reference counterpart: the reference runs one nbfmm::Solver per system
+(include/nbfmm/core/solver.hpp:22).  A batch keeps each system's own particle
+count, level, box and softening; every stage of a step is one launch over all
+systems and `run` replays the step as one hipGraph.  System by system the
+results are bit-identical to separate Solver objects.
+
+Particle arrays are the systems' arrays concatenated in system order; use
+`offsets` (S+1 entries) to slice them.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from ._lib import check, lib
+from .solver import _ptr, _stream
+
+
+class Batch:
+    def __init__(self, num_particle, num_level, position_limits, epsilon, ticks=None):
+        if not torch.cuda.is_available():
+            raise RuntimeError("nbfmm_amd needs a HIP GPU; none is available")
+        n = np.ascontiguousarray(num_particle, np.int32)
+        L = np.ascontiguousarray(num_level, np.int32)
+        lim = np.ascontiguousarray(position_limits, np.float32).reshape(-1, 4)
+        eps = np.ascontiguousarray(np.broadcast_to(np.asarray(epsilon, np.float32), n.shape))
+        if not (n.shape == L.shape == eps.shape and lim.shape[0] == n.shape[0]):
+            raise ValueError("one num_particle, num_level, 4 limits and epsilon per system")
+        h = ctypes.c_void_p()
+        check(lib().nbfmm_batch_create(int(n.size), n.ctypes.data, L.ctypes.data, lim.ctypes.data,
+                                       eps.ctypes.data, ctypes.byref(h)))
+        self._h = h
+        self.num_systems = int(n.size)
+        self.num_particle = n.copy()
+        self.num_level = L.copy()
+        off = np.zeros(self.num_systems + 1, np.int32)
+        check(lib().nbfmm_batch_offsets(self._h, off.ctypes.data))
+        self.offsets = off
+        self.total = int(off[-1])
+        if ticks is not None:
+            self.set_ticks(ticks)
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            check(lib().nbfmm_batch_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_ticks(self, ticks):
+        t = np.ascontiguousarray(np.broadcast_to(np.asarray(ticks, np.float32), (self.num_systems,)))
+        check(lib().nbfmm_batch_set_ticks(self._h, t.ctypes.data))
+
+    def solve(self, position, weight, effect, stream=None):
+        check(lib().nbfmm_batch_solve(self._h, _ptr(position), _ptr(weight), _ptr(effect), _stream(stream)))
+
+    def step(self, position_cur, position_pre, weight, acceleration, stream=None):
+        check(lib().nbfmm_batch_step(self._h, _ptr(position_cur), _ptr(position_pre), _ptr(weight),
+                                     _ptr(acceleration), _stream(stream)))
+
+    def run(self, num_steps, position_cur, position_pre, weight, acceleration, use_graph=True, stream=None):
+        check(lib().nbfmm_batch_run(self._h, int(num_steps), _ptr(position_cur), _ptr(position_pre),
+                                    _ptr(weight), _ptr(acceleration), int(bool(use_graph)), _stream(stream)))
+
+    def flags(self, clear=False):
+        f = np.zeros(self.num_systems, np.uint32)
+        check(lib().nbfmm_batch_flags(self._h, f.ctypes.data, int(bool(clear))))
+        return f

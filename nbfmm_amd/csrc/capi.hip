@@ -4,6 +4,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -86,12 +87,21 @@ void free_all(nbfmm_solver* s) {
   s->allocs.clear();
 }
 
-SysView view(nbfmm_solver* s, int n, const float* pos0, const float* w0) {
+SysView view(nbfmm_solver* s, int n) {
   SysView v = s->v;
   v.n = n;
-  v.pos0 = (const float2*)pos0;
-  v.w0 = w0;
   return v;
+}
+
+CallIO call_io(const float* pos0, const float* w0, float* acc0 = nullptr, float* xcur = nullptr,
+               float* xpre = nullptr) {
+  CallIO io;
+  io.pos0 = (const float2*)pos0;
+  io.w0 = w0;
+  io.acc0 = (float2*)acc0;
+  io.xcur = (float2*)xcur;
+  io.xpre = (float2*)xpre;
+  return io;
 }
 
 int check_n(nbfmm_solver* s, int n) {
@@ -103,7 +113,7 @@ int check_n(nbfmm_solver* s, int n) {
 
 hipStream_t as_stream(void* p) { return (hipStream_t)p; }
 
-int p2p_timed(nbfmm_solver* s, const SysView& v, int mode, const StepArgs& a, hipStream_t st) {
+int p2p_timed(nbfmm_solver* s, const SysView& v, int mode, const CallIO& io, hipStream_t st) {
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (s->profile) {
     if (s->ev_used + 2 > s->ev.size()) {
@@ -118,22 +128,21 @@ int p2p_timed(nbfmm_solver* s, const SysView& v, int mode, const StepArgs& a, hi
     s->ev_used += 2;
     NB_HIP(hipEventRecord(e0, st));
   }
-  NB_HIP(launch_p2p(v, mode, a, s->p2p_grid, st));
+  NB_HIP(launch_p2p(v, mode, io, s->p2p_grid, st));
   if (s->profile) NB_HIP(hipEventRecord(e1, st));
   return NBFMM_OK;
 }
 
 // The fused pipeline shared by solve and step.
-int run_pipeline(nbfmm_solver* s, int n, const float* pos0, const float* w0, int mode, const StepArgs& a,
-                 hipStream_t st) {
-  const SysView v = view(s, n, pos0, w0);
-  NB_HIP(launch_predo(v, st));
+int run_pipeline(nbfmm_solver* s, int n, float tick, int mode, const CallIO& io, hipStream_t st) {
+  SysView v = view(s, n);
+  v.tick = tick;
+  NB_HIP(launch_predo(v, io, 1, st));
   if (n == 0) return NBFMM_OK;
-  NB_HIP(launch_p2m(v, st));
-  NB_HIP(launch_m2m(v, st));
+  NB_HIP(launch_upward(v, std::min(n, s->B * s->B), st));  // P2M + M2M
   NB_HIP(launch_m2l(v, st));
-  NB_HIP(launch_l2l(v, st));
-  return p2p_timed(s, v, mode, a, st);
+  // L2L and L2P are folded into the P2P kernel's epilogue
+  return p2p_timed(s, v, mode, io, st);
 }
 
 }  // namespace
@@ -194,18 +203,15 @@ int nbfmm_solver_create(int num_level, int max_num_particle, const float positio
   int rc = NBFMM_OK;
 #define ALLOC(ptr, cnt) \
   if ((rc = dev_alloc(s, &(ptr), (cnt))) != NBFMM_OK) { free_all(s); delete s; return rc; }
-  ALLOC(v.key0, N);
-  ALLOC(v.keyA, N);
-  ALLOC(v.valA, N);
+  ALLOC(v.tkey, N);
+  ALLOC(v.tpart, N);
   ALLOC(v.key, N);
   ALLOC(v.part, N);
   ALLOC(v.eff, N);
-  ALLOC(v.count, (size_t)C);
   ALLOC(v.head, (size_t)C + 1);
+  ALLOC(v.rowstart, (size_t)s->B + 1);
   ALLOC(v.occ, (size_t)C);
   ALLOC(v.nocc, 1);
-  ALLOC(v.part_cnt, (size_t)scan_blocks(s->B));
-  ALLOC(v.part_occ, (size_t)scan_blocks(s->B));
   ALLOC(v.hist, (size_t)s->B * ntiles);
   ALLOC(v.dtotal, (size_t)s->B);
   ALLOC(v.mpole, (size_t)ncells_tree);
@@ -242,55 +248,52 @@ int nbfmm_solve(nbfmm_solver* s, int n, const float* pos0, const float* w0, floa
   int rc = check_n(s, n);
   if (rc) return rc;
   if (n > 0 && (!pos0 || !w0 || !eff0)) return fail(NBFMM_ERR_INVALID_ARGUMENT, "null array");
-  StepArgs a{};
-  a.acc0 = (float2*)eff0;
-  return run_pipeline(s, n, pos0, w0, kP2PFinal, a, as_stream(stream));
+  return run_pipeline(s, n, 0.0f, kP2PFinal, call_io(pos0, w0, eff0), as_stream(stream));
 }
 
 int nbfmm_predo(nbfmm_solver* s, int n, const float* pos0, const float* w0, void* stream) {
   int rc = check_n(s, n);
   if (rc) return rc;
   if (n > 0 && (!pos0 || !w0)) return fail(NBFMM_ERR_INVALID_ARGUMENT, "null array");
-  NB_HIP(launch_predo(view(s, n, pos0, w0), as_stream(stream)));
+  NB_HIP(launch_predo(view(s, n), call_io(pos0, w0), 0, as_stream(stream)));
   return NBFMM_OK;
 }
 
 int nbfmm_p2p(nbfmm_solver* s, int n, void* stream) {
   int rc = check_n(s, n);
   if (rc) return rc;
-  StepArgs a{};
-  return p2p_timed(s, view(s, n, nullptr, nullptr), kP2POnly, a, as_stream(stream));
+  return p2p_timed(s, view(s, n), kP2POnly, call_io(nullptr, nullptr), as_stream(stream));
 }
 
 int nbfmm_p2m(nbfmm_solver* s, int n, void* stream) {
   int rc = check_n(s, n);
   if (rc) return rc;
-  NB_HIP(launch_p2m(view(s, n, nullptr, nullptr), as_stream(stream)));
+  NB_HIP(launch_p2m(view(s, n), as_stream(stream)));
   return NBFMM_OK;
 }
 
 int nbfmm_m2m(nbfmm_solver* s, void* stream) {
   if (!s) return fail(NBFMM_ERR_INVALID_ARGUMENT, "null solver");
-  NB_HIP(launch_m2m(view(s, 0, nullptr, nullptr), as_stream(stream)));
+  NB_HIP(launch_m2m(view(s, 0), as_stream(stream)));
   return NBFMM_OK;
 }
 
 int nbfmm_m2l(nbfmm_solver* s, void* stream) {
   if (!s) return fail(NBFMM_ERR_INVALID_ARGUMENT, "null solver");
-  NB_HIP(launch_m2l(view(s, 0, nullptr, nullptr), as_stream(stream)));
+  NB_HIP(launch_m2l(view(s, 0), as_stream(stream)));
   return NBFMM_OK;
 }
 
 int nbfmm_l2l(nbfmm_solver* s, void* stream) {
   if (!s) return fail(NBFMM_ERR_INVALID_ARGUMENT, "null solver");
-  NB_HIP(launch_l2l(view(s, 0, nullptr, nullptr), as_stream(stream)));
+  NB_HIP(launch_l2l(view(s, 0), as_stream(stream)));
   return NBFMM_OK;
 }
 
 int nbfmm_l2p(nbfmm_solver* s, int n, void* stream) {
   int rc = check_n(s, n);
   if (rc) return rc;
-  NB_HIP(launch_l2p(view(s, n, nullptr, nullptr), as_stream(stream)));
+  NB_HIP(launch_l2p(view(s, n), as_stream(stream)));
   return NBFMM_OK;
 }
 
@@ -298,7 +301,7 @@ int nbfmm_postdo(nbfmm_solver* s, int n, float* eff0, void* stream) {
   int rc = check_n(s, n);
   if (rc) return rc;
   if (n > 0 && !eff0) return fail(NBFMM_ERR_INVALID_ARGUMENT, "null array");
-  NB_HIP(launch_postdo(view(s, n, nullptr, nullptr), (float2*)eff0, as_stream(stream)));
+  NB_HIP(launch_postdo(view(s, n), (float2*)eff0, as_stream(stream)));
   return NBFMM_OK;
 }
 
@@ -323,7 +326,7 @@ int nbfmm_buffer_bytes(const nbfmm_solver* s, int id, int count, size_t* bytes) 
 
 static int buffer_io(nbfmm_solver* s, int id, int count, void* ptr, bool write, hipStream_t st) {
   if (!s || !ptr) return fail(NBFMM_ERR_INVALID_ARGUMENT, "null argument");
-  const SysView v = view(s, count, nullptr, nullptr);
+  const SysView v = view(s, count);
   switch (id) {
     case NBFMM_BUF_POSITION:
     case NBFMM_BUF_EFFECT:
@@ -382,12 +385,7 @@ int nbfmm_step(nbfmm_solver* s, int n, float tick, float* xcur, float* xpre, con
   int rc = check_n(s, n);
   if (rc) return rc;
   if (n > 0 && (!xcur || !xpre || !w || !acc)) return fail(NBFMM_ERR_INVALID_ARGUMENT, "null array");
-  StepArgs a{};
-  a.acc0 = (float2*)acc;
-  a.xcur = (float2*)xcur;
-  a.xpre = (float2*)xpre;
-  a.tick = tick;
-  return run_pipeline(s, n, xcur, w, kP2PStep, a, as_stream(stream));
+  return run_pipeline(s, n, tick, kP2PStep, call_io(xcur, w, acc, xcur, xpre), as_stream(stream));
 }
 
 int nbfmm_run(nbfmm_solver* s, int n, float tick, int steps, float* xcur, float* xpre, const float* w, float* acc,
@@ -451,13 +449,308 @@ int nbfmm_pair_count(nbfmm_solver* s, int n, double* pairs, int* occupied) {
   if (rc) return rc;
   NB_HIP(hipDeviceSynchronize());
   NB_HIP(hipMemset(s->d_pairs, 0, 8));
-  NB_HIP(launch_pair_count(view(s, n, nullptr, nullptr), s->d_pairs, nullptr));
+  NB_HIP(launch_pair_count(view(s, n), s->d_pairs, nullptr));
   unsigned long long h = 0;
   int occ = 0;
   NB_HIP(hipMemcpy(&h, s->d_pairs, 8, hipMemcpyDeviceToHost));
   NB_HIP(hipMemcpy(&occ, s->v.nocc, 4, hipMemcpyDeviceToHost));
   if (pairs) *pairs = (double)h;
   if (occupied) *occupied = occ;
+  return NBFMM_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ batch
+
+struct nbfmm_batch {
+  int S = 0;
+  int total_n = 0;
+  std::vector<int> part_off;      // S+1
+  std::vector<SysView> hv;        // host copies of the views
+  SysView* dv = nullptr;          // device copies
+  BatchLaunch bl{};
+  std::vector<void*> allocs;
+  hipGraphExec_t graph_exec = nullptr;
+  hipGraph_t graph = nullptr;
+  struct {
+    const void *xcur = nullptr, *xpre = nullptr, *w = nullptr, *acc = nullptr;
+    hipStream_t stream = nullptr;
+    bool valid = false;
+  } gkey;
+};
+
+namespace {
+
+template <class T>
+int batch_alloc(nbfmm_batch* b, T** p, size_t count) {
+  void* q = nullptr;
+  const size_t bytes = count * sizeof(T) > 0 ? count * sizeof(T) : 16;
+  hipError_t e = hipMalloc(&q, bytes);
+  if (e != hipSuccess) return fail(NBFMM_ERR_OUT_OF_MEMORY, std::string("hipMalloc: ") + hipGetErrorString(e));
+  b->allocs.push_back(q);
+  e = hipMemset(q, 0, bytes);
+  if (e != hipSuccess) return hip_fail(e, "hipMemset");
+  *p = (T*)q;
+  return NBFMM_OK;
+}
+
+void batch_free(nbfmm_batch* b) {
+  if (b->graph_exec) hipGraphExecDestroy(b->graph_exec);
+  if (b->graph) hipGraphDestroy(b->graph);
+  for (void* p : b->allocs) hipFree(p);
+  b->allocs.clear();
+}
+
+// Builds a launch map from the per-system block counts.
+int make_map(nbfmm_batch* b, const std::vector<int>& blocks, KernelMap* km) {
+  const int S = (int)blocks.size();
+  std::vector<int> start(S + 1, 0);
+  for (int s = 0; s < S; ++s) start[s + 1] = start[s] + blocks[s];
+  std::vector<int> sys(std::max(start[S], 1), 0);
+  for (int s = 0; s < S; ++s)
+    for (int k = start[s]; k < start[s + 1]; ++k) sys[k] = s;
+  int* d = nullptr;
+  int rc = batch_alloc(b, &d, (size_t)(S + 1) + sys.size());
+  if (rc) return rc;
+  NB_HIP(hipMemcpy(d, start.data(), (S + 1) * sizeof(int), hipMemcpyHostToDevice));
+  NB_HIP(hipMemcpy(d + S + 1, sys.data(), sys.size() * sizeof(int), hipMemcpyHostToDevice));
+  km->sys.vs = b->dv;
+  km->sys.bstart = d;
+  km->sys.bsys = d + S + 1;
+  km->blocks = start[S];
+  return NBFMM_OK;
+}
+
+int batch_pipeline(nbfmm_batch* b, int mode, const CallIO& io, hipStream_t st) {
+  NB_HIP(launch_predo_batch(b->bl, io, st));
+  NB_HIP(launch_tree_batch(b->bl, st));
+  NB_HIP(launch_p2p_batch(b->bl, mode, io, st));
+  return NBFMM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int nbfmm_batch_create(int S, const int* num_particle, const int* num_level, const float* limits,
+                       const float* epsilon, nbfmm_batch** out) {
+  if (!out) return fail(NBFMM_ERR_INVALID_ARGUMENT, "out is null");
+  *out = nullptr;
+  if (S <= 0 || !num_particle || !num_level || !limits || !epsilon)
+    return fail(NBFMM_ERR_INVALID_ARGUMENT, "need num_systems > 0 and non-null arrays");
+  long long total = 0;
+  for (int s = 0; s < S; ++s) {
+    if (num_particle[s] <= 0) return fail(NBFMM_ERR_INVALID_ARGUMENT, "every system needs num_particle > 0");
+    if (num_level[s] < 0 || num_level[s] > kMaxLevel)
+      return fail(NBFMM_ERR_INVALID_ARGUMENT, "num_level must be in [0, 8]");
+    const float* lim = limits + 4 * s;
+    if (!(lim[0] < lim[2] && lim[1] < lim[3]))
+      return fail(NBFMM_ERR_INVALID_ARGUMENT, "position_limits must satisfy x_min < x_max and y_min < y_max");
+    if (!(epsilon[s] > 0.0f) || !std::isfinite(epsilon[s]))
+      return fail(NBFMM_ERR_INVALID_ARGUMENT, "epsilon must be > 0");
+    total += num_particle[s];
+  }
+  if (total > 0x7fffffff) return fail(NBFMM_ERR_INVALID_ARGUMENT, "too many particles in total");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(NBFMM_ERR_NO_DEVICE, "no HIP device");
+  int dev = 0, cus = 256;
+  hipGetDevice(&dev);
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
+    cus = prop.multiProcessorCount;
+
+  nbfmm_batch* b = new nbfmm_batch();
+  b->S = S;
+  b->total_n = (int)total;
+  b->part_off.assign(S + 1, 0);
+  b->hv.resize(S);
+  // sizes of the concatenated scratch arrays
+  size_t n_all = 0, head_all = 0, row_all = 0, cell_all = 0, hist_all = 0, tree_all = 0;
+  std::vector<size_t> head_off(S), row_off(S), cell_off(S), hist_off(S), tree_off(S);
+  for (int s = 0; s < S; ++s) {
+    SysView& v = b->hv[s];
+    v = SysView{};
+    v.n = num_particle[s];
+    v.L = num_level[s];
+    v.B = 1 << (v.L + 1);
+    v.lgB = v.L + 1;
+    const float* lim = limits + 4 * s;
+    v.xmin = lim[0];
+    v.ymin = lim[1];
+    v.cw = (lim[2] - lim[0]) / (float)v.B;  // predo.cu:112-113
+    v.ch = (lim[3] - lim[1]) / (float)v.B;
+    v.eps = epsilon[s];
+    v.part_off = (int)n_all;
+    b->part_off[s] = (int)n_all;
+    const int levels = v.L < 1 ? 1 : v.L;
+    int off = 0;
+    for (int l = 0; l <= kMaxLevel; ++l) {
+      v.lvl_off[l] = off;
+      if (l < levels) off += (v.B >> l) * (v.B >> l);
+    }
+    const size_t C = (size_t)v.B * v.B;
+    head_off[s] = head_all;
+    row_off[s] = row_all;
+    cell_off[s] = cell_all;
+    hist_off[s] = hist_all;
+    tree_off[s] = tree_all;
+    n_all += v.n;
+    head_all += C + 1;
+    row_all += v.B + 1;
+    cell_all += C;
+    hist_all += (size_t)v.B * cdiv(v.n, kSortTile);
+    tree_all += off;
+  }
+  b->part_off[S] = (int)n_all;
+  uint32_t *tkey, *key, *hist, *dtotal, *flags;
+  float4 *tpart, *part, *mpole;
+  float2 *eff, *local;
+  int *head, *rowstart, *occ, *nocc;
+  int rc = NBFMM_OK;
+#define BALLOC(ptr, cnt) \
+  if ((rc = batch_alloc(b, &(ptr), (cnt))) != NBFMM_OK) { batch_free(b); delete b; return rc; }
+  BALLOC(tkey, n_all);
+  BALLOC(tpart, n_all);
+  BALLOC(key, n_all);
+  BALLOC(part, n_all);
+  BALLOC(eff, n_all);
+  BALLOC(head, head_all);
+  BALLOC(rowstart, row_all);
+  BALLOC(occ, cell_all);
+  BALLOC(nocc, (size_t)S);
+  BALLOC(hist, hist_all);
+  BALLOC(dtotal, row_all);
+  BALLOC(mpole, tree_all);
+  BALLOC(local, tree_all);
+  BALLOC(flags, (size_t)S);
+  BALLOC(b->dv, (size_t)S);
+#undef BALLOC
+  for (int s = 0; s < S; ++s) {
+    SysView& v = b->hv[s];
+    const size_t po = (size_t)v.part_off;
+    v.tkey = tkey + po;
+    v.tpart = tpart + po;
+    v.key = key + po;
+    v.part = part + po;
+    v.eff = eff + po;
+    v.head = head + head_off[s];
+    v.rowstart = rowstart + row_off[s];
+    v.occ = occ + cell_off[s];
+    v.nocc = nocc + s;
+    v.hist = hist + hist_off[s];
+    v.dtotal = dtotal + row_off[s];
+    v.mpole = mpole + tree_off[s];
+    v.local = local + tree_off[s];
+    v.flags = flags + s;
+  }
+  hipError_t e = hipMemcpy(b->dv, b->hv.data(), S * sizeof(SysView), hipMemcpyHostToDevice);
+  if (e != hipSuccess) { batch_free(b); delete b; return hip_fail(e, "hipMemcpy views"); }
+  // launch maps
+  std::vector<int> tiles(S), rows(S), p2m(S), mtile(S), mtop(S), m2l(S), p2p(S);
+  const int p2p_total = cus * 16;
+  for (int s = 0; s < S; ++s) {
+    const SysView& v = b->hv[s];
+    const int C = v.B * v.B;
+    tiles[s] = cdiv(v.n, kSortTile);
+    rows[s] = v.B;
+    p2m[s] = v.L > 0 ? cdiv(std::min(v.n, C), 256) : 0;
+    mtile[s] = upward_tile_blocks(v.L, v.B);
+    mtop[s] = upward_top_blocks(v.L, v.B);
+    m2l[s] = m2l_blocks(v);
+    const int cap = cdiv(std::min(v.n, C), kP2PThreads / kWave);
+    const int share = (int)((double)p2p_total * v.n / (double)total + 0.5);
+    p2p[s] = std::max(1, std::min(cap, share));
+  }
+  if ((rc = make_map(b, tiles, &b->bl.tiles)) || (rc = make_map(b, rows, &b->bl.rows)) ||
+      (rc = make_map(b, p2m, &b->bl.p2m)) || (rc = make_map(b, mtile, &b->bl.m2m_tile)) ||
+      (rc = make_map(b, mtop, &b->bl.m2m_top)) || (rc = make_map(b, m2l, &b->bl.m2l)) ||
+      (rc = make_map(b, p2p, &b->bl.p2p))) {
+    batch_free(b);
+    delete b;
+    return rc;
+  }
+  *out = b;
+  return NBFMM_OK;
+}
+
+int nbfmm_batch_destroy(nbfmm_batch* b) {
+  if (!b) return fail(NBFMM_ERR_INVALID_ARGUMENT, "null batch");
+  hipDeviceSynchronize();
+  batch_free(b);
+  delete b;
+  return NBFMM_OK;
+}
+
+int nbfmm_batch_info(const nbfmm_batch* b, int* num_systems, int* total_particles) {
+  if (!b) return fail(NBFMM_ERR_INVALID_ARGUMENT, "null batch");
+  if (num_systems) *num_systems = b->S;
+  if (total_particles) *total_particles = b->total_n;
+  return NBFMM_OK;
+}
+
+int nbfmm_batch_offsets(const nbfmm_batch* b, int* part_off) {
+  if (!b || !part_off) return fail(NBFMM_ERR_INVALID_ARGUMENT, "null argument");
+  std::memcpy(part_off, b->part_off.data(), (b->S + 1) * sizeof(int));
+  return NBFMM_OK;
+}
+
+int nbfmm_batch_set_ticks(nbfmm_batch* b, const float* ticks) {
+  if (!b || !ticks) return fail(NBFMM_ERR_INVALID_ARGUMENT, "null argument");
+  for (int s = 0; s < b->S; ++s) b->hv[s].tick = ticks[s];
+  NB_HIP(hipDeviceSynchronize());
+  NB_HIP(hipMemcpy(b->dv, b->hv.data(), b->S * sizeof(SysView), hipMemcpyHostToDevice));
+  return NBFMM_OK;
+}
+
+int nbfmm_batch_solve(nbfmm_batch* b, const float* pos0, const float* w0, float* eff0, void* stream) {
+  if (!b || !pos0 || !w0 || !eff0) return fail(NBFMM_ERR_INVALID_ARGUMENT, "null argument");
+  return batch_pipeline(b, kP2PFinal, call_io(pos0, w0, eff0), as_stream(stream));
+}
+
+int nbfmm_batch_step(nbfmm_batch* b, float* xcur, float* xpre, const float* w, float* acc, void* stream) {
+  if (!b || !xcur || !xpre || !w || !acc) return fail(NBFMM_ERR_INVALID_ARGUMENT, "null argument");
+  return batch_pipeline(b, kP2PStep, call_io(xcur, w, acc, xcur, xpre), as_stream(stream));
+}
+
+int nbfmm_batch_run(nbfmm_batch* b, int steps, float* xcur, float* xpre, const float* w, float* acc, int use_graph,
+                    void* stream) {
+  if (!b || !xcur || !xpre || !w || !acc) return fail(NBFMM_ERR_INVALID_ARGUMENT, "null argument");
+  if (steps < 0) return fail(NBFMM_ERR_INVALID_ARGUMENT, "num_steps < 0");
+  hipStream_t st = as_stream(stream);
+  int rc = NBFMM_OK;
+  if (!use_graph) {
+    for (int k = 0; k < steps; ++k)
+      if ((rc = nbfmm_batch_step(b, xcur, xpre, w, acc, stream))) return rc;
+    return NBFMM_OK;
+  }
+  if (!st) return fail(NBFMM_ERR_INVALID_ARGUMENT, "graph replay needs a non-null stream");
+  auto& g = b->gkey;
+  const bool same = g.valid && g.xcur == xcur && g.xpre == xpre && g.w == w && g.acc == acc && g.stream == st;
+  if (!same) {
+    if (b->graph_exec) { hipGraphExecDestroy(b->graph_exec); b->graph_exec = nullptr; }
+    if (b->graph) { hipGraphDestroy(b->graph); b->graph = nullptr; }
+    g.valid = false;
+    NB_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    rc = nbfmm_batch_step(b, xcur, xpre, w, acc, stream);
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(st, &graph);
+    if (rc) { if (graph) hipGraphDestroy(graph); return rc; }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    b->graph = graph;
+    NB_HIP(hipGraphInstantiate(&b->graph_exec, graph, nullptr, nullptr, 0));
+    g.xcur = xcur; g.xpre = xpre; g.w = w; g.acc = acc; g.stream = st; g.valid = true;
+  }
+  for (int k = 0; k < steps; ++k) NB_HIP(hipGraphLaunch(b->graph_exec, st));
+  return NBFMM_OK;
+}
+
+int nbfmm_batch_flags(nbfmm_batch* b, unsigned* flags, int clear) {
+  if (!b || !flags) return fail(NBFMM_ERR_INVALID_ARGUMENT, "null argument");
+  NB_HIP(hipDeviceSynchronize());
+  for (int s = 0; s < b->S; ++s) NB_HIP(hipMemcpy(flags + s, b->hv[s].flags, 4, hipMemcpyDeviceToHost));
+  if (clear)
+    for (int s = 0; s < b->S; ++s) NB_HIP(hipMemset(b->hv[s].flags, 0, 4));
   return NBFMM_OK;
 }
 

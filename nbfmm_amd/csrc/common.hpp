@@ -15,6 +15,8 @@ constexpr int kScanThreads = 256;
 constexpr int kScanItems = 8;
 constexpr int kScanChunk = kScanThreads * kScanItems;     // 2048 cells
 constexpr int kP2PThreads = 256;      // 4 waves, one leaf cell each
+constexpr int kP2PStage = 256;        // sources staged in LDS per wave and pass
+constexpr int kP2PUnroll = 4;         // sources per lane per inner iteration
 
 // P2P output modes.
 enum P2PMode : int {
@@ -33,29 +35,61 @@ struct SysView {
   float xmin, ymin; // lower box corner
   float cw, ch;     // leaf cell size, computed on the host as (max-min)/B in float
   float eps;        // kernel softening
-  // caller arrays (original particle order)
-  const float2* pos0;
-  const float* w0;
+  float tick;       // Verlet step (dt*sqrt(G)), step mode only
+  int part_off;     // first particle of this system in the caller's arrays
   // particle scratch (capacity max_n)
-  uint32_t* key0;   // unsorted cell keys iy*B + ix
-  uint32_t* keyA;   // radix pass-1 keys
-  uint32_t* valA;   // radix pass-1 values (original indices)
-  uint32_t* key;    // sorted cell keys
+  uint32_t* tkey;   // row-sorted keys (row pass output)
+  float4* tpart;    // row-sorted particles: x, y, m, original index (bits)
+  uint32_t* key;    // sorted cell keys iy*B + ix
   float4* part;     // sorted particles: x, y, m, original index (bits)
   float2* eff;      // sorted effects (stage API)
   // cell scratch
-  uint32_t* count;  // B*B leaf histogram, kept zero between solves
   int* head;        // B*B+1
-  int* occ;         // occupied leaves in row-major order
+  int* rowstart;    // B+1: first sorted slot of each row
+  int* occ;         // occupied leaves (row by row)
   int* nocc;        // number of occupied leaves
-  uint32_t* part_cnt;  // per scan block: particles, then offsets
-  uint32_t* part_occ;  // per scan block: occupied leaves, then offsets
-  uint32_t* hist;   // radix tile histograms [B][ntiles]
-  uint32_t* dtotal; // radix digit totals [B]
+  uint32_t* hist;   // row-pass tile histograms [B][ntiles]
+  uint32_t* dtotal; // row totals [B]
   float4* mpole;    // per level, dense D_l x D_l: COM x, COM y, mass, 0
   float2* local;    // per level, dense D_l x D_l: M2L effect (L2L accumulates)
   int lvl_off[kMaxLevel + 1];  // start of level l in mpole/local; lvl_off[L] = total
   uint32_t* flags;  // NBFMM_FLAG_* bits
+};
+
+// Caller arrays of one call, original particle order; a system's particles
+// start at its part_off.
+struct CallIO {
+  const float2* pos0;  // positions (input)
+  const float* w0;     // masses (input)
+  float2* acc0;        // effects (output of solve/step)
+  float2* xcur;        // Verlet state (step)
+  float2* xpre;
+};
+
+// Which system a block works on, and its block index / block count inside
+// that system.  OneSys: a single system passed by value.  ManySys: a batch;
+// bsys[b] is the system of block b and bstart[s] the first block of system s.
+struct OneSys {
+  SysView v;
+  __device__ __forceinline__ SysView get(int& blk, int& nblk) const {
+    blk = blockIdx.x;
+    nblk = gridDim.x;
+    return v;
+  }
+};
+
+struct ManySys {
+  const SysView* vs;
+  const int* bsys;
+  const int* bstart;
+  __device__ __forceinline__ SysView get(int& blk, int& nblk) const {
+    const int b = blockIdx.x;
+    const int s = bsys[b];
+    const int b0 = bstart[s];
+    blk = b - b0;
+    nblk = bstart[s + 1] - b0;
+    return vs[s];
+  }
 };
 
 // include/nbfmm/core/kernel_function.hpp:29-40 on the GPU:

@@ -5,13 +5,37 @@
 
 namespace nbfmm_amd {
 
+// Launch map of one kernel over a batch: `blocks` blocks in all, ManySys
+// telling each block its system.
+struct KernelMap {
+  ManySys sys;
+  int blocks;
+};
+
+struct BatchLaunch {
+  KernelMap tiles;     // row pass: cdiv(n_s, kSortTile) per system
+  KernelMap rows;      // row scan / column pass: B_s per system
+  KernelMap p2m;       // P2M of occupied leaves: cdiv(min(n_s, B_s^2), 256)
+  KernelMap m2m_tile;  // M2M tiles
+  KernelMap m2m_top;   // M2M top levels
+  KernelMap m2l;       // M2L: cdiv(tree cells, 256)
+  KernelMap p2p;       // P2P: share of the P2P grid
+};
+
 // sort.hip
-hipError_t launch_predo(const SysView& v, hipStream_t s);
+// zero_empty: also clear the level-0 multipole of empty leaves (fused solve)
+hipError_t launch_predo(const SysView& v, const CallIO& io, int zero_empty, hipStream_t s);
+hipError_t launch_predo_batch(const BatchLaunch& b, const CallIO& io, hipStream_t s);
 hipError_t launch_occ_from_head(const SysView& v, hipStream_t s);
-int scan_blocks(int B);
 
 // tree.hip
 hipError_t launch_p2m(const SysView& v, hipStream_t s);
+// fused solve: P2M of occupied leaves (empty ones cleared by predo) + all M2M levels
+hipError_t launch_upward(const SysView& v, int max_occ, hipStream_t s);
+hipError_t launch_tree_batch(const BatchLaunch& b, hipStream_t s);  // P2M, M2M, M2L
+int upward_tile_blocks(int L, int B);
+int upward_top_blocks(int L, int B);
+int m2l_blocks(const SysView& v);
 hipError_t launch_m2m(const SysView& v, hipStream_t s);
 hipError_t launch_m2l(const SysView& v, hipStream_t s);
 hipError_t launch_l2l(const SysView& v, hipStream_t s);
@@ -20,13 +44,8 @@ hipError_t launch_cube_read(const SysView& v, int which, void* dst, hipStream_t 
 hipError_t launch_cube_write(const SysView& v, int which, const void* src, hipStream_t s);
 
 // p2p.hip
-struct StepArgs {
-  float2* acc0;     // original-order output effect
-  float2* xcur;     // Verlet state (kP2PStep only)
-  float2* xpre;
-  float tick;
-};
-hipError_t launch_p2p(const SysView& v, int mode, const StepArgs& a, int grid_blocks, hipStream_t s);
+hipError_t launch_p2p(const SysView& v, int mode, const CallIO& io, int grid_blocks, hipStream_t s);
+hipError_t launch_p2p_batch(const BatchLaunch& b, int mode, const CallIO& io, hipStream_t s);
 hipError_t launch_l2p(const SysView& v, hipStream_t s);
 hipError_t launch_postdo(const SysView& v, float2* eff0, hipStream_t s);
 hipError_t launch_update(int n, float tick, float2* xcur, float2* xpre, const float2* acc, hipStream_t s);

@@ -7,12 +7,15 @@
 // P2P schedule: one wavefront per occupied leaf cell (grid-stride over the
 // occupied-leaf list).  The sources of a leaf are three contiguous ranges of
 // the sorted particles, one per neighbour row [head[x-1 + yB], head[x+1 + yB + 1])
-// (the row-range idea of p2p.cu:105-118).  For a chunk of nt <= 64 targets
-// the wave is split into K = 64/G copies of G = next_pow2(nt) lanes: lane
-// (t, k) accumulates target t against sources k, k+K, k+2K, ... and the K
-// partial sums are combined by a butterfly over the copies.  Sources are
-// read as one float4 {x, y, m, index} per pair; neighbouring leaves reuse
-// them from L1/L2.
+// (the row-range idea of p2p.cu:105-118).  The wave stages them, 256 at a
+// time, in its own LDS buffer as float4 {x, y, m, -} and pads the buffer
+// with zero-mass sources.  For a chunk of nt <= 64 targets the wave is split
+// into K = 64/G copies of G = next_pow2(nt) lanes: lane (t, k) accumulates
+// target t against staged sources k, k+K, k+2K, ... four per iteration, and
+// the K partial sums are combined by a butterfly over the copies.  In the
+// final modes the wave also forms its leaf's local (L2L folded in), adds it
+// (L2P) and scatters to the original order (postdo), and in the step mode
+// applies the Verlet update.
 #include "common.hpp"
 #include "launch.hpp"
 #include "../../include/nbfmm_amd.h"
@@ -21,34 +24,55 @@ namespace nbfmm_amd {
 
 static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 
-template <int MODE>
-__global__ __launch_bounds__(kP2PThreads) void k_p2p(SysView v, StepArgs sa) {
+// Leaf local of cell (cx, cy): the L2L recursion (l2l.cu:21-36) unrolled
+// from the top, eff_0 = m2l_0 + (m2l_1 + (... + m2l_{L-1})), which gives
+// the bits of the per-level in-place L2L.  Wave-uniform loads.
+__device__ __forceinline__ float2 leaf_local(const SysView& v, int cx, int cy) {
+#pragma clang fp contract(off)
+  int l = v.L - 1;
+  float2 acc = v.local[v.lvl_off[l] + (cy >> l) * (v.B >> l) + (cx >> l)];
+  for (l = v.L - 2; l >= 0; --l) {
+    const float2 e = v.local[v.lvl_off[l] + (cy >> l) * (v.B >> l) + (cx >> l)];
+    acc.x = e.x + acc.x;
+    acc.y = e.y + acc.y;
+  }
+  return acc;
+}
+
+// Wave-local ordering of LDS traffic (the staging buffer is private to the
+// wave, so no block barrier is needed).
+__device__ __forceinline__ void wave_lds_fence() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup"); }
+
+template <int MODE, class Sys>
+__global__ __launch_bounds__(kP2PThreads) void k_p2p(Sys sys, CallIO io) {
+  __shared__ float4 s_src[kP2PThreads / kWave][kP2PStage];
+  int blk, nblk;
+  const SysView v = sys.get(blk, nblk);
   const int lane = lane_id();
-  const int wave = threadIdx.x / kWave;
+  // wave-uniform: keeps the cell bookkeeping and loop counters scalar
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  float4* src = s_src[wave];
+  float2* acc0 = io.acc0 + v.part_off;
+  float2* xcur = io.xcur + v.part_off;
+  float2* xpre = io.xpre + v.part_off;
   const int nocc = *v.nocc;
-  const int stride = gridDim.x * (kP2PThreads / kWave);
+  const int stride = nblk * (kP2PThreads / kWave);
   const int B = v.B;
   const float eps = v.eps;
-  for (int j = blockIdx.x * (kP2PThreads / kWave) + wave; j < nocc; j += stride) {
+  for (int j = blk * (kP2PThreads / kWave) + wave; j < nocc; j += stride) {
     const int c = v.occ[j];
     const int cx = c & (B - 1), cy = c >> v.lgB;
     const int t0 = v.head[c], t1 = v.head[c + 1];
     const int xl = cx > 0 ? cx - 1 : 0;
     const int xr = cx < B - 1 ? cx + 1 : B - 1;
-    int rs[3], re[3];
-#pragma unroll
-    for (int r = 0; r < 3; ++r) {
-      const int y = cy - 1 + r;
-      if (y >= 0 && y < B) {
-        rs[r] = v.head[xl + y * B];
-        re[r] = v.head[xr + y * B + 1];
-      } else {
-        rs[r] = 0;
-        re[r] = 0;
-      }
-    }
+    // the three neighbour rows as contiguous ranges of the sorted particles
+    int rs0 = 0, rs1 = 0, rs2 = 0, n0 = 0, n1 = 0, n2 = 0;
+    if (cy > 0) { rs0 = v.head[xl + (cy - 1) * B]; n0 = v.head[xr + (cy - 1) * B + 1] - rs0; }
+    rs1 = v.head[xl + cy * B]; n1 = v.head[xr + cy * B + 1] - rs1;
+    if (cy < B - 1) { rs2 = v.head[xl + (cy + 1) * B]; n2 = v.head[xr + (cy + 1) * B + 1] - rs2; }
+    const int S = n0 + n1 + n2;
     float2 loc = make_float2(0.0f, 0.0f);
-    if (MODE != kP2POnly && v.L > 0) loc = v.local[c];
+    if (MODE != kP2POnly && v.L > 0) loc = leaf_local(v, cx, cy);
     for (int tb = t0; tb < t1; tb += kWave) {
       const int nt = min(kWave, t1 - tb);
       const int lgG = nt <= 1 ? 0 : 32 - __clz(nt - 1);
@@ -59,12 +83,36 @@ __global__ __launch_bounds__(kP2PThreads) void k_p2p(SysView v, StepArgs sa) {
       const bool valid = t < nt;
       const float4 pt = v.part[tb + (valid ? t : 0)];
       float ax = 0.0f, ay = 0.0f;
+      for (int base = 0; base < S; base += kP2PStage) {
+        const int cnt = min(kP2PStage, S - base);
+        // stage up to kP2PStage sources; slots past cnt hold zero-mass
+        // sources, which add exactly nothing
 #pragma unroll
-      for (int r = 0; r < 3; ++r) {
-        for (int s = rs[r] + k; s < re[r]; s += K) {
-          const float4 ps = v.part[s];
-          kernel_accum(pt.x, pt.y, ps.x, ps.y, ps.z, eps, ax, ay);
+        for (int q = 0; q < kP2PStage / kWave; ++q) {
+          const int slot = q * kWave + lane;
+          const int f = base + slot;
+          float4 val = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+          if (slot < cnt) {
+            const int g = f < n0 ? rs0 + f : (f < n0 + n1 ? rs1 + (f - n0) : rs2 + (f - n0 - n1));
+            val = v.part[g];
+          }
+          src[slot] = val;
         }
+        wave_lds_fence();
+        const int step = kP2PUnroll * K;
+        const int iters = (cnt + step - 1) / step;
+        const float4* p = src + k;
+        for (int i = 0; i < iters; ++i, p += step) {
+          const float4 s0 = p[0];
+          const float4 s1 = p[K];
+          const float4 s2 = p[2 * K];
+          const float4 s3 = p[3 * K];
+          kernel_accum(pt.x, pt.y, s0.x, s0.y, s0.z, eps, ax, ay);
+          kernel_accum(pt.x, pt.y, s1.x, s1.y, s1.z, eps, ax, ay);
+          kernel_accum(pt.x, pt.y, s2.x, s2.y, s2.z, eps, ax, ay);
+          kernel_accum(pt.x, pt.y, s3.x, s3.y, s3.z, eps, ax, ay);
+        }
+        wave_lds_fence();
       }
       for (int off = G; off < kWave; off <<= 1) {
         ax += __shfl_xor(ax, off, kWave);
@@ -80,16 +128,16 @@ __global__ __launch_bounds__(kP2PThreads) void k_p2p(SysView v, StepArgs sa) {
           const float ex = ax + loc.x;
           const float ey = ay + loc.y;
           const uint32_t o = __float_as_uint(pt.w);
-          sa.acc0[o] = make_float2(ex, ey);
+          acc0[o] = make_float2(ex, ey);
           if (MODE == kP2PStep) {
             // update.cu:39-41
-            const float2 x = sa.xcur[o];
-            const float2 xp = sa.xpre[o];
+            const float2 x = xcur[o];
+            const float2 xp = xpre[o];
             float2 nx;
-            nx.x = 2.0f * x.x - xp.x + ex * sa.tick * sa.tick;
-            nx.y = 2.0f * x.y - xp.y + ey * sa.tick * sa.tick;
-            sa.xcur[o] = nx;
-            sa.xpre[o] = x;
+            nx.x = 2.0f * x.x - xp.x + ex * v.tick * v.tick;
+            nx.y = 2.0f * x.y - xp.y + ey * v.tick * v.tick;
+            xcur[o] = nx;
+            xpre[o] = x;
           }
         }
       }
@@ -191,14 +239,24 @@ __global__ __launch_bounds__(256) void k_pair_count(SysView v, unsigned long lon
   if (lane_id() == 0 && pairs) atomicAdd(out, pairs);
 }
 
-hipError_t launch_p2p(const SysView& v, int mode, const StepArgs& a, int grid_blocks, hipStream_t s) {
+hipError_t launch_p2p(const SysView& v, int mode, const CallIO& io, int grid_blocks, hipStream_t s) {
   if (v.n <= 0) return hipSuccess;
   const dim3 g(grid_blocks), b(kP2PThreads);
+  const OneSys one{v};
   switch (mode) {
-    case kP2POnly: hipLaunchKernelGGL(k_p2p<kP2POnly>, g, b, 0, s, v, a); break;
-    case kP2PFinal: hipLaunchKernelGGL(k_p2p<kP2PFinal>, g, b, 0, s, v, a); break;
-    default: hipLaunchKernelGGL(k_p2p<kP2PStep>, g, b, 0, s, v, a); break;
+    case kP2POnly: hipLaunchKernelGGL((k_p2p<kP2POnly, OneSys>), g, b, 0, s, one, io); break;
+    case kP2PFinal: hipLaunchKernelGGL((k_p2p<kP2PFinal, OneSys>), g, b, 0, s, one, io); break;
+    default: hipLaunchKernelGGL((k_p2p<kP2PStep, OneSys>), g, b, 0, s, one, io); break;
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_p2p_batch(const BatchLaunch& bl, int mode, const CallIO& io, hipStream_t s) {
+  const dim3 g(bl.p2p.blocks), b(kP2PThreads);
+  if (mode == kP2PFinal)
+    hipLaunchKernelGGL((k_p2p<kP2PFinal, ManySys>), g, b, 0, s, bl.p2p.sys, io);
+  else
+    hipLaunchKernelGGL((k_p2p<kP2PStep, ManySys>), g, b, 0, s, bl.p2p.sys, io);
   return hipGetLastError();
 }
 

@@ -1,128 +1,304 @@
 // Binning and stable sort of particles by leaf cell (the predo stage).
 //
 // Reference: source/nbfmm/core/solver/predo.cu:26-135.
-//   computeParticleIndex  -> k_bin (same IEEE expression, plus clamp + flag)
-//   thrust::sort_by_key   -> two LSD radix passes (digit ix, then digit iy)
-//   extractHead           -> k_cscan_* (exclusive scan of the leaf histogram)
-//   permuteInputVector    -> fused into the last radix pass
+//   computeParticleIndex  -> bin_key (same IEEE expression, plus clamp + flag)
+//   thrust::sort_by_key   -> a stable counting sort by row (iy), then a stable
+//                            counting sort of every row by column (ix)
+//   extractHead           -> written by the row sort (exclusive scan of the
+//                            per-row column histogram, plus the row base)
+//   permuteInputVector    -> position and weight travel with the keys
 //
-// The sort is a least-significant-digit radix sort on key = iy*B + ix with
-// radix B: pass 0 orders by ix, pass 1 by iy.  Each pass is stable, so the
-// result is the stable argsort by (iy, ix) that thrust::sort_by_key gives
-// (row-major int2 operator<, include/nbfmm/utility/operator.hpp:107-109).
-// Per pass: k_radix_up builds per-tile digit histograms in LDS, k_radix_scan
-// turns them into per-(digit, tile) offsets, and k_radix_down ranks each
-// element inside its tile with wave ballots (a 64-lane match of equal
-// digits) and scatters it.
+// The result is the stable argsort by (iy, ix) that thrust::sort_by_key gives
+// (row-major int2 operator<, include/nbfmm/utility/operator.hpp:107-109): the
+// row pass keeps input order inside a row, and the column pass keeps row-pass
+// order inside a cell.
+//
+// Row pass (tiles of 2048 particles in input order): k_row_up builds each
+// tile's row histogram in LDS, k_row_scan turns them into per-(row, tile)
+// offsets, and k_row_down ranks every particle inside its tile (wave ballots
+// find the lanes with equal rows) and writes {key} and {x, y, m, index} to
+// the row-sorted scratch.  Reads of the caller's arrays are coalesced; no
+// gather.
+// Column pass: one block per row sorts the row's contiguous segment by
+// column with an LDS histogram, ranking groups of particles the same way,
+// and writes the final sorted particles, keys, head for the row's cells and
+// the row's occupied cells.
 #include "common.hpp"
 #include "launch.hpp"
 
 namespace nbfmm_amd {
 
+static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
+
 // predo.cu:37-38: floorf((p - lo) / cell) with IEEE division.  Particles
-// outside the box (or NaN) are clamped into the grid and flagged; the
-// reference has no clamp (a particle at x_max indexes cell B).
-__global__ __launch_bounds__(256) void k_bin(SysView v) {
+// outside the box (or NaN) are clamped into the grid and reported through
+// `bad`; the reference has no clamp (a particle at x_max indexes cell B).
+__device__ __forceinline__ uint32_t bin_key(const SysView& v, float2 p, bool& bad) {
 #pragma clang fp contract(off)
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= v.n) return;
-  const float2 p = v.pos0[i];
   const float fx = floorf((p.x - v.xmin) / v.cw);
   const float fy = floorf((p.y - v.ymin) / v.ch);
   int ix, iy;
-  bool bad = false;
+  bad = false;
   if (!(fx >= 0.0f)) { ix = 0; bad = true; }
   else if (fx >= (float)v.B) { ix = v.B - 1; bad = true; }
   else ix = (int)fx;
   if (!(fy >= 0.0f)) { iy = 0; bad = true; }
   else if (fy >= (float)v.B) { iy = v.B - 1; bad = true; }
   else iy = (int)fy;
-  const uint32_t key = (uint32_t)(iy * v.B + ix);
-  v.key0[i] = key;
-  atomicAdd(&v.count[key], 1u);
-  if (bad) atomicOr(v.flags, 1u);
+  return (uint32_t)(iy * v.B + ix);
 }
 
-// Leaf histogram -> head (exclusive scan, head[B*B] = N) and the compacted
-// list of occupied leaves.  Three kernels: block sums, scan of the block
-// sums, block-local scans.  The last one also clears count for the next
-// solve.
-__global__ __launch_bounds__(kScanThreads) void k_cscan_reduce(SysView v) {
-  __shared__ uint32_t sh[kScanThreads / kWave];
-  const int C = v.B * v.B;
-  const int base = blockIdx.x * kScanChunk;
-  uint32_t cnt = 0, occ = 0;
-#pragma unroll
-  for (int j = 0; j < kScanItems; ++j) {
-    const int c = base + j * kScanThreads + threadIdx.x;
-    if (c < C) {
-      const uint32_t k = v.count[c];
-      cnt += k;
-      occ += k ? 1u : 0u;
-    }
+// Lanes of the wave whose `d` (nbits wide) equals this lane's, among the
+// lanes where `valid` holds.
+__device__ __forceinline__ uint64_t match_mask(uint32_t d, bool valid, int nbits) {
+  uint64_t m = __ballot(valid);
+  for (int b = 0; b < nbits; ++b) {
+    const bool bit = (d >> b) & 1u;
+    const uint64_t bal = __ballot(valid && bit);
+    m &= bit ? bal : ~bal;
   }
-  uint32_t tc, to;
-  block_exclusive_scan<kScanThreads>(cnt, sh, &tc);
-  block_exclusive_scan<kScanThreads>(occ, sh, &to);
-  if (threadIdx.x == 0) {
-    v.part_cnt[blockIdx.x] = tc;
-    v.part_occ[blockIdx.x] = to;
-  }
+  return m;
 }
 
-__global__ __launch_bounds__(kScanThreads) void k_cscan_top(SysView v, int nblk) {
-  __shared__ uint32_t sh[kScanThreads / kWave];
-  __shared__ uint32_t carry[2];
-  if (threadIdx.x == 0) { carry[0] = 0; carry[1] = 0; }
+__device__ __forceinline__ uint64_t lanes_below() {
+  const int lane = lane_id();
+  return lane == 0 ? 0ull : (~0ull >> (kWave - lane));
+}
+
+// Orders this wave's LDS accesses (the counters below are private to a wave,
+// so no block barrier is needed between its chunks).
+__device__ __forceinline__ void wave_fence() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup"); }
+
+// ------------------------------------------------------------- row pass
+
+template <class Sys>
+__global__ __launch_bounds__(kSortThreads) void k_row_up(Sys sys, CallIO io) {
+  __shared__ uint32_t h[512];
+  int tile, ntiles;
+  const SysView v = sys.get(tile, ntiles);
+  const float2* pos0 = io.pos0 + v.part_off;
+  for (int d = threadIdx.x; d < v.B; d += kSortThreads) h[d] = 0;
   __syncthreads();
-  for (int b0 = 0; b0 < nblk; b0 += kScanThreads) {
-    const int b = b0 + threadIdx.x;
-    const uint32_t c = b < nblk ? v.part_cnt[b] : 0u;
-    const uint32_t o = b < nblk ? v.part_occ[b] : 0u;
-    uint32_t tc, to;
-    const uint32_t ec = block_exclusive_scan<kScanThreads>(c, sh, &tc);
-    const uint32_t eo = block_exclusive_scan<kScanThreads>(o, sh, &to);
-    const uint32_t c0 = carry[0], o0 = carry[1];
-    if (b < nblk) {
-      v.part_cnt[b] = c0 + ec;
-      v.part_occ[b] = o0 + eo;
+  const int base = tile * kSortTile;
+  bool any_bad = false;
+#pragma unroll
+  for (int j = 0; j < kSortWaveItems; ++j) {
+    const int i = base + j * kSortThreads + threadIdx.x;
+    if (i < v.n) {
+      bool bad;
+      const uint32_t key = bin_key(v, pos0[i], bad);
+      any_bad |= bad;
+      atomicAdd(&h[key >> v.lgB], 1u);
     }
+  }
+  if (any_bad) atomicOr(v.flags, 1u);
+  __syncthreads();
+  for (int d = threadIdx.x; d < v.B; d += kSortThreads) v.hist[(size_t)d * ntiles + tile] = h[d];
+}
+
+// One block per row: exclusive scan of that row's tile counts, in place; the
+// row total goes to dtotal[row].  Block 0 also clears the occupied-cell
+// counter that the column pass fills.
+template <class Sys>
+__global__ __launch_bounds__(kScanThreads) void k_row_scan(Sys sys) {
+  __shared__ uint32_t sh[kScanThreads / kWave];
+  __shared__ uint32_t carry;
+  int d, nrows;
+  const SysView v = sys.get(d, nrows);
+  const int ntiles = (v.n + kSortTile - 1) / kSortTile;
+  uint32_t* row = v.hist + (size_t)d * ntiles;
+  if (threadIdx.x == 0) {
+    carry = 0;
+    if (d == 0) *v.nocc = 0;
+  }
+  __syncthreads();
+  for (int t0 = 0; t0 < ntiles; t0 += kScanThreads) {
+    const int t = t0 + threadIdx.x;
+    const uint32_t c = t < ntiles ? row[t] : 0u;
+    uint32_t tot;
+    const uint32_t e = block_exclusive_scan<kScanThreads>(c, sh, &tot);
+    const uint32_t base = carry;
+    if (t < ntiles) row[t] = base + e;
     __syncthreads();
-    if (threadIdx.x == 0) { carry[0] = c0 + tc; carry[1] = o0 + to; }
+    if (threadIdx.x == 0) carry = base + tot;
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
-    v.head[v.B * v.B] = (int)carry[0];
-    *v.nocc = (int)carry[1];
+  if (threadIdx.x == 0) v.dtotal[d] = carry;
+}
+
+// Stable scatter of one tile by row.  Wave w owns elements [w*512, w*512+512)
+// of the tile as 8 chunks of 64 in input order; a per-wave LDS counter per row
+// carries the running rank across chunks; per-wave counts are then
+// prefix-summed over waves, so rows keep input order (stable).
+template <class Sys>
+__global__ __launch_bounds__(kSortThreads) void k_row_down(Sys sys, CallIO io) {
+  __shared__ uint32_t s_base[512];
+  __shared__ uint32_t s_wcnt[kSortThreads / kWave][512];
+  __shared__ uint32_t sh[kSortThreads / kWave];
+  int tile, ntiles;
+  const SysView v = sys.get(tile, ntiles);
+  const float2* pos0 = io.pos0 + v.part_off;
+  const float* w0 = io.w0 + v.part_off;
+  const int R = v.B;
+  const int lane = lane_id();
+  const int wave = threadIdx.x / kWave;
+  {
+    // row bases: exclusive scan of the row totals (R <= 512, two per thread)
+    const int d0 = 2 * threadIdx.x, d1 = d0 + 1;
+    const uint32_t t0 = d0 < R ? v.dtotal[d0] : 0u;
+    const uint32_t t1 = d1 < R ? v.dtotal[d1] : 0u;
+    uint32_t tot;
+    const uint32_t e = block_exclusive_scan<kSortThreads>(t0 + t1, sh, &tot);
+    if (d0 < R) s_base[d0] = e + v.hist[(size_t)d0 * ntiles + tile];
+    if (d1 < R) s_base[d1] = e + t0 + v.hist[(size_t)d1 * ntiles + tile];
+    if (tile == 0) {
+      if (d0 < R) v.rowstart[d0] = (int)e;
+      if (d1 < R) v.rowstart[d1] = (int)(e + t0);
+      if (threadIdx.x == 0) v.rowstart[R] = (int)tot;
+    }
+  }
+  for (int d = threadIdx.x; d < R; d += kSortThreads) {
+#pragma unroll
+    for (int w = 0; w < kSortThreads / kWave; ++w) s_wcnt[w][d] = 0;
+  }
+  __syncthreads();
+
+  const int wbase = tile * kSortTile + wave * (kWave * kSortWaveItems);
+  const uint64_t below = lanes_below();
+  uint32_t key[kSortWaveItems], rank[kSortWaveItems];
+#pragma unroll
+  for (int j = 0; j < kSortWaveItems; ++j) {
+    const int i = wbase + j * kWave + lane;
+    const bool valid = i < v.n;
+    bool bad;
+    key[j] = valid ? bin_key(v, pos0[i], bad) : 0u;
+    const uint32_t d = key[j] >> v.lgB;
+    const uint64_t m = match_mask(d, valid, v.lgB);
+    const uint32_t r = (uint32_t)__popcll(m & below);
+    uint32_t prior = 0;
+    if (valid) prior = s_wcnt[wave][d];
+    rank[j] = prior + r;
+    wave_fence();
+    if (valid && r == 0) s_wcnt[wave][d] = prior + (uint32_t)__popcll(m);
+    wave_fence();
+  }
+  __syncthreads();
+  for (int d = threadIdx.x; d < R; d += kSortThreads) {
+    uint32_t run = 0;
+#pragma unroll
+    for (int w = 0; w < kSortThreads / kWave; ++w) {
+      const uint32_t c = s_wcnt[w][d];
+      s_wcnt[w][d] = run;
+      run += c;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kSortWaveItems; ++j) {
+    const int i = wbase + j * kWave + lane;
+    if (i >= v.n) continue;
+    const uint32_t d = key[j] >> v.lgB;
+    const uint32_t p = s_base[d] + s_wcnt[wave][d] + rank[j];
+    const float2 q = pos0[i];
+    v.tkey[p] = key[j];
+    v.tpart[p] = make_float4(q.x, q.y, w0[i], __uint_as_float((uint32_t)i));
   }
 }
 
-__global__ __launch_bounds__(kScanThreads) void k_cscan_down(SysView v) {
-  __shared__ uint32_t sh[kScanThreads / kWave];
-  const int C = v.B * v.B;
-  // each thread owns kScanItems consecutive leaves
-  const int c0 = blockIdx.x * kScanChunk + threadIdx.x * kScanItems;
-  uint32_t k[kScanItems];
-  uint32_t cnt = 0, occ = 0;
-#pragma unroll
-  for (int j = 0; j < kScanItems; ++j) {
-    const int c = c0 + j;
-    k[j] = c < C ? v.count[c] : 0u;
-    cnt += k[j];
-    occ += k[j] ? 1u : 0u;
-  }
-  uint32_t tc, to;
-  uint32_t ec = block_exclusive_scan<kScanThreads>(cnt, sh, &tc) + v.part_cnt[blockIdx.x];
-  uint32_t eo = block_exclusive_scan<kScanThreads>(occ, sh, &to) + v.part_occ[blockIdx.x];
-#pragma unroll
-  for (int j = 0; j < kScanItems; ++j) {
-    const int c = c0 + j;
-    if (c < C) {
-      v.head[c] = (int)ec;
-      if (k[j]) v.occ[eo++] = c;
-      ec += k[j];
-      v.count[c] = 0u;
+// ---------------------------------------------------------- column pass
+
+constexpr int kRowThreads = 512;                         // 8 waves per row block
+constexpr int kRowWaves = kRowThreads / kWave;
+constexpr int kRowGroup = kRowThreads * kSortWaveItems;  // 4096 particles per group
+
+// One block per row y.  zero_empty: also clear the level-0 multipole of the
+// row's empty cells (the fused solve's P2M then visits occupied cells only).
+template <class Sys>
+__global__ __launch_bounds__(kRowThreads) void k_row_sort(Sys sys, int zero_empty) {
+  __shared__ uint32_t s_hist[512];
+  __shared__ uint32_t s_start[512];
+  __shared__ uint32_t s_run[512];
+  __shared__ uint32_t s_wcnt[kRowWaves][512];
+  __shared__ uint32_t sh[kRowWaves];
+  __shared__ uint32_t s_occ_base;
+  int y, nrows;
+  const SysView v = sys.get(y, nrows);
+  const int B = v.B;
+  const int lane = lane_id();
+  const int wave = threadIdx.x / kWave;
+  const int rs = v.rowstart[y], re = v.rowstart[y + 1];
+  const uint32_t cmask = (uint32_t)(B - 1);
+
+  for (int x = threadIdx.x; x < B; x += kRowThreads) { s_hist[x] = 0; s_run[x] = 0; }
+  __syncthreads();
+  for (int p = rs + (int)threadIdx.x; p < re; p += kRowThreads) atomicAdd(&s_hist[v.tkey[p] & cmask], 1u);
+  __syncthreads();
+  {
+    // cell starts inside the row, head, occupied cells (B <= 512: one each)
+    const int x = threadIdx.x;
+    const uint32_t c = x < B ? s_hist[x] : 0u;
+    uint32_t tot;
+    const uint32_t e = block_exclusive_scan<kRowThreads>(c, sh, &tot);
+    const uint32_t occ = (x < B && c > 0) ? 1u : 0u;
+    uint32_t nocc;
+    const uint32_t eo = block_exclusive_scan<kRowThreads>(occ, sh, &nocc);
+    if (threadIdx.x == 0) s_occ_base = nocc ? atomicAdd((uint32_t*)v.nocc, nocc) : 0u;
+    if (x < B) {
+      s_start[x] = e;
+      v.head[y * B + x] = rs + (int)e;
+      if (zero_empty && c == 0) v.mpole[y * B + x] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
+    if (y == B - 1 && threadIdx.x == 0) v.head[B * B] = re;
+    __syncthreads();
+    if (occ) v.occ[s_occ_base + eo] = y * B + x;
+  }
+  const uint64_t below = lanes_below();
+  for (int g = rs; g < re; g += kRowGroup) {
+    for (int x = threadIdx.x; x < B; x += kRowThreads) {
+#pragma unroll
+      for (int w = 0; w < kRowWaves; ++w) s_wcnt[w][x] = 0;
+    }
+    __syncthreads();
+    const int wbase = g + wave * (kWave * kSortWaveItems);
+    uint32_t key[kSortWaveItems], rank[kSortWaveItems];
+#pragma unroll
+    for (int j = 0; j < kSortWaveItems; ++j) {
+      const int e = wbase + j * kWave + lane;
+      const bool valid = e < re;
+      key[j] = valid ? v.tkey[e] : 0u;
+      const uint32_t d = key[j] & cmask;
+      const uint64_t m = match_mask(d, valid, v.lgB);
+      const uint32_t r = (uint32_t)__popcll(m & below);
+      uint32_t prior = 0;
+      if (valid) prior = s_wcnt[wave][d];
+      rank[j] = prior + r;
+      wave_fence();
+      if (valid && r == 0) s_wcnt[wave][d] = prior + (uint32_t)__popcll(m);
+      wave_fence();
+    }
+    __syncthreads();
+    // per cell: offset of each wave = earlier groups + earlier waves
+    for (int x = threadIdx.x; x < B; x += kRowThreads) {
+      uint32_t run = s_run[x];
+#pragma unroll
+      for (int w = 0; w < kRowWaves; ++w) {
+        const uint32_t c = s_wcnt[w][x];
+        s_wcnt[w][x] = run;
+        run += c;
+      }
+      s_run[x] = run;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kSortWaveItems; ++j) {
+      const int e = wbase + j * kWave + lane;
+      if (e >= re) continue;
+      const uint32_t d = key[j] & cmask;
+      const int p = rs + (int)(s_start[d] + s_wcnt[wave][d] + rank[j]);
+      v.key[p] = key[j];
+      v.part[p] = v.tpart[e];
+    }
+    __syncthreads();
   }
 }
 
@@ -148,161 +324,34 @@ __global__ __launch_bounds__(kScanThreads) void k_occ_from_head(SysView v) {
   if (threadIdx.x == 0) *v.nocc = (int)carry;
 }
 
-__device__ __forceinline__ uint32_t digit_of(uint32_t key, int pass, const SysView& v) {
-  return pass == 0 ? (key & (uint32_t)(v.B - 1)) : (key >> v.lgB);
-}
-
-// Per-tile digit histogram, stored digit-major: hist[d * ntiles + tile].
-template <int PASS>
-__global__ __launch_bounds__(kSortThreads) void k_radix_up(SysView v, int ntiles) {
-  __shared__ uint32_t h[512];
-  const int R = v.B;
-  for (int d = threadIdx.x; d < R; d += kSortThreads) h[d] = 0;
-  __syncthreads();
-  const int base = blockIdx.x * kSortTile;
-  const uint32_t* keys = PASS == 0 ? v.key0 : v.keyA;
-#pragma unroll
-  for (int j = 0; j < kSortWaveItems; ++j) {
-    const int i = base + j * kSortThreads + threadIdx.x;
-    if (i < v.n) atomicAdd(&h[digit_of(keys[i], PASS, v)], 1u);
-  }
-  __syncthreads();
-  for (int d = threadIdx.x; d < R; d += kSortThreads) v.hist[(size_t)d * ntiles + blockIdx.x] = h[d];
-}
-
-// One block per digit: exclusive scan of that digit's tile counts, in
-// place; the digit total goes to dtotal[d].
-__global__ __launch_bounds__(kScanThreads) void k_radix_scan(SysView v, int ntiles) {
-  __shared__ uint32_t sh[kScanThreads / kWave];
-  __shared__ uint32_t carry;
-  const int d = blockIdx.x;
-  uint32_t* row = v.hist + (size_t)d * ntiles;
-  if (threadIdx.x == 0) carry = 0;
-  __syncthreads();
-  for (int t0 = 0; t0 < ntiles; t0 += kScanThreads) {
-    const int t = t0 + threadIdx.x;
-    const uint32_t c = t < ntiles ? row[t] : 0u;
-    uint32_t tot;
-    const uint32_t e = block_exclusive_scan<kScanThreads>(c, sh, &tot);
-    const uint32_t base = carry;
-    if (t < ntiles) row[t] = base + e;
-    __syncthreads();
-    if (threadIdx.x == 0) carry = base + tot;
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) v.dtotal[d] = carry;
-}
-
-// Stable scatter of one tile.  Wave w owns elements [w*512, w*512+512) of
-// the tile as 8 chunks of 64 in input order; inside a chunk, lanes with equal
-// digits are found with one ballot per digit bit, and a per-wave LDS counter
-// per digit carries the running rank across chunks.  Per-wave counts are then
-// prefix-summed over waves, so the final order inside a digit is the input
-// order (stable).
-template <int PASS>
-__global__ __launch_bounds__(kSortThreads) void k_radix_down(SysView v, int ntiles) {
-  __shared__ uint32_t s_base[512];
-  __shared__ uint32_t s_wcnt[kSortThreads / kWave][512];
-  __shared__ uint32_t sh[kSortThreads / kWave];
-  const int R = v.B;
-  const int lane = lane_id();
-  const int wave = threadIdx.x / kWave;
-  const int tile = blockIdx.x;
-
-  // digit bases: exclusive scan of the digit totals (R <= 512, two per thread)
-  {
-    const int d0 = 2 * threadIdx.x, d1 = d0 + 1;
-    const uint32_t t0 = d0 < R ? v.dtotal[d0] : 0u;
-    const uint32_t t1 = d1 < R ? v.dtotal[d1] : 0u;
-    uint32_t tot;
-    const uint32_t e = block_exclusive_scan<kSortThreads>(t0 + t1, sh, &tot);
-    if (d0 < R) s_base[d0] = e + v.hist[(size_t)d0 * ntiles + tile];
-    if (d1 < R) s_base[d1] = e + t0 + v.hist[(size_t)d1 * ntiles + tile];
-  }
-  for (int d = threadIdx.x; d < R; d += kSortThreads) {
-#pragma unroll
-    for (int w = 0; w < kSortThreads / kWave; ++w) s_wcnt[w][d] = 0;
-  }
-  __syncthreads();
-
-  const uint32_t* keys = PASS == 0 ? v.key0 : v.keyA;
-  const int wbase = tile * kSortTile + wave * (kWave * kSortWaveItems);
-  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (kWave - lane));
-  uint32_t k[kSortWaveItems], val[kSortWaveItems], rank[kSortWaveItems];
-  int nbits = v.lgB;
-#pragma unroll
-  for (int j = 0; j < kSortWaveItems; ++j) {
-    const int i = wbase + j * kWave + lane;
-    const bool valid = i < v.n;
-    k[j] = valid ? keys[i] : 0u;
-    val[j] = valid ? (PASS == 0 ? (uint32_t)i : v.valA[i]) : 0u;
-    const uint32_t d = digit_of(k[j], PASS, v);
-    uint64_t m = __ballot(valid);
-    for (int b = 0; b < nbits; ++b) {
-      const bool bit = (d >> b) & 1u;
-      const uint64_t bal = __ballot(valid && bit);
-      m &= bit ? bal : ~bal;
-    }
-    const uint32_t r = (uint32_t)__popcll(m & lt_mask);
-    const uint32_t cnt = (uint32_t)__popcll(m);
-    uint32_t prior = 0;
-    if (valid) prior = s_wcnt[wave][d];
-    rank[j] = prior + r;
-    __syncthreads();
-    if (valid && r == 0) s_wcnt[wave][d] = prior + cnt;
-    __syncthreads();
-  }
-  // exclusive prefix of the per-wave counts, per digit
-  for (int d = threadIdx.x; d < R; d += kSortThreads) {
-    uint32_t run = 0;
-#pragma unroll
-    for (int w = 0; w < kSortThreads / kWave; ++w) {
-      const uint32_t c = s_wcnt[w][d];
-      s_wcnt[w][d] = run;
-      run += c;
-    }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int j = 0; j < kSortWaveItems; ++j) {
-    const int i = wbase + j * kWave + lane;
-    if (i >= v.n) continue;
-    const uint32_t d = digit_of(k[j], PASS, v);
-    const uint32_t p = s_base[d] + s_wcnt[wave][d] + rank[j];
-    if (PASS == 0) {
-      v.keyA[p] = k[j];
-      v.valA[p] = val[j];
-    } else {
-      // permuteInputVector (predo.cu:86-100) fused: gather position and weight
-      v.key[p] = k[j];
-      const float2 q = v.pos0[val[j]];
-      v.part[p] = make_float4(q.x, q.y, v.w0[val[j]], __uint_as_float(val[j]));
-    }
-  }
+// An empty system: head is all zero and nothing is occupied.
+__global__ __launch_bounds__(256) void k_empty(SysView v) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c <= v.B * v.B) v.head[c] = 0;
+  if (c == 0) *v.nocc = 0;
 }
 
 // ---------------------------------------------------------------- launchers
 
-static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
+hipError_t launch_predo(const SysView& v, const CallIO& io, int zero_empty, hipStream_t s) {
+  if (v.n <= 0) {
+    hipLaunchKernelGGL(k_empty, dim3(cdiv(v.B * v.B + 1, 256)), dim3(256), 0, s, v);
+    return hipGetLastError();
+  }
+  const OneSys one{v};
+  const int ntiles = cdiv(v.n, kSortTile);
+  hipLaunchKernelGGL(k_row_up<OneSys>, dim3(ntiles), dim3(kSortThreads), 0, s, one, io);
+  hipLaunchKernelGGL(k_row_scan<OneSys>, dim3(v.B), dim3(kScanThreads), 0, s, one);
+  hipLaunchKernelGGL(k_row_down<OneSys>, dim3(ntiles), dim3(kSortThreads), 0, s, one, io);
+  hipLaunchKernelGGL(k_row_sort<OneSys>, dim3(v.B), dim3(kRowThreads), 0, s, one, zero_empty);
+  return hipGetLastError();
+}
 
-hipError_t launch_predo(const SysView& v, hipStream_t s) {
-  const int C = v.B * v.B;
-  if (v.n > 0) {
-    hipLaunchKernelGGL(k_bin, dim3(cdiv(v.n, 256)), dim3(256), 0, s, v);
-  }
-  const int nblk = cdiv(C, kScanChunk);
-  hipLaunchKernelGGL(k_cscan_reduce, dim3(nblk), dim3(kScanThreads), 0, s, v);
-  hipLaunchKernelGGL(k_cscan_top, dim3(1), dim3(kScanThreads), 0, s, v, nblk);
-  hipLaunchKernelGGL(k_cscan_down, dim3(nblk), dim3(kScanThreads), 0, s, v);
-  if (v.n > 0) {
-    const int ntiles = cdiv(v.n, kSortTile);
-    hipLaunchKernelGGL(k_radix_up<0>, dim3(ntiles), dim3(kSortThreads), 0, s, v, ntiles);
-    hipLaunchKernelGGL(k_radix_scan, dim3(v.B), dim3(kScanThreads), 0, s, v, ntiles);
-    hipLaunchKernelGGL(k_radix_down<0>, dim3(ntiles), dim3(kSortThreads), 0, s, v, ntiles);
-    hipLaunchKernelGGL(k_radix_up<1>, dim3(ntiles), dim3(kSortThreads), 0, s, v, ntiles);
-    hipLaunchKernelGGL(k_radix_scan, dim3(v.B), dim3(kScanThreads), 0, s, v, ntiles);
-    hipLaunchKernelGGL(k_radix_down<1>, dim3(ntiles), dim3(kSortThreads), 0, s, v, ntiles);
-  }
+hipError_t launch_predo_batch(const BatchLaunch& b, const CallIO& io, hipStream_t s) {
+  hipLaunchKernelGGL(k_row_up<ManySys>, dim3(b.tiles.blocks), dim3(kSortThreads), 0, s, b.tiles.sys, io);
+  hipLaunchKernelGGL(k_row_scan<ManySys>, dim3(b.rows.blocks), dim3(kScanThreads), 0, s, b.rows.sys);
+  hipLaunchKernelGGL(k_row_down<ManySys>, dim3(b.tiles.blocks), dim3(kSortThreads), 0, s, b.tiles.sys, io);
+  hipLaunchKernelGGL(k_row_sort<ManySys>, dim3(b.rows.blocks), dim3(kRowThreads), 0, s, b.rows.sys, 1);
   return hipGetLastError();
 }
 
@@ -310,7 +359,5 @@ hipError_t launch_occ_from_head(const SysView& v, hipStream_t s) {
   hipLaunchKernelGGL(k_occ_from_head, dim3(1), dim3(kScanThreads), 0, s, v);
   return hipGetLastError();
 }
-
-int scan_blocks(int B) { return cdiv(B * B, kScanChunk); }
 
 }  // namespace nbfmm_amd

@@ -31,10 +31,50 @@ __device__ __forceinline__ void kernel_accum_ieee(float tx, float ty, float sx, 
 // P2M (p2m.cu:24-110, intended semantics): every leaf gets the centre of mass
 // sum(m*x)/sum(m) and mass sum(m) of its particles, empty leaves (0,0) and 0.
 // One thread per leaf, particles summed in sorted order.
+__device__ __forceinline__ float4 p2m_leaf(const SysView& v, int c);
+__device__ __forceinline__ float4 m2m_combine(float4 c00, float4 c10, float4 c01, float4 c11);
+
 __global__ __launch_bounds__(256) void k_p2m(SysView v) {
-#pragma clang fp contract(off)
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= v.B * v.B) return;
+  v.mpole[c] = p2m_leaf(v, c);
+}
+
+// M2M (m2m.cu:22-54): parent COM = (sum of child COM*mass) / (sum of masses),
+// children (0,0),(1,0),(0,1),(1,1) in that order; a massless parent gets
+// COM (0,0).  One thread per parent cell of level l.
+__global__ __launch_bounds__(256) void k_m2m(SysView v, int l) {
+  const int D = v.B >> l;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= D * D) return;
+  const int X = c % D, Y = c / D;
+  const int Dc = D * 2;
+  const float4* ch = v.mpole + v.lvl_off[l - 1];
+  const float4 c00 = ch[(2 * Y) * Dc + 2 * X];
+  const float4 c10 = ch[(2 * Y) * Dc + 2 * X + 1];
+  const float4 c01 = ch[(2 * Y + 1) * Dc + 2 * X];
+  const float4 c11 = ch[(2 * Y + 1) * Dc + 2 * X + 1];
+  v.mpole[v.lvl_off[l] + c] = m2m_combine(c00, c10, c01, c11);
+}
+
+// Parent multipole from its four children (m2m.cu:40-53), the expression
+// shared by k_m2m and the fused upward pass.
+__device__ __forceinline__ float4 m2m_combine(float4 c00, float4 c10, float4 c01, float4 c11) {
+#pragma clang fp contract(off)
+  const float px = c00.x * c00.z + c10.x * c10.z + c01.x * c01.z + c11.x * c11.z;
+  const float py = c00.y * c00.z + c10.y * c10.z + c01.y * c01.z + c11.y * c11.z;
+  const float pw = c00.z + c10.z + c01.z + c11.z;
+  float4 out = make_float4(0.0f, 0.0f, pw, 0.0f);
+  if (pw != 0.0f) {
+    out.x = px / pw;
+    out.y = py / pw;
+  }
+  return out;
+}
+
+// Leaf multipole from its sorted particle range (p2m.cu:24-74, intended).
+__device__ __forceinline__ float4 p2m_leaf(const SysView& v, int c) {
+#pragma clang fp contract(off)
   const int a = v.head[c], b = v.head[c + 1];
   float sx = 0.0f, sy = 0.0f, sw = 0.0f;
   for (int s = a; s < b; ++s) {
@@ -48,40 +88,101 @@ __global__ __launch_bounds__(256) void k_p2m(SysView v) {
     out.x = sx / sw;
     out.y = sy / sw;
   }
-  v.mpole[c] = out;
+  return out;
 }
 
-// M2M (m2m.cu:22-54): parent COM = (sum of child COM*mass) / (sum of masses),
-// children (0,0),(1,0),(0,1),(1,1) in that order; a massless parent gets
-// COM (0,0).  One thread per parent cell of level l.
-__global__ __launch_bounds__(256) void k_m2m(SysView v, int l) {
+constexpr int kUpTile = 16;  // leaves per side of one M2M tile block
+
+// P2M of the occupied leaves only (the fused solve's predo already cleared the
+// empty ones).  One thread per occupied leaf, particles summed in sorted
+// order exactly as p2m_leaf, loads issued four at a time.
+template <class Sys>
+__global__ __launch_bounds__(256) void k_p2m_occ(Sys sys) {
 #pragma clang fp contract(off)
-  const int D = v.B >> l;
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= D * D) return;
-  const int X = c % D, Y = c / D;
-  const int Dc = D * 2;
-  const float4* ch = v.mpole + v.lvl_off[l - 1];
-  const float4 c00 = ch[(2 * Y) * Dc + 2 * X];
-  const float4 c10 = ch[(2 * Y) * Dc + 2 * X + 1];
-  const float4 c01 = ch[(2 * Y + 1) * Dc + 2 * X];
-  const float4 c11 = ch[(2 * Y + 1) * Dc + 2 * X + 1];
-  const float px = c00.x * c00.z + c10.x * c10.z + c01.x * c01.z + c11.x * c11.z;
-  const float py = c00.y * c00.z + c10.y * c10.z + c01.y * c01.z + c11.y * c11.z;
-  const float pw = c00.z + c10.z + c01.z + c11.z;
-  float4 out = make_float4(0.0f, 0.0f, pw, 0.0f);
-  if (pw != 0.0f) {
-    out.x = px / pw;
-    out.y = py / pw;
+  int blk, nblk;
+  const SysView v = sys.get(blk, nblk);
+  const int j = blk * blockDim.x + threadIdx.x;
+  if (j >= *v.nocc) return;
+  const int c = v.occ[j];
+  const int a = v.head[c], b = v.head[c + 1];
+  float sx = 0.0f, sy = 0.0f, sw = 0.0f;
+  int s = a;
+  for (; s + 4 <= b; s += 4) {
+    const float4 p0 = v.part[s], p1 = v.part[s + 1], p2 = v.part[s + 2], p3 = v.part[s + 3];
+    sx += p0.x * p0.z; sy += p0.y * p0.z; sw += p0.z;
+    sx += p1.x * p1.z; sy += p1.y * p1.z; sw += p1.z;
+    sx += p2.x * p2.z; sy += p2.y * p2.z; sw += p2.z;
+    sx += p3.x * p3.z; sy += p3.y * p3.z; sw += p3.z;
   }
-  v.mpole[v.lvl_off[l] + c] = out;
+  for (; s < b; ++s) {
+    const float4 p = v.part[s];
+    sx += p.x * p.z;
+    sy += p.y * p.z;
+    sw += p.z;
+  }
+  v.mpole[c] = make_float4(sx / sw, sy / sw, sw, 0.0f);
+}
+
+// M2M levels 1..min(L-1, 4) for one kUpTile x kUpTile tile of leaves, built
+// in LDS from the tile's level-0 multipoles.
+template <class Sys>
+__global__ __launch_bounds__(kUpTile * kUpTile) void k_m2m_tile(Sys sys) {
+  __shared__ float4 lv[2][kUpTile * kUpTile];
+  int blk, nblk;
+  const SysView v = sys.get(blk, nblk);
+  const int B = v.B;
+  const int T = B < kUpTile ? B : kUpTile;
+  const int tiles = B / T;
+  const int X0 = (blk % tiles) * T, Y0 = (blk / tiles) * T;
+  const int t = threadIdx.x;
+  int lgT = 0;
+  while ((1 << lgT) < T) ++lgT;
+  if (t < T * T) lv[0][t] = v.mpole[(Y0 + t / T) * B + X0 + t % T];
+  __syncthreads();
+  const int top = (v.L - 1) < lgT ? (v.L - 1) : lgT;
+  int cur = 0;
+  for (int l = 1; l <= top; ++l) {
+    const int Dc = T >> (l - 1), Dl = T >> l;
+    if (t < Dl * Dl) {
+      const int x = t % Dl, y = t / Dl;
+      const float4* ch = lv[cur];
+      const float4 m = m2m_combine(ch[(2 * y) * Dc + 2 * x], ch[(2 * y) * Dc + 2 * x + 1],
+                                   ch[(2 * y + 1) * Dc + 2 * x], ch[(2 * y + 1) * Dc + 2 * x + 1]);
+      lv[cur ^ 1][t] = m;
+      v.mpole[v.lvl_off[l] + ((Y0 >> l) + y) * (B >> l) + (X0 >> l) + x] = m;
+    }
+    cur ^= 1;
+    __syncthreads();
+  }
+}
+
+// The M2M levels above the tiles (levels lgT+1..L-1), one block.
+template <class Sys>
+__global__ __launch_bounds__(1024) void k_m2m_top(Sys sys) {
+  int blk, nblk;
+  const SysView v = sys.get(blk, nblk);
+  int first = 1;
+  while ((1 << (first - 1)) < (v.B < kUpTile ? v.B : kUpTile)) ++first;
+  for (int l = first; l < v.L; ++l) {
+    const int D = v.B >> l, Dc = D * 2;
+    const float4* ch = v.mpole + v.lvl_off[l - 1];
+    for (int c = threadIdx.x; c < D * D; c += blockDim.x) {
+      const int x = c % D, y = c / D;
+      v.mpole[v.lvl_off[l] + c] = m2m_combine(ch[(2 * y) * Dc + 2 * x], ch[(2 * y) * Dc + 2 * x + 1],
+                                              ch[(2 * y + 1) * Dc + 2 * x], ch[(2 * y + 1) * Dc + 2 * x + 1]);
+    }
+    __syncthreads();
+  }
 }
 
 // M2L (m2l.cu:24-60), all levels in one launch: each target cell sums the
 // kernel at its own COM over the children of its parent's 3x3 neighbourhood
 // (window [p-2, p+4) per axis) that are not adjacent to it; y outer, x inner.
-__global__ __launch_bounds__(256) void k_m2l(SysView v) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+template <class Sys>
+__global__ __launch_bounds__(256) void k_m2l(Sys sys) {
+  int blk, nblk;
+  const SysView v = sys.get(blk, nblk);
+  const int t = blk * blockDim.x + threadIdx.x;
   if (t >= v.lvl_off[v.L]) return;
   int l = 0;
   while (t >= v.lvl_off[l + 1]) ++l;
@@ -169,6 +270,41 @@ hipError_t launch_p2m(const SysView& v, hipStream_t s) {
   return hipGetLastError();
 }
 
+int upward_tile_blocks(int L, int B) {
+  if (L <= 1) return 0;
+  const int T = B < kUpTile ? B : kUpTile;
+  return (B / T) * (B / T);
+}
+
+int upward_top_blocks(int L, int B) {
+  const int T = B < kUpTile ? B : kUpTile;
+  int lgT = 0;
+  while ((1 << lgT) < T) ++lgT;
+  return (L - 1 > lgT) ? 1 : 0;
+}
+
+int m2l_blocks(const SysView& v) { return v.L > 0 ? cdiv(v.lvl_off[v.L], 256) : 0; }
+
+hipError_t launch_upward(const SysView& v, int max_occ, hipStream_t s) {
+  if (v.L <= 0) return hipSuccess;
+  const OneSys one{v};
+  hipLaunchKernelGGL(k_p2m_occ<OneSys>, dim3(cdiv(max_occ, 256)), dim3(256), 0, s, one);
+  const int tb = upward_tile_blocks(v.L, v.B);
+  if (tb) hipLaunchKernelGGL(k_m2m_tile<OneSys>, dim3(tb), dim3(kUpTile * kUpTile), 0, s, one);
+  if (upward_top_blocks(v.L, v.B)) hipLaunchKernelGGL(k_m2m_top<OneSys>, dim3(1), dim3(1024), 0, s, one);
+  return hipGetLastError();
+}
+
+hipError_t launch_tree_batch(const BatchLaunch& b, hipStream_t s) {
+  if (b.p2m.blocks) hipLaunchKernelGGL(k_p2m_occ<ManySys>, dim3(b.p2m.blocks), dim3(256), 0, s, b.p2m.sys);
+  if (b.m2m_tile.blocks)
+    hipLaunchKernelGGL(k_m2m_tile<ManySys>, dim3(b.m2m_tile.blocks), dim3(kUpTile * kUpTile), 0, s, b.m2m_tile.sys);
+  if (b.m2m_top.blocks)
+    hipLaunchKernelGGL(k_m2m_top<ManySys>, dim3(b.m2m_top.blocks), dim3(1024), 0, s, b.m2m_top.sys);
+  if (b.m2l.blocks) hipLaunchKernelGGL(k_m2l<ManySys>, dim3(b.m2l.blocks), dim3(256), 0, s, b.m2l.sys);
+  return hipGetLastError();
+}
+
 hipError_t launch_m2m(const SysView& v, hipStream_t s) {
   for (int l = 1; l < v.L; ++l) {
     const int D = v.B >> l;
@@ -179,7 +315,7 @@ hipError_t launch_m2m(const SysView& v, hipStream_t s) {
 
 hipError_t launch_m2l(const SysView& v, hipStream_t s) {
   if (v.L <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_m2l, dim3(cdiv(v.lvl_off[v.L], 256)), dim3(256), 0, s, v);
+  hipLaunchKernelGGL(k_m2l<OneSys>, dim3(m2l_blocks(v)), dim3(256), 0, s, OneSys{v});
   return hipGetLastError();
 }
 

@@ -128,6 +128,47 @@ def gaussian_disk_system(n=65536, seed=1003, steps=50, num_level=6, G=None):
     return _finish("gaussian_disk", pos, vel, w, num_level, (-R, -R, R, R), 1e-3, 1e-4, G, steps, G_choice="1/N")
 
 
+@dataclass
+class BatchWorkload:
+    name: str
+    systems: list          # list of Workload, one per system
+    steps: int
+
+    @property
+    def n(self) -> int:
+        return int(sum(w.n for w in self.systems))
+
+    def concat(self):
+        """Concatenated position, position_pre, weight (system order)."""
+        return (np.concatenate([w.position for w in self.systems]),
+                np.concatenate([w.position_pre for w in self.systems]),
+                np.concatenate([w.weight for w in self.systems]))
+
+
+def many_disks(num_systems=256, n_min=1024, n_max=131072, seed=1004, steps=20):
+    """Independent systems: N log-uniform in [n_min, n_max]; a uniform or a
+    Gaussian disk (coin flip) of radius R log-uniform in [0.5, 4] (Gaussian:
+    sigma R/2, truncated at R); levels = clamp(round(log4(N/16)), 3, 9), i.e.
+    num_level = levels - 1; masses U[0.5,1.5]; eps 1e-3; dt 1e-4; box
+    [-1.1R, 1.1R]^2.  G is not specified for this workload; it is 1/N per
+    system, as in the two-disk collision."""
+    rng = np.random.default_rng(seed)
+    systems = []
+    for k in range(num_systems):
+        n = int(np.floor(np.exp(rng.uniform(np.log(n_min), np.log(n_max)))))
+        R = float(np.exp(rng.uniform(np.log(0.5), np.log(4.0))))
+        gaussian = bool(rng.integers(0, 2))
+        levels = int(np.clip(np.round(np.log(n / 16.0) / np.log(4.0)), 3, 9))
+        pos = gaussian_disk(rng, n, R / 2, R) if gaussian else uniform_disk(rng, n, R)
+        w = rng.uniform(0.5, 1.5, n)
+        G = 1.0 / n
+        vel = circular_velocity(pos, w, (0.0, 0.0), G)
+        box = 1.1 * R
+        systems.append(_finish(f"sys{k}", pos, vel, w, levels - 1, (-box, -box, box, box), 1e-3, 1e-4, G, steps,
+                               gaussian=gaussian, R=R))
+    return BatchWorkload("many_disks", systems, steps)
+
+
 def by_name(name: str, **kw) -> Workload:
     return {"disk": disk, "two_disk_collision": two_disk_collision,
             "gaussian_disk": gaussian_disk_system}[name](**kw)
