@@ -32,6 +32,9 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+import numpy as np  # noqa: E402
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -41,6 +44,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU baseline sample length")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-systems leg")
+    ap.add_argument("--batch-systems", type=int, default=256)
+    ap.add_argument("--batch-steps", type=int, default=20)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -150,6 +156,8 @@ def main():
                          f"{tcpu:.1f} s"}
         log(f"[bench] cpu baseline {cpu}")
 
+    batched = None if args.no_batch else batch_leg(args)
+
     out = {
         "metric": "particle-steps/s at N=2^20; batched+adjoint vs plain, force rel err <=1e-4",
         "value": value,
@@ -173,9 +181,68 @@ def main():
                      "occupied_leaves": int(occ0)},
         "cpu_baseline": cpu,
         "accuracy": accuracy,
+        "batched": batched,
     }
     s.close()
     print(json.dumps(out), flush=True)
+
+
+def batch_leg(args):
+    """Many independent systems (N log-uniform in [1024, 131072], uneven
+    depths): one hipGraph replay per step with every stage one launch over
+    all systems, against the same systems stepped one Solver at a time."""
+    import torch
+
+    from nbfmm_amd import Batch, Solver
+    from nbfmm_amd.workloads import many_disks
+
+    bw = many_disks(num_systems=args.batch_systems, steps=args.batch_steps)
+    steps = bw.steps
+    total = bw.n
+    log(f"[bench] batch: {len(bw.systems)} systems, {total} particles, {steps} steps")
+    x, p, w = bw.concat()
+    dev = torch.device("cuda:0")
+    X = torch.as_tensor(x, device=dev)
+    P = torch.as_tensor(p, device=dev)
+    W = torch.as_tensor(w, device=dev)
+    A = torch.zeros_like(X)
+    b = Batch([s.n for s in bw.systems], [s.num_level for s in bw.systems], [s.limits for s in bw.systems],
+              [s.eps for s in bw.systems], ticks=[s.tick for s in bw.systems])
+    stream = torch.cuda.Stream()
+    with torch.cuda.stream(stream):
+        b.run(2, X, P, W, A, use_graph=True)
+    stream.synchronize()
+    t0 = time.perf_counter()
+    with torch.cuda.stream(stream):
+        b.run(steps, X, P, W, A, use_graph=True)
+    stream.synchronize()
+    tb = time.perf_counter() - t0
+    flags = int(b.flags().any())
+    b.close()
+    # per-system launches: one Solver per system, stepped one after the other
+    solvers = [Solver(s.num_level, s.n, s.limits, s.eps) for s in bw.systems]
+    off = np.concatenate([[0], np.cumsum([s.n for s in bw.systems])])
+    views = [(X[off[k]:off[k + 1]], P[off[k]:off[k + 1]], W[off[k]:off[k + 1]], A[off[k]:off[k + 1]])
+             for k in range(len(bw.systems))]
+    with torch.cuda.stream(stream):
+        for k, s in enumerate(bw.systems):
+            solvers[k].step(s.n, s.tick, *views[k])
+    stream.synchronize()
+    t0 = time.perf_counter()
+    with torch.cuda.stream(stream):
+        for _ in range(steps):
+            for k, s in enumerate(bw.systems):
+                solvers[k].step(s.n, s.tick, *views[k])
+    stream.synchronize()
+    ts = time.perf_counter() - t0
+    for sv in solvers:
+        sv.close()
+    out = {"workload": bw.name, "systems": len(bw.systems), "particles": total, "steps": steps,
+           "batched_value": total * steps / tb, "per_system_value": total * steps / ts,
+           "speedup": ts / tb, "unit": "particle-steps/s", "batched_ms_per_step": tb * 1e3 / steps,
+           "per_system_ms_per_step": ts * 1e3 / steps, "flags": flags}
+    log(f"[bench] batch {out}")
+    return out
 
 
 if __name__ == "__main__":

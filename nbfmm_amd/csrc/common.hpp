@@ -113,6 +113,13 @@ __device__ __forceinline__ void kernel_accum(float tx, float ty, float sx, float
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
 
+// Ordering point for LDS data that only the calling wave touches: the
+// hardware executes one wave's LDS instructions in program order, so only
+// the compiler must be kept from moving memory accesses across this point.
+// (A workgroup-scope fence would also wait for every outstanding global
+// load.)
+__device__ __forceinline__ void wave_lds_order() { asm volatile("" ::: "memory"); }
+
 // Inclusive prefix sum over the 64 lanes of a wave.
 __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) {
   const int lane = lane_id();

@@ -39,9 +39,6 @@ __device__ __forceinline__ float2 leaf_local(const SysView& v, int cx, int cy) {
   return acc;
 }
 
-// Wave-local ordering of LDS traffic (the staging buffer is private to the
-// wave, so no block barrier is needed).
-__device__ __forceinline__ void wave_lds_fence() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup"); }
 
 template <int MODE, class Sys>
 __global__ __launch_bounds__(kP2PThreads) void k_p2p(Sys sys, CallIO io) {
@@ -98,7 +95,7 @@ __global__ __launch_bounds__(kP2PThreads) void k_p2p(Sys sys, CallIO io) {
           }
           src[slot] = val;
         }
-        wave_lds_fence();
+        wave_lds_order();
         const int step = kP2PUnroll * K;
         const int iters = (cnt + step - 1) / step;
         const float4* p = src + k;
@@ -112,7 +109,7 @@ __global__ __launch_bounds__(kP2PThreads) void k_p2p(Sys sys, CallIO io) {
           kernel_accum(pt.x, pt.y, s2.x, s2.y, s2.z, eps, ax, ay);
           kernel_accum(pt.x, pt.y, s3.x, s3.y, s3.z, eps, ax, ay);
         }
-        wave_lds_fence();
+        wave_lds_order();
       }
       for (int off = G; off < kWave; off <<= 1) {
         ax += __shfl_xor(ax, off, kWave);

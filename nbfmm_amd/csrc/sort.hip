@@ -65,9 +65,6 @@ __device__ __forceinline__ uint64_t lanes_below() {
   return lane == 0 ? 0ull : (~0ull >> (kWave - lane));
 }
 
-// Orders this wave's LDS accesses (the counters below are private to a wave,
-// so no block barrier is needed between its chunks).
-__device__ __forceinline__ void wave_fence() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup"); }
 
 // ------------------------------------------------------------- row pass
 
@@ -166,21 +163,28 @@ __global__ __launch_bounds__(kSortThreads) void k_row_down(Sys sys, CallIO io) {
   const int wbase = tile * kSortTile + wave * (kWave * kSortWaveItems);
   const uint64_t below = lanes_below();
   uint32_t key[kSortWaveItems], rank[kSortWaveItems];
+  float2 q[kSortWaveItems];
+  // all loads first, so they are in flight together
+#pragma unroll
+  for (int j = 0; j < kSortWaveItems; ++j) {
+    const int i = wbase + j * kWave + lane;
+    q[j] = i < v.n ? pos0[i] : make_float2(0.0f, 0.0f);
+  }
 #pragma unroll
   for (int j = 0; j < kSortWaveItems; ++j) {
     const int i = wbase + j * kWave + lane;
     const bool valid = i < v.n;
     bool bad;
-    key[j] = valid ? bin_key(v, pos0[i], bad) : 0u;
+    key[j] = valid ? bin_key(v, q[j], bad) : 0u;
     const uint32_t d = key[j] >> v.lgB;
     const uint64_t m = match_mask(d, valid, v.lgB);
     const uint32_t r = (uint32_t)__popcll(m & below);
     uint32_t prior = 0;
     if (valid) prior = s_wcnt[wave][d];
     rank[j] = prior + r;
-    wave_fence();
+    wave_lds_order();
     if (valid && r == 0) s_wcnt[wave][d] = prior + (uint32_t)__popcll(m);
-    wave_fence();
+    wave_lds_order();
   }
   __syncthreads();
   for (int d = threadIdx.x; d < R; d += kSortThreads) {
@@ -199,9 +203,8 @@ __global__ __launch_bounds__(kSortThreads) void k_row_down(Sys sys, CallIO io) {
     if (i >= v.n) continue;
     const uint32_t d = key[j] >> v.lgB;
     const uint32_t p = s_base[d] + s_wcnt[wave][d] + rank[j];
-    const float2 q = pos0[i];
     v.tkey[p] = key[j];
-    v.tpart[p] = make_float4(q.x, q.y, w0[i], __uint_as_float((uint32_t)i));
+    v.tpart[p] = make_float4(q[j].x, q[j].y, w0[i], __uint_as_float((uint32_t)i));
   }
 }
 
@@ -231,7 +234,17 @@ __global__ __launch_bounds__(kRowThreads) void k_row_sort(Sys sys, int zero_empt
 
   for (int x = threadIdx.x; x < B; x += kRowThreads) { s_hist[x] = 0; s_run[x] = 0; }
   __syncthreads();
-  for (int p = rs + (int)threadIdx.x; p < re; p += kRowThreads) atomicAdd(&s_hist[v.tkey[p] & cmask], 1u);
+  for (int p0 = rs + (int)threadIdx.x; p0 < re; p0 += kRowThreads * kSortWaveItems) {
+    uint32_t k[kSortWaveItems];
+#pragma unroll
+    for (int j = 0; j < kSortWaveItems; ++j) {
+      const int p = p0 + j * kRowThreads;
+      k[j] = p < re ? v.tkey[p] : 0xffffffffu;
+    }
+#pragma unroll
+    for (int j = 0; j < kSortWaveItems; ++j)
+      if (k[j] != 0xffffffffu) atomicAdd(&s_hist[k[j] & cmask], 1u);
+  }
   __syncthreads();
   {
     // cell starts inside the row, head, occupied cells (B <= 512: one each)
@@ -264,17 +277,21 @@ __global__ __launch_bounds__(kRowThreads) void k_row_sort(Sys sys, int zero_empt
 #pragma unroll
     for (int j = 0; j < kSortWaveItems; ++j) {
       const int e = wbase + j * kWave + lane;
+      key[j] = e < re ? v.tkey[e] : 0u;
+    }
+#pragma unroll
+    for (int j = 0; j < kSortWaveItems; ++j) {
+      const int e = wbase + j * kWave + lane;
       const bool valid = e < re;
-      key[j] = valid ? v.tkey[e] : 0u;
       const uint32_t d = key[j] & cmask;
       const uint64_t m = match_mask(d, valid, v.lgB);
       const uint32_t r = (uint32_t)__popcll(m & below);
       uint32_t prior = 0;
       if (valid) prior = s_wcnt[wave][d];
       rank[j] = prior + r;
-      wave_fence();
+      wave_lds_order();
       if (valid && r == 0) s_wcnt[wave][d] = prior + (uint32_t)__popcll(m);
-      wave_fence();
+      wave_lds_order();
     }
     __syncthreads();
     // per cell: offset of each wave = earlier groups + earlier waves
@@ -289,6 +306,12 @@ __global__ __launch_bounds__(kRowThreads) void k_row_sort(Sys sys, int zero_empt
       s_run[x] = run;
     }
     __syncthreads();
+    float4 pv[kSortWaveItems];
+#pragma unroll
+    for (int j = 0; j < kSortWaveItems; ++j) {
+      const int e = wbase + j * kWave + lane;
+      pv[j] = e < re ? v.tpart[e] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
 #pragma unroll
     for (int j = 0; j < kSortWaveItems; ++j) {
       const int e = wbase + j * kWave + lane;
@@ -296,7 +319,7 @@ __global__ __launch_bounds__(kRowThreads) void k_row_sort(Sys sys, int zero_empt
       const uint32_t d = key[j] & cmask;
       const int p = rs + (int)(s_start[d] + s_wcnt[wave][d] + rank[j]);
       v.key[p] = key[j];
-      v.part[p] = v.tpart[e];
+      v.part[p] = pv[j];
     }
     __syncthreads();
   }
