@@ -146,6 +146,45 @@ int nbfmm_profile_enable(nbfmm_solver* solver, int enable);
 int nbfmm_p2p_time(nbfmm_solver* solver, double* total_ms, int* launches, int reset);
 int nbfmm_pair_count(nbfmm_solver* solver, int num_particle, double* pairs, int* occupied_cells);
 
+/* ---------------------------------------------------------------- adjoint
+ * No reference counterpart (the reference only evaluates forces).  The
+ * gradients treat each particle's cell as fixed (the cell assignment is
+ * piecewise constant in the positions). */
+
+/* Vector-Jacobian product of solve: given d_grad_effect = dloss/d effect
+ * (float2[N], original order) at positions/weights (as for nbfmm_solve),
+ * writes dloss/d position (float2[N]) and dloss/d weight (float[N]). */
+int nbfmm_solve_adjoint(nbfmm_solver* solver, int num_particle, const float* d_position_origin,
+                        const float* d_weight_origin, const float* d_grad_effect, float* d_grad_position,
+                        float* d_grad_weight, void* stream);
+
+/* Adjoint of one Verlet step x_{k+1} = 2 x_k - x_{k-1} + tick^2 a(x_k, m),
+ * at positions d_position = x_k.  In: lambda = dloss/dx_{k+1} and
+ * pi = dloss/d(x_pre)_{k+1}; out (in place): lambda = dloss/dx_k,
+ * pi = dloss/d(x_pre)_k; d_grad_weight += dloss/dm through this step. */
+int nbfmm_step_adjoint(nbfmm_solver* solver, int num_particle, float tick, const float* d_position,
+                       const float* d_weight, float* d_lambda, float* d_pi, float* d_grad_weight, void* stream);
+
+/* Checkpointed trajectories: the forward stores (x, x_pre) every
+ * checkpoint_every steps and the positions at the observation steps; the
+ * backward recomputes each segment from its checkpoint and applies
+ * nbfmm_step_adjoint step by step, adding the observation gradients.
+ * Sizes of the caller-allocated device buffers: */
+int nbfmm_trajectory_workspace(int num_particle, int num_steps, int checkpoint_every, size_t* checkpoint_bytes,
+                               size_t* work_bytes);
+/* obs_steps: host array of n_obs step indices in [0, num_steps];
+ * d_obs_position: n_obs x N float2 (output). */
+int nbfmm_trajectory_forward(nbfmm_solver* solver, int num_particle, float tick, int num_steps,
+                             int checkpoint_every, float* d_position_cur, float* d_position_pre,
+                             const float* d_weight, float* d_acceleration, float* d_checkpoints, int n_obs,
+                             const int* obs_steps, float* d_obs_position, void* stream);
+/* d_obs_grad: n_obs x N float2 = dloss/d(observed positions).  Writes the
+ * gradients wrt the initial positions, initial x_pre and the weights. */
+int nbfmm_trajectory_backward(nbfmm_solver* solver, int num_particle, float tick, int num_steps,
+                              int checkpoint_every, const float* d_weight, const float* d_checkpoints, int n_obs,
+                              const int* obs_steps, const float* d_obs_grad, float* d_grad_position0,
+                              float* d_grad_position_pre0, float* d_grad_weight, float* d_work, void* stream);
+
 /* ---------------------------------------------------------------- batches
  * Many independent systems with their own particle counts, levels, boxes and
  * softenings, advanced together: each stage of the step is one launch over
@@ -175,6 +214,13 @@ int nbfmm_batch_run(nbfmm_batch* batch, int num_steps, float* d_position_cur, fl
                     const float* d_weight, float* d_acceleration, int use_graph, void* stream);
 /* Synchronises; flags[s] = NBFMM_FLAG_* bits of system s. */
 int nbfmm_batch_flags(nbfmm_batch* batch, unsigned* flags, int clear);
+/* nbfmm_solve_adjoint of every system (concatenated arrays). */
+int nbfmm_batch_solve_adjoint(nbfmm_batch* batch, const float* d_position_origin, const float* d_weight_origin,
+                              const float* d_grad_effect, float* d_grad_position, float* d_grad_weight,
+                              void* stream);
+/* nbfmm_step_adjoint of every system, each with its own tick. */
+int nbfmm_batch_step_adjoint(nbfmm_batch* batch, const float* d_position, const float* d_weight, float* d_lambda,
+                             float* d_pi, float* d_grad_weight, void* stream);
 
 #ifdef __cplusplus
 }

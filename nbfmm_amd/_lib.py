@@ -47,6 +47,13 @@ SIGNATURES = {
     "nbfmm_batch_step": [P, P, P, P, P, P],
     "nbfmm_batch_run": [P, c_int, P, P, P, P, c_int, P],
     "nbfmm_batch_flags": [P, P, c_int],
+    "nbfmm_batch_solve_adjoint": [P, P, P, P, P, P, P],
+    "nbfmm_batch_step_adjoint": [P, P, P, P, P, P, P],
+    "nbfmm_solve_adjoint": [P, c_int, P, P, P, P, P, P],
+    "nbfmm_step_adjoint": [P, c_int, c_float, P, P, P, P, P, P],
+    "nbfmm_trajectory_workspace": [c_int, c_int, c_int, ctypes.POINTER(c_size_t), ctypes.POINTER(c_size_t)],
+    "nbfmm_trajectory_forward": [P, c_int, c_float, c_int, c_int, P, P, P, P, P, c_int, P, P, P],
+    "nbfmm_trajectory_backward": [P, c_int, c_float, c_int, c_int, P, P, c_int, P, P, P, P, P, P, P],
 }
 _RESTYPE = {"nbfmm_last_error": ctypes.c_char_p}
 

@@ -70,6 +70,15 @@ class Batch:
         check(lib().nbfmm_batch_run(self._h, int(num_steps), _ptr(position_cur), _ptr(position_pre),
                                     _ptr(weight), _ptr(acceleration), int(bool(use_graph)), _stream(stream)))
 
+    def solve_adjoint(self, position, weight, grad_effect, grad_position, grad_weight, stream=None):
+        check(lib().nbfmm_batch_solve_adjoint(self._h, _ptr(position), _ptr(weight), _ptr(grad_effect),
+                                              _ptr(grad_position), _ptr(grad_weight), _stream(stream)))
+
+    def step_adjoint(self, position, weight, lam, pi, grad_weight, stream=None):
+        """Adjoint of one Verlet step of every system (each with its own tick)."""
+        check(lib().nbfmm_batch_step_adjoint(self._h, _ptr(position), _ptr(weight), _ptr(lam), _ptr(pi),
+                                             _ptr(grad_weight), _stream(stream)))
+
     def flags(self, clear=False):
         f = np.zeros(self.num_systems, np.uint32)
         check(lib().nbfmm_batch_flags(self._h, f.ctypes.data, int(bool(clear))))

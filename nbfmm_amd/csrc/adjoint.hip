@@ -1,0 +1,393 @@
+// Adjoint (vector-Jacobian product) of the force evaluation, and the
+// adjoint of one Verlet step.  No reference counterpart: the reference only
+// evaluates forces (source/nbfmm/core/solver/solve.cu:11-27).
+//
+// Forward being differentiated, per sorted particle i in leaf c:
+//   a_i = sum_{j in 3x3 leaves} K(x_i, x_j, m_j) + sum_l E_l(anc_l(c)),
+//   E_l(t) = sum_{s in IL(t)} K(X_t, X_s, M_s),  K(t, s, m) = m d f(|d|),
+//   d = s - t, f(r) = 1/(r^3 + eps);
+// cell multipoles X (centre of mass) and M (mass) built by P2M/M2M.  The
+// cell assignment is piecewise constant and is held fixed.
+//
+// With upstream g_i = dloss/da_i the gradients are, in gather form (the
+// neighbour relation and the interaction lists are symmetric):
+//   P2P^T:  gx_i += sum_j f h - 3 r f^2 d (d.h),  h = m_i g_j - m_j g_i,
+//           gm_i += sum_j -f d.g_j                        (d = x_j - x_i)
+//   M2L^T:  the same with (X, M, G) per cell, G = sum of g over the cell's
+//           particles (L2P^T and L2L^T), giving dloss/dX_t, dloss/dM_t
+//   M2M^T:  top-down, child c of parent p:
+//           gX_c += (M_c/M_p) gX_p,  gM_c += gM_p + (X_c - X_p).gX_p / M_p
+//   P2M^T:  gx_i += (m_i/M_c) gX_c,  gm_i += gM_c + (x_i - X_c).gX_c / M_c
+// A self pair contributes exactly 0 (d = 0 and h = m_i g_i - m_i g_i = 0 with
+// both products rounded).
+#include "common.hpp"
+#include "launch.hpp"
+
+namespace nbfmm_amd {
+
+static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
+
+// One pair of the gather-form adjoint; target (t, mt, gt), source (s, ms, gs).
+__device__ __forceinline__ void pair_adjoint(float tx, float ty, float mt, float gtx, float gty, float sx, float sy,
+                                             float ms, float gsx, float gsy, float eps, float& gx, float& gy,
+                                             float& gm) {
+#pragma clang fp contract(off)
+  const float dx = sx - tx;
+  const float dy = sy - ty;
+  const float r2 = __builtin_fmaf(dx, dx, dy * dy);
+  const float r = __builtin_amdgcn_sqrtf(r2);
+  const float f = __builtin_amdgcn_rcpf(__builtin_fmaf(r2, r, eps));
+  const float hx = mt * gsx - ms * gtx;
+  const float hy = mt * gsy - ms * gty;
+  const float dh = __builtin_fmaf(dx, hx, dy * hy);
+  const float c2 = 3.0f * r * f * f * dh;
+  gx = __builtin_fmaf(f, hx, gx);
+  gx = __builtin_fmaf(-c2, dx, gx);
+  gy = __builtin_fmaf(f, hy, gy);
+  gy = __builtin_fmaf(-c2, dy, gy);
+  const float dg = __builtin_fmaf(dx, gsx, dy * gsy);
+  gm = __builtin_fmaf(-f, dg, gm);
+}
+
+// Level-0 subtree sums start at zero for every leaf.
+template <class Sys>
+__global__ __launch_bounds__(256) void k_adj_zero(Sys sys) {
+  int blk, nblk;
+  const SysView v = sys.get(blk, nblk);
+  for (int c = blk * blockDim.x + threadIdx.x; c < v.B * v.B; c += nblk * blockDim.x)
+    v.gup[c] = make_float2(0.0f, 0.0f);
+}
+
+// Upstream gradient into sorted order, scaled: one thread per particle.
+template <class Sys>
+__global__ __launch_bounds__(256) void k_adj_sort(Sys sys, CallIO io) {
+  int blk, nblk;
+  const SysView v = sys.get(blk, nblk);
+  const int i = blk * blockDim.x + threadIdx.x;
+  if (i >= v.n) return;
+  const float2 g0 = io.gin[v.part_off + __float_as_uint(v.part[i].w)];
+  // gscale < 0: the system's own tick^2 (Verlet step adjoint of a batch)
+  const float sc = io.gscale < 0.0f ? v.tick * v.tick : io.gscale;
+  v.gsort[i] = make_float2(sc * g0.x, sc * g0.y);
+}
+
+// Sum of the sorted upstream gradient over each occupied leaf (L2P^T).
+template <class Sys>
+__global__ __launch_bounds__(256) void k_adj_leaf(Sys sys) {
+  int blk, nblk;
+  const SysView v = sys.get(blk, nblk);
+  const int j = blk * blockDim.x + threadIdx.x;
+  if (j >= *v.nocc) return;
+  const int c = v.occ[j];
+  const int a = v.head[c], b = v.head[c + 1];
+  float sx = 0.0f, sy = 0.0f;
+  int i = a;
+  for (; i + 4 <= b; i += 4) {
+    const float2 g0 = v.gsort[i], g1 = v.gsort[i + 1], g2 = v.gsort[i + 2], g3 = v.gsort[i + 3];
+    sx += g0.x; sy += g0.y;
+    sx += g1.x; sy += g1.y;
+    sx += g2.x; sy += g2.y;
+    sx += g3.x; sy += g3.y;
+  }
+  for (; i < b; ++i) {
+    const float2 g = v.gsort[i];
+    sx += g.x;
+    sy += g.y;
+  }
+  v.gup[c] = make_float2(sx, sy);
+}
+
+constexpr int kAdjTile = 16;
+
+// Subtree sums of the upper levels: tiles of 16x16 leaves in LDS, then the
+// levels above the tiles in one block (same decomposition as the M2M pass).
+template <class Sys>
+__global__ __launch_bounds__(kAdjTile * kAdjTile) void k_adj_up_tile(Sys sys) {
+  __shared__ float2 lv[2][kAdjTile * kAdjTile];
+  int blk, nblk;
+  const SysView v = sys.get(blk, nblk);
+  const int B = v.B;
+  const int T = B < kAdjTile ? B : kAdjTile;
+  const int tiles = B / T;
+  const int X0 = (blk % tiles) * T, Y0 = (blk / tiles) * T;
+  const int t = threadIdx.x;
+  int lgT = 0;
+  while ((1 << lgT) < T) ++lgT;
+  if (t < T * T) lv[0][t] = v.gup[(Y0 + t / T) * B + X0 + t % T];
+  __syncthreads();
+  const int top = (v.L - 1) < lgT ? (v.L - 1) : lgT;
+  int cur = 0;
+  for (int l = 1; l <= top; ++l) {
+    const int Dc = T >> (l - 1), Dl = T >> l;
+    if (t < Dl * Dl) {
+      const int x = t % Dl, y = t / Dl;
+      const float2* ch = lv[cur];
+      const float2 a = ch[(2 * y) * Dc + 2 * x], b = ch[(2 * y) * Dc + 2 * x + 1];
+      const float2 c = ch[(2 * y + 1) * Dc + 2 * x], d = ch[(2 * y + 1) * Dc + 2 * x + 1];
+      const float2 m = make_float2(a.x + b.x + c.x + d.x, a.y + b.y + c.y + d.y);
+      lv[cur ^ 1][t] = m;
+      v.gup[v.lvl_off[l] + ((Y0 >> l) + y) * (B >> l) + (X0 >> l) + x] = m;
+    }
+    cur ^= 1;
+    __syncthreads();
+  }
+}
+
+template <class Sys>
+__global__ __launch_bounds__(1024) void k_adj_up_top(Sys sys) {
+  int blk, nblk;
+  const SysView v = sys.get(blk, nblk);
+  int first = 1;
+  while ((1 << (first - 1)) < (v.B < kAdjTile ? v.B : kAdjTile)) ++first;
+  for (int l = first; l < v.L; ++l) {
+    const int D = v.B >> l, Dc = D * 2;
+    const float2* ch = v.gup + v.lvl_off[l - 1];
+    for (int c = threadIdx.x; c < D * D; c += blockDim.x) {
+      const int x = c % D, y = c / D;
+      const float2 a = ch[(2 * y) * Dc + 2 * x], b = ch[(2 * y) * Dc + 2 * x + 1];
+      const float2 cc = ch[(2 * y + 1) * Dc + 2 * x], d = ch[(2 * y + 1) * Dc + 2 * x + 1];
+      v.gup[v.lvl_off[l] + c] = make_float2(a.x + b.x + cc.x + d.x, a.y + b.y + cc.y + d.y);
+    }
+    __syncthreads();
+  }
+}
+
+// M2L^T, all levels in one launch: gradient wrt each cell's COM and mass
+// from the M2L interactions it takes part in (as target and as source).
+template <class Sys>
+__global__ __launch_bounds__(256) void k_adj_m2l(Sys sys) {
+  int blk, nblk;
+  const SysView v = sys.get(blk, nblk);
+  const int t = blk * blockDim.x + threadIdx.x;
+  if (t >= v.lvl_off[v.L]) return;
+  int l = 0;
+  while (t >= v.lvl_off[l + 1]) ++l;
+  const int D = v.B >> l;
+  const int c = t - v.lvl_off[l];
+  const int tx = c % D, ty = c / D;
+  const float4* mp = v.mpole + v.lvl_off[l];
+  const float2* gu = v.gup + v.lvl_off[l];
+  const float4 tc = mp[c];
+  const float2 tg = gu[c];
+  const int px = tx & ~1, py = ty & ~1;
+  float gx = 0.0f, gy = 0.0f, gm = 0.0f;
+  for (int y = py - 2; y < py + 4; ++y) {
+    if (y < 0 || y >= D) continue;
+    for (int x = px - 2; x < px + 4; ++x) {
+      if (x < 0 || x >= D) continue;
+      if (abs(x - tx) > 1 || abs(y - ty) > 1) {
+        const float4 sc = mp[y * D + x];
+        const float2 sg = gu[y * D + x];
+        pair_adjoint(tc.x, tc.y, tc.z, tg.x, tg.y, sc.x, sc.y, sc.z, sg.x, sg.y, v.eps, gx, gy, gm);
+      }
+    }
+  }
+  v.gmp[t] = make_float4(gx, gy, gm, 0.0f);
+}
+
+// Gradient wrt the leaf's (X, M): the M2M^T chain from the top level down,
+// on wave-uniform data.
+__device__ __forceinline__ float4 leaf_multipole_grad(const SysView& v, int cx, int cy) {
+  int l = v.L - 1;
+  float4 T = v.gmp[v.lvl_off[l] + (cy >> l) * (v.B >> l) + (cx >> l)];
+  for (l = v.L - 2; l >= 0; --l) {
+    const int ci = v.lvl_off[l] + (cy >> l) * (v.B >> l) + (cx >> l);
+    const int pi = v.lvl_off[l + 1] + (cy >> (l + 1)) * (v.B >> (l + 1)) + (cx >> (l + 1));
+    const float4 mc = v.mpole[ci], mp = v.mpole[pi];
+    const float4 d = v.gmp[ci];
+    float4 n = make_float4(d.x, d.y, d.z + T.z, 0.0f);
+    if (mp.z != 0.0f) {
+      const float ratio = mc.z / mp.z;
+      n.x = __builtin_fmaf(ratio, T.x, d.x);
+      n.y = __builtin_fmaf(ratio, T.y, d.y);
+      n.z += ((mc.x - mp.x) * T.x + (mc.y - mp.y) * T.y) / mp.z;
+    }
+    T = n;
+  }
+  return T;
+}
+
+// P2P^T plus P2M^T per occupied leaf (one wavefront each, sources staged in
+// LDS as in the forward P2P kernel), with the epilogue of the mode:
+//   kAdjSolve: gpos[o] = dloss/dx_o, gw[o] = dloss/dm_o
+//   kAdjStep:  Verlet adjoint; with lambda = gpos, pi = gpre:
+//              lambda_k = 2 lambda + tick^2 (da/dx)^T lambda + pi,
+//              pi_k = -lambda,  gw += tick^2 (da/dm)^T lambda
+//              (the tick^2 scale was applied to the sorted upstream).
+enum AdjMode : int { kAdjSolve = 0, kAdjStep = 1 };
+
+template <int MODE, class Sys>
+__global__ __launch_bounds__(kP2PThreads) void k_adj_p2p(Sys sys, CallIO io) {
+  __shared__ float4 s_pos[kP2PThreads / kWave][kP2PStage];
+  __shared__ float2 s_g[kP2PThreads / kWave][kP2PStage];
+  int blk, nblk;
+  const SysView v = sys.get(blk, nblk);
+  const int lane = lane_id();
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  float4* sp = s_pos[wave];
+  float2* sg = s_g[wave];
+  float2* gpos = io.gpos + v.part_off;
+  float2* gpre = io.gpre + v.part_off;
+  float* gw = io.gw + v.part_off;
+  const int nocc = *v.nocc;
+  const int stride = nblk * (kP2PThreads / kWave);
+  const int B = v.B;
+  const float eps = v.eps;
+  for (int j = blk * (kP2PThreads / kWave) + wave; j < nocc; j += stride) {
+    const int c = v.occ[j];
+    const int cx = c & (B - 1), cy = c >> v.lgB;
+    const int t0 = v.head[c], t1 = v.head[c + 1];
+    const int xl = cx > 0 ? cx - 1 : 0;
+    const int xr = cx < B - 1 ? cx + 1 : B - 1;
+    int rs0 = 0, rs1 = 0, rs2 = 0, n0 = 0, n1 = 0, n2 = 0;
+    if (cy > 0) { rs0 = v.head[xl + (cy - 1) * B]; n0 = v.head[xr + (cy - 1) * B + 1] - rs0; }
+    rs1 = v.head[xl + cy * B]; n1 = v.head[xr + cy * B + 1] - rs1;
+    if (cy < B - 1) { rs2 = v.head[xl + (cy + 1) * B]; n2 = v.head[xr + (cy + 1) * B + 1] - rs2; }
+    const int S = n0 + n1 + n2;
+    float4 leafg = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float4 leafm = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (v.L > 0) {
+      leafg = leaf_multipole_grad(v, cx, cy);
+      leafm = v.mpole[c];
+    }
+    for (int tb = t0; tb < t1; tb += kWave) {
+      const int nt = min(kWave, t1 - tb);
+      const int lgG = nt <= 1 ? 0 : 32 - __clz(nt - 1);
+      const int G = 1 << lgG;
+      const int K = kWave >> lgG;
+      const int t = lane & (G - 1);
+      const int k = lane >> lgG;
+      const bool valid = t < nt;
+      const int ti = tb + (valid ? t : 0);
+      const float4 pt = v.part[ti];
+      const float2 gt = v.gsort[ti];
+      float gx = 0.0f, gy = 0.0f, gm = 0.0f;
+      for (int base = 0; base < S; base += kP2PStage) {
+        const int cnt = min(kP2PStage, S - base);
+#pragma unroll
+        for (int q = 0; q < kP2PStage / kWave; ++q) {
+          const int slot = q * kWave + lane;
+          const int f = base + slot;
+          float4 pv = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+          float2 gv = make_float2(0.0f, 0.0f);
+          if (slot < cnt) {
+            const int g = f < n0 ? rs0 + f : (f < n0 + n1 ? rs1 + (f - n0) : rs2 + (f - n0 - n1));
+            pv = v.part[g];
+            gv = v.gsort[g];
+          }
+          sp[slot] = pv;
+          sg[slot] = gv;
+        }
+        wave_lds_order();
+        const int step = 2 * K;
+        const int iters = (cnt + step - 1) / step;
+        const float4* p = sp + k;
+        const float2* q = sg + k;
+        for (int i = 0; i < iters; ++i, p += step, q += step) {
+          const float4 s0 = p[0], s1 = p[K];
+          const float2 g0 = q[0], g1 = q[K];
+          pair_adjoint(pt.x, pt.y, pt.z, gt.x, gt.y, s0.x, s0.y, s0.z, g0.x, g0.y, eps, gx, gy, gm);
+          pair_adjoint(pt.x, pt.y, pt.z, gt.x, gt.y, s1.x, s1.y, s1.z, g1.x, g1.y, eps, gx, gy, gm);
+        }
+        wave_lds_order();
+      }
+      for (int off = G; off < kWave; off <<= 1) {
+        gx += __shfl_xor(gx, off, kWave);
+        gy += __shfl_xor(gy, off, kWave);
+        gm += __shfl_xor(gm, off, kWave);
+      }
+      if (k == 0 && valid) {
+        if (v.L > 0) {
+          // P2M^T: X_c = sum m x / M_c,  M_c = sum m
+          const float r = pt.z / leafm.z;
+          gx = __builtin_fmaf(r, leafg.x, gx);
+          gy = __builtin_fmaf(r, leafg.y, gy);
+          gm += leafg.z + ((pt.x - leafm.x) * leafg.x + (pt.y - leafm.y) * leafg.y) / leafm.z;
+        }
+        const uint32_t o = __float_as_uint(pt.w);
+        if (MODE == kAdjSolve) {
+          gpos[o] = make_float2(gx, gy);
+          gw[o] = gm;
+        } else {
+          const float2 lam = gpos[o];
+          const float2 pi = gpre[o];
+          gpos[o] = make_float2(2.0f * lam.x + gx + pi.x, 2.0f * lam.y + gy + pi.y);
+          gpre[o] = make_float2(-lam.x, -lam.y);
+          gw[o] += gm;
+        }
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_axpy(int n, float a, const float* x, float* y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = __builtin_fmaf(a, x[i], y[i]);
+}
+
+// ---------------------------------------------------------------- launchers
+
+__global__ __launch_bounds__(256) void k_fill(int n, float a, float* y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = a;
+}
+
+hipError_t launch_fill(int n, float a, float* y, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill, dim3(cdiv(n, 256)), dim3(256), 0, s, n, a, y);
+  return hipGetLastError();
+}
+
+hipError_t launch_axpy(int n, float a, const float* x, float* y, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_axpy, dim3(cdiv(n, 256)), dim3(256), 0, s, n, a, x, y);
+  return hipGetLastError();
+}
+
+hipError_t launch_adjoint(const SysView& v, const CallIO& io, int mode, int grid_blocks, hipStream_t s) {
+  if (v.n <= 0) return hipSuccess;
+  const OneSys one{v};
+  hipLaunchKernelGGL(k_adj_sort<OneSys>, dim3(cdiv(v.n, 256)), dim3(256), 0, s, one, io);
+  if (v.L > 0) {
+    const int C = v.B * v.B;
+    hipLaunchKernelGGL(k_adj_zero<OneSys>, dim3(adj_zero_blocks(v)), dim3(256), 0, s, one);
+    hipLaunchKernelGGL(k_adj_leaf<OneSys>, dim3(cdiv(v.n < C ? v.n : C, 256)), dim3(256), 0, s, one);
+    const int tb = upward_tile_blocks(v.L, v.B);
+    if (tb) hipLaunchKernelGGL(k_adj_up_tile<OneSys>, dim3(tb), dim3(kAdjTile * kAdjTile), 0, s, one);
+    if (upward_top_blocks(v.L, v.B)) hipLaunchKernelGGL(k_adj_up_top<OneSys>, dim3(1), dim3(1024), 0, s, one);
+    hipLaunchKernelGGL(k_adj_m2l<OneSys>, dim3(m2l_blocks(v)), dim3(256), 0, s, one);
+  }
+  if (mode == kAdjSolve)
+    hipLaunchKernelGGL((k_adj_p2p<kAdjSolve, OneSys>), dim3(grid_blocks), dim3(kP2PThreads), 0, s, one, io);
+  else
+    hipLaunchKernelGGL((k_adj_p2p<kAdjStep, OneSys>), dim3(grid_blocks), dim3(kP2PThreads), 0, s, one, io);
+  return hipGetLastError();
+}
+
+int adj_zero_blocks(const SysView& v) {
+  if (v.L <= 0) return 0;
+  const int b = cdiv(v.B * v.B, 256);
+  return b < 1024 ? b : 1024;
+}
+
+hipError_t launch_adjoint_batch(const BatchLaunch& bl, const CallIO& io, int mode, hipStream_t s) {
+  hipLaunchKernelGGL(k_adj_sort<ManySys>, dim3(bl.adj_sort.blocks), dim3(256), 0, s, bl.adj_sort.sys, io);
+  if (bl.adj_zero.blocks)
+    hipLaunchKernelGGL(k_adj_zero<ManySys>, dim3(bl.adj_zero.blocks), dim3(256), 0, s, bl.adj_zero.sys);
+  if (bl.p2m.blocks) hipLaunchKernelGGL(k_adj_leaf<ManySys>, dim3(bl.p2m.blocks), dim3(256), 0, s, bl.p2m.sys);
+  if (bl.m2m_tile.blocks)
+    hipLaunchKernelGGL(k_adj_up_tile<ManySys>, dim3(bl.m2m_tile.blocks), dim3(kAdjTile * kAdjTile), 0, s,
+                       bl.m2m_tile.sys);
+  if (bl.m2m_top.blocks)
+    hipLaunchKernelGGL(k_adj_up_top<ManySys>, dim3(bl.m2m_top.blocks), dim3(1024), 0, s, bl.m2m_top.sys);
+  if (bl.m2l.blocks) hipLaunchKernelGGL(k_adj_m2l<ManySys>, dim3(bl.m2l.blocks), dim3(256), 0, s, bl.m2l.sys);
+  if (mode == kAdjSolve)
+    hipLaunchKernelGGL((k_adj_p2p<kAdjSolve, ManySys>), dim3(bl.p2p.blocks), dim3(kP2PThreads), 0, s, bl.p2p.sys,
+                       io);
+  else
+    hipLaunchKernelGGL((k_adj_p2p<kAdjStep, ManySys>), dim3(bl.p2p.blocks), dim3(kP2PThreads), 0, s, bl.p2p.sys,
+                       io);
+  return hipGetLastError();
+}
+
+}  // namespace nbfmm_amd

@@ -216,6 +216,9 @@ int nbfmm_solver_create(int num_level, int max_num_particle, const float positio
   ALLOC(v.dtotal, (size_t)s->B);
   ALLOC(v.mpole, (size_t)ncells_tree);
   ALLOC(v.local, (size_t)ncells_tree);
+  ALLOC(v.gsort, N);
+  ALLOC(v.gup, (size_t)ncells_tree);
+  ALLOC(v.gmp, (size_t)ncells_tree);
   ALLOC(v.flags, 1);
   ALLOC(s->d_pairs, 1);
 #undef ALLOC
@@ -422,6 +425,138 @@ int nbfmm_run(nbfmm_solver* s, int n, float tick, int steps, float* xcur, float*
   return NBFMM_OK;
 }
 
+// The forward state the adjoint needs (sorted particles, head, multipoles
+// of every level) at the given positions.
+static int adjoint_prepare(nbfmm_solver* s, int n, const CallIO& io, hipStream_t st) {
+  const SysView v = view(s, n);
+  NB_HIP(launch_predo(v, io, 1, st));
+  if (n > 0) NB_HIP(launch_upward(v, std::min(n, s->B * s->B), st));
+  return NBFMM_OK;
+}
+
+int nbfmm_solve_adjoint(nbfmm_solver* s, int n, const float* pos0, const float* w0, const float* grad_eff,
+                        float* grad_pos, float* grad_w, void* stream) {
+  int rc = check_n(s, n);
+  if (rc) return rc;
+  if (n > 0 && (!pos0 || !w0 || !grad_eff || !grad_pos || !grad_w))
+    return fail(NBFMM_ERR_INVALID_ARGUMENT, "null array");
+  hipStream_t st = as_stream(stream);
+  CallIO io = call_io(pos0, w0);
+  io.gin = (const float2*)grad_eff;
+  io.gpos = (float2*)grad_pos;
+  io.gw = grad_w;
+  io.gscale = 1.0f;
+  if ((rc = adjoint_prepare(s, n, io, st))) return rc;
+  NB_HIP(launch_adjoint(view(s, n), io, 0, s->p2p_grid, st));
+  return NBFMM_OK;
+}
+
+int nbfmm_step_adjoint(nbfmm_solver* s, int n, float tick, const float* x, const float* w, float* lam, float* pi,
+                       float* grad_w, void* stream) {
+  int rc = check_n(s, n);
+  if (rc) return rc;
+  if (n > 0 && (!x || !w || !lam || !pi || !grad_w)) return fail(NBFMM_ERR_INVALID_ARGUMENT, "null array");
+  hipStream_t st = as_stream(stream);
+  CallIO io = call_io(x, w);
+  io.gin = (const float2*)lam;
+  io.gpos = (float2*)lam;
+  io.gpre = (float2*)pi;
+  io.gw = grad_w;
+  io.gscale = tick * tick;
+  if ((rc = adjoint_prepare(s, n, io, st))) return rc;
+  NB_HIP(launch_adjoint(view(s, n), io, 1, s->p2p_grid, st));
+  return NBFMM_OK;
+}
+
+int nbfmm_trajectory_workspace(int n, int num_steps, int checkpoint_every, size_t* checkpoint_bytes,
+                               size_t* work_bytes) {
+  if (n < 0 || num_steps < 0 || checkpoint_every <= 0) return fail(NBFMM_ERR_INVALID_ARGUMENT, "bad sizes");
+  const size_t state = (size_t)n * 2 * sizeof(float);
+  const size_t nck = (size_t)(num_steps / checkpoint_every) + 1;
+  if (checkpoint_bytes) *checkpoint_bytes = nck * 2 * state;
+  // recompute buffer: positions of one segment, plus x, x_pre and acc scratch
+  if (work_bytes) *work_bytes = ((size_t)checkpoint_every + 3) * state;
+  return NBFMM_OK;
+}
+
+int nbfmm_trajectory_forward(nbfmm_solver* s, int n, float tick, int num_steps, int K, float* xcur, float* xpre,
+                             const float* w, float* acc, float* ckpt, int n_obs, const int* obs_steps, float* obs_pos,
+                             void* stream) {
+  int rc = check_n(s, n);
+  if (rc) return rc;
+  if (num_steps < 0 || K <= 0 || n_obs < 0 || (n_obs && (!obs_steps || !obs_pos)) || !ckpt)
+    return fail(NBFMM_ERR_INVALID_ARGUMENT, "bad trajectory arguments");
+  hipStream_t st = as_stream(stream);
+  const size_t sb = (size_t)n * 2 * sizeof(float);
+  auto observe = [&](int k) -> int {
+    for (int o = 0; o < n_obs; ++o)
+      if (obs_steps[o] == k) NB_HIP(hipMemcpyAsync(obs_pos + (size_t)o * n * 2, xcur, sb, hipMemcpyDeviceToDevice, st));
+    return NBFMM_OK;
+  };
+  for (int k = 0; k <= num_steps; ++k) {
+    if (k % K == 0) {
+      float* c = ckpt + (size_t)(k / K) * 2 * n * 2;
+      NB_HIP(hipMemcpyAsync(c, xcur, sb, hipMemcpyDeviceToDevice, st));
+      NB_HIP(hipMemcpyAsync(c + (size_t)n * 2, xpre, sb, hipMemcpyDeviceToDevice, st));
+    }
+    if ((rc = observe(k))) return rc;
+    if (k == num_steps) break;
+    if ((rc = nbfmm_step(s, n, tick, xcur, xpre, w, acc, stream))) return rc;
+  }
+  return NBFMM_OK;
+}
+
+// Reverse pass of a checkpointed trajectory.  In: the checkpoints written by
+// nbfmm_trajectory_forward, the loss gradient at each observation step
+// (obs_grad, n_obs x n float2).  Out: gradients wrt the initial x, x_pre and
+// the masses.  Each segment of K steps is recomputed from its checkpoint
+// (same kernels, so the same bits as the forward) and then stepped back.
+int nbfmm_trajectory_backward(nbfmm_solver* s, int n, float tick, int num_steps, int K, const float* w,
+                              const float* ckpt, int n_obs, const int* obs_steps, const float* obs_grad,
+                              float* grad_x0, float* grad_xpre0, float* grad_w, float* work, void* stream) {
+  int rc = check_n(s, n);
+  if (rc) return rc;
+  if (num_steps < 0 || K <= 0 || n_obs < 0 || (n_obs && (!obs_steps || !obs_grad)) || !ckpt || !work ||
+      !grad_x0 || !grad_xpre0 || !grad_w)
+    return fail(NBFMM_ERR_INVALID_ARGUMENT, "bad trajectory arguments");
+  hipStream_t st = as_stream(stream);
+  const size_t sb = (size_t)n * 2 * sizeof(float);
+  float* lam = grad_x0;
+  float* pi = grad_xpre0;
+  float* seg = work;                         // K positions
+  float* xc = work + (size_t)K * n * 2;      // recompute state
+  float* xp = xc + (size_t)n * 2;
+  float* ac = xp + (size_t)n * 2;
+  NB_HIP(launch_fill(2 * n, 0.0f, lam, st));
+  NB_HIP(launch_fill(2 * n, 0.0f, pi, st));
+  NB_HIP(launch_fill(n, 0.0f, grad_w, st));
+  auto add_obs = [&](int k) -> int {
+    for (int o = 0; o < n_obs; ++o)
+      if (obs_steps[o] == k)
+        NB_HIP(launch_axpy(2 * n, 1.0f, obs_grad + (size_t)o * n * 2, lam, st));
+    return NBFMM_OK;
+  };
+  if ((rc = add_obs(num_steps))) return rc;
+  const int nseg = (num_steps + K - 1) / K;
+  for (int sg = nseg - 1; sg >= 0; --sg) {
+    const int k0 = sg * K;
+    const int k1 = std::min(k0 + K, num_steps);
+    const float* c = ckpt + (size_t)sg * 2 * n * 2;
+    NB_HIP(hipMemcpyAsync(xc, c, sb, hipMemcpyDeviceToDevice, st));
+    NB_HIP(hipMemcpyAsync(xp, c + (size_t)n * 2, sb, hipMemcpyDeviceToDevice, st));
+    for (int k = k0; k < k1; ++k) {
+      NB_HIP(hipMemcpyAsync(seg + (size_t)(k - k0) * n * 2, xc, sb, hipMemcpyDeviceToDevice, st));
+      if (k + 1 < k1 && (rc = nbfmm_step(s, n, tick, xc, xp, w, ac, stream))) return rc;
+    }
+    for (int k = k1 - 1; k >= k0; --k) {
+      if ((rc = nbfmm_step_adjoint(s, n, tick, seg + (size_t)(k - k0) * n * 2, w, lam, pi, grad_w, stream)))
+        return rc;
+      if ((rc = add_obs(k))) return rc;
+    }
+  }
+  return NBFMM_OK;
+}
+
 int nbfmm_profile_enable(nbfmm_solver* s, int enable) {
   if (!s) return fail(NBFMM_ERR_INVALID_ARGUMENT, "null solver");
   s->profile = enable != 0;
@@ -605,7 +740,8 @@ int nbfmm_batch_create(int S, const int* num_particle, const int* num_level, con
   b->part_off[S] = (int)n_all;
   uint32_t *tkey, *key, *hist, *dtotal, *flags;
   float4 *tpart, *part, *mpole;
-  float2 *eff, *local;
+  float2 *eff, *local, *gsort, *gup;
+  float4* gmp;
   int *head, *rowstart, *occ, *nocc;
   int rc = NBFMM_OK;
 #define BALLOC(ptr, cnt) \
@@ -623,6 +759,9 @@ int nbfmm_batch_create(int S, const int* num_particle, const int* num_level, con
   BALLOC(dtotal, row_all);
   BALLOC(mpole, tree_all);
   BALLOC(local, tree_all);
+  BALLOC(gsort, n_all);
+  BALLOC(gup, tree_all);
+  BALLOC(gmp, tree_all);
   BALLOC(flags, (size_t)S);
   BALLOC(b->dv, (size_t)S);
 #undef BALLOC
@@ -642,12 +781,15 @@ int nbfmm_batch_create(int S, const int* num_particle, const int* num_level, con
     v.dtotal = dtotal + row_off[s];
     v.mpole = mpole + tree_off[s];
     v.local = local + tree_off[s];
+    v.gsort = gsort + po;
+    v.gup = gup + tree_off[s];
+    v.gmp = gmp + tree_off[s];
     v.flags = flags + s;
   }
   hipError_t e = hipMemcpy(b->dv, b->hv.data(), S * sizeof(SysView), hipMemcpyHostToDevice);
   if (e != hipSuccess) { batch_free(b); delete b; return hip_fail(e, "hipMemcpy views"); }
   // launch maps
-  std::vector<int> tiles(S), rows(S), p2m(S), mtile(S), mtop(S), m2l(S), p2p(S);
+  std::vector<int> tiles(S), rows(S), p2m(S), mtile(S), mtop(S), m2l(S), p2p(S), asort(S), azero(S);
   const int p2p_total = cus * 16;
   for (int s = 0; s < S; ++s) {
     const SysView& v = b->hv[s];
@@ -661,11 +803,14 @@ int nbfmm_batch_create(int S, const int* num_particle, const int* num_level, con
     const int cap = cdiv(std::min(v.n, C), kP2PThreads / kWave);
     const int share = (int)((double)p2p_total * v.n / (double)total + 0.5);
     p2p[s] = std::max(1, std::min(cap, share));
+    asort[s] = cdiv(v.n, 256);
+    azero[s] = adj_zero_blocks(v);
   }
   if ((rc = make_map(b, tiles, &b->bl.tiles)) || (rc = make_map(b, rows, &b->bl.rows)) ||
       (rc = make_map(b, p2m, &b->bl.p2m)) || (rc = make_map(b, mtile, &b->bl.m2m_tile)) ||
       (rc = make_map(b, mtop, &b->bl.m2m_top)) || (rc = make_map(b, m2l, &b->bl.m2l)) ||
-      (rc = make_map(b, p2p, &b->bl.p2p))) {
+      (rc = make_map(b, p2p, &b->bl.p2p)) || (rc = make_map(b, asort, &b->bl.adj_sort)) ||
+      (rc = make_map(b, azero, &b->bl.adj_zero))) {
     batch_free(b);
     delete b;
     return rc;
@@ -742,6 +887,37 @@ int nbfmm_batch_run(nbfmm_batch* b, int steps, float* xcur, float* xpre, const f
     g.xcur = xcur; g.xpre = xpre; g.w = w; g.acc = acc; g.stream = st; g.valid = true;
   }
   for (int k = 0; k < steps; ++k) NB_HIP(hipGraphLaunch(b->graph_exec, st));
+  return NBFMM_OK;
+}
+
+int nbfmm_batch_solve_adjoint(nbfmm_batch* b, const float* pos0, const float* w0, const float* grad_eff,
+                              float* grad_pos, float* grad_w, void* stream) {
+  if (!b || !pos0 || !w0 || !grad_eff || !grad_pos || !grad_w) return fail(NBFMM_ERR_INVALID_ARGUMENT, "null argument");
+  hipStream_t st = as_stream(stream);
+  CallIO io = call_io(pos0, w0);
+  io.gin = (const float2*)grad_eff;
+  io.gpos = (float2*)grad_pos;
+  io.gw = grad_w;
+  io.gscale = 1.0f;
+  NB_HIP(launch_predo_batch(b->bl, io, st));
+  NB_HIP(launch_tree_batch(b->bl, st));
+  NB_HIP(launch_adjoint_batch(b->bl, io, 0, st));
+  return NBFMM_OK;
+}
+
+int nbfmm_batch_step_adjoint(nbfmm_batch* b, const float* x, const float* w, float* lam, float* pi, float* grad_w,
+                             void* stream) {
+  if (!b || !x || !w || !lam || !pi || !grad_w) return fail(NBFMM_ERR_INVALID_ARGUMENT, "null argument");
+  hipStream_t st = as_stream(stream);
+  CallIO io = call_io(x, w);
+  io.gin = (const float2*)lam;
+  io.gpos = (float2*)lam;
+  io.gpre = (float2*)pi;
+  io.gw = grad_w;
+  io.gscale = -1.0f;  // each system's tick^2
+  NB_HIP(launch_predo_batch(b->bl, io, st));
+  NB_HIP(launch_tree_batch(b->bl, st));
+  NB_HIP(launch_adjoint_batch(b->bl, io, 1, st));
   return NBFMM_OK;
 }
 

@@ -52,6 +52,10 @@ struct SysView {
   uint32_t* dtotal; // row totals [B]
   float4* mpole;    // per level, dense D_l x D_l: COM x, COM y, mass, 0
   float2* local;    // per level, dense D_l x D_l: M2L effect (L2L accumulates)
+  // adjoint scratch
+  float2* gsort;    // upstream gradient of the effects, sorted order
+  float2* gup;      // per level: sum of the upstream gradient over the cell's particles
+  float4* gmp;      // per level: gradient wrt the cell's COM (x, y) and mass (z) from M2L
   int lvl_off[kMaxLevel + 1];  // start of level l in mpole/local; lvl_off[L] = total
   uint32_t* flags;  // NBFMM_FLAG_* bits
 };
@@ -64,6 +68,14 @@ struct CallIO {
   float2* acc0;        // effects (output of solve/step)
   float2* xcur;        // Verlet state (step)
   float2* xpre;
+  // adjoint
+  const float2* gin;   // upstream gradient (solve adjoint: d loss / d effect;
+                       // step adjoint: lambda = d loss / d x_{k+1})
+  float2* gpos;        // solve adjoint: d loss / d position (out);
+                       // step adjoint: lambda, overwritten with lambda_k
+  float2* gpre;        // step adjoint: pi (in/out)
+  float* gw;           // d loss / d mass (solve adjoint: out; step adjoint: += )
+  float gscale;        // upstream scale (step adjoint: tick^2)
 };
 
 // Which system a block works on, and its block index / block count inside

@@ -20,6 +20,8 @@ struct BatchLaunch {
   KernelMap m2m_top;   // M2M top levels
   KernelMap m2l;       // M2L: cdiv(tree cells, 256)
   KernelMap p2p;       // P2P: share of the P2P grid
+  KernelMap adj_sort;  // adjoint: upstream gradient to sorted order, cdiv(n_s, 256)
+  KernelMap adj_zero;  // adjoint: clear the level-0 subtree sums
 };
 
 // sort.hip
@@ -47,6 +49,15 @@ hipError_t launch_cube_write(const SysView& v, int which, const void* src, hipSt
 hipError_t launch_p2p(const SysView& v, int mode, const CallIO& io, int grid_blocks, hipStream_t s);
 hipError_t launch_p2p_batch(const BatchLaunch& b, int mode, const CallIO& io, hipStream_t s);
 hipError_t launch_l2p(const SysView& v, hipStream_t s);
+
+// adjoint.hip; mode 0: solve adjoint, 1: Verlet step adjoint
+hipError_t launch_adjoint(const SysView& v, const CallIO& io, int mode, int grid_blocks, hipStream_t s);
+hipError_t launch_adjoint_batch(const BatchLaunch& b, const CallIO& io, int mode, hipStream_t s);
+int adj_zero_blocks(const SysView& v);
+// y += a * x over n floats
+hipError_t launch_axpy(int n, float a, const float* x, float* y, hipStream_t s);
+// y = a over n floats (a kernel, so that callers may capture it in a graph)
+hipError_t launch_fill(int n, float a, float* y, hipStream_t s);
 hipError_t launch_postdo(const SysView& v, float2* eff0, hipStream_t s);
 hipError_t launch_update(int n, float tick, float2* xcur, float2* xpre, const float2* acc, hipStream_t s);
 // particle buffers <-> reference layouts; which = NBFMM_BUF_* id

@@ -160,6 +160,44 @@ class Solver:
                               _ptr(position_pre), _ptr(weight), _ptr(acceleration), int(bool(use_graph)),
                               _stream(stream)))
 
+    # -- adjoint ----------------------------------------------------------
+    def solve_adjoint(self, num_particle, position_origin, weight_origin, grad_effect, grad_position, grad_weight,
+                      stream=None):
+        """dloss/dposition and dloss/dweight from dloss/deffect (cells held fixed)."""
+        check(lib().nbfmm_solve_adjoint(self._h, int(num_particle), _ptr(position_origin), _ptr(weight_origin),
+                                        _ptr(grad_effect), _ptr(grad_position), _ptr(grad_weight),
+                                        _stream(stream)))
+
+    def step_adjoint(self, num_particle, tick, position, weight, lam, pi, grad_weight, stream=None):
+        """Adjoint of one Verlet step at positions x_k; lam, pi updated in place."""
+        check(lib().nbfmm_step_adjoint(self._h, int(num_particle), float(tick), _ptr(position), _ptr(weight),
+                                       _ptr(lam), _ptr(pi), _ptr(grad_weight), _stream(stream)))
+
+    def trajectory_workspace(self, num_particle, num_steps, checkpoint_every):
+        a = ctypes.c_size_t()
+        b = ctypes.c_size_t()
+        check(lib().nbfmm_trajectory_workspace(int(num_particle), int(num_steps), int(checkpoint_every),
+                                               ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def trajectory_forward(self, num_particle, tick, num_steps, checkpoint_every, position_cur, position_pre, weight,
+                           acceleration, checkpoints, obs_steps, obs_position, stream=None):
+        obs = (ctypes.c_int * max(len(obs_steps), 1))(*[int(k) for k in obs_steps])
+        check(lib().nbfmm_trajectory_forward(self._h, int(num_particle), float(tick), int(num_steps),
+                                             int(checkpoint_every), _ptr(position_cur), _ptr(position_pre),
+                                             _ptr(weight), _ptr(acceleration), _ptr(checkpoints), len(obs_steps),
+                                             obs, _ptr(obs_position) if len(obs_steps) else None,
+                                             _stream(stream)))
+
+    def trajectory_backward(self, num_particle, tick, num_steps, checkpoint_every, weight, checkpoints, obs_steps,
+                            obs_grad, grad_position0, grad_position_pre0, grad_weight, work, stream=None):
+        obs = (ctypes.c_int * max(len(obs_steps), 1))(*[int(k) for k in obs_steps])
+        check(lib().nbfmm_trajectory_backward(self._h, int(num_particle), float(tick), int(num_steps),
+                                              int(checkpoint_every), _ptr(weight), _ptr(checkpoints),
+                                              len(obs_steps), obs, _ptr(obs_grad) if len(obs_steps) else None,
+                                              _ptr(grad_position0), _ptr(grad_position_pre0), _ptr(grad_weight),
+                                              _ptr(work), _stream(stream)))
+
     # -- measurement ------------------------------------------------------
     def profile(self, enable: bool = True) -> None:
         check(lib().nbfmm_profile_enable(self._h, int(bool(enable))))

@@ -422,6 +422,16 @@ int orc_run_f32(int L, int n, const float limits[4], float eps, float tick, int 
   return clamped;
 }
 
+int orc_run_f64(int L, int n, const double limits[4], double eps, double tick, int steps, double* xcur,
+                double* xpre, const double* w, double* acc) {
+  int clamped = 0;
+  for (int s = 0; s < steps; ++s) {
+    clamped += solve<double>(L, n, limits, eps, xcur, w, acc);
+    update<double>(n, tick, xcur, xpre, acc);
+  }
+  return clamped;
+}
+
 // fp64 O(N^2) direct sum with the same kernel form and epsilon
 // (main/demo.cu:59-66 is the reference's only end-to-end comparison).
 // Evaluates the targets listed in `targets` (all particles when targets is

@@ -59,6 +59,8 @@ def lib():
         L.orc_update_f64.argtypes = [c_int, c_double, _f64p, _f64p, _f64p]
         L.orc_run_f32.argtypes = [c_int, c_int, _f32p, c_float, c_float, c_int, _f32p, _f32p, _f32p, _f32p]
         L.orc_run_f32.restype = c_int
+        L.orc_run_f64.argtypes = [c_int, c_int, _f64p, c_double, c_double, c_int, _f64p, _f64p, _f64p, _f64p]
+        L.orc_run_f64.restype = c_int
         L.orc_direct_f64.argtypes = [c_int, _f32p, _f32p, c_double, c_int, ctypes.c_void_p, _f64p]
         _lib = L
     return _lib
@@ -196,6 +198,16 @@ def run(L, limits, eps, tick, steps, xcur, xpre, w):
     clamped = lib().orc_run_f32(L, xcur.shape[0], _f32(limits), float(eps), float(tick), int(steps), xcur, xpre,
                                 _f32(w), acc)
     return xcur, xpre, acc, clamped
+
+
+def run_f64(L, limits, eps, tick, steps, xcur, xpre, w):
+    """fp64 Stars::update loop (solve + Verlet), for finite differences."""
+    xcur = np.ascontiguousarray(xcur, np.float64).copy()
+    xpre = np.ascontiguousarray(xpre, np.float64).copy()
+    acc = np.zeros_like(xcur)
+    lib().orc_run_f64(L, xcur.shape[0], np.ascontiguousarray(limits, np.float64), float(eps), float(tick),
+                      int(steps), xcur, xpre, np.ascontiguousarray(w, np.float64), acc)
+    return xcur, xpre
 
 
 def direct_f64(pos, w, eps, targets=None):

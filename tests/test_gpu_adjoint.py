@@ -1,0 +1,320 @@
+"""Adjoint of force evaluation and of the Verlet trajectory (no reference
+counterpart; the reference only evaluates forces).
+
+Pins:
+  * central finite differences of the CPU oracle's fp64 solve / fp64 Verlet
+    loop (same algorithm) with every particle kept inside its cell, to 1e-3
+    relative (the GPU adjoint runs in fp32);
+  * the exact regimes where the FMM equals the direct sum (num_level 0, and at
+    most one particle per cell on every level): torch autograd of an fp64
+    direct sum;
+  * size-independent identities at 65536 particles: translation invariance
+    (sum_i dloss/dx_i = 0) and degree-1 homogeneity in the masses
+    (sum_j m_j dloss/dm_j = sum_i g_i . a_i);
+  * checkpointing: K = 1, K = 4 and K = steps give the same bits;
+  * batches: system by system the same bits as separate solvers.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import pyoracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(a):
+    return torch.as_tensor(np.ascontiguousarray(a, dtype=np.float32), device="cuda")
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64).ravel()
+    b = np.asarray(b, np.float64).ravel()
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def inside_cells(rng, n, L, lim, margin=0.2):
+    """n points in distinct random leaves, each at least `margin` of a leaf
+    width from the leaf's edges."""
+    B = 1 << (L + 1)
+    cw = (lim[2] - lim[0]) / B
+    ch = (lim[3] - lim[1]) / B
+    cells = rng.choice(B * B, size=n, replace=n > B * B)
+    ix, iy = cells % B, cells // B
+    x = lim[0] + (ix + rng.uniform(margin, 1 - margin, n)) * cw
+    y = lim[1] + (iy + rng.uniform(margin, 1 - margin, n)) * ch
+    return np.stack([x, y], 1)
+
+
+def gpu_adjoint(L, pos, w, lim, eps, g):
+    from nbfmm_amd import Solver
+    n = pos.shape[0]
+    s = Solver(L, n, lim, eps)
+    gx = torch.zeros((n, 2), device="cuda")
+    gm = torch.zeros(n, device="cuda")
+    s.solve_adjoint(n, dev(pos), dev(w), dev(g), gx, gm)
+    torch.cuda.synchronize()
+    out = gx.cpu().numpy(), gm.cpu().numpy()
+    s.close()
+    return out
+
+
+def fd_solve(L, pos, w, lim, eps, g, h=1e-5):
+    """Central differences of sum_i g_i . a_i with the fp64 oracle."""
+    pos = pos.astype(np.float64)
+    w = w.astype(np.float64)
+    lim64 = np.asarray(lim, np.float64)
+
+    def loss(p, m):
+        return float((g * orc.solve_f64(L, p, m, lim64, eps)).sum())
+
+    gx = np.zeros_like(pos)
+    for i in range(pos.shape[0]):
+        for d in range(2):
+            pp, pm = pos.copy(), pos.copy()
+            pp[i, d] += h
+            pm[i, d] -= h
+            gx[i, d] = (loss(pp, w) - loss(pm, w)) / (2 * h)
+    gm = np.zeros_like(w)
+    for i in range(w.shape[0]):
+        wp, wm = w.copy(), w.copy()
+        wp[i] += h
+        wm[i] -= h
+        gm[i] = (loss(pos, wp) - loss(pos, wm)) / (2 * h)
+    return gx, gm
+
+
+@pytest.mark.parametrize("L", [1, 3, 4])
+def test_solve_adjoint_matches_fd(L):
+    rng = np.random.default_rng(300 + L)
+    lim = (-1.0, -1.0, 1.0, 1.0)
+    n = 256
+    pos = inside_cells(rng, n, L, lim).astype(np.float32)
+    w = rng.uniform(0.5, 1.5, n).astype(np.float32)
+    g = rng.normal(0, 1, (n, 2))
+    eps = 1e-3
+    gx, gm = gpu_adjoint(L, pos, w, lim, eps, g)
+    fx, fm = fd_solve(L, pos, w, lim, eps, g)
+    assert rel(gx, fx) < 1e-3, rel(gx, fx)
+    assert rel(gm, fm) < 1e-3, rel(gm, fm)
+
+
+def direct_torch(pos, w, eps):
+    d = pos[None, :, :] - pos[:, None, :]
+    r2 = (d * d).sum(-1)
+    eye = torch.eye(pos.shape[0], dtype=pos.dtype)
+    r = torch.sqrt(r2 + eye) * (1 - eye)
+    f = w[None, :] / (r ** 3 + eps) * (1 - eye)
+    return (d * f[..., None]).sum(1)
+
+
+@pytest.mark.parametrize("L,n", [(0, 300), (2, 16), (5, 16)])
+def test_adjoint_exact_regime_matches_direct_autograd(L, n):
+    """num_level 0 is the direct sum; with one particle per top-level cell
+    every level has at most one particle per cell and the FMM equals the
+    direct sum, so its gradient is the direct sum's."""
+    rng = np.random.default_rng(400 + L)
+    lim = (-1.0, -1.0, 1.0, 1.0)
+    if L == 0:
+        pos = rng.uniform(-0.95, 0.95, (n, 2))
+    else:
+        top = 2.0 / 4
+        pos = np.array([[-1 + (i + rng.uniform(0.2, 0.8)) * top, -1 + (j + rng.uniform(0.2, 0.8)) * top]
+                        for i in range(4) for j in range(4)])
+    pos = pos.astype(np.float32)
+    w = rng.uniform(0.5, 1.5, n).astype(np.float32)
+    g = rng.normal(0, 1, (n, 2))
+    eps = 1e-3
+    gx, gm = gpu_adjoint(L, pos, w, lim, eps, g)
+    P = torch.tensor(pos, dtype=torch.float64, requires_grad=True)
+    W = torch.tensor(w, dtype=torch.float64, requires_grad=True)
+    (direct_torch(P, W, eps) * torch.tensor(g)).sum().backward()
+    assert rel(gx, P.grad.numpy()) < 1e-4, rel(gx, P.grad.numpy())
+    assert rel(gm, W.grad.numpy()) < 1e-4, rel(gm, W.grad.numpy())
+
+
+def test_adjoint_identities_full_size():
+    """65536 particles, 128x128 leaves: translation invariance and mass
+    homogeneity of the FMM hold for its gradient."""
+    from nbfmm_amd import Solver
+    from nbfmm_amd.workloads import gaussian_disk_system
+    wl = gaussian_disk_system()
+    n = wl.n
+    rng = np.random.default_rng(5)
+    g = rng.normal(0, 1, (n, 2)).astype(np.float32)
+    s = Solver(wl.num_level, n, wl.limits, wl.eps)
+    x, w, G = dev(wl.position), dev(wl.weight), dev(g)
+    a = torch.zeros_like(x)
+    gx = torch.zeros_like(x)
+    gm = torch.zeros_like(w)
+    s.solve(n, x, w, a)
+    s.solve_adjoint(n, x, w, G, gx, gm)
+    torch.cuda.synchronize()
+    gx, gm, a = gx.double().cpu().numpy(), gm.double().cpu().numpy(), a.double().cpu().numpy()
+    assert np.all(np.abs(gx.sum(0)) < 1e-4 * np.abs(gx).sum(0))
+    lhs = float((wl.weight.astype(np.float64) * gm).sum())
+    rhs = float((g.astype(np.float64) * a).sum())
+    assert abs(lhs - rhs) < 1e-4 * (np.abs(wl.weight * gm).sum())
+    s.close()
+
+
+def _trajectory_case(seed=11):
+    """Small system whose fp64 trajectory keeps every particle at least 5% of
+    a leaf away from leaf edges at every step (finite differences valid)."""
+    L, n, steps = 2, 96, 12
+    lim = (-1.0, -1.0, 1.0, 1.0)
+    B = 1 << (L + 1)
+    for s in range(seed, seed + 50):
+        rng = np.random.default_rng(s)
+        pos = inside_cells(rng, n, L, lim, margin=0.3)
+        vel = rng.normal(0, 0.5, (n, 2))
+        w = rng.uniform(0.5, 1.5, n) * 1e-2
+        tick = 2e-3
+        pre = pos - vel * tick
+        ok = True
+        x, p = pos.copy(), pre.copy()
+        for _ in range(steps + 1):
+            fx = (x[:, 0] - lim[0]) / ((lim[2] - lim[0]) / B)
+            fy = (x[:, 1] - lim[1]) / ((lim[3] - lim[1]) / B)
+            frac = np.minimum(np.minimum(fx % 1, 1 - fx % 1), np.minimum(fy % 1, 1 - fy % 1))
+            if frac.min() < 0.05 or np.abs(x).max() > 0.99:
+                ok = False
+                break
+            x, p = orc.run_f64(L, lim, 1e-3, tick, 1, x, p, w)
+        if ok:
+            return L, lim, pos.astype(np.float32), pre.astype(np.float32), w.astype(np.float32), tick, steps
+    pytest.skip("no trajectory without cell crossings found")
+
+
+def test_trajectory_adjoint_matches_fd_and_checkpoint_invariance():
+    from nbfmm_amd import Solver
+    from nbfmm_amd.autograd import fmm_trajectory
+    L, lim, pos, pre, w, tick, steps = _trajectory_case()
+    n = pos.shape[0]
+    obs = [6, steps]
+    rng = np.random.default_rng(3)
+    c = rng.normal(0, 1, (len(obs), n, 2))
+    s = Solver(L, n, lim, 1e-3)
+    grads = {}
+    for K in (1, 4, steps):
+        x0 = dev(pos).requires_grad_(True)
+        p0 = dev(pre).requires_grad_(True)
+        m = dev(w).requires_grad_(True)
+        out = fmm_trajectory(s, tick, steps, K, obs, x0, p0, m)
+        (out * dev(c)).sum().backward()
+        grads[K] = (x0.grad.clone(), p0.grad.clone(), m.grad.clone())
+    for K in (4, steps):
+        for a, b in zip(grads[1], grads[K]):
+            assert torch.equal(a, b), K
+    gx, gp, gm = (t.cpu().numpy() for t in grads[4])
+
+    def loss(x, p, m):
+        tot = 0.0
+        prev = 0
+        xc, pc = x.astype(np.float64), p.astype(np.float64)
+        for o, k in enumerate(obs):
+            xc, pc = orc.run_f64(L, lim, 1e-3, tick, k - prev, xc, pc, m.astype(np.float64))
+            prev = k
+            tot += float((c[o] * xc).sum())
+        return tot
+
+    h = 1e-6
+    idx = rng.choice(n, 12, replace=False)
+    for i in idx:
+        for d in range(2):
+            for arr, grad in ((pos, gx), (pre, gp)):
+                a_p = arr.astype(np.float64).copy()
+                a_m = a_p.copy()
+                a_p[i, d] += h
+                a_m[i, d] -= h
+                if arr is pos:
+                    fd = (loss(a_p, pre, w) - loss(a_m, pre, w)) / (2 * h)
+                else:
+                    fd = (loss(pos, a_p, w) - loss(pos, a_m, w)) / (2 * h)
+                assert abs(grad[i, d] - fd) <= 1e-3 * max(abs(fd), 1.0), (i, d, grad[i, d], fd)
+        wp = w.astype(np.float64).copy()
+        wm = wp.copy()
+        wp[i] += h
+        wm[i] -= h
+        fd = (loss(pos, pre, wp) - loss(pos, pre, wm)) / (2 * h)
+        assert abs(gm[i] - fd) <= 1e-3 * max(abs(fd), 1.0), (i, gm[i], fd)
+    s.close()
+
+
+def test_forces_autograd_function():
+    from nbfmm_amd import Solver
+    from nbfmm_amd.autograd import fmm_forces
+    rng = np.random.default_rng(8)
+    n, L = 3000, 4
+    lim = (-1.0, -1.0, 1.0, 1.0)
+    pos = rng.uniform(-0.9, 0.9, (n, 2)).astype(np.float32)
+    w = rng.uniform(0.5, 1.5, n).astype(np.float32)
+    g = rng.normal(0, 1, (n, 2)).astype(np.float32)
+    s = Solver(L, n, lim, 1e-3)
+    x = dev(pos).requires_grad_(True)
+    m = dev(w).requires_grad_(True)
+    (fmm_forces(s, x, m) * dev(g)).sum().backward()
+    gx, gm = gpu_adjoint(L, pos, w, lim, 1e-3, g)
+    assert np.array_equal(x.grad.cpu().numpy(), gx)
+    assert np.array_equal(m.grad.cpu().numpy(), gm)
+    s.close()
+
+
+def test_mass_inference_loss_decreases():
+    """Fit masses to a trajectory made with perturbed masses (small version
+    of the mass-inference workload): a few gradient steps reduce the loss."""
+    from nbfmm_amd import Solver
+    from nbfmm_amd.autograd import fmm_trajectory
+    from nbfmm_amd.workloads import gaussian_disk_system
+    wl = gaussian_disk_system(n=4096, num_level=4, steps=20)
+    rng = np.random.default_rng(21)
+    s = Solver(wl.num_level, wl.n, wl.limits, wl.eps)
+    true_m = dev(wl.weight)
+    obs = [10, 20]
+    tick = wl.dt * np.sqrt(wl.G) * 30  # longer steps so the masses matter
+    with torch.no_grad():
+        target = fmm_trajectory(s, tick, 20, 5, obs, dev(wl.position), dev(wl.position_pre), true_m)
+    m = (true_m * dev(rng.uniform(0.8, 1.2, wl.n))).requires_grad_(True)
+    losses = []
+    for _ in range(4):
+        out = fmm_trajectory(s, tick, 20, 5, obs, dev(wl.position), dev(wl.position_pre), m)
+        loss = ((out - target) ** 2).mean()
+        losses.append(float(loss))
+        m.grad = None
+        loss.backward()
+        with torch.no_grad():
+            m -= 0.5 * losses[0] / max(float((m.grad ** 2).sum()), 1e-30) * m.grad
+    assert losses[-1] < losses[0]
+    s.close()
+
+
+def test_batch_adjoint_equals_single():
+    from nbfmm_amd import Batch, Solver
+    from nbfmm_amd.workloads import many_disks
+    bw = many_disks(num_systems=6, n_min=300, n_max=20000, seed=17, steps=3)
+    x, p, w = bw.concat()
+    rng = np.random.default_rng(4)
+    g = rng.normal(0, 1, x.shape).astype(np.float32)
+    b = Batch([s.n for s in bw.systems], [s.num_level for s in bw.systems], [s.limits for s in bw.systems],
+              [s.eps for s in bw.systems], ticks=[s.tick for s in bw.systems])
+    GX = torch.zeros_like(dev(x))
+    GM = torch.zeros_like(dev(w))
+    b.solve_adjoint(dev(x), dev(w), dev(g), GX, GM)
+    # one Verlet step adjoint from lambda = g, pi = 0
+    LAM, PI, GW = dev(g), torch.zeros_like(dev(x)), torch.zeros_like(dev(w))
+    b.step_adjoint(dev(x), dev(w), LAM, PI, GW)
+    torch.cuda.synchronize()
+    off = b.offsets
+    for k, s in enumerate(bw.systems):
+        one = Solver(s.num_level, s.n, s.limits, s.eps)
+        sl = slice(off[k], off[k + 1])
+        gx = torch.zeros((s.n, 2), device="cuda")
+        gm = torch.zeros(s.n, device="cuda")
+        one.solve_adjoint(s.n, dev(s.position), dev(s.weight), dev(g[sl]), gx, gm)
+        lam, pi, gw = dev(g[sl]), torch.zeros((s.n, 2), device="cuda"), torch.zeros(s.n, device="cuda")
+        one.step_adjoint(s.n, s.tick, dev(s.position), dev(s.weight), lam, pi, gw)
+        torch.cuda.synchronize()
+        assert torch.equal(GX[sl], gx) and torch.equal(GM[sl], gm), k
+        assert torch.equal(LAM[sl], lam) and torch.equal(PI[sl], pi) and torch.equal(GW[sl], gw), k
+        one.close()
+    b.close()
