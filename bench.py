@@ -47,6 +47,9 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-systems leg")
     ap.add_argument("--batch-systems", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=20)
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint legs")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="also run one 256 MiB device-to-device copy (scale reference for HBM counters)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -156,7 +159,16 @@ def main():
                          f"{tcpu:.1f} s"}
         log(f"[bench] cpu baseline {cpu}")
 
+    if args.calib_copy:
+        src = torch.ones(64 << 20, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     batched = None if args.no_batch else batch_leg(args)
+    adjoint = None if args.no_adjoint else adjoint_leg()
+    batched_adjoint = None if (args.no_adjoint or args.no_batch) else batch_adjoint_leg()
 
     out = {
         "metric": "particle-steps/s at N=2^20; batched+adjoint vs plain, force rel err <=1e-4",
@@ -182,6 +194,8 @@ def main():
         "cpu_baseline": cpu,
         "accuracy": accuracy,
         "batched": batched,
+        "adjoint": adjoint,
+        "batched_adjoint": batched_adjoint,
     }
     s.close()
     print(json.dumps(out), flush=True)
@@ -243,6 +257,109 @@ def batch_leg(args):
            "per_system_ms_per_step": ts * 1e3 / steps, "flags": flags}
     log(f"[bench] batch {out}")
     return out
+
+
+def adjoint_leg():
+    """Differentiable trajectory: 65536-particle Gaussian disk, 128x128
+    leaves, 50 Verlet steps checkpointed every 10; loss = mean squared
+    distance of the final positions to targets from a run with masses
+    perturbed by 5% Gaussian noise; gradients wrt initial positions, x_pre
+    and masses.  Reports forward and backward time."""
+    import torch
+
+    from nbfmm_amd import Solver
+    from nbfmm_amd.autograd import fmm_trajectory
+    from nbfmm_amd.workloads import gaussian_disk_system
+
+    wl = gaussian_disk_system()
+    n, steps, K = wl.n, wl.steps, 10
+    dev = torch.device("cuda:0")
+    s = Solver(wl.num_level, n, wl.limits, wl.eps)
+    x0 = torch.as_tensor(wl.position, device=dev)
+    p0 = torch.as_tensor(wl.position_pre, device=dev)
+    m_true = torch.as_tensor(wl.weight, device=dev)
+    rng = np.random.default_rng(31)
+    m_pert = m_true * torch.as_tensor((1 + 0.05 * rng.normal(size=n)).astype(np.float32), device=dev)
+    with torch.no_grad():
+        target = fmm_trajectory(s, wl.tick, steps, K, [steps], x0, p0, m_pert)
+
+    def fwd():
+        xg = x0.clone().requires_grad_(True)
+        pg = p0.clone().requires_grad_(True)
+        mg = m_true.clone().requires_grad_(True)
+        out = fmm_trajectory(s, wl.tick, steps, K, [steps], xg, pg, mg)
+        loss = ((out - target) ** 2).sum(-1).mean()
+        return loss, (xg, pg, mg)
+
+    loss, leaves = fwd()  # warm-up
+    loss.backward()
+    torch.cuda.synchronize()
+    reps = 3
+    tf = tb = 0.0
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        loss, leaves = fwd()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        loss.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        tf += t1 - t0
+        tb += t2 - t1
+    tf /= reps
+    tb /= reps
+    gm = leaves[2].grad
+    s.close()
+    out = {"workload": wl.name, "n": n, "num_level": wl.num_level, "steps": steps, "checkpoint_every": K,
+           "forward_ms": tf * 1e3, "backward_ms": tb * 1e3, "backward_over_forward": tb / tf,
+           "forward_particle_steps_per_s": n * steps / tf,
+           "forward_backward_particle_steps_per_s": n * steps / (tf + tb),
+           "loss": float(loss), "grad_mass_finite": bool(torch.isfinite(gm).all())}
+    log(f"[bench] adjoint {out}")
+    return out
+
+
+def batch_adjoint_leg():
+    """Batched mass inference: 64 two-disk collisions (N log-uniform in
+    [4096, 262144]), 100 steps checkpointed every 10, loss = position
+    mismatch at steps 25/50/75/100 against a run with the true masses,
+    starting masses = true x U[0.8,1.2]; gradients wrt the masses of every
+    system in one batched backward."""
+    import torch
+
+    from nbfmm_amd import Batch
+    from nbfmm_amd.autograd import fmm_batch_trajectory
+    from nbfmm_amd.workloads import many_collisions
+
+    bw = many_collisions()
+    steps, K, obs = bw.steps, 10, [25, 50, 75, 100]
+    x, p, w = bw.concat()
+    dev = torch.device("cuda:0")
+    X, P, W = (torch.as_tensor(a, device=dev) for a in (x, p, w))
+    b = Batch([s.n for s in bw.systems], [s.num_level for s in bw.systems], [s.limits for s in bw.systems],
+              [s.eps for s in bw.systems], ticks=[s.tick for s in bw.systems])
+    rng = np.random.default_rng(41)
+    with torch.no_grad():
+        target = fmm_batch_trajectory(b, steps, K, obs, X, P, W)
+    torch.cuda.synchronize()
+    m = (W * torch.as_tensor(rng.uniform(0.8, 1.2, W.shape[0]).astype(np.float32), device=dev)).requires_grad_(True)
+    t0 = time.perf_counter()
+    out = fmm_batch_trajectory(b, steps, K, obs, X, P, m)
+    loss = ((out - target) ** 2).sum(-1).mean()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    loss.backward()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    b.close()
+    total = bw.n
+    res = {"workload": bw.name, "systems": len(bw.systems), "particles": total, "steps": steps,
+           "checkpoint_every": K, "forward_ms": (t1 - t0) * 1e3, "backward_ms": (t2 - t1) * 1e3,
+           "backward_over_forward": (t2 - t1) / (t1 - t0),
+           "forward_backward_particle_steps_per_s": total * steps / (t2 - t0), "loss": float(loss),
+           "grad_mass_finite": bool(torch.isfinite(m.grad).all())}
+    log(f"[bench] batched adjoint {res}")
+    return res
 
 
 if __name__ == "__main__":

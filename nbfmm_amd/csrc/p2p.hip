@@ -40,6 +40,27 @@ __device__ __forceinline__ float2 leaf_local(const SysView& v, int cx, int cy) {
 }
 
 
+// Lane (t, k) of a chunk: target pt against staged sources k, k+K, k+2K, ...
+// (cnt sources, zero-mass padded to a multiple of kP2PUnroll*K).  K is a
+// compile-time constant so the four LDS reads use immediate offsets.
+template <int K>
+__device__ __forceinline__ void p2p_lane_sum(const float4* __restrict__ src, int k, int cnt, float4 pt, float eps,
+                                             float& ax, float& ay) {
+  constexpr int step = kP2PUnroll * K;
+  const int iters = __builtin_amdgcn_readfirstlane((cnt + step - 1) / step);
+  const float4* p = src + k;
+  for (int i = 0; i < iters; ++i, p += step) {
+    const float4 s0 = p[0];
+    const float4 s1 = p[K];
+    const float4 s2 = p[2 * K];
+    const float4 s3 = p[3 * K];
+    kernel_accum(pt.x, pt.y, s0.x, s0.y, s0.z, eps, ax, ay);
+    kernel_accum(pt.x, pt.y, s1.x, s1.y, s1.z, eps, ax, ay);
+    kernel_accum(pt.x, pt.y, s2.x, s2.y, s2.z, eps, ax, ay);
+    kernel_accum(pt.x, pt.y, s3.x, s3.y, s3.z, eps, ax, ay);
+  }
+}
+
 template <int MODE, class Sys>
 __global__ __launch_bounds__(kP2PThreads) void k_p2p(Sys sys, CallIO io) {
   __shared__ float4 s_src[kP2PThreads / kWave][kP2PStage];
@@ -96,18 +117,14 @@ __global__ __launch_bounds__(kP2PThreads) void k_p2p(Sys sys, CallIO io) {
           src[slot] = val;
         }
         wave_lds_order();
-        const int step = kP2PUnroll * K;
-        const int iters = (cnt + step - 1) / step;
-        const float4* p = src + k;
-        for (int i = 0; i < iters; ++i, p += step) {
-          const float4 s0 = p[0];
-          const float4 s1 = p[K];
-          const float4 s2 = p[2 * K];
-          const float4 s3 = p[3 * K];
-          kernel_accum(pt.x, pt.y, s0.x, s0.y, s0.z, eps, ax, ay);
-          kernel_accum(pt.x, pt.y, s1.x, s1.y, s1.z, eps, ax, ay);
-          kernel_accum(pt.x, pt.y, s2.x, s2.y, s2.z, eps, ax, ay);
-          kernel_accum(pt.x, pt.y, s3.x, s3.y, s3.z, eps, ax, ay);
+        switch (lgG) {
+          case 0: p2p_lane_sum<64>(src, k, cnt, pt, eps, ax, ay); break;
+          case 1: p2p_lane_sum<32>(src, k, cnt, pt, eps, ax, ay); break;
+          case 2: p2p_lane_sum<16>(src, k, cnt, pt, eps, ax, ay); break;
+          case 3: p2p_lane_sum<8>(src, k, cnt, pt, eps, ax, ay); break;
+          case 4: p2p_lane_sum<4>(src, k, cnt, pt, eps, ax, ay); break;
+          case 5: p2p_lane_sum<2>(src, k, cnt, pt, eps, ax, ay); break;
+          default: p2p_lane_sum<1>(src, k, cnt, pt, eps, ax, ay); break;
         }
         wave_lds_order();
       }

@@ -169,6 +169,21 @@ def many_disks(num_systems=256, n_min=1024, n_max=131072, seed=1004, steps=20):
     return BatchWorkload("many_disks", systems, steps)
 
 
+def many_collisions(num_systems=64, n_min=4096, n_max=262144, seed=1005, steps=100):
+    """Independent two-disk collisions (the two-disk geometry with N_s
+    particles): N log-uniform in [n_min, n_max], levels as in many_disks,
+    G = 1/N per system."""
+    rng = np.random.default_rng(seed)
+    systems = []
+    for k in range(num_systems):
+        n = int(np.floor(np.exp(rng.uniform(np.log(n_min), np.log(n_max)))))
+        levels = int(np.clip(np.round(np.log(n / 16.0) / np.log(4.0)), 3, 9))
+        w = two_disk_collision(n=n, seed=int(rng.integers(1 << 30)), steps=steps, num_level=levels - 1)
+        w.name = f"sys{k}"
+        systems.append(w)
+    return BatchWorkload("many_collisions", systems, steps)
+
+
 def by_name(name: str, **kw) -> Workload:
     return {"disk": disk, "two_disk_collision": two_disk_collision,
             "gaussian_disk": gaussian_disk_system}[name](**kw)
