@@ -26,6 +26,24 @@ sys.path.insert(0, ROOT)
 FP32_PEAK_TFLOPS = 157.3   # MI355X published FP32 vector peak
 HBM_PEAK_GBS = 8000.0
 P2P_FLOPS_PER_PAIR = 14    # sub 2, r^2 3, sqrt 1, r^3 2, +eps 1, div 1, scale 2, accumulate 2
+# VALU issue capacity: 256 CUs x 64 lanes x 2.4 GHz lane-instructions per second
+VALU_ISSUE_PER_S = 256 * 64 * 2.4e9
+# instruction slots per pair of the P2P inner loop: 8.25 VALU + 2 transcendentals
+# at 1.74 slots each (measured transcendental cost on gfx950 relative to an FMA)
+P2P_SLOTS_PER_PAIR = 8.25 + 2 * 1.74
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_two_disk_N1048576.md")
+
+
+def measured_traffic():
+    """HBM bytes per P2P launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary of this workload (None if absent)."""
+    try:
+        for line in open(TRAFFIC_PROFILE):
+            if line.startswith("traffic_bytes_per_launch k_p2p<kP2PStep>:"):
+                return float(line.split(":")[1])
+    except OSError:
+        return None
+    return None
 
 
 def log(msg):
@@ -186,10 +204,13 @@ def main():
         "config": {"workload": wl.name, "n": n, "num_level": wl.num_level, "leaf_grid": 1 << (wl.num_level + 1),
                    "limits": list(wl.limits), "eps": wl.eps, "dt": wl.dt, "G": wl.G, "mode": "plain, hipGraph",
                    "parallelism": "single GPU"},
-        "roofline": {"kernel": "k_p2p<kP2PStep>", "bound": "valu-fp32", "achieved": achieved,
+        "roofline": {"kernel": "k_p2p<kP2PStep>", "bound": "valu", "achieved": achieved,
                      "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
-                     "traffic": None, "avg_launch_us": p2p_avg_ms * 1e3, "launches": launches,
+                     "traffic": (measured_traffic() if wl.name == "two_disk_collision" and n == 1 << 20
+                                 else None),
+                     "avg_launch_us": p2p_avg_ms * 1e3, "launches": launches,
                      "pairs_per_launch": pairs, "flops_per_pair": P2P_FLOPS_PER_PAIR,
+                     "valu_issue_frac": pairs * P2P_SLOTS_PER_PAIR / (p2p_avg_ms * 1e-3 * VALU_ISSUE_PER_S),
                      "occupied_leaves": int(occ0)},
         "cpu_baseline": cpu,
         "accuracy": accuracy,
@@ -314,7 +335,7 @@ def adjoint_leg():
            "forward_ms": tf * 1e3, "backward_ms": tb * 1e3, "backward_over_forward": tb / tf,
            "forward_particle_steps_per_s": n * steps / tf,
            "forward_backward_particle_steps_per_s": n * steps / (tf + tb),
-           "loss": float(loss), "grad_mass_finite": bool(torch.isfinite(gm).all())}
+           "loss": float(loss.detach()), "grad_mass_finite": bool(torch.isfinite(gm).all())}
     log(f"[bench] adjoint {out}")
     return out
 
@@ -356,7 +377,7 @@ def batch_adjoint_leg():
     res = {"workload": bw.name, "systems": len(bw.systems), "particles": total, "steps": steps,
            "checkpoint_every": K, "forward_ms": (t1 - t0) * 1e3, "backward_ms": (t2 - t1) * 1e3,
            "backward_over_forward": (t2 - t1) / (t1 - t0),
-           "forward_backward_particle_steps_per_s": total * steps / (t2 - t0), "loss": float(loss),
+           "forward_backward_particle_steps_per_s": total * steps / (t2 - t0), "loss": float(loss.detach()),
            "grad_mass_finite": bool(torch.isfinite(m.grad).all())}
     log(f"[bench] batched adjoint {res}")
     return res
