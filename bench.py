@@ -66,6 +66,7 @@ def main():
     ap.add_argument("--batch-systems", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint legs")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default P2P configuration")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one 256 MiB device-to-device copy (scale reference for HBM counters)")
     args = ap.parse_args()
@@ -94,7 +95,17 @@ def main():
     x0, p0 = x.clone(), p.clone()
     stream = torch.cuda.Stream()
 
-    # pair count at the initial state (the P2P work figure)
+    # P2P configuration: timed on this input (all configurations give the
+    # same bits), then the pair count at the initial state (the P2P work figure)
+    tuned = None
+    if not args.no_tune:
+        from nbfmm_amd.autotune import tune, variants
+        with torch.cuda.stream(stream):
+            v, bpc, tms = tune(s, n, x, w, cache_path=None, repeats=3)
+        stage, unroll, waves = variants()[v]
+        tuned = {"variant": v, "stage": stage, "unroll": unroll, "waves_per_block": waves, "blocks_per_cu": bpc,
+                 "p2p_ms": tms}
+        log(f"[bench] tuned P2P: {tuned}")
     with torch.cuda.stream(stream):
         s.solve(n, x, w, a)
     stream.synchronize()
@@ -203,7 +214,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": wl.name, "n": n, "num_level": wl.num_level, "leaf_grid": 1 << (wl.num_level + 1),
                    "limits": list(wl.limits), "eps": wl.eps, "dt": wl.dt, "G": wl.G, "mode": "plain, hipGraph",
-                   "parallelism": "single GPU"},
+                   "parallelism": "single GPU", "p2p_config": tuned},
         "roofline": {"kernel": "k_p2p<kP2PStep>", "bound": "valu", "achieved": achieved,
                      "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
                      "traffic": (measured_traffic() if wl.name == "two_disk_collision" and n == 1 << 20

@@ -146,6 +146,21 @@ int nbfmm_profile_enable(nbfmm_solver* solver, int enable);
 int nbfmm_p2p_time(nbfmm_solver* solver, double* total_ms, int* launches, int reset);
 int nbfmm_pair_count(nbfmm_solver* solver, int num_particle, double* pairs, int* occupied_cells);
 
+/* ---------------------------------------------------------------- tuning
+ * The P2P kernel comes in variants (LDS staging size, unroll, waves per
+ * block) and runs on a grid of blocks_per_cu blocks per compute unit.  Every configuration gives
+ * bit-identical results; only the time differs.  No reference counterpart
+ * (the reference fixes 64-thread blocks, source/nbfmm/core/solver/p2p.cu:16). */
+int nbfmm_p2p_variant_count(void);
+int nbfmm_p2p_variant_info(int variant, int* stage, int* unroll, int* waves_per_block);
+int nbfmm_set_p2p_config(nbfmm_solver* solver, int variant, int blocks_per_cu);
+int nbfmm_get_p2p_config(const nbfmm_solver* solver, int* variant, int* blocks_per_cu);
+/* Times every configuration on this input (repeats launches each), keeps the
+ * fastest in the solver and returns it.  Synchronises. */
+int nbfmm_autotune(nbfmm_solver* solver, int num_particle, const float* d_position_origin,
+                   const float* d_weight_origin, int repeats, int* best_variant, int* best_blocks_per_cu,
+                   float* best_ms, void* stream);
+
 /* ---------------------------------------------------------------- adjoint
  * No reference counterpart (the reference only evaluates forces).  The
  * gradients treat each particle's cell as fixed (the cell assignment is
@@ -214,6 +229,11 @@ int nbfmm_batch_run(nbfmm_batch* batch, int num_steps, float* d_position_cur, fl
                     const float* d_weight, float* d_acceleration, int use_graph, void* stream);
 /* Synchronises; flags[s] = NBFMM_FLAG_* bits of system s. */
 int nbfmm_batch_flags(nbfmm_batch* batch, unsigned* flags, int clear);
+/* P2P configuration of the batch (see nbfmm_set_p2p_config): the grid of
+ * blocks_per_cu blocks per CU is shared among the systems in proportion to
+ * their particle counts. */
+int nbfmm_batch_set_p2p_config(nbfmm_batch* batch, int variant, int blocks_per_cu);
+int nbfmm_batch_get_p2p_config(const nbfmm_batch* batch, int* variant, int* blocks_per_cu);
 /* nbfmm_solve_adjoint of every system (concatenated arrays). */
 int nbfmm_batch_solve_adjoint(nbfmm_batch* batch, const float* d_position_origin, const float* d_weight_origin,
                               const float* d_grad_effect, float* d_grad_position, float* d_grad_weight,

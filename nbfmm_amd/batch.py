@@ -59,6 +59,16 @@ class Batch:
         t = np.ascontiguousarray(np.broadcast_to(np.asarray(ticks, np.float32), (self.num_systems,)))
         check(lib().nbfmm_batch_set_ticks(self._h, t.ctypes.data))
 
+    def set_p2p_config(self, variant, blocks_per_cu):
+        """P2P kernel variant (see nbfmm_amd.autotune.variants) and grid of
+        every launch; all configurations give identical results."""
+        check(lib().nbfmm_batch_set_p2p_config(self._h, int(variant), int(blocks_per_cu)))
+
+    def p2p_config(self):
+        v, b = ctypes.c_int(), ctypes.c_int()
+        check(lib().nbfmm_batch_get_p2p_config(self._h, ctypes.byref(v), ctypes.byref(b)))
+        return v.value, b.value
+
     def solve(self, position, weight, effect, stream=None):
         check(lib().nbfmm_batch_solve(self._h, _ptr(position), _ptr(weight), _ptr(effect), _stream(stream)))
 

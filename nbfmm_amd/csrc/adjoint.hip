@@ -217,9 +217,10 @@ __device__ __forceinline__ float4 leaf_multipole_grad(const SysView& v, int cx, 
 enum AdjMode : int { kAdjSolve = 0, kAdjStep = 1 };
 
 template <int MODE, class Sys>
-__global__ __launch_bounds__(kP2PThreads) void k_adj_p2p(Sys sys, CallIO io) {
-  __shared__ float4 s_pos[kP2PThreads / kWave][kP2PStage];
-  __shared__ float2 s_g[kP2PThreads / kWave][kP2PStage];
+__global__ __launch_bounds__(kAdjP2PWaves * kWave) void k_adj_p2p(Sys sys, CallIO io) {
+  __shared__ float4 s_pos[kAdjP2PWaves][kP2PStage];
+  __shared__ float2 s_g[kAdjP2PWaves][kP2PStage];
+  __shared__ int s_next;
   int blk, nblk;
   const SysView v = sys.get(blk, nblk);
   const int lane = lane_id();
@@ -229,12 +230,15 @@ __global__ __launch_bounds__(kP2PThreads) void k_adj_p2p(Sys sys, CallIO io) {
   float2* gpos = io.gpos + v.part_off;
   float2* gpre = io.gpre + v.part_off;
   float* gw = io.gw + v.part_off;
-  const int nocc = *v.nocc;
-  const int stride = nblk * (kP2PThreads / kWave);
   const int B = v.B;
   const float eps = v.eps;
-  for (int j = blk * (kP2PThreads / kWave) + wave; j < nocc; j += stride) {
-    const int c = v.occ[j];
+  LeafQueue queue;
+  queue.load(v);
+  if (threadIdx.x == 0) s_next = 0;
+  __syncthreads();
+  for (int j = blk + block_take(&s_next) * nblk; j < queue.total;) {
+    const int c = queue.leaf(v, j);
+    j = blk + block_take(&s_next) * nblk;
     const int cx = c & (B - 1), cy = c >> v.lgB;
     const int t0 = v.head[c], t1 = v.head[c + 1];
     const int xl = cx > 0 ? cx - 1 : 0;
@@ -358,9 +362,9 @@ hipError_t launch_adjoint(const SysView& v, const CallIO& io, int mode, int grid
     hipLaunchKernelGGL(k_adj_m2l<OneSys>, dim3(m2l_blocks(v)), dim3(256), 0, s, one);
   }
   if (mode == kAdjSolve)
-    hipLaunchKernelGGL((k_adj_p2p<kAdjSolve, OneSys>), dim3(grid_blocks), dim3(kP2PThreads), 0, s, one, io);
+    hipLaunchKernelGGL((k_adj_p2p<kAdjSolve, OneSys>), dim3(grid_blocks), dim3(kAdjP2PWaves * kWave), 0, s, one, io);
   else
-    hipLaunchKernelGGL((k_adj_p2p<kAdjStep, OneSys>), dim3(grid_blocks), dim3(kP2PThreads), 0, s, one, io);
+    hipLaunchKernelGGL((k_adj_p2p<kAdjStep, OneSys>), dim3(grid_blocks), dim3(kAdjP2PWaves * kWave), 0, s, one, io);
   return hipGetLastError();
 }
 
@@ -382,10 +386,10 @@ hipError_t launch_adjoint_batch(const BatchLaunch& bl, const CallIO& io, int mod
     hipLaunchKernelGGL(k_adj_up_top<ManySys>, dim3(bl.m2m_top.blocks), dim3(1024), 0, s, bl.m2m_top.sys);
   if (bl.m2l.blocks) hipLaunchKernelGGL(k_adj_m2l<ManySys>, dim3(bl.m2l.blocks), dim3(256), 0, s, bl.m2l.sys);
   if (mode == kAdjSolve)
-    hipLaunchKernelGGL((k_adj_p2p<kAdjSolve, ManySys>), dim3(bl.p2p.blocks), dim3(kP2PThreads), 0, s, bl.p2p.sys,
+    hipLaunchKernelGGL((k_adj_p2p<kAdjSolve, ManySys>), dim3(bl.adj_p2p.blocks), dim3(kAdjP2PWaves * kWave), 0, s, bl.adj_p2p.sys,
                        io);
   else
-    hipLaunchKernelGGL((k_adj_p2p<kAdjStep, ManySys>), dim3(bl.p2p.blocks), dim3(kP2PThreads), 0, s, bl.p2p.sys,
+    hipLaunchKernelGGL((k_adj_p2p<kAdjStep, ManySys>), dim3(bl.adj_p2p.blocks), dim3(kAdjP2PWaves * kWave), 0, s, bl.adj_p2p.sys,
                        io);
   return hipGetLastError();
 }

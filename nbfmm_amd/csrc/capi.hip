@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <algorithm>
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -43,7 +44,11 @@ struct nbfmm_solver {
   float limits[4] = {0, 0, 0, 0};
   float cw = 0, ch = 0, eps = 0;
   int device = 0;
-  int p2p_grid = 0;
+  int cus = 256;            // compute units of the device
+  int p2p_grid = 0;         // P2P blocks (each takes every p2p_grid-th leaf of the size-ordered list)
+  int adj_grid = 0;         // adjoint P2P blocks
+  int p2p_variant = kDefaultP2PVariant;     // P2P kernel variant (staging, unroll, waves per block)
+  int p2p_bpc = kDefaultP2PBlocksPerCU;     // P2P blocks per compute unit
   SysView v{};  // arrays and sizes; v.n, v.pos0, v.w0 are set per call
   std::vector<void*> allocs;
   // profiling of the P2P launch
@@ -113,6 +118,37 @@ int check_n(nbfmm_solver* s, int n) {
 
 hipStream_t as_stream(void* p) { return (hipStream_t)p; }
 
+int variant_waves(int variant) {
+  int stage = 0, unroll = 0, waves = 0;
+  p2p_variant_info(variant, &stage, &unroll, &waves);
+  return waves;
+}
+
+// P2P grid of `variant` for `bpc` blocks per CU, never more blocks than
+// occupied leaves can use.
+int p2p_grid_for(const nbfmm_solver* s, int variant, int bpc) {
+  const int C = s->B * s->B;
+  const int max_occ = C < s->max_n ? C : s->max_n;
+  const int want = cdiv(max_occ, variant_waves(variant));
+  int g = want < s->cus * bpc ? want : s->cus * bpc;
+  return g < 1 ? 1 : g;
+}
+
+int adj_grid_for(const nbfmm_solver* s) {
+  const int C = s->B * s->B;
+  const int max_occ = C < s->max_n ? C : s->max_n;
+  const int want = cdiv(max_occ, kAdjP2PWaves);
+  const int g = want < s->cus * kAdjP2PBlocksPerCU ? want : s->cus * kAdjP2PBlocksPerCU;
+  return g < 1 ? 1 : g;
+}
+
+// A captured step graph holds the launch configuration it was made with.
+void drop_graph(nbfmm_solver* s) {
+  if (s->graph_exec) { hipGraphExecDestroy(s->graph_exec); s->graph_exec = nullptr; }
+  if (s->graph) { hipGraphDestroy(s->graph); s->graph = nullptr; }
+  s->gkey.n = -1;
+}
+
 int p2p_timed(nbfmm_solver* s, const SysView& v, int mode, const CallIO& io, hipStream_t st) {
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (s->profile) {
@@ -128,7 +164,7 @@ int p2p_timed(nbfmm_solver* s, const SysView& v, int mode, const CallIO& io, hip
     s->ev_used += 2;
     NB_HIP(hipEventRecord(e0, st));
   }
-  NB_HIP(launch_p2p(v, mode, io, s->p2p_grid, st));
+  NB_HIP(launch_p2p(v, mode, io, s->p2p_grid, s->p2p_variant, st));
   if (s->profile) NB_HIP(hipEventRecord(e1, st));
   return NBFMM_OK;
 }
@@ -149,7 +185,7 @@ int run_pipeline(nbfmm_solver* s, int n, float tick, int mode, const CallIO& io,
 
 extern "C" {
 
-int nbfmm_version(void) { return 10000; }
+int nbfmm_version(void) { return 10100; }
 
 const char* nbfmm_last_error(void) { return g_last_error.c_str(); }
 
@@ -212,6 +248,9 @@ int nbfmm_solver_create(int num_level, int max_num_particle, const float positio
   ALLOC(v.rowstart, (size_t)s->B + 1);
   ALLOC(v.occ, (size_t)C);
   ALLOC(v.nocc, 1);
+  v.ncap = max_num_particle;
+  ALLOC(v.ord, p2p_class_total(C, v.ncap));
+  ALLOC(v.sched, (size_t)kSchedInts);
   ALLOC(v.hist, (size_t)s->B * ntiles);
   ALLOC(v.dtotal, (size_t)s->B);
   ALLOC(v.mpole, (size_t)ncells_tree);
@@ -222,11 +261,10 @@ int nbfmm_solver_create(int num_level, int max_num_particle, const float positio
   ALLOC(v.flags, 1);
   ALLOC(s->d_pairs, 1);
 #undef ALLOC
-  // P2P: grid-stride over occupied leaves, enough waves to fill every CU
-  const int max_occ = C < max_num_particle ? C : max_num_particle;
-  const int want = cdiv(max_occ, kP2PThreads / kWave);
-  s->p2p_grid = want < cus * 16 ? want : cus * 16;
-  if (s->p2p_grid < 1) s->p2p_grid = 1;
+  // P2P: enough waves to fill every CU
+  s->cus = cus;
+  s->p2p_grid = p2p_grid_for(s, s->p2p_variant, s->p2p_bpc);
+  s->adj_grid = adj_grid_for(s);
   *out = s;
   return NBFMM_OK;
 }
@@ -447,7 +485,7 @@ int nbfmm_solve_adjoint(nbfmm_solver* s, int n, const float* pos0, const float* 
   io.gw = grad_w;
   io.gscale = 1.0f;
   if ((rc = adjoint_prepare(s, n, io, st))) return rc;
-  NB_HIP(launch_adjoint(view(s, n), io, 0, s->p2p_grid, st));
+  NB_HIP(launch_adjoint(view(s, n), io, 0, s->adj_grid, st));
   return NBFMM_OK;
 }
 
@@ -464,7 +502,7 @@ int nbfmm_step_adjoint(nbfmm_solver* s, int n, float tick, const float* x, const
   io.gw = grad_w;
   io.gscale = tick * tick;
   if ((rc = adjoint_prepare(s, n, io, st))) return rc;
-  NB_HIP(launch_adjoint(view(s, n), io, 1, s->p2p_grid, st));
+  NB_HIP(launch_adjoint(view(s, n), io, 1, s->adj_grid, st));
   return NBFMM_OK;
 }
 
@@ -557,6 +595,101 @@ int nbfmm_trajectory_backward(nbfmm_solver* s, int n, float tick, int num_steps,
   return NBFMM_OK;
 }
 
+int nbfmm_p2p_variant_count(void) { return p2p_num_variants(); }
+
+int nbfmm_p2p_variant_info(int variant, int* stage, int* unroll, int* waves_per_block) {
+  if (variant < 0 || variant >= p2p_num_variants()) return fail(NBFMM_ERR_INVALID_ARGUMENT, "unknown P2P variant");
+  int a = 0, b = 0, c = 0;
+  p2p_variant_info(variant, &a, &b, &c);
+  if (stage) *stage = a;
+  if (unroll) *unroll = b;
+  if (waves_per_block) *waves_per_block = c;
+  return NBFMM_OK;
+}
+
+int nbfmm_set_p2p_config(nbfmm_solver* s, int variant, int blocks_per_cu) {
+  if (!s) return fail(NBFMM_ERR_INVALID_ARGUMENT, "null solver");
+  if (variant < 0 || variant >= p2p_num_variants()) return fail(NBFMM_ERR_INVALID_ARGUMENT, "unknown P2P variant");
+  if (blocks_per_cu < 1 || blocks_per_cu > 64) return fail(NBFMM_ERR_INVALID_ARGUMENT, "blocks_per_cu in [1, 64]");
+  s->p2p_variant = variant;
+  s->p2p_bpc = blocks_per_cu;
+  s->p2p_grid = p2p_grid_for(s, variant, blocks_per_cu);
+  drop_graph(s);
+  return NBFMM_OK;
+}
+
+int nbfmm_get_p2p_config(const nbfmm_solver* s, int* variant, int* blocks_per_cu) {
+  if (!s) return fail(NBFMM_ERR_INVALID_ARGUMENT, "null solver");
+  if (variant) *variant = s->p2p_variant;
+  if (blocks_per_cu) *blocks_per_cu = s->p2p_bpc;
+  return NBFMM_OK;
+}
+
+// Times every P2P configuration (variant x blocks per CU) on this input and
+// keeps the fastest.  Synchronises.  The configurations give identical
+// results, so tuning never changes an answer.
+int nbfmm_autotune(nbfmm_solver* s, int n, const float* pos0, const float* w0, int repeats, int* best_variant,
+                   int* best_bpc, float* best_ms, void* stream) {
+  int rc = check_n(s, n);
+  if (rc) return rc;
+  if (n <= 0 || !pos0 || !w0) return fail(NBFMM_ERR_INVALID_ARGUMENT, "autotune needs particles");
+  if (repeats < 1) repeats = 1;
+  hipStream_t st = as_stream(stream);
+  const SysView v = view(s, n);
+  const CallIO io = call_io(pos0, w0);
+  NB_HIP(launch_predo(v, io, 1, st));
+  NB_HIP(launch_upward(v, std::min(n, s->B * s->B), st));
+  NB_HIP(launch_m2l(v, st));
+  // Time the step kernel itself (L2L walk, scatter and Verlet epilogue
+  // included) with its outputs pointed at scratch that the next predo
+  // rewrites: accelerations into eff, x and x_pre into the two halves of the
+  // row-pass particle buffer.  The caller's arrays are not touched.
+  CallIO tio = io;
+  tio.acc0 = v.eff;
+  tio.xcur = (float2*)v.tpart;
+  tio.xpre = (float2*)v.tpart + s->max_n;
+  hipEvent_t e0, e1;
+  NB_HIP(hipEventCreate(&e0));
+  NB_HIP(hipEventCreate(&e1));
+  // grids of 16, 32 and 64 waves per CU (8 per SIMD is the hardware's
+  // maximum residency; 64 queues a second round of blocks)
+  static const int wpcs[] = {16, 32, 64};
+  float best = 3.4e38f;
+  int bv = s->p2p_variant, bb = s->p2p_bpc;
+  hipError_t err = hipSuccess;
+  for (int var = 0; var < p2p_num_variants() && err == hipSuccess; ++var) {
+    for (int wpc : wpcs) {
+      const int bpc = std::max(1, wpc / variant_waves(var));
+      const int grid = p2p_grid_for(s, var, bpc);
+      err = launch_p2p(v, kP2PStep, tio, grid, var, st);  // warm-up
+      if (err == hipSuccess) err = hipEventRecord(e0, st);
+      for (int r = 0; r < repeats && err == hipSuccess; ++r) err = launch_p2p(v, kP2PStep, tio, grid, var, st);
+      if (err == hipSuccess) err = hipEventRecord(e1, st);
+      if (err == hipSuccess) err = hipEventSynchronize(e1);
+      float ms = 0.0f;
+      if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+      if (err != hipSuccess) break;
+      ms /= repeats;
+      if (ms < best) {
+        best = ms;
+        bv = var;
+        bb = bpc;
+      }
+    }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (err != hipSuccess) return hip_fail(err, "autotune");
+  s->p2p_variant = bv;
+  s->p2p_bpc = bb;
+  s->p2p_grid = p2p_grid_for(s, bv, bb);
+  drop_graph(s);
+  if (best_variant) *best_variant = bv;
+  if (best_bpc) *best_bpc = bb;
+  if (best_ms) *best_ms = best;
+  return NBFMM_OK;
+}
+
 int nbfmm_profile_enable(nbfmm_solver* s, int enable) {
   if (!s) return fail(NBFMM_ERR_INVALID_ARGUMENT, "null solver");
   s->profile = enable != 0;
@@ -605,6 +738,10 @@ struct nbfmm_batch {
   std::vector<SysView> hv;        // host copies of the views
   SysView* dv = nullptr;          // device copies
   BatchLaunch bl{};
+  int cus = 256;
+  int p2p_variant = kDefaultP2PVariant;
+  int p2p_bpc = kDefaultBatchP2PBlocksPerCU;
+  std::map<std::pair<int, int>, KernelMap> p2p_maps;  // (waves per block, blocks per CU) -> map
   std::vector<void*> allocs;
   hipGraphExec_t graph_exec = nullptr;
   hipGraph_t graph = nullptr;
@@ -657,10 +794,40 @@ int make_map(nbfmm_batch* b, const std::vector<int>& blocks, KernelMap* km) {
   return NBFMM_OK;
 }
 
+// P2P blocks of `waves` waves, `total_blocks` in all, shared among the
+// systems in proportion to n and never more than a system's leaves can use.
+std::vector<int> p2p_share(const nbfmm_batch* b, int waves, int total_blocks) {
+  std::vector<int> blocks(b->S);
+  for (int s = 0; s < b->S; ++s) {
+    const SysView& v = b->hv[s];
+    const int cap = cdiv(std::min(v.n, v.B * v.B), waves);
+    const int share = (int)((double)total_blocks * v.n / (double)b->total_n + 0.5);
+    blocks[s] = std::max(1, std::min(cap, share));
+  }
+  return blocks;
+}
+
+// Points bl.p2p at the launch map of (variant, blocks per CU), built on
+// first use and kept until the batch is destroyed.
+int batch_use_p2p(nbfmm_batch* b, int variant, int bpc) {
+  const std::pair<int, int> key(variant_waves(variant), bpc);
+  auto it = b->p2p_maps.find(key);
+  if (it == b->p2p_maps.end()) {
+    KernelMap km{};
+    const int rc = make_map(b, p2p_share(b, key.first, b->cus * bpc), &km);
+    if (rc) return rc;
+    it = b->p2p_maps.emplace(key, km).first;
+  }
+  b->bl.p2p = it->second;
+  b->p2p_variant = variant;
+  b->p2p_bpc = bpc;
+  return NBFMM_OK;
+}
+
 int batch_pipeline(nbfmm_batch* b, int mode, const CallIO& io, hipStream_t st) {
   NB_HIP(launch_predo_batch(b->bl, io, st));
   NB_HIP(launch_tree_batch(b->bl, st));
-  NB_HIP(launch_p2p_batch(b->bl, mode, io, st));
+  NB_HIP(launch_p2p_batch(b->bl, mode, io, b->p2p_variant, st));
   return NBFMM_OK;
 }
 
@@ -696,13 +863,14 @@ int nbfmm_batch_create(int S, const int* num_particle, const int* num_level, con
     cus = prop.multiProcessorCount;
 
   nbfmm_batch* b = new nbfmm_batch();
+  b->cus = cus;
   b->S = S;
   b->total_n = (int)total;
   b->part_off.assign(S + 1, 0);
   b->hv.resize(S);
   // sizes of the concatenated scratch arrays
-  size_t n_all = 0, head_all = 0, row_all = 0, cell_all = 0, hist_all = 0, tree_all = 0;
-  std::vector<size_t> head_off(S), row_off(S), cell_off(S), hist_off(S), tree_off(S);
+  size_t n_all = 0, head_all = 0, row_all = 0, cell_all = 0, hist_all = 0, tree_all = 0, ord_all = 0;
+  std::vector<size_t> head_off(S), row_off(S), cell_off(S), hist_off(S), tree_off(S), ord_off(S);
   for (int s = 0; s < S; ++s) {
     SysView& v = b->hv[s];
     v = SysView{};
@@ -730,7 +898,9 @@ int nbfmm_batch_create(int S, const int* num_particle, const int* num_level, con
     cell_off[s] = cell_all;
     hist_off[s] = hist_all;
     tree_off[s] = tree_all;
+    ord_off[s] = ord_all;
     n_all += v.n;
+    ord_all += p2p_class_total((int)C, v.n);
     head_all += C + 1;
     row_all += v.B + 1;
     cell_all += C;
@@ -742,7 +912,7 @@ int nbfmm_batch_create(int S, const int* num_particle, const int* num_level, con
   float4 *tpart, *part, *mpole;
   float2 *eff, *local, *gsort, *gup;
   float4* gmp;
-  int *head, *rowstart, *occ, *nocc;
+  int *head, *rowstart, *occ, *nocc, *ord, *sched;
   int rc = NBFMM_OK;
 #define BALLOC(ptr, cnt) \
   if ((rc = batch_alloc(b, &(ptr), (cnt))) != NBFMM_OK) { batch_free(b); delete b; return rc; }
@@ -755,6 +925,8 @@ int nbfmm_batch_create(int S, const int* num_particle, const int* num_level, con
   BALLOC(rowstart, row_all);
   BALLOC(occ, cell_all);
   BALLOC(nocc, (size_t)S);
+  BALLOC(ord, ord_all);
+  BALLOC(sched, (size_t)S * kSchedInts);
   BALLOC(hist, hist_all);
   BALLOC(dtotal, row_all);
   BALLOC(mpole, tree_all);
@@ -777,6 +949,9 @@ int nbfmm_batch_create(int S, const int* num_particle, const int* num_level, con
     v.rowstart = rowstart + row_off[s];
     v.occ = occ + cell_off[s];
     v.nocc = nocc + s;
+    v.ncap = v.n;
+    v.ord = ord + ord_off[s];
+    v.sched = sched + (size_t)s * kSchedInts;
     v.hist = hist + hist_off[s];
     v.dtotal = dtotal + row_off[s];
     v.mpole = mpole + tree_off[s];
@@ -789,8 +964,7 @@ int nbfmm_batch_create(int S, const int* num_particle, const int* num_level, con
   hipError_t e = hipMemcpy(b->dv, b->hv.data(), S * sizeof(SysView), hipMemcpyHostToDevice);
   if (e != hipSuccess) { batch_free(b); delete b; return hip_fail(e, "hipMemcpy views"); }
   // launch maps
-  std::vector<int> tiles(S), rows(S), p2m(S), mtile(S), mtop(S), m2l(S), p2p(S), asort(S), azero(S);
-  const int p2p_total = cus * 16;
+  std::vector<int> tiles(S), rows(S), p2m(S), mtile(S), mtop(S), m2l(S), asort(S), azero(S);
   for (int s = 0; s < S; ++s) {
     const SysView& v = b->hv[s];
     const int C = v.B * v.B;
@@ -800,17 +974,15 @@ int nbfmm_batch_create(int S, const int* num_particle, const int* num_level, con
     mtile[s] = upward_tile_blocks(v.L, v.B);
     mtop[s] = upward_top_blocks(v.L, v.B);
     m2l[s] = m2l_blocks(v);
-    const int cap = cdiv(std::min(v.n, C), kP2PThreads / kWave);
-    const int share = (int)((double)p2p_total * v.n / (double)total + 0.5);
-    p2p[s] = std::max(1, std::min(cap, share));
     asort[s] = cdiv(v.n, 256);
     azero[s] = adj_zero_blocks(v);
   }
   if ((rc = make_map(b, tiles, &b->bl.tiles)) || (rc = make_map(b, rows, &b->bl.rows)) ||
       (rc = make_map(b, p2m, &b->bl.p2m)) || (rc = make_map(b, mtile, &b->bl.m2m_tile)) ||
       (rc = make_map(b, mtop, &b->bl.m2m_top)) || (rc = make_map(b, m2l, &b->bl.m2l)) ||
-      (rc = make_map(b, p2p, &b->bl.p2p)) || (rc = make_map(b, asort, &b->bl.adj_sort)) ||
-      (rc = make_map(b, azero, &b->bl.adj_zero))) {
+      (rc = make_map(b, asort, &b->bl.adj_sort)) || (rc = make_map(b, azero, &b->bl.adj_zero)) ||
+      (rc = make_map(b, p2p_share(b, kAdjP2PWaves, cus * kAdjP2PBlocksPerCU), &b->bl.adj_p2p)) ||
+      (rc = batch_use_p2p(b, b->p2p_variant, b->p2p_bpc))) {
     batch_free(b);
     delete b;
     return rc;
@@ -918,6 +1090,26 @@ int nbfmm_batch_step_adjoint(nbfmm_batch* b, const float* x, const float* w, flo
   NB_HIP(launch_predo_batch(b->bl, io, st));
   NB_HIP(launch_tree_batch(b->bl, st));
   NB_HIP(launch_adjoint_batch(b->bl, io, 1, st));
+  return NBFMM_OK;
+}
+
+int nbfmm_batch_set_p2p_config(nbfmm_batch* b, int variant, int blocks_per_cu) {
+  if (!b) return fail(NBFMM_ERR_INVALID_ARGUMENT, "null batch");
+  if (variant < 0 || variant >= p2p_num_variants()) return fail(NBFMM_ERR_INVALID_ARGUMENT, "unknown P2P variant");
+  if (blocks_per_cu < 1 || blocks_per_cu > 64) return fail(NBFMM_ERR_INVALID_ARGUMENT, "blocks_per_cu in [1, 64]");
+  const int rc = batch_use_p2p(b, variant, blocks_per_cu);
+  if (rc) return rc;
+  // a captured step graph holds the old kernel
+  if (b->graph_exec) { hipGraphExecDestroy(b->graph_exec); b->graph_exec = nullptr; }
+  if (b->graph) { hipGraphDestroy(b->graph); b->graph = nullptr; }
+  b->gkey.valid = false;
+  return NBFMM_OK;
+}
+
+int nbfmm_batch_get_p2p_config(const nbfmm_batch* b, int* variant, int* blocks_per_cu) {
+  if (!b) return fail(NBFMM_ERR_INVALID_ARGUMENT, "null batch");
+  if (variant) *variant = b->p2p_variant;
+  if (blocks_per_cu) *blocks_per_cu = b->p2p_bpc;
   return NBFMM_OK;
 }
 

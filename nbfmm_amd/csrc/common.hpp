@@ -14,9 +14,38 @@ constexpr int kSortTile = kSortThreads * kSortWaveItems;  // 2048 elements
 constexpr int kScanThreads = 256;
 constexpr int kScanItems = 8;
 constexpr int kScanChunk = kScanThreads * kScanItems;     // 2048 cells
-constexpr int kP2PThreads = 256;      // 4 waves, one leaf cell each
-constexpr int kP2PStage = 256;        // sources staged in LDS per wave and pass
-constexpr int kP2PUnroll = 4;         // sources per lane per inner iteration
+constexpr int kP2PStage = 256;        // adjoint P2P: sources staged in LDS per wave and pass
+constexpr int kAdjP2PWaves = 8;       // adjoint P2P: waves per block (48 KiB of LDS)
+constexpr int kAdjP2PBlocksPerCU = 3; // adjoint P2P: resident blocks per CU (LDS bound)
+
+// P2P work distribution.  Occupied leaves are listed by size class (class
+// k holds the leaves with 2^k <= count < 2^(k+1); the last class is open
+// above) and numbered largest class first.  Block b of nblk takes the
+// numbers b, b + nblk, b + 2 nblk, ..., so every block gets the same mix of
+// leaf sizes, and its waves draw from that sequence through an LDS counter:
+// a wave that finishes early takes the next leaf, and the few dense leaves,
+// which come first, never land last on a wave that is already loaded.
+// sched[] of a system holds the class sizes.
+constexpr int kP2PClasses = 12;
+constexpr int kSchedInts = kP2PClasses;
+
+// Class k can hold at most min(C, ncap >> k) leaves (each has >= 2^k
+// particles), so class k's list is that long and the lists together hold
+// at most C + 2 * ncap entries.
+__host__ __device__ inline int p2p_class_cap(int C, int ncap, int k) {
+  const int c = ncap >> k;
+  return c < C ? c : C;
+}
+__host__ __device__ inline int p2p_class_off(int C, int ncap, int k) {
+  int off = 0;
+  for (int i = 0; i < k; ++i) off += p2p_class_cap(C, ncap, i);
+  return off;
+}
+inline size_t p2p_class_total(int C, int ncap) { return (size_t)p2p_class_off(C, ncap, kP2PClasses); }
+__device__ __forceinline__ int p2p_class_of(uint32_t count) {
+  const int k = 31 - __clz((int)count);
+  return k < kP2PClasses - 1 ? k : kP2PClasses - 1;
+}
 
 // P2P output modes.
 enum P2PMode : int {
@@ -48,6 +77,9 @@ struct SysView {
   int* rowstart;    // B+1: first sorted slot of each row
   int* occ;         // occupied leaves (row by row)
   int* nocc;        // number of occupied leaves
+  int* ord;         // occupied leaves by size class, class k at p2p_class_off(C, ncap, k)
+  int* sched;       // kSchedInts: size of each class list
+  int ncap;         // particle capacity the class lists are sized for
   uint32_t* hist;   // row-pass tile histograms [B][ntiles]
   uint32_t* dtotal; // row totals [B]
   float4* mpole;    // per level, dense D_l x D_l: COM x, COM y, mass, 0
@@ -124,6 +156,51 @@ __device__ __forceinline__ void kernel_accum(float tx, float ty, float sx, float
 }
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
+
+// The occupied leaves of a system in size-class order, largest first
+// (leaf(t) for t < total).
+struct LeafQueue {
+  int cnt[kP2PClasses];
+  int off[kP2PClasses];
+  int total;
+
+  __device__ __forceinline__ void load(const SysView& v) {
+    const int C = v.B * v.B;
+    total = 0;
+    int o = 0;
+#pragma unroll
+    for (int k = 0; k < kP2PClasses; ++k) {
+      cnt[k] = __builtin_amdgcn_readfirstlane(v.sched[k]);
+      off[k] = o;
+      o += p2p_class_cap(C, v.ncap, k);
+      total += cnt[k];
+    }
+  }
+  __device__ __forceinline__ int leaf(const SysView& v, int t) const {
+    int c = 0;
+    bool found = false;
+#pragma unroll
+    for (int k = kP2PClasses - 1; k >= 0; --k) {
+      if (!found) {
+        if (t < cnt[k]) {
+          c = off[k] + t;
+          found = true;
+        } else {
+          t -= cnt[k];
+        }
+      }
+    }
+    return v.ord[c];
+  }
+};
+
+// Next position in the calling block's share of a LeafQueue: lane 0 bumps
+// the block's LDS counter, the whole wave gets the value.
+__device__ __forceinline__ int block_take(int* s_next) {
+  int i = 0;
+  if (lane_id() == 0) i = atomicAdd(s_next, 1);
+  return __builtin_amdgcn_readfirstlane(i);
+}
 
 // Ordering point for LDS data that only the calling wave touches: the
 // hardware executes one wave's LDS instructions in program order, so only

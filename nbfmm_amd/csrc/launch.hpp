@@ -19,7 +19,8 @@ struct BatchLaunch {
   KernelMap m2m_tile;  // M2M tiles
   KernelMap m2m_top;   // M2M top levels
   KernelMap m2l;       // M2L: cdiv(tree cells, 256)
-  KernelMap p2p;       // P2P: share of the P2P grid
+  KernelMap p2p;       // P2P: share of the P2P grid (blocks of the chosen variant)
+  KernelMap adj_p2p;   // adjoint P2P: share of its grid (kAdjP2PWaves-wave blocks)
   KernelMap adj_sort;  // adjoint: upstream gradient to sorted order, cdiv(n_s, 256)
   KernelMap adj_zero;  // adjoint: clear the level-0 subtree sums
 };
@@ -46,8 +47,16 @@ hipError_t launch_cube_read(const SysView& v, int which, void* dst, hipStream_t 
 hipError_t launch_cube_write(const SysView& v, int which, const void* src, hipStream_t s);
 
 // p2p.hip
-hipError_t launch_p2p(const SysView& v, int mode, const CallIO& io, int grid_blocks, hipStream_t s);
-hipError_t launch_p2p_batch(const BatchLaunch& b, int mode, const CallIO& io, hipStream_t s);
+hipError_t launch_p2p(const SysView& v, int mode, const CallIO& io, int grid_blocks, int variant, hipStream_t s);
+hipError_t launch_p2p_batch(const BatchLaunch& b, int mode, const CallIO& io, int variant, hipStream_t s);
+int p2p_num_variants();
+// default P2P configuration: variant {256 staged, unroll 4, 4 waves} on 16
+// blocks per CU (two rounds of blocks at 8 waves per SIMD); a batch spreads
+// 32 blocks per CU over its systems
+constexpr int kDefaultP2PVariant = 5;
+constexpr int kDefaultP2PBlocksPerCU = 16;
+constexpr int kDefaultBatchP2PBlocksPerCU = 32;
+void p2p_variant_info(int variant, int* stage, int* unroll, int* waves);
 hipError_t launch_l2p(const SysView& v, hipStream_t s);
 
 // adjoint.hip; mode 0: solve adjoint, 1: Verlet step adjoint

@@ -4,10 +4,11 @@
 // Reference: source/nbfmm/core/solver/p2p.cu:30-176 (p2pNaive / p2pBlock),
 // l2p.cu:23-48, postdo.cu:22-45, source/nbfmm/display/stars/update.cu:26-42.
 //
-// P2P schedule: one wavefront per occupied leaf cell (grid-stride over the
-// occupied-leaf list).  The sources of a leaf are three contiguous ranges of
-// the sorted particles, one per neighbour row [head[x-1 + yB], head[x+1 + yB + 1])
-// (the row-range idea of p2p.cu:105-118).  The wave stages them, 256 at a
+// P2P schedule: one wavefront per occupied leaf cell; the waves take the
+// leaves from the per-system work queue, largest size class first.  The
+// sources of a leaf are three contiguous ranges of the sorted particles, one
+// per neighbour row [head[x-1 + yB], head[x+1 + yB + 1]) (the row-range idea
+// of p2p.cu:105-118).  The wave stages them, 256 at a
 // time, in its own LDS buffer as float4 {x, y, m, -} and pads the buffer
 // with zero-mass sources.  For a chunk of nt <= 64 targets the wave is split
 // into K = 64/G copies of G = next_pow2(nt) lanes: lane (t, k) accumulates
@@ -41,29 +42,33 @@ __device__ __forceinline__ float2 leaf_local(const SysView& v, int cx, int cy) {
 
 
 // Lane (t, k) of a chunk: target pt against staged sources k, k+K, k+2K, ...
-// (cnt sources, zero-mass padded to a multiple of kP2PUnroll*K).  K is a
-// compile-time constant so the four LDS reads use immediate offsets.
-template <int K>
+// (cnt sources, zero-mass padded to a multiple of U*K).  K is a compile-time
+// constant so the LDS reads use immediate offsets.  Whatever U and the
+// staging size, a lane adds its sources in the same order (k, k+K, ...) and
+// the padding zeros only at the end, so every variant gives the same bits.
+template <int K, int U>
 __device__ __forceinline__ void p2p_lane_sum(const float4* __restrict__ src, int k, int cnt, float4 pt, float eps,
                                              float& ax, float& ay) {
-  constexpr int step = kP2PUnroll * K;
+  constexpr int step = U * K;
   const int iters = __builtin_amdgcn_readfirstlane((cnt + step - 1) / step);
   const float4* p = src + k;
   for (int i = 0; i < iters; ++i, p += step) {
-    const float4 s0 = p[0];
-    const float4 s1 = p[K];
-    const float4 s2 = p[2 * K];
-    const float4 s3 = p[3 * K];
-    kernel_accum(pt.x, pt.y, s0.x, s0.y, s0.z, eps, ax, ay);
-    kernel_accum(pt.x, pt.y, s1.x, s1.y, s1.z, eps, ax, ay);
-    kernel_accum(pt.x, pt.y, s2.x, s2.y, s2.z, eps, ax, ay);
-    kernel_accum(pt.x, pt.y, s3.x, s3.y, s3.z, eps, ax, ay);
+    float4 s[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) s[u] = p[u * K];
+#pragma unroll
+    for (int u = 0; u < U; ++u) kernel_accum(pt.x, pt.y, s[u].x, s[u].y, s[u].z, eps, ax, ay);
   }
 }
 
-template <int MODE, class Sys>
-__global__ __launch_bounds__(kP2PThreads) void k_p2p(Sys sys, CallIO io) {
-  __shared__ float4 s_src[kP2PThreads / kWave][kP2PStage];
+// STAGE sources staged per pass (a multiple of 64*U, so every lane's
+// padding falls in the last pass only), U sources per lane per iteration,
+// WAVES waves per block.
+template <int MODE, class Sys, int STAGE, int U, int WAVES>
+__global__ __launch_bounds__(WAVES * kWave) void k_p2p(Sys sys, CallIO io) {
+  static_assert(STAGE % (kWave * U) == 0, "staging size must be a multiple of 64*U");
+  __shared__ float4 s_src[WAVES][STAGE];
+  __shared__ int s_next;
   int blk, nblk;
   const SysView v = sys.get(blk, nblk);
   const int lane = lane_id();
@@ -73,12 +78,15 @@ __global__ __launch_bounds__(kP2PThreads) void k_p2p(Sys sys, CallIO io) {
   float2* acc0 = io.acc0 + v.part_off;
   float2* xcur = io.xcur + v.part_off;
   float2* xpre = io.xpre + v.part_off;
-  const int nocc = *v.nocc;
-  const int stride = nblk * (kP2PThreads / kWave);
   const int B = v.B;
   const float eps = v.eps;
-  for (int j = blk * (kP2PThreads / kWave) + wave; j < nocc; j += stride) {
-    const int c = v.occ[j];
+  LeafQueue queue;
+  queue.load(v);
+  if (threadIdx.x == 0) s_next = 0;
+  __syncthreads();
+  for (int j = blk + block_take(&s_next) * nblk; j < queue.total;) {
+    const int c = queue.leaf(v, j);
+    j = blk + block_take(&s_next) * nblk;
     const int cx = c & (B - 1), cy = c >> v.lgB;
     const int t0 = v.head[c], t1 = v.head[c + 1];
     const int xl = cx > 0 ? cx - 1 : 0;
@@ -101,12 +109,12 @@ __global__ __launch_bounds__(kP2PThreads) void k_p2p(Sys sys, CallIO io) {
       const bool valid = t < nt;
       const float4 pt = v.part[tb + (valid ? t : 0)];
       float ax = 0.0f, ay = 0.0f;
-      for (int base = 0; base < S; base += kP2PStage) {
-        const int cnt = min(kP2PStage, S - base);
-        // stage up to kP2PStage sources; slots past cnt hold zero-mass
-        // sources, which add exactly nothing
+      for (int base = 0; base < S; base += STAGE) {
+        const int cnt = min(STAGE, S - base);
+        // stage up to STAGE sources; slots past cnt hold zero-mass sources,
+        // which add exactly nothing
 #pragma unroll
-        for (int q = 0; q < kP2PStage / kWave; ++q) {
+        for (int q = 0; q < STAGE / kWave; ++q) {
           const int slot = q * kWave + lane;
           const int f = base + slot;
           float4 val = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -118,13 +126,13 @@ __global__ __launch_bounds__(kP2PThreads) void k_p2p(Sys sys, CallIO io) {
         }
         wave_lds_order();
         switch (lgG) {
-          case 0: p2p_lane_sum<64>(src, k, cnt, pt, eps, ax, ay); break;
-          case 1: p2p_lane_sum<32>(src, k, cnt, pt, eps, ax, ay); break;
-          case 2: p2p_lane_sum<16>(src, k, cnt, pt, eps, ax, ay); break;
-          case 3: p2p_lane_sum<8>(src, k, cnt, pt, eps, ax, ay); break;
-          case 4: p2p_lane_sum<4>(src, k, cnt, pt, eps, ax, ay); break;
-          case 5: p2p_lane_sum<2>(src, k, cnt, pt, eps, ax, ay); break;
-          default: p2p_lane_sum<1>(src, k, cnt, pt, eps, ax, ay); break;
+          case 0: p2p_lane_sum<64, U>(src, k, cnt, pt, eps, ax, ay); break;
+          case 1: p2p_lane_sum<32, U>(src, k, cnt, pt, eps, ax, ay); break;
+          case 2: p2p_lane_sum<16, U>(src, k, cnt, pt, eps, ax, ay); break;
+          case 3: p2p_lane_sum<8, U>(src, k, cnt, pt, eps, ax, ay); break;
+          case 4: p2p_lane_sum<4, U>(src, k, cnt, pt, eps, ax, ay); break;
+          case 5: p2p_lane_sum<2, U>(src, k, cnt, pt, eps, ax, ay); break;
+          default: p2p_lane_sum<1, U>(src, k, cnt, pt, eps, ax, ay); break;
         }
         wave_lds_order();
       }
@@ -253,24 +261,54 @@ __global__ __launch_bounds__(256) void k_pair_count(SysView v, unsigned long lon
   if (lane_id() == 0 && pairs) atomicAdd(out, pairs);
 }
 
-hipError_t launch_p2p(const SysView& v, int mode, const CallIO& io, int grid_blocks, hipStream_t s) {
+// P2P kernel variants (staging size, unroll, waves per block); the
+// autotuner picks one.  All give identical results.
+struct P2PVariant {
+  int stage, unroll, waves;
+};
+static const P2PVariant kVariants[] = {{256, 4, 16}, {256, 4, 8}, {256, 2, 16}, {512, 4, 8},
+                                       {512, 8, 8},  {256, 4, 4}, {128, 2, 16}};
+constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
+
+int p2p_num_variants() { return kNumVariants; }
+void p2p_variant_info(int v, int* stage, int* unroll, int* waves) {
+  *stage = kVariants[v].stage;
+  *unroll = kVariants[v].unroll;
+  *waves = kVariants[v].waves;
+}
+
+template <int MODE, class Sys>
+static void launch_p2p_variant(int variant, dim3 g, const Sys& sys, const CallIO& io, hipStream_t s) {
+  const dim3 b(kVariants[variant < 0 || variant >= kNumVariants ? 0 : variant].waves * kWave);
+  switch (variant) {
+    case 1: hipLaunchKernelGGL((k_p2p<MODE, Sys, 256, 4, 8>), g, b, 0, s, sys, io); break;
+    case 2: hipLaunchKernelGGL((k_p2p<MODE, Sys, 256, 2, 16>), g, b, 0, s, sys, io); break;
+    case 3: hipLaunchKernelGGL((k_p2p<MODE, Sys, 512, 4, 8>), g, b, 0, s, sys, io); break;
+    case 4: hipLaunchKernelGGL((k_p2p<MODE, Sys, 512, 8, 8>), g, b, 0, s, sys, io); break;
+    case 5: hipLaunchKernelGGL((k_p2p<MODE, Sys, 256, 4, 4>), g, b, 0, s, sys, io); break;
+    case 6: hipLaunchKernelGGL((k_p2p<MODE, Sys, 128, 2, 16>), g, b, 0, s, sys, io); break;
+    default: hipLaunchKernelGGL((k_p2p<MODE, Sys, 256, 4, 16>), g, b, 0, s, sys, io); break;
+  }
+}
+
+hipError_t launch_p2p(const SysView& v, int mode, const CallIO& io, int grid_blocks, int variant, hipStream_t s) {
   if (v.n <= 0) return hipSuccess;
-  const dim3 g(grid_blocks), b(kP2PThreads);
+  const dim3 g(grid_blocks);
   const OneSys one{v};
   switch (mode) {
-    case kP2POnly: hipLaunchKernelGGL((k_p2p<kP2POnly, OneSys>), g, b, 0, s, one, io); break;
-    case kP2PFinal: hipLaunchKernelGGL((k_p2p<kP2PFinal, OneSys>), g, b, 0, s, one, io); break;
-    default: hipLaunchKernelGGL((k_p2p<kP2PStep, OneSys>), g, b, 0, s, one, io); break;
+    case kP2POnly: launch_p2p_variant<kP2POnly, OneSys>(variant, g, one, io, s); break;
+    case kP2PFinal: launch_p2p_variant<kP2PFinal, OneSys>(variant, g, one, io, s); break;
+    default: launch_p2p_variant<kP2PStep, OneSys>(variant, g, one, io, s); break;
   }
   return hipGetLastError();
 }
 
-hipError_t launch_p2p_batch(const BatchLaunch& bl, int mode, const CallIO& io, hipStream_t s) {
-  const dim3 g(bl.p2p.blocks), b(kP2PThreads);
+hipError_t launch_p2p_batch(const BatchLaunch& bl, int mode, const CallIO& io, int variant, hipStream_t s) {
+  const dim3 g(bl.p2p.blocks);
   if (mode == kP2PFinal)
-    hipLaunchKernelGGL((k_p2p<kP2PFinal, ManySys>), g, b, 0, s, bl.p2p.sys, io);
+    launch_p2p_variant<kP2PFinal, ManySys>(variant, g, bl.p2p.sys, io, s);
   else
-    hipLaunchKernelGGL((k_p2p<kP2PStep, ManySys>), g, b, 0, s, bl.p2p.sys, io);
+    launch_p2p_variant<kP2PStep, ManySys>(variant, g, bl.p2p.sys, io, s);
   return hipGetLastError();
 }
 

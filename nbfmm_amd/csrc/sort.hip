@@ -108,6 +108,7 @@ __global__ __launch_bounds__(kScanThreads) void k_row_scan(Sys sys) {
     carry = 0;
     if (d == 0) *v.nocc = 0;
   }
+  if (d == 0 && threadIdx.x < kP2PClasses) v.sched[threadIdx.x] = 0;
   __syncthreads();
   for (int t0 = 0; t0 < ntiles; t0 += kScanThreads) {
     const int t = t0 + threadIdx.x;
@@ -224,6 +225,8 @@ __global__ __launch_bounds__(kRowThreads) void k_row_sort(Sys sys, int zero_empt
   __shared__ uint32_t s_wcnt[kRowWaves][512];
   __shared__ uint32_t sh[kRowWaves];
   __shared__ uint32_t s_occ_base;
+  __shared__ uint32_t s_ccnt[kP2PClasses];
+  __shared__ uint32_t s_cbase[kP2PClasses];
   int y, nrows;
   const SysView v = sys.get(y, nrows);
   const int B = v.B;
@@ -233,6 +236,7 @@ __global__ __launch_bounds__(kRowThreads) void k_row_sort(Sys sys, int zero_empt
   const uint32_t cmask = (uint32_t)(B - 1);
 
   for (int x = threadIdx.x; x < B; x += kRowThreads) { s_hist[x] = 0; s_run[x] = 0; }
+  if (threadIdx.x < kP2PClasses) s_ccnt[threadIdx.x] = 0;
   __syncthreads();
   for (int p0 = rs + (int)threadIdx.x; p0 < re; p0 += kRowThreads * kSortWaveItems) {
     uint32_t k[kSortWaveItems];
@@ -250,12 +254,19 @@ __global__ __launch_bounds__(kRowThreads) void k_row_sort(Sys sys, int zero_empt
     // cell starts inside the row, head, occupied cells (B <= 512: one each)
     const int x = threadIdx.x;
     const uint32_t c = x < B ? s_hist[x] : 0u;
+    const uint32_t occ = (x < B && c > 0) ? 1u : 0u;
+    // slot in the leaf's size class (the order inside a class is free)
+    const int cls = occ ? p2p_class_of(c) : 0;
+    const uint32_t cslot = occ ? atomicAdd(&s_ccnt[cls], 1u) : 0u;
     uint32_t tot;
     const uint32_t e = block_exclusive_scan<kRowThreads>(c, sh, &tot);
-    const uint32_t occ = (x < B && c > 0) ? 1u : 0u;
     uint32_t nocc;
     const uint32_t eo = block_exclusive_scan<kRowThreads>(occ, sh, &nocc);
     if (threadIdx.x == 0) s_occ_base = nocc ? atomicAdd((uint32_t*)v.nocc, nocc) : 0u;
+    if (threadIdx.x < kP2PClasses) {
+      const uint32_t m = s_ccnt[threadIdx.x];
+      s_cbase[threadIdx.x] = m ? atomicAdd((uint32_t*)&v.sched[threadIdx.x], m) : 0u;
+    }
     if (x < B) {
       s_start[x] = e;
       v.head[y * B + x] = rs + (int)e;
@@ -263,7 +274,10 @@ __global__ __launch_bounds__(kRowThreads) void k_row_sort(Sys sys, int zero_empt
     }
     if (y == B - 1 && threadIdx.x == 0) v.head[B * B] = re;
     __syncthreads();
-    if (occ) v.occ[s_occ_base + eo] = y * B + x;
+    if (occ) {
+      v.occ[s_occ_base + eo] = y * B + x;
+      v.ord[p2p_class_off(B * B, v.ncap, cls) + s_cbase[cls] + cslot] = y * B + x;
+    }
   }
   const uint64_t below = lanes_below();
   for (int g = rs; g < re; g += kRowGroup) {
@@ -325,26 +339,40 @@ __global__ __launch_bounds__(kRowThreads) void k_row_sort(Sys sys, int zero_empt
   }
 }
 
-// Occupied-leaf list rebuilt from head alone (after head was written through
-// the buffer API).  Single block; only used by the stage API.
+// Occupied-leaf lists rebuilt from head alone (after head was written through
+// the buffer API).  Single block; only used by the stage API.  A head that
+// is not a valid partition of ncap particles can overfill a size class: the
+// surplus leaves are dropped rather than written past the class list.
 __global__ __launch_bounds__(kScanThreads) void k_occ_from_head(SysView v) {
   __shared__ uint32_t sh[kScanThreads / kWave];
   __shared__ uint32_t carry;
+  __shared__ uint32_t s_ccnt[kP2PClasses];
   const int C = v.B * v.B;
   if (threadIdx.x == 0) carry = 0;
+  if (threadIdx.x < kP2PClasses) s_ccnt[threadIdx.x] = 0;
   __syncthreads();
   for (int b0 = 0; b0 < C; b0 += kScanThreads) {
     const int c = b0 + threadIdx.x;
-    const uint32_t o = (c < C && v.head[c + 1] > v.head[c]) ? 1u : 0u;
+    const int cnt = c < C ? v.head[c + 1] - v.head[c] : 0;
+    const uint32_t o = cnt > 0 ? 1u : 0u;
     uint32_t tot;
     const uint32_t e = block_exclusive_scan<kScanThreads>(o, sh, &tot);
     const uint32_t base = carry;
-    if (o) v.occ[base + e] = c;
+    if (o) {
+      v.occ[base + e] = c;
+      const int cls = p2p_class_of((uint32_t)cnt);
+      const uint32_t slot = atomicAdd(&s_ccnt[cls], 1u);
+      if ((int)slot < p2p_class_cap(C, v.ncap, cls)) v.ord[p2p_class_off(C, v.ncap, cls) + slot] = c;
+    }
     __syncthreads();
     if (threadIdx.x == 0) carry = base + tot;
     __syncthreads();
   }
   if (threadIdx.x == 0) *v.nocc = (int)carry;
+  if (threadIdx.x < kP2PClasses) {
+    const int cap = p2p_class_cap(C, v.ncap, threadIdx.x);
+    v.sched[threadIdx.x] = min((int)s_ccnt[threadIdx.x], cap);
+  }
 }
 
 // An empty system: head is all zero and nothing is occupied.
@@ -352,6 +380,7 @@ __global__ __launch_bounds__(256) void k_empty(SysView v) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c <= v.B * v.B) v.head[c] = 0;
   if (c == 0) *v.nocc = 0;
+  if (c < kP2PClasses) v.sched[c] = 0;
 }
 
 // ---------------------------------------------------------------- launchers

@@ -39,7 +39,7 @@ def test_binding_table_matches_header():
 
 
 def test_version(lib):
-    assert lib.nbfmm_version() == 10000
+    assert lib.nbfmm_version() == 10100
 
 
 def _create(lib, L, n, lim, eps):

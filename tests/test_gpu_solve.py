@@ -255,3 +255,53 @@ def test_solve_equals_stage_pipeline_bitwise(L, n):
     assert torch.equal(a1, a2)
     assert torch.equal(cp1, cp2) and torch.equal(cw1, cw2)
     s.close()
+
+
+def _leaf_layout(L, counts, seed):
+    """Particles placed inside chosen leaves: counts[(ix, iy)] particles in
+    leaf (ix, iy) of the box [-1, 1]^2, uniform inside the leaf."""
+    rng = np.random.default_rng(seed)
+    B = 1 << (L + 1)
+    w = 2.0 / B
+    pts = []
+    for (ix, iy), c in counts.items():
+        lo = np.array([-1.0 + ix * w, -1.0 + iy * w])
+        pts.append(lo + w * (0.1 + 0.8 * rng.random((c, 2))))
+    pos = np.concatenate(pts).astype(np.float32)
+    perm = rng.permutation(pos.shape[0])
+    return pos[perm], (0.5 + rng.random(pos.shape[0])).astype(np.float32)
+
+
+@pytest.mark.parametrize("case", ["one_leaf", "exact_powers", "mixed_classes"])
+def test_leaf_size_classes(case):
+    """The P2P schedule lists occupied leaves by size class (2^k <= count <
+    2^(k+1), the last class open above), each class sized for its largest
+    possible population n >> k.  Layouts that fill a class to that bound,
+    or put everything into the open top class, match the oracle."""
+    from nbfmm_amd import Solver
+    L = 4
+    B = 1 << (L + 1)
+    if case == "one_leaf":
+        counts = {(7, 9): 5000}                       # top class holds one leaf of 5000
+    elif case == "exact_powers":
+        k = 3                                         # every leaf exactly 2^3: class 3 holds n >> 3 leaves
+        counts = {(ix, iy): 1 << k for ix in range(B) for iy in range(0, B, 2)}
+    else:
+        rng = np.random.default_rng(5)
+        counts = {}
+        for j in range(300):
+            ix, iy = int(rng.integers(B)), int(rng.integers(B))
+            counts[(ix, iy)] = int(2 ** rng.integers(0, 12) + rng.integers(0, 3))
+    pos, m = _leaf_layout(L, counts, seed=17)
+    n = pos.shape[0]
+    lim = np.array([-1, -1, 1, 1], np.float32)
+    eps = 1e-3
+    s = Solver(L, n, lim, eps)
+    acc = torch.zeros((n, 2), device="cuda")
+    s.solve(n, dev(pos), dev(m), acc)
+    torch.cuda.synchronize()
+    ref, _ = orc.solve(L, pos, m, lim, eps)
+    got = acc.cpu().numpy()
+    assert rel_l2(got, ref) <= RTOL_L2, (case, rel_l2(got, ref))
+    assert max_rel(got, ref) <= RTOL_MAX, (case, max_rel(got, ref))
+    s.close()
