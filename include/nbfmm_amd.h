@@ -146,6 +146,18 @@ int nbfmm_profile_enable(nbfmm_solver* solver, int enable);
 int nbfmm_p2p_time(nbfmm_solver* solver, double* total_ms, int* launches, int reset);
 int nbfmm_pair_count(nbfmm_solver* solver, int num_particle, double* pairs, int* occupied_cells);
 
+/* ---------------------------------------------------------------- warm start
+ * No reference counterpart (the reference sorts from scratch every call,
+ * source/nbfmm/core/solver/predo.cu:122-127).  solve, step, run and the
+ * adjoints start their sort from the solver's previous sort when that was of
+ * the same num_particle: particles whose leaf did not change keep their order,
+ * and only the others are placed.  The result is the same stable sort, bit
+ * for bit; when too many particles changed leaf the cold sort runs instead
+ * (decided on the device).  Writing a buffer through nbfmm_buffer_write
+ * makes the next sort cold.  Enabled by default; disabling it forces the
+ * cold sort every call. */
+int nbfmm_set_warm_start(nbfmm_solver* solver, int enable);
+
 /* ---------------------------------------------------------------- tuning
  * The P2P kernel comes in variants (LDS staging size, unroll, waves per
  * block) and runs on a grid of blocks_per_cu blocks per compute unit.  Every configuration gives

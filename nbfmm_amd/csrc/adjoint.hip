@@ -154,6 +154,9 @@ __global__ __launch_bounds__(1024) void k_adj_up_top(Sys sys) {
 
 // M2L^T, all levels in one launch: gradient wrt each cell's COM and mass
 // from the M2L interactions it takes part in (as target and as source).
+// Consecutive threads take consecutive cells of a row; the six cells of a
+// window row are loaded together (clamped addresses, used only where the
+// window cell is in the box and not adjacent).
 template <class Sys>
 __global__ __launch_bounds__(256) void k_adj_m2l(Sys sys) {
   int blk, nblk;
@@ -170,16 +173,26 @@ __global__ __launch_bounds__(256) void k_adj_m2l(Sys sys) {
   const float4 tc = mp[c];
   const float2 tg = gu[c];
   const int px = tx & ~1, py = ty & ~1;
+  const float eps = v.eps;
   float gx = 0.0f, gy = 0.0f, gm = 0.0f;
-  for (int y = py - 2; y < py + 4; ++y) {
-    if (y < 0 || y >= D) continue;
-    for (int x = px - 2; x < px + 4; ++x) {
-      if (x < 0 || x >= D) continue;
-      if (abs(x - tx) > 1 || abs(y - ty) > 1) {
-        const float4 sc = mp[y * D + x];
-        const float2 sg = gu[y * D + x];
-        pair_adjoint(tc.x, tc.y, tc.z, tg.x, tg.y, sc.x, sc.y, sc.z, sg.x, sg.y, v.eps, gx, gy, gm);
-      }
+#pragma clang loop unroll(full)
+  for (int wy = -2; wy < 4; ++wy) {
+    const int y = py + wy;
+    const bool row_in = y >= 0 && y < D;
+    const int yc = min(max(y, 0), D - 1);
+    float4 sc[6];
+    float2 sg[6];
+#pragma clang loop unroll(full)
+    for (int k = 0; k < 6; ++k) {
+      const int a = yc * D + min(max(px - 2 + k, 0), D - 1);
+      sc[k] = mp[a];
+      sg[k] = gu[a];
+    }
+#pragma clang loop unroll(full)
+    for (int k = 0; k < 6; ++k) {
+      const int x = px - 2 + k;
+      if (row_in && x >= 0 && x < D && (abs(x - tx) > 1 || abs(y - ty) > 1))
+        pair_adjoint(tc.x, tc.y, tc.z, tg.x, tg.y, sc[k].x, sc[k].y, sc[k].z, sg[k].x, sg[k].y, eps, gx, gy, gm);
     }
   }
   v.gmp[t] = make_float4(gx, gy, gm, 0.0f);

@@ -58,6 +58,11 @@ struct nbfmm_solver {
   double p2p_ms = 0.0;
   int p2p_launches = 0;
   unsigned long long* d_pairs = nullptr;
+  // warm-started sort: key/part/head hold a stable sort of warm_n particles
+  // (-1: none) that the next sort of the same n starts from
+  bool warm_enabled = true;
+  int warm_n = -1;
+  float2* xs = nullptr;  // N: new positions in sorted order, handed from one step of a run to the next
   // graph of one step
   hipGraphExec_t graph_exec = nullptr;
   hipGraph_t graph = nullptr;
@@ -66,6 +71,7 @@ struct nbfmm_solver {
     float tick = 0;
     const void *xcur = nullptr, *xpre = nullptr, *w = nullptr, *acc = nullptr;
     hipStream_t stream = nullptr;
+    bool warm = false;  // captured with the warm-started sort
   } gkey;
 };
 
@@ -169,11 +175,24 @@ int p2p_timed(nbfmm_solver* s, const SysView& v, int mode, const CallIO& io, hip
   return NBFMM_OK;
 }
 
+// Sort stage of the fused paths: warm-started from the previous sort when
+// that was of the same n, else the cold sort.  Either way key/part/head then
+// hold the sort of these n particles.
+int sort_stage(nbfmm_solver* s, const SysView& v, const CallIO& io, hipStream_t st) {
+  if (s->warm_enabled && v.n > 0 && s->warm_n == v.n)
+    NB_HIP(launch_predo_warm(v, io, 1, st));
+  else
+    NB_HIP(launch_predo(v, io, 1, st));
+  s->warm_n = v.n > 0 ? v.n : -1;
+  return NBFMM_OK;
+}
+
 // The fused pipeline shared by solve and step.
 int run_pipeline(nbfmm_solver* s, int n, float tick, int mode, const CallIO& io, hipStream_t st) {
   SysView v = view(s, n);
   v.tick = tick;
-  NB_HIP(launch_predo(v, io, 1, st));
+  int rc = sort_stage(s, v, io, st);
+  if (rc) return rc;
   if (n == 0) return NBFMM_OK;
   NB_HIP(launch_upward(v, std::min(n, s->B * s->B), st));  // P2M + M2M
   NB_HIP(launch_m2l(v, st));
@@ -260,6 +279,19 @@ int nbfmm_solver_create(int num_level, int max_num_particle, const float positio
   ALLOC(v.gmp, (size_t)ncells_tree);
   ALLOC(v.flags, 1);
   ALLOC(s->d_pairs, 1);
+  ALLOC(s->xs, N);
+  ALLOC(v.wstate, (size_t)kWarmInts);
+  ALLOC(v.wrel, N);
+  ALLOC(v.wmgrp, N);
+  ALLOC(v.wcin, (size_t)C);
+  ALLOC(v.wcout, (size_t)C);
+  ALLOC(v.wtouch, (size_t)C);
+  ALLOC(v.wtold, (size_t)C);
+  ALLOC(v.wtlist, (size_t)C);
+  ALLOC(v.wcnew, (size_t)C);
+  ALLOC(v.winoff, (size_t)C);
+  ALLOC(v.wfill, (size_t)C);
+  ALLOC(v.wrowtot, 2 * ((size_t)s->B + 1));
 #undef ALLOC
   // P2P: enough waves to fill every CU
   s->cus = cus;
@@ -297,6 +329,7 @@ int nbfmm_predo(nbfmm_solver* s, int n, const float* pos0, const float* w0, void
   if (rc) return rc;
   if (n > 0 && (!pos0 || !w0)) return fail(NBFMM_ERR_INVALID_ARGUMENT, "null array");
   NB_HIP(launch_predo(view(s, n), call_io(pos0, w0), 0, as_stream(stream)));
+  s->warm_n = n > 0 ? n : -1;
   return NBFMM_OK;
 }
 
@@ -403,7 +436,20 @@ int nbfmm_buffer_read(nbfmm_solver* s, int id, int count, void* dst, void* strea
 }
 
 int nbfmm_buffer_write(nbfmm_solver* s, int id, int count, const void* src, void* stream) {
+  if (s) {
+    // the stored sort may no longer be one: the next sort starts cold, and a
+    // step graph captured with the warm start is dropped
+    s->warm_n = -1;
+    drop_graph(s);
+  }
   return buffer_io(s, id, count, const_cast<void*>(src), true, as_stream(stream));
+}
+
+int nbfmm_set_warm_start(nbfmm_solver* s, int enable) {
+  if (!s) return fail(NBFMM_ERR_INVALID_ARGUMENT, "null solver");
+  s->warm_enabled = enable != 0;
+  drop_graph(s);
+  return NBFMM_OK;
 }
 
 int nbfmm_solver_flags(nbfmm_solver* s, unsigned* flags, int clear) {
@@ -435,7 +481,7 @@ int nbfmm_run(nbfmm_solver* s, int n, float tick, int steps, float* xcur, float*
   if (rc) return rc;
   if (steps < 0) return fail(NBFMM_ERR_INVALID_ARGUMENT, "num_steps < 0");
   hipStream_t st = as_stream(stream);
-  if (!use_graph || s->profile) {
+  if (!use_graph || s->profile || n == 0) {
     for (int k = 0; k < steps; ++k) {
       rc = nbfmm_step(s, n, tick, xcur, xpre, w, acc, stream);
       if (rc) return rc;
@@ -443,21 +489,33 @@ int nbfmm_run(nbfmm_solver* s, int n, float tick, int steps, float* xcur, float*
     return NBFMM_OK;
   }
   if (!st) return fail(NBFMM_ERR_INVALID_ARGUMENT, "graph replay needs a non-null stream");
+  if (steps == 0) return NBFMM_OK;
+  // The first step is eager: it sorts from the caller's arrays (warm from an
+  // earlier sort of n, else cold) and leaves the new positions in sorted
+  // order in xs.  The other steps replay one captured step; with the warm
+  // start on, its sort reads positions from xs and masses from the previous
+  // sort (w is constant during the call), which the previous step left.
+  CallIO io = call_io(xcur, w, acc, xcur, xpre);
+  io.xs_out = s->xs;
+  if ((rc = run_pipeline(s, n, tick, kP2PStep, io, st))) return rc;
+  if (--steps == 0) return NBFMM_OK;
+  const bool warm = s->warm_enabled && s->warm_n == n;
   auto& g = s->gkey;
   const bool same = s->graph_exec && g.n == n && g.tick == tick && g.xcur == xcur && g.xpre == xpre &&
-                    g.w == w && g.acc == acc && g.stream == st;
+                    g.w == w && g.acc == acc && g.stream == st && g.warm == warm;
   if (!same) {
     if (s->graph_exec) { hipGraphExecDestroy(s->graph_exec); s->graph_exec = nullptr; }
     if (s->graph) { hipGraphDestroy(s->graph); s->graph = nullptr; }
+    if (warm) io.xs_in = s->xs;
     NB_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    rc = nbfmm_step(s, n, tick, xcur, xpre, w, acc, stream);
+    rc = run_pipeline(s, n, tick, kP2PStep, io, st);
     hipGraph_t graph = nullptr;
     hipError_t e = hipStreamEndCapture(st, &graph);
     if (rc) { if (graph) hipGraphDestroy(graph); return rc; }
     if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
     s->graph = graph;
     NB_HIP(hipGraphInstantiate(&s->graph_exec, graph, nullptr, nullptr, 0));
-    g.n = n; g.tick = tick; g.xcur = xcur; g.xpre = xpre; g.w = w; g.acc = acc; g.stream = st;
+    g.n = n; g.tick = tick; g.xcur = xcur; g.xpre = xpre; g.w = w; g.acc = acc; g.stream = st; g.warm = warm;
   }
   for (int k = 0; k < steps; ++k) NB_HIP(hipGraphLaunch(s->graph_exec, st));
   return NBFMM_OK;
@@ -467,7 +525,8 @@ int nbfmm_run(nbfmm_solver* s, int n, float tick, int steps, float* xcur, float*
 // of every level) at the given positions.
 static int adjoint_prepare(nbfmm_solver* s, int n, const CallIO& io, hipStream_t st) {
   const SysView v = view(s, n);
-  NB_HIP(launch_predo(v, io, 1, st));
+  const int rc = sort_stage(s, v, io, st);
+  if (rc) return rc;
   if (n > 0) NB_HIP(launch_upward(v, std::min(n, s->B * s->B), st));
   return NBFMM_OK;
 }
@@ -637,7 +696,7 @@ int nbfmm_autotune(nbfmm_solver* s, int n, const float* pos0, const float* w0, i
   hipStream_t st = as_stream(stream);
   const SysView v = view(s, n);
   const CallIO io = call_io(pos0, w0);
-  NB_HIP(launch_predo(v, io, 1, st));
+  if ((rc = sort_stage(s, v, io, st))) return rc;
   NB_HIP(launch_upward(v, std::min(n, s->B * s->B), st));
   NB_HIP(launch_m2l(v, st));
   // Time the step kernel itself (L2L walk, scatter and Verlet epilogue

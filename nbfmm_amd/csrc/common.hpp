@@ -90,24 +90,52 @@ struct SysView {
   float4* gmp;      // per level: gradient wrt the cell's COM (x, y) and mass (z) from M2L
   int lvl_off[kMaxLevel + 1];  // start of level l in mpole/local; lvl_off[L] = total
   uint32_t* flags;  // NBFMM_FLAG_* bits
+  // warm-started sort (the previous sort's key/part/head as the starting
+  // order); every array is left cleared by the pass that last uses it
+  int* wstate;      // kWarmInts: fallback flag, touched-leaf count
+  int* wrel;        // N: rank of a staying particle inside its old leaf
+  int* wmgrp;       // N: old slots of the movers grouped by new leaf
+  int* wcin;        // C: movers into each leaf
+  int* wcout;       // C: movers out of each leaf
+  int* wtouch;      // C: leaf gained or lost a mover
+  int2* wtold;      // C: old (first slot, count) of each touched leaf
+  int* wtlist;      // C: touched leaves (appended one row at a time)
+  int* wcnew;       // C: row-local exclusive offset of the new leaf counts
+  int* winoff;      // C: first slot of each leaf's movers in wmgrp
+  int* wfill;       // C: movers placed into each leaf's group
+  int* wrowtot;     // 2 (B + 1): row totals of the new counts and of the movers
+  const int* cold_if;  // when set, the row/column sort kernels run only if *cold_if != 0
 };
+
+// Warm-started sort.
+constexpr uint32_t kMoverBit = 0x80000000u;  // tkey flag: the particle changed leaf
+constexpr int kWarmMaxIn = 64;               // movers into one leaf the warm pass places
+constexpr int kWarmFallback = 0, kWarmTouched = 1, kWarmInts = 4;
+constexpr int kWarmMergeWaves = 16384;       // merge grid: one wave per touched leaf, grid-stride beyond
 
 // Caller arrays of one call, original particle order; a system's particles
 // start at its part_off.
 struct CallIO {
-  const float2* pos0;  // positions (input)
-  const float* w0;     // masses (input)
-  float2* acc0;        // effects (output of solve/step)
-  float2* xcur;        // Verlet state (step)
-  float2* xpre;
+  const float2* pos0 = nullptr;  // positions (input)
+  const float* w0 = nullptr;     // masses (input)
+  float2* acc0 = nullptr;        // effects (output of solve/step)
+  float2* xcur = nullptr;        // Verlet state (step)
+  float2* xpre = nullptr;
   // adjoint
-  const float2* gin;   // upstream gradient (solve adjoint: d loss / d effect;
-                       // step adjoint: lambda = d loss / d x_{k+1})
-  float2* gpos;        // solve adjoint: d loss / d position (out);
-                       // step adjoint: lambda, overwritten with lambda_k
-  float2* gpre;        // step adjoint: pi (in/out)
-  float* gw;           // d loss / d mass (solve adjoint: out; step adjoint: += )
-  float gscale;        // upstream scale (step adjoint: tick^2)
+  const float2* gin = nullptr;   // upstream gradient (solve adjoint: d loss / d effect;
+                                 // step adjoint: lambda = d loss / d x_{k+1})
+  float2* gpos = nullptr;        // solve adjoint: d loss / d position (out);
+                                 // step adjoint: lambda, overwritten with lambda_k
+  float2* gpre = nullptr;        // step adjoint: pi (in/out)
+  float* gw = nullptr;           // d loss / d mass (solve adjoint: out; step adjoint: += )
+  float gscale = 0.0f;           // upstream scale (step adjoint: tick^2)
+  // inside one multi-step run (the caller's arrays are not touched between
+  // steps): the step kernel also writes each new position at its sorted slot
+  // (xs_out), and the next warm-started sort reads positions from there in
+  // the old sorted order (xs_in) and masses from the previous sort, instead
+  // of gathering both from the caller's arrays
+  float2* xs_out = nullptr;
+  const float2* xs_in = nullptr;
 };
 
 // Which system a block works on, and its block index / block count inside

@@ -30,6 +30,8 @@ struct BatchLaunch {
 hipError_t launch_predo(const SysView& v, const CallIO& io, int zero_empty, hipStream_t s);
 hipError_t launch_predo_batch(const BatchLaunch& b, const CallIO& io, hipStream_t s);
 hipError_t launch_occ_from_head(const SysView& v, hipStream_t s);
+// warm start from the previous sort of the same n (falls back to the cold sort on the device)
+hipError_t launch_predo_warm(const SysView& v, const CallIO& io, int zero_empty, hipStream_t s);
 
 // tree.hip
 hipError_t launch_p2m(const SysView& v, hipStream_t s);

@@ -160,6 +160,7 @@ __global__ __launch_bounds__(WAVES * kWave) void k_p2p(Sys sys, CallIO io) {
             nx.y = 2.0f * x.y - xp.y + ey * v.tick * v.tick;
             xcur[o] = nx;
             xpre[o] = x;
+            if (io.xs_out) io.xs_out[v.part_off + i] = nx;
           }
         }
       }

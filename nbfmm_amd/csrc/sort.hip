@@ -73,6 +73,7 @@ __global__ __launch_bounds__(kSortThreads) void k_row_up(Sys sys, CallIO io) {
   __shared__ uint32_t h[512];
   int tile, ntiles;
   const SysView v = sys.get(tile, ntiles);
+  if (v.cold_if && *v.cold_if == 0) return;
   const float2* pos0 = io.pos0 + v.part_off;
   for (int d = threadIdx.x; d < v.B; d += kSortThreads) h[d] = 0;
   __syncthreads();
@@ -102,6 +103,7 @@ __global__ __launch_bounds__(kScanThreads) void k_row_scan(Sys sys) {
   __shared__ uint32_t carry;
   int d, nrows;
   const SysView v = sys.get(d, nrows);
+  if (v.cold_if && *v.cold_if == 0) return;
   const int ntiles = (v.n + kSortTile - 1) / kSortTile;
   uint32_t* row = v.hist + (size_t)d * ntiles;
   if (threadIdx.x == 0) {
@@ -135,6 +137,7 @@ __global__ __launch_bounds__(kSortThreads) void k_row_down(Sys sys, CallIO io) {
   __shared__ uint32_t sh[kSortThreads / kWave];
   int tile, ntiles;
   const SysView v = sys.get(tile, ntiles);
+  if (v.cold_if && *v.cold_if == 0) return;
   const float2* pos0 = io.pos0 + v.part_off;
   const float* w0 = io.w0 + v.part_off;
   const int R = v.B;
@@ -229,6 +232,7 @@ __global__ __launch_bounds__(kRowThreads) void k_row_sort(Sys sys, int zero_empt
   __shared__ uint32_t s_cbase[kP2PClasses];
   int y, nrows;
   const SysView v = sys.get(y, nrows);
+  if (v.cold_if && *v.cold_if == 0) return;
   const int B = v.B;
   const int lane = lane_id();
   const int wave = threadIdx.x / kWave;
@@ -383,6 +387,261 @@ __global__ __launch_bounds__(256) void k_empty(SysView v) {
   if (c < kP2PClasses) v.sched[c] = 0;
 }
 
+// --------------------------------------------------------- warm start
+//
+// When key/part/head hold the stable sort of an earlier call with the same
+// N, that order is the starting point.  Particles are gathered in the old
+// sorted order; one whose leaf is unchanged (a stayer) keeps its place
+// relative to the other stayers of its leaf, because both orders are by
+// (leaf, original index).  Only the movers, whose leaf changed, need
+// placing: a leaf's new content is its stayers merged with its incoming
+// movers by original index.  The result is the same stable argsort as the
+// cold path, bit for bit.
+//   k_warm_gather  gather, bin, mark movers and the leaves they touch
+//   k_warm_rows    new leaf counts, row-local offsets (all rows, one block each)
+//   k_warm_finish  head, occupied-leaf lists, movers' group offsets, list
+//                  of touched leaves
+//   k_warm_place   stayers of untouched leaves go straight to their slot;
+//                  movers are grouped by new leaf
+//   k_warm_merge   one wave per touched leaf merges its stayers and movers
+//   k_warm_end     clears the counters
+// Per-particle atomics are per leaf; an atomic that the whole grid shares
+// serialises across the chip, so the only shared counters are bumped once
+// per row block.  More than kWarmMaxIn movers into one leaf sets
+// the fallback flag: the warm passes stop after cleaning their scratch and
+// the cold row/column sort, launched after them with cold_if pointing at
+// that flag, sorts from scratch.
+
+// First touch of leaf c records its old slot range.
+__device__ __forceinline__ void warm_touch(const SysView& v, uint32_t c) {
+  if (atomicExch(&v.wtouch[c], 1) == 0) v.wtold[c] = make_int2(v.head[c], v.head[c + 1] - v.head[c]);
+}
+
+template <class Sys>
+__global__ __launch_bounds__(256) void k_warm_gather(Sys sys, CallIO io) {
+  int blk, nblk;
+  const SysView v = sys.get(blk, nblk);
+  const int i = blk * 256 + threadIdx.x;
+  bool bad = false;
+  if (i < v.n) {
+    const float4 old = v.part[i];
+    const uint32_t o = __float_as_uint(old.w);
+    float2 p;
+    float m;
+    if (io.xs_in) {
+      p = io.xs_in[v.part_off + i];
+      m = old.z;
+    } else {
+      p = io.pos0[v.part_off + o];
+      m = io.w0[v.part_off + o];
+    }
+    const uint32_t k = bin_key(v, p, bad);
+    const uint32_t kold = v.key[i];
+    v.tpart[i] = make_float4(p.x, p.y, m, old.w);
+    if (k == kold) {
+      v.tkey[i] = k;
+      v.wrel[i] = i - v.head[kold];
+    } else {
+      v.tkey[i] = k | kMoverBit;
+      atomicAdd(&v.wcout[kold], 1);
+      if (atomicAdd(&v.wcin[k], 1) == kWarmMaxIn) atomicExch(&v.wstate[kWarmFallback], 1);
+      warm_touch(v, kold);
+      warm_touch(v, k);
+    }
+  }
+  if (__any(bad) && lane_id() == 0) atomicOr(v.flags, 1u);
+}
+
+// One block per row: new count of every leaf (old count - out + in), its
+// row-local exclusive offset, and the row-local offset of its movers.
+// Clears wcin/wcout (also on fallback).
+template <class Sys>
+__global__ __launch_bounds__(kRowThreads) void k_warm_rows(Sys sys) {
+  __shared__ uint32_t sh[kRowWaves];
+  int y, nrows;
+  const SysView v = sys.get(y, nrows);
+  const int B = v.B;
+  const int x = threadIdx.x;
+  const int c = y * B + x;
+  uint32_t cnt = 0, cin = 0;
+  if (x < B) {
+    const int ci = v.wcin[c], co = v.wcout[c];
+    if (ci | co) {
+      v.wcin[c] = 0;
+      v.wcout[c] = 0;
+    }
+    cnt = (uint32_t)(v.head[c + 1] - v.head[c] - co + ci);
+    cin = (uint32_t)ci;
+  }
+  if (v.wstate[kWarmFallback]) {  // block-uniform: clear the touch flags too, then stop
+    if (x < B && v.wtouch[c]) v.wtouch[c] = 0;
+    return;
+  }
+  if (y == 0) {  // the occupied-leaf counters k_warm_finish fills
+    if (x == 0) *v.nocc = 0;
+    if (x < kP2PClasses) v.sched[x] = 0;
+  }
+  uint32_t tot, itot;
+  const uint32_t e = block_exclusive_scan<kRowThreads>(cnt, sh, &tot);
+  const uint32_t ei = block_exclusive_scan<kRowThreads>(cin, sh, &itot);
+  if (x < B) {
+    v.wcnew[c] = (int)e;
+    v.winoff[c] = (int)ei;
+  }
+  if (x == 0) {
+    v.wrowtot[y] = (int)tot;
+    v.wrowtot[B + 1 + y] = (int)itot;
+  }
+}
+
+// One block per row: each block scans the B row totals itself, then writes
+// head and the movers' group offsets for its row and appends its occupied
+// leaves to occ and to the size-class lists (as the column pass does).
+// zero_empty: also clear the level-0 multipole of the row's empty leaves.
+template <class Sys>
+__global__ __launch_bounds__(kRowThreads) void k_warm_finish(Sys sys, int zero_empty) {
+  __shared__ uint32_t sh[kRowWaves];
+  __shared__ uint32_t s_base, s_ibase, s_occ_base, s_touch_base;
+  __shared__ uint32_t s_ccnt[kP2PClasses];
+  __shared__ uint32_t s_cbase[kP2PClasses];
+  int y, nrows;
+  const SysView v = sys.get(y, nrows);
+  if (v.wstate[kWarmFallback]) return;
+  const int B = v.B;
+  const int x = threadIdx.x;
+  if (x < kP2PClasses) s_ccnt[x] = 0;
+  // row starts: exclusive scan of the row totals, up to row y
+  {
+    const uint32_t t = x < B ? (uint32_t)v.wrowtot[x] : 0u;
+    const uint32_t ti = x < B ? (uint32_t)v.wrowtot[B + 1 + x] : 0u;
+    uint32_t tot, itot;
+    const uint32_t e = block_exclusive_scan<kRowThreads>(t, sh, &tot);
+    const uint32_t ei = block_exclusive_scan<kRowThreads>(ti, sh, &itot);
+    if (x == y) {
+      s_base = e;
+      s_ibase = ei;
+      v.rowstart[y] = (int)e;
+    }
+    if (y == B - 1 && x == 0) {
+      v.rowstart[B] = (int)tot;
+      v.head[B * B] = (int)tot;
+    }
+  }
+  __syncthreads();
+  const int c = y * B + x;
+  uint32_t cnt = 0;
+  if (x < B) {
+    const int e = v.wcnew[c];
+    const int e1 = x + 1 < B ? v.wcnew[c + 1] : v.wrowtot[y];
+    cnt = (uint32_t)(e1 - e);
+    v.head[c] = (int)s_base + e;
+    v.winoff[c] += (int)s_ibase;
+    if (zero_empty && cnt == 0) v.mpole[c] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  }
+  const uint32_t occ = cnt > 0 ? 1u : 0u;
+  const int cls = occ ? p2p_class_of(cnt) : 0;
+  const uint32_t cslot = occ ? atomicAdd(&s_ccnt[cls], 1u) : 0u;
+  const uint32_t touched = (x < B && v.wtouch[c]) ? 1u : 0u;
+  uint32_t nocc, ntouch;
+  const uint32_t eo = block_exclusive_scan<kRowThreads>(occ, sh, &nocc);
+  const uint32_t et = block_exclusive_scan<kRowThreads>(touched, sh, &ntouch);
+  if (x == 0) {
+    s_occ_base = nocc ? atomicAdd((uint32_t*)v.nocc, nocc) : 0u;
+    s_touch_base = ntouch ? atomicAdd((uint32_t*)&v.wstate[kWarmTouched], ntouch) : 0u;
+  }
+  if (x < kP2PClasses) {
+    const uint32_t m = s_ccnt[x];
+    s_cbase[x] = m ? atomicAdd((uint32_t*)&v.sched[x], m) : 0u;
+  }
+  __syncthreads();
+  if (occ) {
+    v.occ[s_occ_base + eo] = c;
+    v.ord[p2p_class_off(B * B, v.ncap, cls) + s_cbase[cls] + cslot] = c;
+  }
+  if (touched) v.wtlist[s_touch_base + et] = c;
+}
+
+// Stayers of untouched leaves go to head[leaf] + their old rank; movers are
+// put into their new leaf's group (slots of the group in any order).
+template <class Sys>
+__global__ __launch_bounds__(256) void k_warm_place(Sys sys) {
+  int blk, nblk;
+  const SysView v = sys.get(blk, nblk);
+  if (v.wstate[kWarmFallback]) return;
+  const int i = blk * 256 + threadIdx.x;
+  if (i >= v.n) return;
+  const uint32_t t = v.tkey[i];
+  if (t & kMoverBit) {
+    const uint32_t k = t & ~kMoverBit;
+    v.wmgrp[v.winoff[k] + atomicAdd(&v.wfill[k], 1)] = i;
+  } else if (!v.wtouch[t]) {
+    const int dst = v.head[t] + v.wrel[i];
+    v.key[dst] = t;
+    v.part[dst] = v.tpart[i];
+  }
+}
+
+// One wave per touched leaf (grid-stride over the list): the leaf's stayers
+// (old slots [first, first+count) whose tkey is still this leaf) and its
+// incoming movers (at most kWarmMaxIn, one per lane) are merged by original
+// index.  Clears the leaf's touch flag and group count.
+template <class Sys>
+__global__ __launch_bounds__(256) void k_warm_merge(Sys sys) {
+  int blk, nblk;
+  const SysView v = sys.get(blk, nblk);
+  if (v.wstate[kWarmFallback]) return;
+  const int lane = lane_id();
+  const int waves = nblk * (256 / kWave);
+  const int nt = v.wstate[kWarmTouched];
+  const uint64_t below = lanes_below();
+  for (int d = blk * (256 / kWave) + (int)(threadIdx.x / kWave); d < nt; d += waves) {
+    const uint32_t c = (uint32_t)v.wtlist[d];
+    const int2 od = v.wtold[c];
+    const int h = v.head[c];
+    const int ni = v.wfill[c];
+    const int ms = lane < ni ? v.wmgrp[v.winoff[c] + lane] : 0;
+    const uint32_t om = lane < ni ? __float_as_uint(v.tpart[ms].w) : 0xffffffffu;
+    int rank_m = 0, cnt_m = 0;
+    for (int l = 0; l < ni; ++l) rank_m += __shfl(om, l, kWave) < om ? 1 : 0;
+    int before = 0;
+    for (int b = 0; b < od.y; b += kWave) {
+      const int slot = od.x + b + lane;
+      const bool stay = b + lane < od.y && v.tkey[slot] == c;
+      const uint32_t os = stay ? __float_as_uint(v.tpart[slot].w) : 0u;
+      int cnt_s = 0;
+      for (int l = 0; l < ni; ++l) {
+        const uint32_t o = __shfl(om, l, kWave);
+        cnt_s += o < os ? 1 : 0;
+        const int lt = __popcll(__ballot(stay && os < o));
+        if (lane == l) cnt_m += lt;
+      }
+      const uint64_t sm = __ballot(stay);
+      if (stay) {
+        const int dst = h + before + __popcll(sm & below) + cnt_s;
+        v.key[dst] = c;
+        v.part[dst] = v.tpart[slot];
+      }
+      before += __popcll(sm);
+    }
+    if (lane < ni) {
+      const int dst = h + rank_m + cnt_m;
+      v.key[dst] = c;
+      v.part[dst] = v.tpart[ms];
+    }
+    if (lane == 0) {
+      v.wtouch[c] = 0;
+      v.wfill[c] = 0;
+    }
+  }
+}
+
+template <class Sys>
+__global__ void k_warm_end(Sys sys) {
+  int blk, nblk;
+  const SysView v = sys.get(blk, nblk);
+  if (threadIdx.x < kWarmInts) v.wstate[threadIdx.x] = 0;
+}
+
 // ---------------------------------------------------------------- launchers
 
 hipError_t launch_predo(const SysView& v, const CallIO& io, int zero_empty, hipStream_t s) {
@@ -404,6 +663,26 @@ hipError_t launch_predo_batch(const BatchLaunch& b, const CallIO& io, hipStream_
   hipLaunchKernelGGL(k_row_scan<ManySys>, dim3(b.rows.blocks), dim3(kScanThreads), 0, s, b.rows.sys);
   hipLaunchKernelGGL(k_row_down<ManySys>, dim3(b.tiles.blocks), dim3(kSortThreads), 0, s, b.tiles.sys, io);
   hipLaunchKernelGGL(k_row_sort<ManySys>, dim3(b.rows.blocks), dim3(kRowThreads), 0, s, b.rows.sys, 1);
+  return hipGetLastError();
+}
+
+// Warm-started predo (key/part/head hold an earlier sort of the same n),
+// followed by the cold passes, which run only if the warm passes fell back.
+hipError_t launch_predo_warm(const SysView& v, const CallIO& io, int zero_empty, hipStream_t s) {
+  if (v.n <= 0) return launch_predo(v, io, zero_empty, s);
+  const OneSys one{v};
+  const int nb = cdiv(v.n, 256);
+  hipLaunchKernelGGL(k_warm_gather<OneSys>, dim3(nb), dim3(256), 0, s, one, io);
+  hipLaunchKernelGGL(k_warm_rows<OneSys>, dim3(v.B), dim3(kRowThreads), 0, s, one);
+  hipLaunchKernelGGL(k_warm_finish<OneSys>, dim3(v.B), dim3(kRowThreads), 0, s, one, zero_empty);
+  hipLaunchKernelGGL(k_warm_place<OneSys>, dim3(nb), dim3(256), 0, s, one);
+  hipLaunchKernelGGL(k_warm_merge<OneSys>, dim3(cdiv(std::min(kWarmMergeWaves, v.B * v.B), 256 / kWave)),
+                     dim3(256), 0, s, one);
+  SysView cv = v;
+  cv.cold_if = v.wstate + kWarmFallback;
+  const hipError_t e = launch_predo(cv, io, zero_empty, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_warm_end<OneSys>, dim3(1), dim3(64), 0, s, one);
   return hipGetLastError();
 }
 

@@ -178,6 +178,10 @@ __global__ __launch_bounds__(1024) void k_m2m_top(Sys sys) {
 // M2L (m2l.cu:24-60), all levels in one launch: each target cell sums the
 // kernel at its own COM over the children of its parent's 3x3 neighbourhood
 // (window [p-2, p+4) per axis) that are not adjacent to it; y outer, x inner.
+// Consecutive threads take consecutive cells of a row, so a wave shares the
+// window rows.  The six cells of a window row are loaded together (clamped
+// addresses, used only where the window cell is in the box and not
+// adjacent), so a row costs one wait instead of one per interaction.
 template <class Sys>
 __global__ __launch_bounds__(256) void k_m2l(Sys sys) {
   int blk, nblk;
@@ -192,15 +196,21 @@ __global__ __launch_bounds__(256) void k_m2l(Sys sys) {
   const float4* mp = v.mpole + v.lvl_off[l];
   const float4 tc = mp[c];
   const int px = tx & ~1, py = ty & ~1;
+  const float eps = v.eps;
   float ax = 0.0f, ay = 0.0f;
-  for (int y = py - 2; y < py + 4; ++y) {
-    if (y < 0 || y >= D) continue;
-    for (int x = px - 2; x < px + 4; ++x) {
-      if (x < 0 || x >= D) continue;
-      if (abs(x - tx) > 1 || abs(y - ty) > 1) {
-        const float4 sc = mp[y * D + x];
-        kernel_accum_ieee(tc.x, tc.y, sc.x, sc.y, sc.z, v.eps, ax, ay);
-      }
+#pragma clang loop unroll(full)
+  for (int wy = -2; wy < 4; ++wy) {
+    const int y = py + wy;
+    const bool row_in = y >= 0 && y < D;
+    const int yc = min(max(y, 0), D - 1);
+    float4 sc[6];
+#pragma clang loop unroll(full)
+    for (int k = 0; k < 6; ++k) sc[k] = mp[yc * D + min(max(px - 2 + k, 0), D - 1)];
+#pragma clang loop unroll(full)
+    for (int k = 0; k < 6; ++k) {
+      const int x = px - 2 + k;
+      if (row_in && x >= 0 && x < D && (abs(x - tx) > 1 || abs(y - ty) > 1))
+        kernel_accum_ieee(tc.x, tc.y, sc[k].x, sc[k].y, sc[k].z, eps, ax, ay);
     }
   }
   v.local[t] = make_float2(ax, ay);

@@ -198,6 +198,11 @@ class Solver:
                                               _ptr(grad_position0), _ptr(grad_position_pre0), _ptr(grad_weight),
                                               _ptr(work), _stream(stream)))
 
+    def warm_start(self, enable: bool = True) -> None:
+        """Start each sort from the previous one of the same N (default on);
+        results are identical either way."""
+        check(lib().nbfmm_set_warm_start(self._h, int(bool(enable))))
+
     # -- measurement ------------------------------------------------------
     def profile(self, enable: bool = True) -> None:
         check(lib().nbfmm_profile_enable(self._h, int(bool(enable))))

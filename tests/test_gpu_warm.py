@@ -1,0 +1,180 @@
+"""Warm-started sort: solve/step/run start their sort from the solver's
+previous sort of the same N.  Everything they produce (forces, states, the
+sorted buffers) must equal, bit for bit, a solver that sorts from scratch
+every call, whatever the particles did in between: small drifts (the warm
+passes place the few movers), large jumps and crowded leaves (the device
+falls back to the cold sort), buffer writes and changes of N (cold again)."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import pyoracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(a):
+    return torch.as_tensor(np.ascontiguousarray(a, dtype=np.float32), device="cuda")
+
+
+def pair(wl):
+    from nbfmm_amd import Solver
+    warm = Solver(wl.num_level, wl.n, wl.limits, wl.eps)
+    cold = Solver(wl.num_level, wl.n, wl.limits, wl.eps)
+    cold.warm_start(False)
+    return warm, cold
+
+
+def sorted_state(s, n):
+    return [s.read_buffer(b, n).cpu() for b in ("position", "weight", "perm", "index")] + \
+           [s.read_buffer("head").cpu()]
+
+
+def assert_same_state(a, b, n):
+    for x, y in zip(sorted_state(a, n), sorted_state(b, n)):
+        assert torch.equal(x, y)
+
+
+def test_steps_bitwise_equal_cold():
+    from nbfmm_amd.workloads import two_disk_collision
+    wl = two_disk_collision(n=1 << 16, num_level=6, steps=12)
+    warm, cold = pair(wl)
+    Xw, Pw, W = dev(wl.position), dev(wl.position_pre), dev(wl.weight)
+    Xc, Pc = Xw.clone(), Pw.clone()
+    Aw, Ac = torch.zeros_like(Xw), torch.zeros_like(Xw)
+    for k in range(wl.steps):
+        warm.step(wl.n, wl.tick, Xw, Pw, W, Aw)
+        cold.step(wl.n, wl.tick, Xc, Pc, W, Ac)
+        torch.cuda.synchronize()
+        assert torch.equal(Aw, Ac) and torch.equal(Xw, Xc) and torch.equal(Pw, Pc), k
+    assert_same_state(warm, cold, wl.n)
+    warm.close()
+    cold.close()
+
+
+def test_graph_run_bitwise_equal_cold():
+    from nbfmm_amd.workloads import two_disk_collision
+    wl = two_disk_collision(n=1 << 15, num_level=5, steps=10)
+    warm, cold = pair(wl)
+    Xw, Pw, W = dev(wl.position), dev(wl.position_pre), dev(wl.weight)
+    Xc, Pc = Xw.clone(), Pw.clone()
+    Aw, Ac = torch.zeros_like(Xw), torch.zeros_like(Xw)
+    st = torch.cuda.Stream()
+    with torch.cuda.stream(st):
+        warm.run(wl.n, wl.tick, 4, Xw, Pw, W, Aw, use_graph=True)   # cold eager step, then warm replays
+        warm.run(wl.n, wl.tick, 6, Xw, Pw, W, Aw, use_graph=True)   # same graph again
+    st.synchronize()
+    for _ in range(10):
+        cold.step(wl.n, wl.tick, Xc, Pc, W, Ac)
+    torch.cuda.synchronize()
+    assert torch.equal(Aw, Ac) and torch.equal(Xw, Xc) and torch.equal(Pw, Pc)
+    warm.close()
+    cold.close()
+
+
+@pytest.mark.parametrize("jump", ["drift", "shuffle", "crowd", "weights"])
+def test_solve_sequences(jump):
+    """solve at x1, then at x2: small drift (warm placement), unrelated
+    positions (fallback on the mover count), half the particles into one
+    leaf (fallback on a leaf's movers), new masses only (no movers)."""
+    from nbfmm_amd.workloads import disk
+    wl = disk(n=20000)
+    warm, cold = pair(wl)
+    rng = np.random.default_rng(9)
+    x1, m1 = wl.position, wl.weight
+    if jump == "drift":
+        x2 = x1 + 2e-3 * rng.standard_normal(x1.shape).astype(np.float32)
+        m2 = m1
+    elif jump == "shuffle":
+        x2 = x1[rng.permutation(wl.n)]
+        m2 = m1
+    elif jump == "crowd":
+        x2 = x1.copy()
+        x2[: wl.n // 2] = x1[0] + 1e-6 * rng.random((wl.n // 2, 2)).astype(np.float32)
+        m2 = m1
+    else:
+        x2 = x1
+        m2 = (m1 * (0.5 + rng.random(wl.n))).astype(np.float32)
+    for x, m in ((x1, m1), (x2, m2), (x1, m1)):
+        aw, ac = torch.zeros((wl.n, 2), device="cuda"), torch.zeros((wl.n, 2), device="cuda")
+        warm.solve(wl.n, dev(x), dev(m), aw)
+        cold.solve(wl.n, dev(x), dev(m), ac)
+        torch.cuda.synchronize()
+        assert torch.equal(aw, ac)
+        assert_same_state(warm, cold, wl.n)
+    ref, _ = orc.solve(wl.num_level, x1, m1, np.array(wl.limits, np.float32), wl.eps)
+    got = aw.cpu().numpy()
+    assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 1e-5
+    warm.close()
+    cold.close()
+
+
+def test_buffer_write_and_n_change_start_cold():
+    from nbfmm_amd.workloads import disk
+    wl = disk(n=12000)
+    warm, cold = pair(wl)
+    x, m = dev(wl.position), dev(wl.weight)
+    seq = [wl.n, wl.n, wl.n, 7000, 7000, wl.n]
+    for k, n in enumerate(seq):
+        if k == 2:
+            # overwrite the stored sort through the buffer API: the next sort must start cold
+            warm.write_buffer("perm", torch.arange(wl.n, dtype=torch.int32, device="cuda"), wl.n)
+        aw, ac = torch.zeros((n, 2), device="cuda"), torch.zeros((n, 2), device="cuda")
+        warm.solve(n, x[:n], m[:n], aw)
+        cold.solve(n, x[:n], m[:n], ac)
+        torch.cuda.synchronize()
+        assert torch.equal(aw, ac), (k, n)
+        assert_same_state(warm, cold, n)
+    warm.close()
+    cold.close()
+
+
+def test_adjoint_after_warm_sort():
+    from nbfmm_amd.workloads import disk
+    wl = disk(n=8192)
+    warm, cold = pair(wl)
+    rng = np.random.default_rng(4)
+    x = dev(wl.position)
+    m = dev(wl.weight)
+    g = dev(rng.standard_normal((wl.n, 2)))
+    a = torch.zeros_like(x)
+    warm.solve(wl.n, x, m, a)                      # leaves a sort to start from
+    x2 = x + 1e-3 * dev(rng.standard_normal((wl.n, 2)))
+    out = []
+    for s in (warm, cold):
+        gp, gw = torch.zeros_like(x), torch.zeros_like(m)
+        s.solve_adjoint(wl.n, x2, m, g, gp, gw)
+        torch.cuda.synchronize()
+        out.append((gp, gw))
+    assert torch.equal(out[0][0], out[1][0]) and torch.equal(out[0][1], out[1][1])
+    warm.close()
+    cold.close()
+
+
+def test_run_after_caller_edits_arrays():
+    """Between two runs the caller moves every particle and changes masses:
+    the next run's first step reads the caller's arrays again (its later
+    steps hand positions over in sorted order)."""
+    from nbfmm_amd.workloads import two_disk_collision
+    wl = two_disk_collision(n=1 << 14, num_level=5, steps=6)
+    warm, cold = pair(wl)
+    rng = np.random.default_rng(21)
+    Xw, Pw, Ww = dev(wl.position), dev(wl.position_pre), dev(wl.weight)
+    Xc, Pc, Wc = Xw.clone(), Pw.clone(), Ww.clone()
+    Aw, Ac = torch.zeros_like(Xw), torch.zeros_like(Xw)
+    st = torch.cuda.Stream()
+    for rnd in range(3):
+        with torch.cuda.stream(st):
+            warm.run(wl.n, wl.tick, 3, Xw, Pw, Ww, Aw, use_graph=True)
+        st.synchronize()
+        for _ in range(3):
+            cold.step(wl.n, wl.tick, Xc, Pc, Wc, Ac)
+        torch.cuda.synchronize()
+        assert torch.equal(Aw, Ac) and torch.equal(Xw, Xc) and torch.equal(Pw, Pc), rnd
+        shift = dev(1e-2 * rng.standard_normal((wl.n, 2)))
+        scale = dev(0.5 + rng.random(wl.n))
+        for X, W in ((Xw, Ww), (Xc, Wc)):
+            X += shift
+            W *= scale
+    warm.close()
+    cold.close()
