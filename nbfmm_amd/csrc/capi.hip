@@ -798,7 +798,7 @@ struct nbfmm_batch {
   SysView* dv = nullptr;          // device copies
   BatchLaunch bl{};
   int cus = 256;
-  int p2p_variant = kDefaultP2PVariant;
+  int p2p_variant = kDefaultBatchP2PVariant;
   int p2p_bpc = kDefaultBatchP2PBlocksPerCU;
   std::map<std::pair<int, int>, KernelMap> p2p_maps;  // (waves per block, blocks per CU) -> map
   std::vector<void*> allocs;

@@ -52,11 +52,12 @@ hipError_t launch_cube_write(const SysView& v, int which, const void* src, hipSt
 hipError_t launch_p2p(const SysView& v, int mode, const CallIO& io, int grid_blocks, int variant, hipStream_t s);
 hipError_t launch_p2p_batch(const BatchLaunch& b, int mode, const CallIO& io, int variant, hipStream_t s);
 int p2p_num_variants();
-// default P2P configuration: variant {256 staged, unroll 4, 4 waves} on 16
-// blocks per CU (two rounds of blocks at 8 waves per SIMD); a batch spreads
-// 32 blocks per CU over its systems
+// default P2P configurations (measured): a solver runs variant {384 staged,
+// unroll 4, 8 waves} on 4 blocks per CU; a batch spreads variant {384, 4, 4
+// waves} on 32 blocks per CU over its systems
 constexpr int kDefaultP2PVariant = 5;
-constexpr int kDefaultP2PBlocksPerCU = 16;
+constexpr int kDefaultP2PBlocksPerCU = 4;
+constexpr int kDefaultBatchP2PVariant = 0;
 constexpr int kDefaultBatchP2PBlocksPerCU = 32;
 void p2p_variant_info(int variant, int* stage, int* unroll, int* waves);
 hipError_t launch_l2p(const SysView& v, hipStream_t s);

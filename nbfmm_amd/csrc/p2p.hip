@@ -4,19 +4,15 @@
 // Reference: source/nbfmm/core/solver/p2p.cu:30-176 (p2pNaive / p2pBlock),
 // l2p.cu:23-48, postdo.cu:22-45, source/nbfmm/display/stars/update.cu:26-42.
 //
-// P2P schedule: one wavefront per occupied leaf cell; the waves take the
-// leaves from the per-system work queue, largest size class first.  The
-// sources of a leaf are three contiguous ranges of the sorted particles, one
-// per neighbour row [head[x-1 + yB], head[x+1 + yB + 1]) (the row-range idea
-// of p2p.cu:105-118).  The wave stages them, 256 at a
-// time, in its own LDS buffer as float4 {x, y, m, -} and pads the buffer
-// with zero-mass sources.  For a chunk of nt <= 64 targets the wave is split
-// into K = 64/G copies of G = next_pow2(nt) lanes: lane (t, k) accumulates
-// target t against staged sources k, k+K, k+2K, ... four per iteration, and
-// the K partial sums are combined by a butterfly over the copies.  In the
-// final modes the wave also forms its leaf's local (L2L folded in), adds it
-// (L2P) and scatters to the original order (postdo), and in the step mode
-// applies the Verlet update.
+// P2P schedule: the sorted particles of every row of leaves are cut into
+// runs of 64, one wavefront per run and one target per lane, so a wave's
+// lanes are busy whatever the leaf sizes.  The sources of a target are three
+// contiguous ranges of the sorted particles, one per neighbour row
+// [head[x-1 + yB], head[x+1 + yB + 1]) (the row-range idea of
+// p2p.cu:105-118); a run stages the union of its targets' ranges in LDS and
+// each lane sums its own sources.  In the final modes the wave also forms
+// each target's leaf local (L2L folded in), adds it (L2P) and scatters to the
+// original order (postdo), and in the step mode applies the Verlet update.
 #include "common.hpp"
 #include "launch.hpp"
 #include "../../include/nbfmm_amd.h"
@@ -41,38 +37,56 @@ __device__ __forceinline__ float2 leaf_local(const SysView& v, int cx, int cy) {
 }
 
 
-// Lane (t, k) of a chunk: target pt against staged sources k, k+K, k+2K, ...
-// (cnt sources, zero-mass padded to a multiple of U*K).  K is a compile-time
-// constant so the LDS reads use immediate offsets.  Whatever U and the
-// staging size, a lane adds its sources in the same order (k, k+K, ...) and
-// the padding zeros only at the end, so every variant gives the same bits.
-template <int K, int U>
-__device__ __forceinline__ void p2p_lane_sum(const float4* __restrict__ src, int k, int cnt, float4 pt, float eps,
-                                             float& ax, float& ay) {
-  constexpr int step = U * K;
-  const int iters = __builtin_amdgcn_readfirstlane((cnt + step - 1) / step);
-  const float4* p = src + k;
-  for (int i = 0; i < iters; ++i, p += step) {
-    float4 s[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) s[u] = p[u * K];
-#pragma unroll
-    for (int u = 0; u < U; ++u) kernel_accum(pt.x, pt.y, s[u].x, s[u].y, s[u].z, eps, ax, ay);
+// Target runs: every row of leaves is cut into runs of 64 consecutive sorted
+// particles (the last run of a row may be shorter).  s_runs[y] is the number
+// of runs in the rows before y; s_runs[B] is the total.  All threads of the
+// block call this.
+template <int NT>
+__device__ __forceinline__ void row_runs(const SysView& v, int* s_runs, uint32_t* sh) {
+  const int B = v.B;
+  uint32_t carry = 0;
+  for (int r0 = 0; r0 < B; r0 += NT) {
+    const int r = r0 + (int)threadIdx.x;
+    const uint32_t cnt = r < B ? (uint32_t)((v.head[(r + 1) * B] - v.head[r * B] + kWave - 1) / kWave) : 0u;
+    uint32_t tot;
+    const uint32_t e = block_exclusive_scan<NT>(cnt, sh, &tot);
+    if (r < B) s_runs[r] = (int)(carry + e);
+    carry += tot;
   }
+  if (threadIdx.x == 0) s_runs[B] = (int)carry;
 }
 
-// STAGE sources staged per pass (a multiple of 64*U, so every lane's
-// padding falls in the last pass only), U sources per lane per iteration,
-// WAVES waves per block.
+// Row of run j (j < total): the last row whose first run is <= j.
+__device__ __forceinline__ int run_row(const int* s_runs, int B, int j) {
+  int lo = 0, hi = B;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (s_runs[mid] <= j) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// One wavefront works one run of up to 64 targets of row y, one target per
+// lane.  Target t in leaf x_t needs, in each neighbour row r, the sorted
+// range [head[r, x_t - 1], head[r, x_t + 2]) (the row-range idea of
+// p2p.cu:105-118).  Those ranges lie inside the run's union
+// [head[r, x_first - 1], head[r, x_last + 2]), which the wave stages in its
+// LDS buffer STAGE sources at a time; each lane then loops over its own part
+// of the window, U sources per iteration.  A lane adds its sources in sorted
+// order, rows y-1, y, y+1, whatever STAGE and U: every variant gives the
+// same bits.  The modes after the sum: kP2POnly stores the sorted effect;
+// kP2PFinal adds the leaf local (L2L folded in, L2P) and scatters to the
+// original order (postdo); kP2PStep also applies the Verlet update.
 template <int MODE, class Sys, int STAGE, int U, int WAVES>
 __global__ __launch_bounds__(WAVES * kWave) void k_p2p(Sys sys, CallIO io) {
-  static_assert(STAGE % (kWave * U) == 0, "staging size must be a multiple of 64*U");
+  static_assert(STAGE % kWave == 0, "staging size must be a multiple of 64");
   __shared__ float4 s_src[WAVES][STAGE];
+  __shared__ int s_runs[(1 << (kMaxLevel + 1)) + 1];
+  __shared__ uint32_t sh[WAVES];
   __shared__ int s_next;
   int blk, nblk;
   const SysView v = sys.get(blk, nblk);
   const int lane = lane_id();
-  // wave-uniform: keeps the cell bookkeeping and loop counters scalar
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   float4* src = s_src[wave];
   float2* acc0 = io.acc0 + v.part_off;
@@ -80,88 +94,79 @@ __global__ __launch_bounds__(WAVES * kWave) void k_p2p(Sys sys, CallIO io) {
   float2* xpre = io.xpre + v.part_off;
   const int B = v.B;
   const float eps = v.eps;
-  LeafQueue queue;
-  queue.load(v);
+  row_runs<WAVES * kWave>(v, s_runs, sh);
   if (threadIdx.x == 0) s_next = 0;
   __syncthreads();
-  for (int j = blk + block_take(&s_next) * nblk; j < queue.total;) {
-    const int c = queue.leaf(v, j);
-    j = blk + block_take(&s_next) * nblk;
-    const int cx = c & (B - 1), cy = c >> v.lgB;
-    const int t0 = v.head[c], t1 = v.head[c + 1];
-    const int xl = cx > 0 ? cx - 1 : 0;
-    const int xr = cx < B - 1 ? cx + 1 : B - 1;
-    // the three neighbour rows as contiguous ranges of the sorted particles
-    int rs0 = 0, rs1 = 0, rs2 = 0, n0 = 0, n1 = 0, n2 = 0;
-    if (cy > 0) { rs0 = v.head[xl + (cy - 1) * B]; n0 = v.head[xr + (cy - 1) * B + 1] - rs0; }
-    rs1 = v.head[xl + cy * B]; n1 = v.head[xr + cy * B + 1] - rs1;
-    if (cy < B - 1) { rs2 = v.head[xl + (cy + 1) * B]; n2 = v.head[xr + (cy + 1) * B + 1] - rs2; }
-    const int S = n0 + n1 + n2;
-    float2 loc = make_float2(0.0f, 0.0f);
-    if (MODE != kP2POnly && v.L > 0) loc = leaf_local(v, cx, cy);
-    for (int tb = t0; tb < t1; tb += kWave) {
-      const int nt = min(kWave, t1 - tb);
-      const int lgG = nt <= 1 ? 0 : 32 - __clz(nt - 1);
-      const int G = 1 << lgG;
-      const int K = kWave >> lgG;
-      const int t = lane & (G - 1);
-      const int k = lane >> lgG;
-      const bool valid = t < nt;
-      const float4 pt = v.part[tb + (valid ? t : 0)];
-      float ax = 0.0f, ay = 0.0f;
-      for (int base = 0; base < S; base += STAGE) {
-        const int cnt = min(STAGE, S - base);
-        // stage up to STAGE sources; slots past cnt hold zero-mass sources,
-        // which add exactly nothing
+  const int total = s_runs[B];
+  for (int j = blk + block_take(&s_next) * nblk; j < total; j = blk + block_take(&s_next) * nblk) {
+    const int y = run_row(s_runs, B, j);
+    const int rs = v.head[y * B];
+    const int T0 = rs + kWave * (j - s_runs[y]);
+    const int nt = min(kWave, v.head[(y + 1) * B] - T0);
+    const bool valid = lane < nt;
+    const int i = T0 + (valid ? lane : 0);
+    const float4 pt = v.part[i];
+    const int xt = (int)(v.key[i] & (uint32_t)(B - 1));
+    const int xf = __shfl(xt, 0, kWave);
+    const int xl = __shfl(xt, nt - 1, kWave);
+    float ax = 0.0f, ay = 0.0f;
+#pragma unroll
+    for (int dr = -1; dr <= 1; ++dr) {
+      const int r = y + dr;
+      if (r < 0 || r >= B) continue;
+      const int* hr = v.head + r * B;
+      const int us = hr[xf > 0 ? xf - 1 : 0];
+      const int ulen = hr[(xl < B - 1 ? xl + 1 : B - 1) + 1] - us;
+      // this lane's range inside the union (empty for idle lanes)
+      const int a = valid ? hr[xt > 0 ? xt - 1 : 0] - us : 0;
+      const int b = valid ? hr[(xt < B - 1 ? xt + 1 : B - 1) + 1] - us : 0;
+      for (int w0 = 0; w0 < ulen; w0 += STAGE) {
+        const int cnt = min(STAGE, ulen - w0);
 #pragma unroll
         for (int q = 0; q < STAGE / kWave; ++q) {
           const int slot = q * kWave + lane;
-          const int f = base + slot;
-          float4 val = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-          if (slot < cnt) {
-            const int g = f < n0 ? rs0 + f : (f < n0 + n1 ? rs1 + (f - n0) : rs2 + (f - n0 - n1));
-            val = v.part[g];
-          }
-          src[slot] = val;
+          if (slot < cnt) src[slot] = v.part[us + w0 + slot];
         }
         wave_lds_order();
-        switch (lgG) {
-          case 0: p2p_lane_sum<64, U>(src, k, cnt, pt, eps, ax, ay); break;
-          case 1: p2p_lane_sum<32, U>(src, k, cnt, pt, eps, ax, ay); break;
-          case 2: p2p_lane_sum<16, U>(src, k, cnt, pt, eps, ax, ay); break;
-          case 3: p2p_lane_sum<8, U>(src, k, cnt, pt, eps, ax, ay); break;
-          case 4: p2p_lane_sum<4, U>(src, k, cnt, pt, eps, ax, ay); break;
-          case 5: p2p_lane_sum<2, U>(src, k, cnt, pt, eps, ax, ay); break;
-          default: p2p_lane_sum<1, U>(src, k, cnt, pt, eps, ax, ay); break;
+        const int lo = min(max(a - w0, 0), cnt);
+        const int hi = min(max(b - w0, 0), cnt);
+        const float4* p = src + lo;
+        int n = hi - lo;
+        for (; n >= U; n -= U, p += U) {
+          float4 sv[U];
+#pragma unroll
+          for (int u = 0; u < U; ++u) sv[u] = p[u];
+#pragma unroll
+          for (int u = 0; u < U; ++u) kernel_accum(pt.x, pt.y, sv[u].x, sv[u].y, sv[u].z, eps, ax, ay);
+        }
+        for (; n > 0; --n, ++p) {
+          const float4 s1 = *p;
+          kernel_accum(pt.x, pt.y, s1.x, s1.y, s1.z, eps, ax, ay);
         }
         wave_lds_order();
       }
-      for (int off = G; off < kWave; off <<= 1) {
-        ax += __shfl_xor(ax, off, kWave);
-        ay += __shfl_xor(ay, off, kWave);
-      }
-      if (k == 0 && valid) {
-        const int i = tb + t;
-        if (MODE == kP2POnly) {
-          v.eff[i] = make_float2(ax, ay);
-        } else {
+    }
+    if (valid) {
+      if (MODE == kP2POnly) {
+        v.eff[i] = make_float2(ax, ay);
+      } else {
 #pragma clang fp contract(off)
-          // l2p.cu:34: effect += cell effect; postdo.cu:32: scatter by perm
-          const float ex = ax + loc.x;
-          const float ey = ay + loc.y;
-          const uint32_t o = __float_as_uint(pt.w);
-          acc0[o] = make_float2(ex, ey);
-          if (MODE == kP2PStep) {
-            // update.cu:39-41
-            const float2 x = xcur[o];
-            const float2 xp = xpre[o];
-            float2 nx;
-            nx.x = 2.0f * x.x - xp.x + ex * v.tick * v.tick;
-            nx.y = 2.0f * x.y - xp.y + ey * v.tick * v.tick;
-            xcur[o] = nx;
-            xpre[o] = x;
-            if (io.xs_out) io.xs_out[v.part_off + i] = nx;
-          }
+        // l2p.cu:34: effect += cell effect; postdo.cu:32: scatter by perm
+        const float2 loc = v.L > 0 ? leaf_local(v, xt, y) : make_float2(0.0f, 0.0f);
+        const float ex = ax + loc.x;
+        const float ey = ay + loc.y;
+        const uint32_t o = __float_as_uint(pt.w);
+        acc0[o] = make_float2(ex, ey);
+        if (MODE == kP2PStep) {
+          // update.cu:39-41
+          const float2 x = xcur[o];
+          const float2 xp = xpre[o];
+          float2 nx;
+          nx.x = 2.0f * x.x - xp.x + ex * v.tick * v.tick;
+          nx.y = 2.0f * x.y - xp.y + ey * v.tick * v.tick;
+          xcur[o] = nx;
+          xpre[o] = x;
+          if (io.xs_out) io.xs_out[v.part_off + i] = nx;
         }
       }
     }
@@ -267,8 +272,8 @@ __global__ __launch_bounds__(256) void k_pair_count(SysView v, unsigned long lon
 struct P2PVariant {
   int stage, unroll, waves;
 };
-static const P2PVariant kVariants[] = {{256, 4, 16}, {256, 4, 8}, {256, 2, 16}, {512, 4, 8},
-                                       {512, 8, 8},  {256, 4, 4}, {128, 2, 16}};
+static const P2PVariant kVariants[] = {{384, 4, 4}, {256, 4, 4}, {512, 4, 4}, {384, 8, 4},
+                                       {256, 2, 4}, {384, 4, 8}, {320, 4, 4}};
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 
 int p2p_num_variants() { return kNumVariants; }
@@ -282,13 +287,13 @@ template <int MODE, class Sys>
 static void launch_p2p_variant(int variant, dim3 g, const Sys& sys, const CallIO& io, hipStream_t s) {
   const dim3 b(kVariants[variant < 0 || variant >= kNumVariants ? 0 : variant].waves * kWave);
   switch (variant) {
-    case 1: hipLaunchKernelGGL((k_p2p<MODE, Sys, 256, 4, 8>), g, b, 0, s, sys, io); break;
-    case 2: hipLaunchKernelGGL((k_p2p<MODE, Sys, 256, 2, 16>), g, b, 0, s, sys, io); break;
-    case 3: hipLaunchKernelGGL((k_p2p<MODE, Sys, 512, 4, 8>), g, b, 0, s, sys, io); break;
-    case 4: hipLaunchKernelGGL((k_p2p<MODE, Sys, 512, 8, 8>), g, b, 0, s, sys, io); break;
-    case 5: hipLaunchKernelGGL((k_p2p<MODE, Sys, 256, 4, 4>), g, b, 0, s, sys, io); break;
-    case 6: hipLaunchKernelGGL((k_p2p<MODE, Sys, 128, 2, 16>), g, b, 0, s, sys, io); break;
-    default: hipLaunchKernelGGL((k_p2p<MODE, Sys, 256, 4, 16>), g, b, 0, s, sys, io); break;
+    case 1: hipLaunchKernelGGL((k_p2p<MODE, Sys, 256, 4, 4>), g, b, 0, s, sys, io); break;
+    case 2: hipLaunchKernelGGL((k_p2p<MODE, Sys, 512, 4, 4>), g, b, 0, s, sys, io); break;
+    case 3: hipLaunchKernelGGL((k_p2p<MODE, Sys, 384, 8, 4>), g, b, 0, s, sys, io); break;
+    case 4: hipLaunchKernelGGL((k_p2p<MODE, Sys, 256, 2, 4>), g, b, 0, s, sys, io); break;
+    case 5: hipLaunchKernelGGL((k_p2p<MODE, Sys, 384, 4, 8>), g, b, 0, s, sys, io); break;
+    case 6: hipLaunchKernelGGL((k_p2p<MODE, Sys, 320, 4, 4>), g, b, 0, s, sys, io); break;
+    default: hipLaunchKernelGGL((k_p2p<MODE, Sys, 384, 4, 4>), g, b, 0, s, sys, io); break;
   }
 }
 
