@@ -26,11 +26,12 @@ sys.path.insert(0, ROOT)
 FP32_PEAK_TFLOPS = 157.3   # MI355X published FP32 vector peak
 HBM_PEAK_GBS = 8000.0
 P2P_FLOPS_PER_PAIR = 14    # sub 2, r^2 3, sqrt 1, r^3 2, +eps 1, div 1, scale 2, accumulate 2
-# VALU issue capacity: 256 CUs x 64 lanes x 2.4 GHz lane-instructions per second
-VALU_ISSUE_PER_S = 256 * 64 * 2.4e9
-# instruction slots per pair of the P2P inner loop: 8.25 VALU + 2 transcendentals
-# at 1.74 slots each (measured transcendental cost on gfx950 relative to an FMA)
-P2P_SLOTS_PER_PAIR = 8.25 + 2 * 1.74
+# Pair-rate bound of the P2P arithmetic on MI355X: a throughput probe running
+# the inner loop's instruction mix per pair (8 FMA-class VALU + v_sqrt_f32 +
+# v_rcp_f32, independent chains, every CU busy) sustains 3.19e12 lane-steps/s,
+# one pair per lane-step (8 FMA alone: 4.45e12; transcendentals cost ~1.9 FMA
+# slots each and mostly add to the rest).
+P2P_PAIR_BOUND_PER_S = 3.19e12
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_two_disk_N1048576.md")
 
 
@@ -221,7 +222,7 @@ def main():
                                  else None),
                      "avg_launch_us": p2p_avg_ms * 1e3, "launches": launches,
                      "pairs_per_launch": pairs, "flops_per_pair": P2P_FLOPS_PER_PAIR,
-                     "valu_issue_frac": pairs * P2P_SLOTS_PER_PAIR / (p2p_avg_ms * 1e-3 * VALU_ISSUE_PER_S),
+                     "valu_issue_frac": pairs / (p2p_avg_ms * 1e-3 * P2P_PAIR_BOUND_PER_S),
                      "occupied_leaves": int(occ0)},
         "cpu_baseline": cpu,
         "accuracy": accuracy,

@@ -130,16 +130,20 @@ __global__ __launch_bounds__(WAVES * kWave) void k_p2p(Sys sys, CallIO io) {
         wave_lds_order();
         const int lo = min(max(a - w0, 0), cnt);
         const int hi = min(max(b - w0, 0), cnt);
+        // loop control on the LDS address alone: one compare and one add
+        // per U sources (pfull >= p, so the unsigned address compare never
+        // sees a bound below the buffer)
         const float4* p = src + lo;
-        int n = hi - lo;
-        for (; n >= U; n -= U, p += U) {
+        const float4* const pend = src + hi;
+        const float4* const pfull = src + max(hi - (U - 1), lo);
+        for (; p < pfull; p += U) {
           float4 sv[U];
 #pragma unroll
           for (int u = 0; u < U; ++u) sv[u] = p[u];
 #pragma unroll
           for (int u = 0; u < U; ++u) kernel_accum(pt.x, pt.y, sv[u].x, sv[u].y, sv[u].z, eps, ax, ay);
         }
-        for (; n > 0; --n, ++p) {
+        for (; p < pend; ++p) {
           const float4 s1 = *p;
           kernel_accum(pt.x, pt.y, s1.x, s1.y, s1.z, eps, ax, ay);
         }
