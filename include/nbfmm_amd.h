@@ -246,6 +246,9 @@ int nbfmm_batch_flags(nbfmm_batch* batch, unsigned* flags, int clear);
  * their particle counts. */
 int nbfmm_batch_set_p2p_config(nbfmm_batch* batch, int variant, int blocks_per_cu);
 int nbfmm_batch_get_p2p_config(const nbfmm_batch* batch, int* variant, int* blocks_per_cu);
+/* Warm-started sort of every system (see nbfmm_set_warm_start); off by default
+ * for batches (many systems already keep the cold sort busy). */
+int nbfmm_batch_set_warm_start(nbfmm_batch* batch, int enable);
 /* nbfmm_solve_adjoint of every system (concatenated arrays). */
 int nbfmm_batch_solve_adjoint(nbfmm_batch* batch, const float* d_position_origin, const float* d_weight_origin,
                               const float* d_grad_effect, float* d_grad_position, float* d_grad_weight,

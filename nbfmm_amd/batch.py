@@ -64,6 +64,11 @@ class Batch:
         every launch; all configurations give identical results."""
         check(lib().nbfmm_batch_set_p2p_config(self._h, int(variant), int(blocks_per_cu)))
 
+    def warm_start(self, enable=True):
+        """Start each sort from the previous one (default off for batches);
+        results are identical either way."""
+        check(lib().nbfmm_batch_set_warm_start(self._h, int(bool(enable))))
+
     def p2p_config(self):
         v, b = ctypes.c_int(), ctypes.c_int()
         check(lib().nbfmm_batch_get_p2p_config(self._h, ctypes.byref(v), ctypes.byref(b)))

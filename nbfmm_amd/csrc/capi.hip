@@ -796,7 +796,14 @@ struct nbfmm_batch {
   std::vector<int> part_off;      // S+1
   std::vector<SysView> hv;        // host copies of the views
   SysView* dv = nullptr;          // device copies
+  SysView* dv_guard = nullptr;    // device copies with cold_if set (guarded cold sort passes)
   BatchLaunch bl{};
+  // Off by default for batches: measured on 256 systems (7.1M particles) the
+  // warm passes cost as much as the cold sort (about 0.34 ms a step each),
+  // whose per-row blocks are already balanced by the many systems.
+  bool warm_enabled = false;
+  bool warm_valid = false;        // key/part/head hold a sort of every system
+  float2* xs = nullptr;           // sorted positions handed from one step of a run to the next
   int cus = 256;
   int p2p_variant = kDefaultBatchP2PVariant;
   int p2p_bpc = kDefaultBatchP2PBlocksPerCU;
@@ -808,6 +815,7 @@ struct nbfmm_batch {
     const void *xcur = nullptr, *xpre = nullptr, *w = nullptr, *acc = nullptr;
     hipStream_t stream = nullptr;
     bool valid = false;
+    bool warm = false;
   } gkey;
 };
 
@@ -833,8 +841,9 @@ void batch_free(nbfmm_batch* b) {
   b->allocs.clear();
 }
 
-// Builds a launch map from the per-system block counts.
-int make_map(nbfmm_batch* b, const std::vector<int>& blocks, KernelMap* km) {
+// Builds a launch map from the per-system block counts (over the device
+// views vs, by default the plain ones).
+int make_map(nbfmm_batch* b, const std::vector<int>& blocks, KernelMap* km, const SysView* vs = nullptr) {
   const int S = (int)blocks.size();
   std::vector<int> start(S + 1, 0);
   for (int s = 0; s < S; ++s) start[s + 1] = start[s] + blocks[s];
@@ -846,7 +855,7 @@ int make_map(nbfmm_batch* b, const std::vector<int>& blocks, KernelMap* km) {
   if (rc) return rc;
   NB_HIP(hipMemcpy(d, start.data(), (S + 1) * sizeof(int), hipMemcpyHostToDevice));
   NB_HIP(hipMemcpy(d + S + 1, sys.data(), sys.size() * sizeof(int), hipMemcpyHostToDevice));
-  km->sys.vs = b->dv;
+  km->sys.vs = vs ? vs : b->dv;
   km->sys.bstart = d;
   km->sys.bsys = d + S + 1;
   km->blocks = start[S];
@@ -883,8 +892,19 @@ int batch_use_p2p(nbfmm_batch* b, int variant, int bpc) {
   return NBFMM_OK;
 }
 
+// Sort stage of a batch: warm-started from its previous sort when there is
+// one (every system keeps its N), else cold.
+int batch_sort(nbfmm_batch* b, const CallIO& io, hipStream_t st) {
+  if (b->warm_enabled && b->warm_valid)
+    NB_HIP(launch_predo_warm_batch(b->bl, io, st));
+  else
+    NB_HIP(launch_predo_batch(b->bl, io, st));
+  b->warm_valid = true;
+  return NBFMM_OK;
+}
+
 int batch_pipeline(nbfmm_batch* b, int mode, const CallIO& io, hipStream_t st) {
-  NB_HIP(launch_predo_batch(b->bl, io, st));
+  if (int rc = batch_sort(b, io, st)) return rc;
   NB_HIP(launch_tree_batch(b->bl, st));
   NB_HIP(launch_p2p_batch(b->bl, mode, io, b->p2p_variant, st));
   return NBFMM_OK;
@@ -995,6 +1015,22 @@ int nbfmm_batch_create(int S, const int* num_particle, const int* num_level, con
   BALLOC(gmp, tree_all);
   BALLOC(flags, (size_t)S);
   BALLOC(b->dv, (size_t)S);
+  BALLOC(b->dv_guard, (size_t)S);
+  int *wstate, *wrel, *wmgrp, *wcin, *wcout, *wtouch, *wtlist, *wcnew, *winoff, *wfill, *wrowtot;
+  int2* wtold;
+  BALLOC(b->xs, n_all);
+  BALLOC(wstate, (size_t)S * kWarmInts);
+  BALLOC(wrel, n_all);
+  BALLOC(wmgrp, n_all);
+  BALLOC(wcin, cell_all);
+  BALLOC(wcout, cell_all);
+  BALLOC(wtouch, cell_all);
+  BALLOC(wtlist, cell_all);
+  BALLOC(wcnew, cell_all);
+  BALLOC(winoff, cell_all);
+  BALLOC(wfill, cell_all);
+  BALLOC(wtold, cell_all);
+  BALLOC(wrowtot, 2 * row_all);
 #undef BALLOC
   for (int s = 0; s < S; ++s) {
     SysView& v = b->hv[s];
@@ -1019,11 +1055,29 @@ int nbfmm_batch_create(int S, const int* num_particle, const int* num_level, con
     v.gup = gup + tree_off[s];
     v.gmp = gmp + tree_off[s];
     v.flags = flags + s;
+    v.wstate = wstate + (size_t)s * kWarmInts;
+    v.wrel = wrel + po;
+    v.wmgrp = wmgrp + po;
+    v.wcin = wcin + cell_off[s];
+    v.wcout = wcout + cell_off[s];
+    v.wtouch = wtouch + cell_off[s];
+    v.wtlist = wtlist + cell_off[s];
+    v.wcnew = wcnew + cell_off[s];
+    v.winoff = winoff + cell_off[s];
+    v.wfill = wfill + cell_off[s];
+    v.wtold = wtold + cell_off[s];
+    v.wrowtot = wrowtot + 2 * row_off[s];
   }
   hipError_t e = hipMemcpy(b->dv, b->hv.data(), S * sizeof(SysView), hipMemcpyHostToDevice);
   if (e != hipSuccess) { batch_free(b); delete b; return hip_fail(e, "hipMemcpy views"); }
+  {
+    std::vector<SysView> guard(b->hv);
+    for (auto& v : guard) v.cold_if = v.wstate + kWarmFallback;
+    e = hipMemcpy(b->dv_guard, guard.data(), S * sizeof(SysView), hipMemcpyHostToDevice);
+    if (e != hipSuccess) { batch_free(b); delete b; return hip_fail(e, "hipMemcpy views"); }
+  }
   // launch maps
-  std::vector<int> tiles(S), rows(S), p2m(S), mtile(S), mtop(S), m2l(S), asort(S), azero(S);
+  std::vector<int> tiles(S), rows(S), p2m(S), mtile(S), mtop(S), m2l(S), asort(S), azero(S), wmerge(S), one(S, 1);
   for (int s = 0; s < S; ++s) {
     const SysView& v = b->hv[s];
     const int C = v.B * v.B;
@@ -1035,11 +1089,16 @@ int nbfmm_batch_create(int S, const int* num_particle, const int* num_level, con
     m2l[s] = m2l_blocks(v);
     asort[s] = cdiv(v.n, 256);
     azero[s] = adj_zero_blocks(v);
+    // one wave per 64 particles (touched leaves are about twice the movers;
+    // the waves grid-stride over the list)
+    wmerge[s] = cdiv(std::min(C, v.n / 64 + 64), 256 / kWave);
   }
   if ((rc = make_map(b, tiles, &b->bl.tiles)) || (rc = make_map(b, rows, &b->bl.rows)) ||
       (rc = make_map(b, p2m, &b->bl.p2m)) || (rc = make_map(b, mtile, &b->bl.m2m_tile)) ||
       (rc = make_map(b, mtop, &b->bl.m2m_top)) || (rc = make_map(b, m2l, &b->bl.m2l)) ||
       (rc = make_map(b, asort, &b->bl.adj_sort)) || (rc = make_map(b, azero, &b->bl.adj_zero)) ||
+      (rc = make_map(b, tiles, &b->bl.tiles_g, b->dv_guard)) || (rc = make_map(b, rows, &b->bl.rows_g, b->dv_guard)) ||
+      (rc = make_map(b, wmerge, &b->bl.warm_merge)) || (rc = make_map(b, one, &b->bl.one)) ||
       (rc = make_map(b, p2p_share(b, kAdjP2PWaves, cus * kAdjP2PBlocksPerCU), &b->bl.adj_p2p)) ||
       (rc = batch_use_p2p(b, b->p2p_variant, b->p2p_bpc))) {
     batch_free(b);
@@ -1101,23 +1160,43 @@ int nbfmm_batch_run(nbfmm_batch* b, int steps, float* xcur, float* xpre, const f
     return NBFMM_OK;
   }
   if (!st) return fail(NBFMM_ERR_INVALID_ARGUMENT, "graph replay needs a non-null stream");
+  if (steps == 0) return NBFMM_OK;
+  // As nbfmm_run: the first step is eager and leaves the new positions in
+  // sorted order; the captured step replayed for the others reads them
+  // (with the warm start on) instead of gathering from the caller's arrays.
+  CallIO io = call_io(xcur, w, acc, xcur, xpre);
+  io.xs_out = b->xs;
+  if ((rc = batch_pipeline(b, kP2PStep, io, st))) return rc;
+  if (--steps == 0) return NBFMM_OK;
+  const bool warm = b->warm_enabled;
   auto& g = b->gkey;
-  const bool same = g.valid && g.xcur == xcur && g.xpre == xpre && g.w == w && g.acc == acc && g.stream == st;
+  const bool same = g.valid && g.xcur == xcur && g.xpre == xpre && g.w == w && g.acc == acc && g.stream == st &&
+                    g.warm == warm;
   if (!same) {
     if (b->graph_exec) { hipGraphExecDestroy(b->graph_exec); b->graph_exec = nullptr; }
     if (b->graph) { hipGraphDestroy(b->graph); b->graph = nullptr; }
     g.valid = false;
+    if (warm) io.xs_in = b->xs;
     NB_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    rc = nbfmm_batch_step(b, xcur, xpre, w, acc, stream);
+    rc = batch_pipeline(b, kP2PStep, io, st);
     hipGraph_t graph = nullptr;
     hipError_t e = hipStreamEndCapture(st, &graph);
     if (rc) { if (graph) hipGraphDestroy(graph); return rc; }
     if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
     b->graph = graph;
     NB_HIP(hipGraphInstantiate(&b->graph_exec, graph, nullptr, nullptr, 0));
-    g.xcur = xcur; g.xpre = xpre; g.w = w; g.acc = acc; g.stream = st; g.valid = true;
+    g.xcur = xcur; g.xpre = xpre; g.w = w; g.acc = acc; g.stream = st; g.valid = true; g.warm = warm;
   }
   for (int k = 0; k < steps; ++k) NB_HIP(hipGraphLaunch(b->graph_exec, st));
+  return NBFMM_OK;
+}
+
+int nbfmm_batch_set_warm_start(nbfmm_batch* b, int enable) {
+  if (!b) return fail(NBFMM_ERR_INVALID_ARGUMENT, "null batch");
+  b->warm_enabled = enable != 0;
+  if (b->graph_exec) { hipGraphExecDestroy(b->graph_exec); b->graph_exec = nullptr; }
+  if (b->graph) { hipGraphDestroy(b->graph); b->graph = nullptr; }
+  b->gkey.valid = false;
   return NBFMM_OK;
 }
 
@@ -1130,7 +1209,7 @@ int nbfmm_batch_solve_adjoint(nbfmm_batch* b, const float* pos0, const float* w0
   io.gpos = (float2*)grad_pos;
   io.gw = grad_w;
   io.gscale = 1.0f;
-  NB_HIP(launch_predo_batch(b->bl, io, st));
+  if (int rc = batch_sort(b, io, st)) return rc;
   NB_HIP(launch_tree_batch(b->bl, st));
   NB_HIP(launch_adjoint_batch(b->bl, io, 0, st));
   return NBFMM_OK;
@@ -1146,7 +1225,7 @@ int nbfmm_batch_step_adjoint(nbfmm_batch* b, const float* x, const float* w, flo
   io.gpre = (float2*)pi;
   io.gw = grad_w;
   io.gscale = -1.0f;  // each system's tick^2
-  NB_HIP(launch_predo_batch(b->bl, io, st));
+  if (int rc = batch_sort(b, io, st)) return rc;
   NB_HIP(launch_tree_batch(b->bl, st));
   NB_HIP(launch_adjoint_batch(b->bl, io, 1, st));
   return NBFMM_OK;

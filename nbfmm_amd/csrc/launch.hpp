@@ -23,6 +23,11 @@ struct BatchLaunch {
   KernelMap adj_p2p;   // adjoint P2P: share of its grid (kAdjP2PWaves-wave blocks)
   KernelMap adj_sort;  // adjoint: upstream gradient to sorted order, cdiv(n_s, 256)
   KernelMap adj_zero;  // adjoint: clear the level-0 subtree sums
+  // warm-started sort: the particle kernels use adj_sort, the row kernels rows
+  KernelMap tiles_g;   // as tiles / rows, over views whose cold_if points at the
+  KernelMap rows_g;    //   system's fallback flag (the guarded cold passes)
+  KernelMap warm_merge;  // one wave per touched leaf (grid-stride): its share of waves
+  KernelMap one;       // one block per system
 };
 
 // sort.hip
@@ -32,6 +37,7 @@ hipError_t launch_predo_batch(const BatchLaunch& b, const CallIO& io, hipStream_
 hipError_t launch_occ_from_head(const SysView& v, hipStream_t s);
 // warm start from the previous sort of the same n (falls back to the cold sort on the device)
 hipError_t launch_predo_warm(const SysView& v, const CallIO& io, int zero_empty, hipStream_t s);
+hipError_t launch_predo_warm_batch(const BatchLaunch& b, const CallIO& io, hipStream_t s);
 
 // tree.hip
 hipError_t launch_p2m(const SysView& v, hipStream_t s);

@@ -494,12 +494,40 @@ __global__ __launch_bounds__(kRowThreads) void k_warm_rows(Sys sys) {
   }
 }
 
-// One block per row: each block scans the B row totals itself, then writes
-// head and the movers' group offsets for its row and appends its occupied
-// leaves to occ and to the size-class lists (as the column pass does).
-// zero_empty: also clear the level-0 multipole of the row's empty leaves.
+// One block per system: exclusive scans of the row totals of the new
+// counts (-> rowstart, head[B*B]) and of the movers (-> wrowtot[B+1+y]),
+// when k_warm_finish does not scan them itself (batches: one scan per system
+// instead of one per row block).
 template <class Sys>
-__global__ __launch_bounds__(kRowThreads) void k_warm_finish(Sys sys, int zero_empty) {
+__global__ __launch_bounds__(kRowThreads) void k_warm_rowscan(Sys sys) {
+  __shared__ uint32_t sh[kRowWaves];
+  int blk, nblk;
+  const SysView v = sys.get(blk, nblk);
+  if (v.wstate[kWarmFallback]) return;
+  const int B = v.B;
+  const int x = threadIdx.x;
+  const uint32_t t = x < B ? (uint32_t)v.wrowtot[x] : 0u;
+  const uint32_t ti = x < B ? (uint32_t)v.wrowtot[B + 1 + x] : 0u;
+  uint32_t tot, itot;
+  const uint32_t e = block_exclusive_scan<kRowThreads>(t, sh, &tot);
+  const uint32_t ei = block_exclusive_scan<kRowThreads>(ti, sh, &itot);
+  if (x < B) {
+    v.rowstart[x] = (int)e;
+    v.wrowtot[B + 1 + x] = (int)ei;
+  }
+  if (x == 0) {
+    v.rowstart[B] = (int)tot;
+    v.head[B * B] = (int)tot;
+  }
+}
+
+// One block per row: the row's start (scanned here from the B row totals,
+// or read from rowstart when k_warm_rowscan ran), then head and the movers'
+// group offsets for the row; appends the row's occupied leaves to occ and to
+// the size-class lists (as the column pass does).  zero_empty: also clear
+// the level-0 multipole of the row's empty leaves.
+template <class Sys>
+__global__ __launch_bounds__(kRowThreads) void k_warm_finish(Sys sys, int zero_empty, int prescanned) {
   __shared__ uint32_t sh[kRowWaves];
   __shared__ uint32_t s_base, s_ibase, s_occ_base, s_touch_base;
   __shared__ uint32_t s_ccnt[kP2PClasses];
@@ -510,8 +538,13 @@ __global__ __launch_bounds__(kRowThreads) void k_warm_finish(Sys sys, int zero_e
   const int B = v.B;
   const int x = threadIdx.x;
   if (x < kP2PClasses) s_ccnt[x] = 0;
-  // row starts: exclusive scan of the row totals, up to row y
-  {
+  if (prescanned) {
+    if (x == 0) {
+      s_base = (uint32_t)v.rowstart[y];
+      s_ibase = (uint32_t)v.wrowtot[B + 1 + y];
+    }
+  } else {
+    // row starts: exclusive scan of the row totals, up to row y
     const uint32_t t = x < B ? (uint32_t)v.wrowtot[x] : 0u;
     const uint32_t ti = x < B ? (uint32_t)v.wrowtot[B + 1 + x] : 0u;
     uint32_t tot, itot;
@@ -674,7 +707,7 @@ hipError_t launch_predo_warm(const SysView& v, const CallIO& io, int zero_empty,
   const int nb = cdiv(v.n, 256);
   hipLaunchKernelGGL(k_warm_gather<OneSys>, dim3(nb), dim3(256), 0, s, one, io);
   hipLaunchKernelGGL(k_warm_rows<OneSys>, dim3(v.B), dim3(kRowThreads), 0, s, one);
-  hipLaunchKernelGGL(k_warm_finish<OneSys>, dim3(v.B), dim3(kRowThreads), 0, s, one, zero_empty);
+  hipLaunchKernelGGL(k_warm_finish<OneSys>, dim3(v.B), dim3(kRowThreads), 0, s, one, zero_empty, 0);
   hipLaunchKernelGGL(k_warm_place<OneSys>, dim3(nb), dim3(256), 0, s, one);
   hipLaunchKernelGGL(k_warm_merge<OneSys>, dim3(cdiv(std::min(kWarmMergeWaves, v.B * v.B), 256 / kWave)),
                      dim3(256), 0, s, one);
@@ -683,6 +716,21 @@ hipError_t launch_predo_warm(const SysView& v, const CallIO& io, int zero_empty,
   const hipError_t e = launch_predo(cv, io, zero_empty, s);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_warm_end<OneSys>, dim3(1), dim3(64), 0, s, one);
+  return hipGetLastError();
+}
+
+hipError_t launch_predo_warm_batch(const BatchLaunch& b, const CallIO& io, hipStream_t s) {
+  hipLaunchKernelGGL(k_warm_gather<ManySys>, dim3(b.adj_sort.blocks), dim3(256), 0, s, b.adj_sort.sys, io);
+  hipLaunchKernelGGL(k_warm_rows<ManySys>, dim3(b.rows.blocks), dim3(kRowThreads), 0, s, b.rows.sys);
+  hipLaunchKernelGGL(k_warm_rowscan<ManySys>, dim3(b.one.blocks), dim3(kRowThreads), 0, s, b.one.sys);
+  hipLaunchKernelGGL(k_warm_finish<ManySys>, dim3(b.rows.blocks), dim3(kRowThreads), 0, s, b.rows.sys, 1, 1);
+  hipLaunchKernelGGL(k_warm_place<ManySys>, dim3(b.adj_sort.blocks), dim3(256), 0, s, b.adj_sort.sys);
+  hipLaunchKernelGGL(k_warm_merge<ManySys>, dim3(b.warm_merge.blocks), dim3(256), 0, s, b.warm_merge.sys);
+  hipLaunchKernelGGL(k_row_up<ManySys>, dim3(b.tiles_g.blocks), dim3(kSortThreads), 0, s, b.tiles_g.sys, io);
+  hipLaunchKernelGGL(k_row_scan<ManySys>, dim3(b.rows_g.blocks), dim3(kScanThreads), 0, s, b.rows_g.sys);
+  hipLaunchKernelGGL(k_row_down<ManySys>, dim3(b.tiles_g.blocks), dim3(kSortThreads), 0, s, b.tiles_g.sys, io);
+  hipLaunchKernelGGL(k_row_sort<ManySys>, dim3(b.rows_g.blocks), dim3(kRowThreads), 0, s, b.rows_g.sys, 1);
+  hipLaunchKernelGGL(k_warm_end<ManySys>, dim3(b.one.blocks), dim3(64), 0, s, b.one.sys);
   return hipGetLastError();
 }
 

@@ -178,3 +178,52 @@ def test_run_after_caller_edits_arrays():
             W *= scale
     warm.close()
     cold.close()
+
+
+def _batch_pair(bw):
+    from nbfmm_amd import Batch
+    args = ([s.n for s in bw.systems], [s.num_level for s in bw.systems], [s.limits for s in bw.systems],
+            [s.eps for s in bw.systems])
+    warm = Batch(*args, ticks=[s.tick for s in bw.systems])
+    warm.warm_start(True)
+    cold = Batch(*args, ticks=[s.tick for s in bw.systems])
+    cold.warm_start(False)
+    return warm, cold
+
+
+def test_batch_warm_equals_cold():
+    """Every system of a batch starts its sort from the batch's previous sort:
+    solves with drift and with shuffled positions (fallback in that system),
+    then graph runs, all bitwise equal to a batch that sorts cold."""
+    from nbfmm_amd.workloads import many_disks
+    bw = many_disks(num_systems=6, n_min=500, n_max=30000, seed=13, steps=4)
+    for w, L in zip(bw.systems, [0, 2, 3, 5, 6, 7]):
+        w.num_level = L
+    warm, cold = _batch_pair(bw)
+    x, p, m = bw.concat()
+    rng = np.random.default_rng(2)
+    off = warm.offsets
+    xs = [x, x + 1e-3 * rng.standard_normal(x.shape).astype(np.float32)]
+    shuffled = xs[1].copy()
+    k = 3  # scramble one system only
+    shuffled[off[k]:off[k + 1]] = shuffled[off[k]:off[k + 1]][rng.permutation(off[k + 1] - off[k])]
+    xs.append(shuffled)
+    M = dev(m)
+    for xi in xs:
+        aw, ac = torch.zeros((x.shape[0], 2), device="cuda"), torch.zeros((x.shape[0], 2), device="cuda")
+        warm.solve(dev(xi), M, aw)
+        cold.solve(dev(xi), M, ac)
+        torch.cuda.synchronize()
+        assert torch.equal(aw, ac)
+    Xw, Pw = dev(x), dev(p)
+    Xc, Pc = Xw.clone(), Pw.clone()
+    Aw, Ac = torch.zeros_like(Xw), torch.zeros_like(Xw)
+    st = torch.cuda.Stream()
+    with torch.cuda.stream(st):
+        warm.run(3, Xw, Pw, M, Aw, use_graph=True)
+        warm.run(4, Xw, Pw, M, Aw, use_graph=True)
+        cold.run(7, Xc, Pc, M, Ac, use_graph=True)
+    st.synchronize()
+    assert torch.equal(Aw, Ac) and torch.equal(Xw, Xc) and torch.equal(Pw, Pc)
+    warm.close()
+    cold.close()
