@@ -199,7 +199,7 @@ __global__ __launch_bounds__(256) void k_adj_m2l(Sys sys) {
 }
 
 // Gradient wrt the leaf's (X, M): the M2M^T chain from the top level down,
-// on wave-uniform data.
+// per lane (lanes of one leaf load the same cells).
 __device__ __forceinline__ float4 leaf_multipole_grad(const SysView& v, int cx, int cy) {
   int l = v.L - 1;
   float4 T = v.gmp[v.lvl_off[l] + (cy >> l) * (v.B >> l) + (cx >> l)];
@@ -220,8 +220,10 @@ __device__ __forceinline__ float4 leaf_multipole_grad(const SysView& v, int cx, 
   return T;
 }
 
-// P2P^T plus P2M^T per occupied leaf (one wavefront each, sources staged in
-// LDS as in the forward P2P kernel), with the epilogue of the mode:
+// P2P^T plus P2M^T, over the same runs as the forward P2P (64 consecutive
+// sorted targets of one row per wavefront, one per lane; each lane loops
+// over its own sources inside the staged union of the run's ranges, in
+// sorted order), with the epilogue of the mode:
 //   kAdjSolve: gpos[o] = dloss/dx_o, gw[o] = dloss/dm_o
 //   kAdjStep:  Verlet adjoint; with lambda = gpos, pi = gpre:
 //              lambda_k = 2 lambda + tick^2 (da/dx)^T lambda + pi,
@@ -233,6 +235,8 @@ template <int MODE, class Sys>
 __global__ __launch_bounds__(kAdjP2PWaves * kWave) void k_adj_p2p(Sys sys, CallIO io) {
   __shared__ float4 s_pos[kAdjP2PWaves][kP2PStage];
   __shared__ float2 s_g[kAdjP2PWaves][kP2PStage];
+  __shared__ int s_runs[(1 << (kMaxLevel + 1)) + 1];
+  __shared__ uint32_t sh[kAdjP2PWaves];
   __shared__ int s_next;
   int blk, nblk;
   const SysView v = sys.get(blk, nblk);
@@ -245,93 +249,72 @@ __global__ __launch_bounds__(kAdjP2PWaves * kWave) void k_adj_p2p(Sys sys, CallI
   float* gw = io.gw + v.part_off;
   const int B = v.B;
   const float eps = v.eps;
-  LeafQueue queue;
-  queue.load(v);
+  row_runs<kAdjP2PWaves * kWave>(v, s_runs, sh);
   if (threadIdx.x == 0) s_next = 0;
   __syncthreads();
-  for (int j = blk + block_take(&s_next) * nblk; j < queue.total;) {
-    const int c = queue.leaf(v, j);
-    j = blk + block_take(&s_next) * nblk;
-    const int cx = c & (B - 1), cy = c >> v.lgB;
-    const int t0 = v.head[c], t1 = v.head[c + 1];
-    const int xl = cx > 0 ? cx - 1 : 0;
-    const int xr = cx < B - 1 ? cx + 1 : B - 1;
-    int rs0 = 0, rs1 = 0, rs2 = 0, n0 = 0, n1 = 0, n2 = 0;
-    if (cy > 0) { rs0 = v.head[xl + (cy - 1) * B]; n0 = v.head[xr + (cy - 1) * B + 1] - rs0; }
-    rs1 = v.head[xl + cy * B]; n1 = v.head[xr + cy * B + 1] - rs1;
-    if (cy < B - 1) { rs2 = v.head[xl + (cy + 1) * B]; n2 = v.head[xr + (cy + 1) * B + 1] - rs2; }
-    const int S = n0 + n1 + n2;
-    float4 leafg = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    float4 leafm = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if (v.L > 0) {
-      leafg = leaf_multipole_grad(v, cx, cy);
-      leafm = v.mpole[c];
-    }
-    for (int tb = t0; tb < t1; tb += kWave) {
-      const int nt = min(kWave, t1 - tb);
-      const int lgG = nt <= 1 ? 0 : 32 - __clz(nt - 1);
-      const int G = 1 << lgG;
-      const int K = kWave >> lgG;
-      const int t = lane & (G - 1);
-      const int k = lane >> lgG;
-      const bool valid = t < nt;
-      const int ti = tb + (valid ? t : 0);
-      const float4 pt = v.part[ti];
-      const float2 gt = v.gsort[ti];
-      float gx = 0.0f, gy = 0.0f, gm = 0.0f;
-      for (int base = 0; base < S; base += kP2PStage) {
-        const int cnt = min(kP2PStage, S - base);
+  const int total = s_runs[B];
+  for (int j = blk + block_take(&s_next) * nblk; j < total; j = blk + block_take(&s_next) * nblk) {
+    const int y = run_row(s_runs, B, j);
+    const int T0 = v.head[y * B] + kWave * (j - s_runs[y]);
+    const int nt = min(kWave, v.head[(y + 1) * B] - T0);
+    const bool valid = lane < nt;
+    const int ti = T0 + (valid ? lane : 0);
+    const float4 pt = v.part[ti];
+    const float2 gt = v.gsort[ti];
+    const int xt = (int)(v.key[ti] & (uint32_t)(B - 1));
+    const int xf = __shfl(xt, 0, kWave);
+    const int xl = __shfl(xt, nt - 1, kWave);
+    float gx = 0.0f, gy = 0.0f, gm = 0.0f;
+#pragma unroll
+    for (int dr = -1; dr <= 1; ++dr) {
+      const int r = y + dr;
+      if (r < 0 || r >= B) continue;
+      const int* hr = v.head + r * B;
+      const int us = hr[xf > 0 ? xf - 1 : 0];
+      const int ulen = hr[(xl < B - 1 ? xl + 1 : B - 1) + 1] - us;
+      const int a = valid ? hr[xt > 0 ? xt - 1 : 0] - us : 0;
+      const int b = valid ? hr[(xt < B - 1 ? xt + 1 : B - 1) + 1] - us : 0;
+      for (int w0 = 0; w0 < ulen; w0 += kP2PStage) {
+        const int cnt = min(kP2PStage, ulen - w0);
 #pragma unroll
         for (int q = 0; q < kP2PStage / kWave; ++q) {
           const int slot = q * kWave + lane;
-          const int f = base + slot;
-          float4 pv = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-          float2 gv = make_float2(0.0f, 0.0f);
           if (slot < cnt) {
-            const int g = f < n0 ? rs0 + f : (f < n0 + n1 ? rs1 + (f - n0) : rs2 + (f - n0 - n1));
-            pv = v.part[g];
-            gv = v.gsort[g];
+            sp[slot] = v.part[us + w0 + slot];
+            sg[slot] = v.gsort[us + w0 + slot];
           }
-          sp[slot] = pv;
-          sg[slot] = gv;
         }
         wave_lds_order();
-        const int step = 2 * K;
-        const int iters = (cnt + step - 1) / step;
-        const float4* p = sp + k;
-        const float2* q = sg + k;
-        for (int i = 0; i < iters; ++i, p += step, q += step) {
-          const float4 s0 = p[0], s1 = p[K];
-          const float2 g0 = q[0], g1 = q[K];
-          pair_adjoint(pt.x, pt.y, pt.z, gt.x, gt.y, s0.x, s0.y, s0.z, g0.x, g0.y, eps, gx, gy, gm);
-          pair_adjoint(pt.x, pt.y, pt.z, gt.x, gt.y, s1.x, s1.y, s1.z, g1.x, g1.y, eps, gx, gy, gm);
+        const int lo = min(max(a - w0, 0), cnt);
+        const int hi = min(max(b - w0, 0), cnt);
+        for (int k = lo; k < hi; ++k) {
+          const float4 s = sp[k];
+          const float2 g = sg[k];
+          pair_adjoint(pt.x, pt.y, pt.z, gt.x, gt.y, s.x, s.y, s.z, g.x, g.y, eps, gx, gy, gm);
         }
         wave_lds_order();
       }
-      for (int off = G; off < kWave; off <<= 1) {
-        gx += __shfl_xor(gx, off, kWave);
-        gy += __shfl_xor(gy, off, kWave);
-        gm += __shfl_xor(gm, off, kWave);
+    }
+    if (valid) {
+      if (v.L > 0) {
+        // P2M^T: X_c = sum m x / M_c,  M_c = sum m
+        const float4 leafg = leaf_multipole_grad(v, xt, y);
+        const float4 leafm = v.mpole[y * B + xt];
+        const float rr = pt.z / leafm.z;
+        gx = __builtin_fmaf(rr, leafg.x, gx);
+        gy = __builtin_fmaf(rr, leafg.y, gy);
+        gm += leafg.z + ((pt.x - leafm.x) * leafg.x + (pt.y - leafm.y) * leafg.y) / leafm.z;
       }
-      if (k == 0 && valid) {
-        if (v.L > 0) {
-          // P2M^T: X_c = sum m x / M_c,  M_c = sum m
-          const float r = pt.z / leafm.z;
-          gx = __builtin_fmaf(r, leafg.x, gx);
-          gy = __builtin_fmaf(r, leafg.y, gy);
-          gm += leafg.z + ((pt.x - leafm.x) * leafg.x + (pt.y - leafm.y) * leafg.y) / leafm.z;
-        }
-        const uint32_t o = __float_as_uint(pt.w);
-        if (MODE == kAdjSolve) {
-          gpos[o] = make_float2(gx, gy);
-          gw[o] = gm;
-        } else {
-          const float2 lam = gpos[o];
-          const float2 pi = gpre[o];
-          gpos[o] = make_float2(2.0f * lam.x + gx + pi.x, 2.0f * lam.y + gy + pi.y);
-          gpre[o] = make_float2(-lam.x, -lam.y);
-          gw[o] += gm;
-        }
+      const uint32_t o = __float_as_uint(pt.w);
+      if (MODE == kAdjSolve) {
+        gpos[o] = make_float2(gx, gy);
+        gw[o] = gm;
+      } else {
+        const float2 lam = gpos[o];
+        const float2 pi = gpre[o];
+        gpos[o] = make_float2(2.0f * lam.x + gx + pi.x, 2.0f * lam.y + gy + pi.y);
+        gpre[o] = make_float2(-lam.x, -lam.y);
+        gw[o] += gm;
       }
     }
   }

@@ -267,9 +267,6 @@ int nbfmm_solver_create(int num_level, int max_num_particle, const float positio
   ALLOC(v.rowstart, (size_t)s->B + 1);
   ALLOC(v.occ, (size_t)C);
   ALLOC(v.nocc, 1);
-  v.ncap = max_num_particle;
-  ALLOC(v.ord, p2p_class_total(C, v.ncap));
-  ALLOC(v.sched, (size_t)kSchedInts);
   ALLOC(v.hist, (size_t)s->B * ntiles);
   ALLOC(v.dtotal, (size_t)s->B);
   ALLOC(v.mpole, (size_t)ncells_tree);
@@ -948,8 +945,8 @@ int nbfmm_batch_create(int S, const int* num_particle, const int* num_level, con
   b->part_off.assign(S + 1, 0);
   b->hv.resize(S);
   // sizes of the concatenated scratch arrays
-  size_t n_all = 0, head_all = 0, row_all = 0, cell_all = 0, hist_all = 0, tree_all = 0, ord_all = 0;
-  std::vector<size_t> head_off(S), row_off(S), cell_off(S), hist_off(S), tree_off(S), ord_off(S);
+  size_t n_all = 0, head_all = 0, row_all = 0, cell_all = 0, hist_all = 0, tree_all = 0;
+  std::vector<size_t> head_off(S), row_off(S), cell_off(S), hist_off(S), tree_off(S);
   for (int s = 0; s < S; ++s) {
     SysView& v = b->hv[s];
     v = SysView{};
@@ -977,9 +974,7 @@ int nbfmm_batch_create(int S, const int* num_particle, const int* num_level, con
     cell_off[s] = cell_all;
     hist_off[s] = hist_all;
     tree_off[s] = tree_all;
-    ord_off[s] = ord_all;
     n_all += v.n;
-    ord_all += p2p_class_total((int)C, v.n);
     head_all += C + 1;
     row_all += v.B + 1;
     cell_all += C;
@@ -991,7 +986,7 @@ int nbfmm_batch_create(int S, const int* num_particle, const int* num_level, con
   float4 *tpart, *part, *mpole;
   float2 *eff, *local, *gsort, *gup;
   float4* gmp;
-  int *head, *rowstart, *occ, *nocc, *ord, *sched;
+  int *head, *rowstart, *occ, *nocc;
   int rc = NBFMM_OK;
 #define BALLOC(ptr, cnt) \
   if ((rc = batch_alloc(b, &(ptr), (cnt))) != NBFMM_OK) { batch_free(b); delete b; return rc; }
@@ -1004,8 +999,6 @@ int nbfmm_batch_create(int S, const int* num_particle, const int* num_level, con
   BALLOC(rowstart, row_all);
   BALLOC(occ, cell_all);
   BALLOC(nocc, (size_t)S);
-  BALLOC(ord, ord_all);
-  BALLOC(sched, (size_t)S * kSchedInts);
   BALLOC(hist, hist_all);
   BALLOC(dtotal, row_all);
   BALLOC(mpole, tree_all);
@@ -1044,9 +1037,6 @@ int nbfmm_batch_create(int S, const int* num_particle, const int* num_level, con
     v.rowstart = rowstart + row_off[s];
     v.occ = occ + cell_off[s];
     v.nocc = nocc + s;
-    v.ncap = v.n;
-    v.ord = ord + ord_off[s];
-    v.sched = sched + (size_t)s * kSchedInts;
     v.hist = hist + hist_off[s];
     v.dtotal = dtotal + row_off[s];
     v.mpole = mpole + tree_off[s];

@@ -18,35 +18,6 @@ constexpr int kP2PStage = 256;        // adjoint P2P: sources staged in LDS per 
 constexpr int kAdjP2PWaves = 8;       // adjoint P2P: waves per block (48 KiB of LDS)
 constexpr int kAdjP2PBlocksPerCU = 3; // adjoint P2P: resident blocks per CU (LDS bound)
 
-// P2P work distribution.  Occupied leaves are listed by size class (class
-// k holds the leaves with 2^k <= count < 2^(k+1); the last class is open
-// above) and numbered largest class first.  Block b of nblk takes the
-// numbers b, b + nblk, b + 2 nblk, ..., so every block gets the same mix of
-// leaf sizes, and its waves draw from that sequence through an LDS counter:
-// a wave that finishes early takes the next leaf, and the few dense leaves,
-// which come first, never land last on a wave that is already loaded.
-// sched[] of a system holds the class sizes.
-constexpr int kP2PClasses = 12;
-constexpr int kSchedInts = kP2PClasses;
-
-// Class k can hold at most min(C, ncap >> k) leaves (each has >= 2^k
-// particles), so class k's list is that long and the lists together hold
-// at most C + 2 * ncap entries.
-__host__ __device__ inline int p2p_class_cap(int C, int ncap, int k) {
-  const int c = ncap >> k;
-  return c < C ? c : C;
-}
-__host__ __device__ inline int p2p_class_off(int C, int ncap, int k) {
-  int off = 0;
-  for (int i = 0; i < k; ++i) off += p2p_class_cap(C, ncap, i);
-  return off;
-}
-inline size_t p2p_class_total(int C, int ncap) { return (size_t)p2p_class_off(C, ncap, kP2PClasses); }
-__device__ __forceinline__ int p2p_class_of(uint32_t count) {
-  const int k = 31 - __clz((int)count);
-  return k < kP2PClasses - 1 ? k : kP2PClasses - 1;
-}
-
 // P2P output modes.
 enum P2PMode : int {
   kP2POnly = 0,     // sorted effect = P2P sum (stage p2p)
@@ -77,9 +48,6 @@ struct SysView {
   int* rowstart;    // B+1: first sorted slot of each row
   int* occ;         // occupied leaves (row by row)
   int* nocc;        // number of occupied leaves
-  int* ord;         // occupied leaves by size class, class k at p2p_class_off(C, ncap, k)
-  int* sched;       // kSchedInts: size of each class list
-  int ncap;         // particle capacity the class lists are sized for
   uint32_t* hist;   // row-pass tile histograms [B][ntiles]
   uint32_t* dtotal; // row totals [B]
   float4* mpole;    // per level, dense D_l x D_l: COM x, COM y, mass, 0
@@ -185,44 +153,7 @@ __device__ __forceinline__ void kernel_accum(float tx, float ty, float sx, float
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
 
-// The occupied leaves of a system in size-class order, largest first
-// (leaf(t) for t < total).
-struct LeafQueue {
-  int cnt[kP2PClasses];
-  int off[kP2PClasses];
-  int total;
-
-  __device__ __forceinline__ void load(const SysView& v) {
-    const int C = v.B * v.B;
-    total = 0;
-    int o = 0;
-#pragma unroll
-    for (int k = 0; k < kP2PClasses; ++k) {
-      cnt[k] = __builtin_amdgcn_readfirstlane(v.sched[k]);
-      off[k] = o;
-      o += p2p_class_cap(C, v.ncap, k);
-      total += cnt[k];
-    }
-  }
-  __device__ __forceinline__ int leaf(const SysView& v, int t) const {
-    int c = 0;
-    bool found = false;
-#pragma unroll
-    for (int k = kP2PClasses - 1; k >= 0; --k) {
-      if (!found) {
-        if (t < cnt[k]) {
-          c = off[k] + t;
-          found = true;
-        } else {
-          t -= cnt[k];
-        }
-      }
-    }
-    return v.ord[c];
-  }
-};
-
-// Next position in the calling block's share of a LeafQueue: lane 0 bumps
+// Next position in the calling block's share of a work list: lane 0 bumps
 // the block's LDS counter, the whole wave gets the value.
 __device__ __forceinline__ int block_take(int* s_next) {
   int i = 0;
@@ -268,6 +199,35 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s
   __syncthreads();
   *total = tot;
   return wave_off + inc - v;
+}
+
+// Target runs: every row of leaves is cut into runs of 64 consecutive sorted
+// particles (the last run of a row may be shorter).  s_runs[y] is the number
+// of runs in the rows before y; s_runs[B] is the total.  All threads of the
+// block call this.
+template <int NT>
+__device__ __forceinline__ void row_runs(const SysView& v, int* s_runs, uint32_t* sh) {
+  const int B = v.B;
+  uint32_t carry = 0;
+  for (int r0 = 0; r0 < B; r0 += NT) {
+    const int r = r0 + (int)threadIdx.x;
+    const uint32_t cnt = r < B ? (uint32_t)((v.head[(r + 1) * B] - v.head[r * B] + kWave - 1) / kWave) : 0u;
+    uint32_t tot;
+    const uint32_t e = block_exclusive_scan<NT>(cnt, sh, &tot);
+    if (r < B) s_runs[r] = (int)(carry + e);
+    carry += tot;
+  }
+  if (threadIdx.x == 0) s_runs[B] = (int)carry;
+}
+
+// Row of run j (j < total): the last row whose first run is <= j.
+__device__ __forceinline__ int run_row(const int* s_runs, int B, int j) {
+  int lo = 0, hi = B;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (s_runs[mid] <= j) lo = mid; else hi = mid;
+  }
+  return lo;
 }
 
 inline int level_dim(int B, int l) { return B >> l; }

@@ -37,35 +37,6 @@ __device__ __forceinline__ float2 leaf_local(const SysView& v, int cx, int cy) {
 }
 
 
-// Target runs: every row of leaves is cut into runs of 64 consecutive sorted
-// particles (the last run of a row may be shorter).  s_runs[y] is the number
-// of runs in the rows before y; s_runs[B] is the total.  All threads of the
-// block call this.
-template <int NT>
-__device__ __forceinline__ void row_runs(const SysView& v, int* s_runs, uint32_t* sh) {
-  const int B = v.B;
-  uint32_t carry = 0;
-  for (int r0 = 0; r0 < B; r0 += NT) {
-    const int r = r0 + (int)threadIdx.x;
-    const uint32_t cnt = r < B ? (uint32_t)((v.head[(r + 1) * B] - v.head[r * B] + kWave - 1) / kWave) : 0u;
-    uint32_t tot;
-    const uint32_t e = block_exclusive_scan<NT>(cnt, sh, &tot);
-    if (r < B) s_runs[r] = (int)(carry + e);
-    carry += tot;
-  }
-  if (threadIdx.x == 0) s_runs[B] = (int)carry;
-}
-
-// Row of run j (j < total): the last row whose first run is <= j.
-__device__ __forceinline__ int run_row(const int* s_runs, int B, int j) {
-  int lo = 0, hi = B;
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (s_runs[mid] <= j) lo = mid; else hi = mid;
-  }
-  return lo;
-}
-
 // One wavefront works one run of up to 64 targets of row y, one target per
 // lane.  Target t in leaf x_t needs, in each neighbour row r, the sorted
 // range [head[r, x_t - 1], head[r, x_t + 2]) (the row-range idea of

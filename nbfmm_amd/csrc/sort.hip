@@ -110,7 +110,6 @@ __global__ __launch_bounds__(kScanThreads) void k_row_scan(Sys sys) {
     carry = 0;
     if (d == 0) *v.nocc = 0;
   }
-  if (d == 0 && threadIdx.x < kP2PClasses) v.sched[threadIdx.x] = 0;
   __syncthreads();
   for (int t0 = 0; t0 < ntiles; t0 += kScanThreads) {
     const int t = t0 + threadIdx.x;
@@ -228,8 +227,6 @@ __global__ __launch_bounds__(kRowThreads) void k_row_sort(Sys sys, int zero_empt
   __shared__ uint32_t s_wcnt[kRowWaves][512];
   __shared__ uint32_t sh[kRowWaves];
   __shared__ uint32_t s_occ_base;
-  __shared__ uint32_t s_ccnt[kP2PClasses];
-  __shared__ uint32_t s_cbase[kP2PClasses];
   int y, nrows;
   const SysView v = sys.get(y, nrows);
   if (v.cold_if && *v.cold_if == 0) return;
@@ -240,7 +237,6 @@ __global__ __launch_bounds__(kRowThreads) void k_row_sort(Sys sys, int zero_empt
   const uint32_t cmask = (uint32_t)(B - 1);
 
   for (int x = threadIdx.x; x < B; x += kRowThreads) { s_hist[x] = 0; s_run[x] = 0; }
-  if (threadIdx.x < kP2PClasses) s_ccnt[threadIdx.x] = 0;
   __syncthreads();
   for (int p0 = rs + (int)threadIdx.x; p0 < re; p0 += kRowThreads * kSortWaveItems) {
     uint32_t k[kSortWaveItems];
@@ -259,18 +255,11 @@ __global__ __launch_bounds__(kRowThreads) void k_row_sort(Sys sys, int zero_empt
     const int x = threadIdx.x;
     const uint32_t c = x < B ? s_hist[x] : 0u;
     const uint32_t occ = (x < B && c > 0) ? 1u : 0u;
-    // slot in the leaf's size class (the order inside a class is free)
-    const int cls = occ ? p2p_class_of(c) : 0;
-    const uint32_t cslot = occ ? atomicAdd(&s_ccnt[cls], 1u) : 0u;
     uint32_t tot;
     const uint32_t e = block_exclusive_scan<kRowThreads>(c, sh, &tot);
     uint32_t nocc;
     const uint32_t eo = block_exclusive_scan<kRowThreads>(occ, sh, &nocc);
     if (threadIdx.x == 0) s_occ_base = nocc ? atomicAdd((uint32_t*)v.nocc, nocc) : 0u;
-    if (threadIdx.x < kP2PClasses) {
-      const uint32_t m = s_ccnt[threadIdx.x];
-      s_cbase[threadIdx.x] = m ? atomicAdd((uint32_t*)&v.sched[threadIdx.x], m) : 0u;
-    }
     if (x < B) {
       s_start[x] = e;
       v.head[y * B + x] = rs + (int)e;
@@ -278,10 +267,7 @@ __global__ __launch_bounds__(kRowThreads) void k_row_sort(Sys sys, int zero_empt
     }
     if (y == B - 1 && threadIdx.x == 0) v.head[B * B] = re;
     __syncthreads();
-    if (occ) {
-      v.occ[s_occ_base + eo] = y * B + x;
-      v.ord[p2p_class_off(B * B, v.ncap, cls) + s_cbase[cls] + cslot] = y * B + x;
-    }
+    if (occ) v.occ[s_occ_base + eo] = y * B + x;
   }
   const uint64_t below = lanes_below();
   for (int g = rs; g < re; g += kRowGroup) {
@@ -343,40 +329,26 @@ __global__ __launch_bounds__(kRowThreads) void k_row_sort(Sys sys, int zero_empt
   }
 }
 
-// Occupied-leaf lists rebuilt from head alone (after head was written through
-// the buffer API).  Single block; only used by the stage API.  A head that
-// is not a valid partition of ncap particles can overfill a size class: the
-// surplus leaves are dropped rather than written past the class list.
+// Occupied-leaf list rebuilt from head alone (after head was written through
+// the buffer API).  Single block; only used by the stage API.
 __global__ __launch_bounds__(kScanThreads) void k_occ_from_head(SysView v) {
   __shared__ uint32_t sh[kScanThreads / kWave];
   __shared__ uint32_t carry;
-  __shared__ uint32_t s_ccnt[kP2PClasses];
   const int C = v.B * v.B;
   if (threadIdx.x == 0) carry = 0;
-  if (threadIdx.x < kP2PClasses) s_ccnt[threadIdx.x] = 0;
   __syncthreads();
   for (int b0 = 0; b0 < C; b0 += kScanThreads) {
     const int c = b0 + threadIdx.x;
-    const int cnt = c < C ? v.head[c + 1] - v.head[c] : 0;
-    const uint32_t o = cnt > 0 ? 1u : 0u;
+    const uint32_t o = (c < C && v.head[c + 1] > v.head[c]) ? 1u : 0u;
     uint32_t tot;
     const uint32_t e = block_exclusive_scan<kScanThreads>(o, sh, &tot);
     const uint32_t base = carry;
-    if (o) {
-      v.occ[base + e] = c;
-      const int cls = p2p_class_of((uint32_t)cnt);
-      const uint32_t slot = atomicAdd(&s_ccnt[cls], 1u);
-      if ((int)slot < p2p_class_cap(C, v.ncap, cls)) v.ord[p2p_class_off(C, v.ncap, cls) + slot] = c;
-    }
+    if (o) v.occ[base + e] = c;
     __syncthreads();
     if (threadIdx.x == 0) carry = base + tot;
     __syncthreads();
   }
   if (threadIdx.x == 0) *v.nocc = (int)carry;
-  if (threadIdx.x < kP2PClasses) {
-    const int cap = p2p_class_cap(C, v.ncap, threadIdx.x);
-    v.sched[threadIdx.x] = min((int)s_ccnt[threadIdx.x], cap);
-  }
 }
 
 // An empty system: head is all zero and nothing is occupied.
@@ -384,7 +356,6 @@ __global__ __launch_bounds__(256) void k_empty(SysView v) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c <= v.B * v.B) v.head[c] = 0;
   if (c == 0) *v.nocc = 0;
-  if (c < kP2PClasses) v.sched[c] = 0;
 }
 
 // --------------------------------------------------------- warm start
@@ -399,7 +370,7 @@ __global__ __launch_bounds__(256) void k_empty(SysView v) {
 // cold path, bit for bit.
 //   k_warm_gather  gather, bin, mark movers and the leaves they touch
 //   k_warm_rows    new leaf counts, row-local offsets (all rows, one block each)
-//   k_warm_finish  head, occupied-leaf lists, movers' group offsets, list
+//   k_warm_finish  head, occupied-leaf list, movers' group offsets, list
 //                  of touched leaves
 //   k_warm_place   stayers of untouched leaves go straight to their slot;
 //                  movers are grouped by new leaf
@@ -477,10 +448,7 @@ __global__ __launch_bounds__(kRowThreads) void k_warm_rows(Sys sys) {
     if (x < B && v.wtouch[c]) v.wtouch[c] = 0;
     return;
   }
-  if (y == 0) {  // the occupied-leaf counters k_warm_finish fills
-    if (x == 0) *v.nocc = 0;
-    if (x < kP2PClasses) v.sched[x] = 0;
-  }
+  if (y == 0 && x == 0) *v.nocc = 0;  // the occupied-leaf counter k_warm_finish fills
   uint32_t tot, itot;
   const uint32_t e = block_exclusive_scan<kRowThreads>(cnt, sh, &tot);
   const uint32_t ei = block_exclusive_scan<kRowThreads>(cin, sh, &itot);
@@ -523,21 +491,18 @@ __global__ __launch_bounds__(kRowThreads) void k_warm_rowscan(Sys sys) {
 
 // One block per row: the row's start (scanned here from the B row totals,
 // or read from rowstart when k_warm_rowscan ran), then head and the movers'
-// group offsets for the row; appends the row's occupied leaves to occ and to
-// the size-class lists (as the column pass does).  zero_empty: also clear
+// group offsets for the row; appends the row's occupied leaves to occ (as
+// the column pass does).  zero_empty: also clear
 // the level-0 multipole of the row's empty leaves.
 template <class Sys>
 __global__ __launch_bounds__(kRowThreads) void k_warm_finish(Sys sys, int zero_empty, int prescanned) {
   __shared__ uint32_t sh[kRowWaves];
   __shared__ uint32_t s_base, s_ibase, s_occ_base, s_touch_base;
-  __shared__ uint32_t s_ccnt[kP2PClasses];
-  __shared__ uint32_t s_cbase[kP2PClasses];
   int y, nrows;
   const SysView v = sys.get(y, nrows);
   if (v.wstate[kWarmFallback]) return;
   const int B = v.B;
   const int x = threadIdx.x;
-  if (x < kP2PClasses) s_ccnt[x] = 0;
   if (prescanned) {
     if (x == 0) {
       s_base = (uint32_t)v.rowstart[y];
@@ -572,8 +537,6 @@ __global__ __launch_bounds__(kRowThreads) void k_warm_finish(Sys sys, int zero_e
     if (zero_empty && cnt == 0) v.mpole[c] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   }
   const uint32_t occ = cnt > 0 ? 1u : 0u;
-  const int cls = occ ? p2p_class_of(cnt) : 0;
-  const uint32_t cslot = occ ? atomicAdd(&s_ccnt[cls], 1u) : 0u;
   const uint32_t touched = (x < B && v.wtouch[c]) ? 1u : 0u;
   uint32_t nocc, ntouch;
   const uint32_t eo = block_exclusive_scan<kRowThreads>(occ, sh, &nocc);
@@ -582,15 +545,8 @@ __global__ __launch_bounds__(kRowThreads) void k_warm_finish(Sys sys, int zero_e
     s_occ_base = nocc ? atomicAdd((uint32_t*)v.nocc, nocc) : 0u;
     s_touch_base = ntouch ? atomicAdd((uint32_t*)&v.wstate[kWarmTouched], ntouch) : 0u;
   }
-  if (x < kP2PClasses) {
-    const uint32_t m = s_ccnt[x];
-    s_cbase[x] = m ? atomicAdd((uint32_t*)&v.sched[x], m) : 0u;
-  }
   __syncthreads();
-  if (occ) {
-    v.occ[s_occ_base + eo] = c;
-    v.ord[p2p_class_off(B * B, v.ncap, cls) + s_cbase[cls] + cslot] = c;
-  }
+  if (occ) v.occ[s_occ_base + eo] = c;
   if (touched) v.wtlist[s_touch_base + et] = c;
 }
 

@@ -274,10 +274,10 @@ def _leaf_layout(L, counts, seed):
 
 @pytest.mark.parametrize("case", ["one_leaf", "exact_powers", "mixed_classes"])
 def test_leaf_size_classes(case):
-    """The P2P schedule lists occupied leaves by size class (2^k <= count <
-    2^(k+1), the last class open above), each class sized for its largest
-    possible population n >> k.  Layouts that fill a class to that bound,
-    or put everything into the open top class, match the oracle."""
+    """Extreme leaf populations: everything in one leaf (runs of 64 inside a
+    single leaf, a long P2M sum), every occupied leaf holding exactly 2^k
+    particles, and a mix of leaf sizes from 1 to 2^11 scattered over the
+    grid.  All match the oracle."""
     from nbfmm_amd import Solver
     L = 4
     B = 1 << (L + 1)
