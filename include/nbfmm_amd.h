@@ -132,6 +132,15 @@ int nbfmm_update(int num_star, float tick, float* d_position_cur, float* d_posit
 int nbfmm_step(nbfmm_solver* solver, int num_particle, float tick, float* d_position_cur,
                float* d_position_pre, const float* d_weight, float* d_acceleration, void* stream);
 
+/* nbfmm::Stars::prune (source/nbfmm/display/stars/prune.cu:28-62): stars with
+ * x < x_min, y < y_min, x > x_max or y > y_max (the solver's position limits;
+ * NaN stays) are moved behind the others; both groups keep their order, and
+ * d_position_cur, d_position_pre and d_weight are permuted in place.  The
+ * number kept (the new num_star) is written to *num_kept.  Synchronises the
+ * stream to return it, as the reference's thrust::find does. */
+int nbfmm_prune(nbfmm_solver* solver, int num_star, float* d_position_cur, float* d_position_pre,
+                float* d_weight, int* num_kept, void* stream);
+
 /* num_steps calls of nbfmm_step.  use_graph = 1 captures one step into a
  * hipGraph on the first call and replays it (stream must not be NULL). */
 int nbfmm_run(nbfmm_solver* solver, int num_particle, float tick, int num_steps, float* d_position_cur,

@@ -108,6 +108,20 @@ class Solver {
                      (float*)acceleration, stream_));
   }
 
+  // Stars::prune (prune.cu:28-62) with this solver's limits: returns the new star count
+  int prune(int num_star, float2* position_cur, float2* position_pre, float* weight) {
+    int kept = 0;
+    check(nbfmm_prune(h_, num_star, (float*)position_cur, (float*)position_pre, weight, &kept, stream_));
+    return kept;
+  }
+
+  // Stars::update (update.cu:47-55): solve, Verlet update, prune; returns the new star count
+  int update_stars(int num_star, float tick, float2* position_cur, float2* position_pre, float* weight,
+                   float2* acceleration) {
+    step(num_star, tick, position_cur, position_pre, weight, acceleration);
+    return prune(num_star, position_cur, position_pre, weight);
+  }
+
   unsigned flags(bool clear = false) {
     unsigned f = 0;
     check(nbfmm_solver_flags(h_, &f, clear ? 1 : 0));

@@ -54,6 +54,7 @@ SIGNATURES = {
     "nbfmm_p2p_variant_info": [c_int, ctypes.POINTER(c_int), ctypes.POINTER(c_int), ctypes.POINTER(c_int)],
     "nbfmm_set_p2p_config": [P, c_int, c_int],
     "nbfmm_set_warm_start": [P, c_int],
+    "nbfmm_prune": [P, c_int, P, P, P, ctypes.POINTER(c_int), P],
     "nbfmm_get_p2p_config": [P, ctypes.POINTER(c_int), ctypes.POINTER(c_int)],
     "nbfmm_autotune": [P, c_int, P, P, c_int, ctypes.POINTER(c_int), ctypes.POINTER(c_int),
                        ctypes.POINTER(c_float), P],

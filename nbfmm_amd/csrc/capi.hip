@@ -58,6 +58,7 @@ struct nbfmm_solver {
   double p2p_ms = 0.0;
   int p2p_launches = 0;
   unsigned long long* d_pairs = nullptr;
+  int* d_kept = nullptr;  // star count after a prune
   // warm-started sort: key/part/head hold a stable sort of warm_n particles
   // (-1: none) that the next sort of the same n starts from
   bool warm_enabled = true;
@@ -276,6 +277,7 @@ int nbfmm_solver_create(int num_level, int max_num_particle, const float positio
   ALLOC(v.gmp, (size_t)ncells_tree);
   ALLOC(v.flags, 1);
   ALLOC(s->d_pairs, 1);
+  ALLOC(s->d_kept, 1);
   ALLOC(s->xs, N);
   ALLOC(v.wstate, (size_t)kWarmInts);
   ALLOC(v.wrel, N);
@@ -440,6 +442,24 @@ int nbfmm_buffer_write(nbfmm_solver* s, int id, int count, const void* src, void
     drop_graph(s);
   }
   return buffer_io(s, id, count, const_cast<void*>(src), true, as_stream(stream));
+}
+
+int nbfmm_prune(nbfmm_solver* s, int n, float* xcur, float* xpre, float* w, int* num_kept, void* stream) {
+  int rc = check_n(s, n);
+  if (rc) return rc;
+  if (!num_kept) return fail(NBFMM_ERR_INVALID_ARGUMENT, "num_kept is null");
+  if (n > 0 && (!xcur || !xpre || !w)) return fail(NBFMM_ERR_INVALID_ARGUMENT, "null array");
+  *num_kept = 0;
+  if (n == 0) return NBFMM_OK;
+  hipStream_t st = as_stream(stream);
+  const SysView& v = s->v;
+  const float4 lim = make_float4(s->limits[0], s->limits[1], s->limits[2], s->limits[3]);
+  NB_HIP(launch_prune(n, lim, (float2*)xcur, (float2*)xpre, w, v.hist, v.tpart, (float*)v.gsort, s->d_kept, st));
+  int kept = 0;
+  NB_HIP(hipMemcpyAsync(&kept, s->d_kept, sizeof(int), hipMemcpyDeviceToHost, st));
+  NB_HIP(hipStreamSynchronize(st));
+  *num_kept = kept;
+  return NBFMM_OK;
 }
 
 int nbfmm_set_warm_start(nbfmm_solver* s, int enable) {

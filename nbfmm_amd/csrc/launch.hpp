@@ -83,4 +83,10 @@ hipError_t launch_part_read(const SysView& v, int which, int count, void* dst, h
 hipError_t launch_part_write(const SysView& v, int which, int count, const void* src, hipStream_t s);
 hipError_t launch_pair_count(const SysView& v, unsigned long long* out, hipStream_t s);
 
+// prune.hip: Stars::prune on (xcur, xpre, w), n > 0; tcount needs prune_tiles(n)
+// entries, tmp and tmpw n each; *kept (device) receives the new count
+hipError_t launch_prune(int n, float4 lim, float2* xcur, float2* xpre, float* w, uint32_t* tcount, float4* tmp,
+                        float* tmpw, int* kept, hipStream_t s);
+int prune_tiles(int n);
+
 }  // namespace nbfmm_amd

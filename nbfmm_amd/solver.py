@@ -198,6 +198,21 @@ class Solver:
                                               _ptr(grad_position0), _ptr(grad_position_pre0), _ptr(grad_weight),
                                               _ptr(work), _stream(stream)))
 
+    def prune(self, num_star, position_cur, position_pre, weight, stream=None) -> int:
+        """Stars::prune (prune.cu:28-62): stars outside the position limits move
+        behind the others (both groups keep their order, arrays permuted in
+        place); returns the number kept.  Synchronises."""
+        kept = ctypes.c_int()
+        check(lib().nbfmm_prune(self._h, int(num_star), _ptr(position_cur), _ptr(position_pre), _ptr(weight),
+                                ctypes.byref(kept), _stream(stream)))
+        return kept.value
+
+    def update_stars(self, num_star, tick, position_cur, position_pre, weight, acceleration, stream=None) -> int:
+        """Stars::update (update.cu:47-55): solve, Verlet update, prune;
+        returns the new star count."""
+        self.step(num_star, tick, position_cur, position_pre, weight, acceleration, stream=stream)
+        return self.prune(num_star, position_cur, position_pre, weight, stream=stream)
+
     def warm_start(self, enable: bool = True) -> None:
         """Start each sort from the previous one of the same N (default on);
         results are identical either way."""

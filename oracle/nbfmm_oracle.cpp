@@ -337,11 +337,46 @@ void update(int n, T tick, T* xcur, T* xpre, const T* acc) {
   }
 }
 
+// source/nbfmm/display/stars/prune.cu:28-62 (pruneDevice + Stars::prune):
+// a star is eliminated when x < x_min || y < y_min || x > x_max || y > y_max
+// (a NaN compares false and stays); thrust::sort_by_key on the bool flags
+// moves (x_cur, x_pre, weight) so that the kept stars come first, both groups
+// in their previous order, and the new count is the number kept.
+// Pinned by: numpy argsort(flags, kind="stable") (the order a radix sort of
+// bool keys gives) and hand-built cases on the box edges.
+int prune(int n, const float limits[4], float* xcur, float* xpre, float* w) {
+  std::vector<float> xc(xcur, xcur + 2 * n), xp(xpre, xpre + 2 * n), ww(w, w + n);
+  std::vector<int> order;
+  order.reserve(n);
+  std::vector<char> gone(n);
+  for (int i = 0; i < n; ++i) {
+    const float x = xc[2 * i], y = xc[2 * i + 1];
+    gone[i] = (x < limits[0] || y < limits[1] || x > limits[2] || y > limits[3]) ? 1 : 0;
+    if (!gone[i]) order.push_back(i);
+  }
+  const int kept = (int)order.size();
+  for (int i = 0; i < n; ++i)
+    if (gone[i]) order.push_back(i);
+  for (int j = 0; j < n; ++j) {
+    const int i = order[j];
+    xcur[2 * j] = xc[2 * i];
+    xcur[2 * j + 1] = xc[2 * i + 1];
+    xpre[2 * j] = xp[2 * i];
+    xpre[2 * j + 1] = xp[2 * i + 1];
+    w[j] = ww[i];
+  }
+  return kept;
+}
+
 }  // namespace
 
 extern "C" {
 
 int orc_version(void) { return 1; }
+
+int orc_prune_f32(int n, const float limits[4], float* xcur, float* xpre, float* w) {
+  return prune(n, limits, xcur, xpre, w);
+}
 
 int orc_max_threads(void) {
 #ifdef _OPENMP

@@ -62,6 +62,8 @@ def lib():
         L.orc_run_f64.argtypes = [c_int, c_int, _f64p, c_double, c_double, c_int, _f64p, _f64p, _f64p, _f64p]
         L.orc_run_f64.restype = c_int
         L.orc_direct_f64.argtypes = [c_int, _f32p, _f32p, c_double, c_int, ctypes.c_void_p, _f64p]
+        L.orc_prune_f32.argtypes = [c_int, _f32p, _f32p, _f32p, _f32p]
+        L.orc_prune_f32.restype = c_int
         _lib = L
     return _lib
 
@@ -221,3 +223,13 @@ def direct_f64(pos, w, eps, targets=None):
         out = np.zeros((t.shape[0], 2), np.float64)
         lib().orc_direct_f64(n, pos, _f32(w), float(eps), t.shape[0], t.ctypes.data_as(ctypes.c_void_p), out)
     return out
+
+
+def prune(limits, xcur, xpre, w):
+    """Stars::prune: (xcur, xpre, w) with the kept stars first (both groups in
+    their previous order) and the number kept."""
+    xcur = _f32(xcur).copy()
+    xpre = _f32(xpre).copy()
+    w = _f32(w).copy()
+    kept = lib().orc_prune_f32(w.shape[0], _f32(limits), xcur, xpre, w)
+    return xcur, xpre, w, kept

@@ -356,6 +356,51 @@ void test_update(nbfmm::Solver&, const Fixture& f) {
   }
 }
 
+// Stars::prune (prune.cu:28-62): every third star is moved outside the box
+// on one side or another (one lands exactly on x_max and stays); the kept
+// ones come first, both groups in their previous order.
+void test_prune(nbfmm::Solver& s, const Fixture& f) {
+  const int N = Fixture::N;
+  std::vector<float2> cur(f.position), pre(N);
+  std::vector<float> w(f.weight);
+  for (int i = 0; i < N; ++i) {
+    pre[i] = make_float2(-f.position[i].y, f.position[i].x);
+    if (i % 3 == 0) {
+      const int side = (i / 3) % 5;
+      if (side == 0) cur[i].x = f.limits.x - 0.5f;
+      if (side == 1) cur[i].y = f.limits.y - 0.5f;
+      if (side == 2) cur[i].x = f.limits.z + 0.5f;
+      if (side == 3) cur[i].y = f.limits.w + 0.5f;
+      if (side == 4) cur[i].x = f.limits.z;  // on the edge: kept
+    }
+  }
+  std::vector<int> order;
+  for (int pass = 0; pass < 2; ++pass)
+    for (int i = 0; i < N; ++i) {
+      const bool out = cur[i].x < f.limits.x || cur[i].y < f.limits.y || cur[i].x > f.limits.z || cur[i].y > f.limits.w;
+      if (out == (pass == 1)) order.push_back(i);
+    }
+  int expect_kept = 0;
+  for (int i = 0; i < N; ++i)
+    expect_kept += !(cur[i].x < f.limits.x || cur[i].y < f.limits.y || cur[i].x > f.limits.z || cur[i].y > f.limits.w);
+  DeviceArray<float2> d_cur(N), d_pre(N);
+  DeviceArray<float> d_w(N);
+  d_cur.put(cur);
+  d_pre.put(pre);
+  d_w.put(w);
+  const int kept = s.prune(N, d_cur.p, d_pre.p, d_w.p);
+  EXPECT(kept == expect_kept);
+  const auto C = d_cur.get();
+  const auto P = d_pre.get();
+  const auto W = d_w.get();
+  for (int j = 0; j < N; ++j) {
+    const int i = order[j];
+    EXPECT(C[j].x == cur[i].x && C[j].y == cur[i].y);
+    EXPECT(P[j].x == pre[i].x && P[j].y == pre[i].y);
+    EXPECT(W[j] == w[i]);
+  }
+}
+
 }  // namespace
 
 int main() {
@@ -371,7 +416,7 @@ int main() {
     void (*fn)(nbfmm::Solver&, const Fixture&);
   } tests[] = {{"predo", test_predo}, {"postdo", test_postdo}, {"p2p", test_p2p}, {"p2m", test_p2m},
                {"m2m", test_m2m},     {"m2l", test_m2l},       {"l2l", test_l2l}, {"l2p", test_l2p},
-               {"update", test_update}};
+               {"update", test_update}, {"prune", test_prune}};
   for (auto& t : tests) {
     const int before = g_failures;
     t.fn(solver, f);

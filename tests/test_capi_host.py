@@ -71,3 +71,5 @@ def test_null_arguments_rejected(lib):
     assert lib.nbfmm_update(-1, 0.1, None, None, None, None) == 1
     assert lib.nbfmm_update(5, 0.1, None, None, None, None) == 1
     assert lib.nbfmm_solver_create(3, 10, None, 0.1, None) == 1
+    kept = ctypes.c_int(7)
+    assert lib.nbfmm_prune(None, 10, None, None, None, ctypes.byref(kept), None) == 1

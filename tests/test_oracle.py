@@ -280,3 +280,38 @@ def test_verlet_constant_acceleration_closed_form():
     n = 50
     exact = x0 + n * v * dt + 0.5 * (n * dt) ** 2 * a
     np.testing.assert_allclose(xcur, exact, rtol=0, atol=1e-13)
+
+
+def test_prune_is_stable_bool_partition():
+    """Stars::prune (prune.cu:28-62): flags x < x_min | y < y_min | x > x_max |
+    y > y_max; kept stars first, both groups in their previous order (what a
+    radix sort of the bool keys gives: numpy's stable argsort)."""
+    rng = np.random.default_rng(31)
+    n = 5000
+    lim = np.array([-1.0, -2.0, 3.0, 1.5], np.float32)
+    xc = rng.uniform(-2, 4, (n, 2)).astype(np.float32)
+    xp = rng.standard_normal((n, 2)).astype(np.float32)
+    w = rng.random(n).astype(np.float32)
+    # the box edges themselves stay; NaN compares false and stays
+    xc[0] = (3.0, 1.5)
+    xc[1] = (-1.0, -2.0)
+    xc[2] = (np.nextafter(np.float32(3.0), np.float32(4.0)), 0.0)
+    xc[3] = (0.0, np.nextafter(np.float32(-2.0), np.float32(-3.0)))
+    xc[4] = (np.nan, 0.0)
+    flags = (xc[:, 0] < lim[0]) | (xc[:, 1] < lim[1]) | (xc[:, 0] > lim[2]) | (xc[:, 1] > lim[3])
+    order = np.argsort(flags, kind="stable")
+    c2, p2, w2, kept = orc.prune(lim, xc, xp, w)
+    assert kept == int((~flags).sum())
+    assert flags[2] and flags[3] and not flags[0] and not flags[1] and not flags[4]
+    np.testing.assert_array_equal(c2, xc[order])
+    np.testing.assert_array_equal(p2, xp[order])
+    np.testing.assert_array_equal(w2, w[order])
+
+
+def test_prune_nothing_and_everything():
+    lim = np.array([0, 0, 1, 1], np.float32)
+    xc = np.array([[0.5, 0.5], [0.1, 0.9]], np.float32)
+    c2, _, _, kept = orc.prune(lim, xc, xc, np.ones(2, np.float32))
+    assert kept == 2 and np.array_equal(c2, xc)
+    c2, _, _, kept = orc.prune(lim, xc + 5, xc, np.ones(2, np.float32))
+    assert kept == 0 and np.array_equal(c2, xc + 5)
