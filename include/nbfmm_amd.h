@@ -167,6 +167,39 @@ int nbfmm_pair_count(nbfmm_solver* solver, int num_particle, double* pairs, int*
  * cold sort every call. */
 int nbfmm_set_warm_start(nbfmm_solver* solver, int enable);
 
+/* ---------------------------------------------------------------- models
+ * Initial conditions of the reference's star models (the .cu files under source/nbfmm/model/,
+ * selected by name in main/main.cu:78-222), written into device arrays
+ * (position_cur, position_pre: float2[n]; weight: float[n]).  Uniform draws
+ * come from a counter-based hash of (seed, particle, draw) in (0, 1] instead
+ * of curand_init(0, i, 0) streams, so a model is reproducible across
+ * devices; the geometry, the chord-angle position_pre of the rotating models
+ * and the orbit offsets of the double disks follow the reference.  eps is
+ * the kernel function's softening (the reference's kernelFunction uses 0.1);
+ * tick already includes sqrt(G) as Stars passes it. */
+int nbfmm_model_disk(int num_particle, float center_x, float center_y, float radius, float weight, float tick,
+                     float eps, unsigned seed, float* d_position_cur, float* d_position_pre, float* d_weight,
+                     void* stream);  /* model/disk.cu:27-81 */
+int nbfmm_model_disk_static(int num_particle, float center_x, float center_y, float radius, float weight,
+                            unsigned seed, float* d_position_cur, float* d_position_pre, float* d_weight,
+                            void* stream);  /* model/disk_static.cu */
+int nbfmm_model_disk_center(int num_particle, float center_x, float center_y, float radius, float weight,
+                            float center_weight, float tick, float eps, unsigned seed, float* d_position_cur,
+                            float* d_position_pre, float* d_weight, void* stream);  /* model/disk_center.cu */
+/* uniform_angles = 0: model/circle_center.cu; 1: model/circle_uniform_center.cu */
+int nbfmm_model_circle_center(int num_particle, float center_x, float center_y, float radius, float weight,
+                              float center_weight, float tick, float eps, int uniform_angles, unsigned seed,
+                              float* d_position_cur, float* d_position_pre, float* d_weight, void* stream);
+int nbfmm_model_rectangle(int num_particle, float center_x, float center_y, float width, float height,
+                          float max_weight, unsigned seed, float* d_position_cur, float* d_position_pre,
+                          float* d_weight, void* stream);  /* model/rectangle.cu */
+/* centered = 0: model/double_disk.cu (center weights unused); 1: model/double_disk_center.cu.
+ * Disk 1 fills entries [0, n1), disk 2 [n1, n1 + n2). */
+int nbfmm_model_double_disk(int num_particle1, int num_particle2, const float center1[2], const float center2[2],
+                            float radius1, float radius2, float weight, float center_weight1, float center_weight2,
+                            float eccentricity, float tick, float eps, int centered, unsigned seed,
+                            float* d_position_cur, float* d_position_pre, float* d_weight, void* stream);
+
 /* ---------------------------------------------------------------- tuning
  * The P2P kernel comes in variants (LDS staging size, unroll, waves per
  * block) and runs on a grid of blocks_per_cu blocks per compute unit.  Every configuration gives

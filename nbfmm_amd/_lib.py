@@ -55,6 +55,15 @@ SIGNATURES = {
     "nbfmm_set_p2p_config": [P, c_int, c_int],
     "nbfmm_set_warm_start": [P, c_int],
     "nbfmm_prune": [P, c_int, P, P, P, ctypes.POINTER(c_int), P],
+    "nbfmm_model_disk": [c_int, c_float, c_float, c_float, c_float, c_float, c_float, ctypes.c_uint, P, P, P, P],
+    "nbfmm_model_disk_static": [c_int, c_float, c_float, c_float, c_float, ctypes.c_uint, P, P, P, P],
+    "nbfmm_model_disk_center": [c_int, c_float, c_float, c_float, c_float, c_float, c_float, c_float, ctypes.c_uint,
+                                P, P, P, P],
+    "nbfmm_model_circle_center": [c_int, c_float, c_float, c_float, c_float, c_float, c_float, c_float, c_int,
+                                  ctypes.c_uint, P, P, P, P],
+    "nbfmm_model_rectangle": [c_int, c_float, c_float, c_float, c_float, c_float, ctypes.c_uint, P, P, P, P],
+    "nbfmm_model_double_disk": [c_int, c_int, P, P, c_float, c_float, c_float, c_float, c_float, c_float, c_float,
+                                c_float, c_int, ctypes.c_uint, P, P, P, P],
     "nbfmm_get_p2p_config": [P, ctypes.POINTER(c_int), ctypes.POINTER(c_int)],
     "nbfmm_autotune": [P, c_int, P, P, c_int, ctypes.POINTER(c_int), ctypes.POINTER(c_int),
                        ctypes.POINTER(c_float), P],

@@ -30,7 +30,7 @@ FLAGS = [
     "-I", os.path.join(ROOT, "include"),
 ]
 
-SOURCES = ["sort.hip", "tree.hip", "p2p.hip", "adjoint.hip", "prune.hip", "capi.hip"]
+SOURCES = ["sort.hip", "tree.hip", "p2p.hip", "adjoint.hip", "prune.hip", "model.hip", "capi.hip"]
 
 
 def _compile(src: str) -> str:

@@ -368,11 +368,130 @@ int prune(int n, const float limits[4], float* xcur, float* xpre, float* w) {
   return kept;
 }
 
+// ---- initial-condition models (source/nbfmm/model/*.cu) ----
+// Uniform draws: h = lowbias32(lowbias32(seed ^ 0x9e3779b9) ^ (4 i + k))
+// (lowbias32: C. Wellons, hash-prospector) -> ((h >> 8) + 1) / 2^24 in
+// (0, 1], the k-th draw of particle i (the reference uses curand_init(0, i, 0)
+// streams).
+// Pinned by: the closed-form orbit checks in tests/test_oracle.py (a star on
+// its circle, one Verlet step under the central pull stays on it; the
+// double-disk offsets are the circular-orbit step perpendicular to the pull).
+uint32_t lowbias32(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x7feb352du;
+  x ^= x >> 15;
+  x *= 0x846ca68bu;
+  x ^= x >> 16;
+  return x;
+}
+
+float model_uniform(uint32_t seed, int i, int k) {
+  const uint32_t h = lowbias32(lowbias32(seed ^ 0x9e3779b9u) ^ (uint32_t)(4 * i + k));
+  return (float)((h >> 8) + 1u) * (1.0f / 16777216.0f);
+}
+
+// kernelFunction(origin, (r, 0), m).x with IEEE sqrt and division
+float radial_effect(float r, float m, float eps) {
+  const float dx = r;
+  const float rr = std::sqrt(dx * dx);
+  const float tmp = m / (rr * rr * rr + eps);
+  return dx * tmp;
+}
+
+// disk.cu:49-50
+float chord_angle(float a, float r, float tick) {
+  const float q = a * tick * tick / r / 2.0f;
+  return (float)std::acos(1.0 - (double)q);
+}
+
+void place(float cx, float cy, float r, float ang, float* p) {
+  p[0] = cx + r * std::cos(ang);
+  p[1] = cy + r * std::sin(ang);
+}
+
+constexpr double kTwoPi = 2.0 * 3.14159265358979323846;
+
+void model_disk(int n, float cx, float cy, float R, float weight, float cw, int centered, int rotate, float tick,
+                float eps, uint32_t seed, float* xcur, float* xpre, float* w) {
+  for (int i = 0; i < n; ++i) {
+    if (centered && i == 0) {
+      xcur[0] = xpre[0] = cx;
+      xcur[1] = xpre[1] = cy;
+      w[0] = cw;
+      continue;
+    }
+    const float r = ((float)(i + 1) / (float)n) * R;
+    float dang = 0.0f;
+    if (rotate) {
+      const float m = centered ? weight * (float)(i - 1) + cw : weight * (float)i;
+      dang = chord_angle(radial_effect(r, m, eps), r, tick);
+    }
+    const float ang = (float)(kTwoPi * (double)model_uniform(seed, i, 0));
+    place(cx, cy, r, ang, xcur + 2 * i);
+    place(cx, cy, r, ang - dang, xpre + 2 * i);
+    w[i] = weight;
+  }
+}
+
 }  // namespace
 
 extern "C" {
 
 int orc_version(void) { return 1; }
+
+float orc_model_uniform(unsigned seed, int i, int k) { return model_uniform(seed, i, k); }
+
+// model/disk.cu (centered 0, rotate 1), disk_center.cu (1, 1), disk_static.cu (0, 0)
+void orc_model_disk(int n, float cx, float cy, float R, float weight, float cw, int centered, int rotate, float tick,
+                    float eps, unsigned seed, float* xcur, float* xpre, float* w) {
+  model_disk(n, cx, cy, R, weight, cw, centered, rotate, tick, eps, seed, xcur, xpre, w);
+}
+
+// model/circle_center.cu (uniform 0), circle_uniform_center.cu (uniform 1)
+void orc_model_circle(int n, float cx, float cy, float R, float weight, float cw, float tick, float eps, int uniform,
+                      unsigned seed, float* xcur, float* xpre, float* w) {
+  const float dang = chord_angle(radial_effect(R, cw, eps), R, tick);
+  for (int i = 0; i < n; ++i) {
+    if (i == 0) {
+      xcur[0] = xpre[0] = cx;
+      xcur[1] = xpre[1] = cy;
+      w[0] = cw;
+      continue;
+    }
+    const float ang = uniform ? (float)((kTwoPi * (double)i) / (double)n) : (float)(kTwoPi * (double)model_uniform(seed, i, 0));
+    place(cx, cy, R, ang, xcur + 2 * i);
+    place(cx, cy, R, ang - dang, xpre + 2 * i);
+    w[i] = weight;
+  }
+}
+
+// model/rectangle.cu
+void orc_model_rectangle(int n, float cx, float cy, float width, float height, float max_weight, unsigned seed,
+                         float* xcur, float* xpre, float* w) {
+  for (int i = 0; i < n; ++i) {
+    xcur[2 * i] = xpre[2 * i] = cx + (model_uniform(seed, i, 0) - 0.5f) * width;
+    xcur[2 * i + 1] = xpre[2 * i + 1] = cy + (model_uniform(seed, i, 1) - 0.5f) * height;
+    w[i] = max_weight * model_uniform(seed, i, 2);
+  }
+}
+
+// model/double_disk.cu:55-86 and double_disk_center.cu: the offsets added to
+// the two disks' previous positions (o[0..1] disk 1, o[2..3] disk 2).
+void orc_double_disk_offsets(float c1x, float c1y, float c2x, float c2y, float m1, float m2, float q1, float q2,
+                             float ecc, float tick, float eps, float* o) {
+  const float dx = c2x - c1x, dy = c2y - c1y;
+  const float r = std::sqrt(dx * dx + dy * dy);
+  const float t1 = m2 / (r * r * r + eps), t2 = m1 / (r * r * r + eps);
+  const float e1x = dx * t1, e1y = dy * t1, e2x = -dx * t2, e2y = -dy * t2;
+  const float a1 = std::sqrt(e1x * e1x + e1y * e1y), a2 = std::sqrt(e2x * e2x + e2y * e2y);
+  const float r1 = r * q2 / (q1 + q2), r2 = r * q1 / (q1 + q2);
+  const float s1 = std::sqrt(r1 / a1) * tick / std::exp2(ecc);
+  const float s2 = std::sqrt(r2 / a2) * tick / std::exp2(ecc);
+  o[0] = -e1y * s1 - e1x * tick * tick * ecc;
+  o[1] = e1x * s1 - e1y * tick * tick * ecc;
+  o[2] = -e2y * s2 - e2x * tick * tick * ecc;
+  o[3] = e2x * s2 - e2y * tick * tick * ecc;
+}
 
 int orc_prune_f32(int n, const float limits[4], float* xcur, float* xpre, float* w) {
   return prune(n, limits, xcur, xpre, w);

@@ -64,6 +64,16 @@ def lib():
         L.orc_direct_f64.argtypes = [c_int, _f32p, _f32p, c_double, c_int, ctypes.c_void_p, _f64p]
         L.orc_prune_f32.argtypes = [c_int, _f32p, _f32p, _f32p, _f32p]
         L.orc_prune_f32.restype = c_int
+        c_uint = ctypes.c_uint
+        L.orc_model_uniform.argtypes = [c_uint, c_int, c_int]
+        L.orc_model_uniform.restype = c_float
+        L.orc_model_disk.argtypes = [c_int, c_float, c_float, c_float, c_float, c_float, c_int, c_int, c_float,
+                                     c_float, c_uint, _f32p, _f32p, _f32p]
+        L.orc_model_circle.argtypes = [c_int, c_float, c_float, c_float, c_float, c_float, c_float, c_float, c_int,
+                                       c_uint, _f32p, _f32p, _f32p]
+        L.orc_model_rectangle.argtypes = [c_int, c_float, c_float, c_float, c_float, c_float, c_uint, _f32p, _f32p,
+                                          _f32p]
+        L.orc_double_disk_offsets.argtypes = [c_float] * 11 + [_f32p]
         _lib = L
     return _lib
 
@@ -233,3 +243,56 @@ def prune(limits, xcur, xpre, w):
     w = _f32(w).copy()
     kept = lib().orc_prune_f32(w.shape[0], _f32(limits), xcur, xpre, w)
     return xcur, xpre, w, kept
+
+
+# ---- initial-condition models (source/nbfmm/model/*.cu) ----
+
+def model_uniform(seed, i, k):
+    return float(lib().orc_model_uniform(int(seed), int(i), int(k)))
+
+
+def _model_out(n):
+    return np.zeros((n, 2), np.float32), np.zeros((n, 2), np.float32), np.zeros(n, np.float32)
+
+
+def model_disk(n, center, radius, weight, tick, eps=0.1, seed=0, center_weight=0.0, centered=False, rotate=True):
+    """generateDisk / generateDiskCenter (centered) / generateDiskStatic (rotate=False)."""
+    xc, xp, w = _model_out(n)
+    lib().orc_model_disk(n, float(center[0]), float(center[1]), float(radius), float(weight), float(center_weight),
+                         int(centered), int(rotate), float(tick), float(eps), int(seed), xc, xp, w)
+    return xc, xp, w
+
+
+def model_circle(n, center, radius, weight, center_weight, tick, eps=0.1, uniform=False, seed=0):
+    """generateCircleCenter / generateCircleUniformCenter (uniform)."""
+    xc, xp, w = _model_out(n)
+    lib().orc_model_circle(n, float(center[0]), float(center[1]), float(radius), float(weight), float(center_weight),
+                           float(tick), float(eps), int(uniform), int(seed), xc, xp, w)
+    return xc, xp, w
+
+
+def model_rectangle(n, center, width, height, max_weight, seed=0):
+    xc, xp, w = _model_out(n)
+    lib().orc_model_rectangle(n, float(center[0]), float(center[1]), float(width), float(height), float(max_weight),
+                              int(seed), xc, xp, w)
+    return xc, xp, w
+
+
+def double_disk_offsets(c1, c2, m1, m2, q1, q2, ecc, tick, eps=0.1):
+    o = np.zeros(4, np.float32)
+    lib().orc_double_disk_offsets(float(c1[0]), float(c1[1]), float(c2[0]), float(c2[1]), float(m1), float(m2),
+                                  float(q1), float(q2), float(ecc), float(tick), float(eps), o)
+    return o[:2].copy(), o[2:].copy()
+
+
+def model_double_disk(n1, n2, c1, c2, r1, r2, weight, ecc, tick, eps=0.1, seed=0, centered=False, cw1=0.0, cw2=0.0):
+    """generateDoubleDisk / generateDoubleDiskCenter: disk 1 then disk 2."""
+    a = model_disk(n1, c1, r1, weight, tick, eps, seed, cw1, centered)
+    b = model_disk(n2, c2, r2, weight, tick, eps, seed, cw2, centered)
+    f = np.float32
+    m1 = f(weight) * f(n1 - 1) + f(cw1) if centered else f(weight) * f(n1)
+    m2 = f(weight) * f(n2 - 1) + f(cw2) if centered else f(weight) * f(n2)
+    q1, q2 = (m1, m2) if centered else (f(n1), f(n2))
+    o1, o2 = double_disk_offsets(c1, c2, m1, m2, q1, q2, ecc, tick, eps)
+    xp = np.concatenate([a[1] + o1, b[1] + o2]).astype(np.float32)
+    return np.concatenate([a[0], b[0]]), xp, np.concatenate([a[2], b[2]])

@@ -315,3 +315,84 @@ def test_prune_nothing_and_everything():
     assert kept == 2 and np.array_equal(c2, xc)
     c2, _, _, kept = orc.prune(lim, xc + 5, xc, np.ones(2, np.float32))
     assert kept == 0 and np.array_equal(c2, xc + 5)
+
+
+# ---- initial-condition models (source/nbfmm/model/*.cu) ----
+
+def test_model_uniform_draws():
+    u = np.array([orc.model_uniform(s, i, k) for s in (0, 5) for i in range(2000) for k in range(3)])
+    assert u.min() > 0.0 and u.max() <= 1.0
+    assert abs(u.mean() - 0.5) < 0.01 and abs(u.var() - 1 / 12) < 0.005
+    assert len(np.unique(u)) > 0.99 * u.size
+
+
+@pytest.mark.parametrize("centered", [False, True])
+def test_model_disk_first_step_stays_on_circle(centered):
+    """disk.cu:45-55: radius (i+1)/n * R, previous position rotated back by
+    the chord angle acos(1 - a tau^2 / 2r).  That angle makes the first Verlet
+    step under the central pull a (kernel of the enclosed mass) land back on
+    the circle exactly: |2x - x_prev - a tau^2 x/r| = r."""
+    n, R, wt, tau, eps, cw = 3000, 2.0, 0.01, 0.02, 0.1, 5.0
+    c = np.array([0.3, -0.7])
+    xc, xp, w = orc.model_disk(n, c, R, wt, tau, eps, seed=3, center_weight=cw, centered=centered)
+    i = np.arange(n)
+    r32 = ((i + 1).astype(np.float32) / np.float32(n)) * np.float32(R)
+    start = 1 if centered else 0
+    d = xc.astype(np.float64) - c
+    dp = xp.astype(np.float64) - c
+    rr = np.sqrt((d ** 2).sum(1))
+    # positions are c + r (cos, sin) in float32: a few ulps of |c| absolute
+    np.testing.assert_allclose(rr[start:], r32[start:], rtol=2e-6, atol=3e-7)
+    np.testing.assert_allclose(np.sqrt((dp ** 2).sum(1))[start:], r32[start:], rtol=2e-6, atol=3e-7)
+    m = (wt * (i - 1) + cw) if centered else wt * i
+    r = r32.astype(np.float64)
+    a = r * m / (r ** 3 + eps)
+    x2 = 2 * d - dp - (a * tau * tau / r)[:, None] * d
+    dev = np.abs(np.sqrt((x2 ** 2).sum(1)) - r)
+    assert np.all(dev[start:] < 1e-6 + 2e-5 * r[start:])
+    assert np.all(w[start:] == np.float32(wt))
+    if centered:
+        assert np.all(xc[0] == c.astype(np.float32)) and w[0] == np.float32(cw)
+
+
+def test_model_disk_static_and_circles():
+    c = (1.0, 2.0)
+    xc, xp, w = orc.model_disk(500, c, 1.5, 0.2, 0.01, rotate=False, seed=1)
+    assert np.array_equal(xc, xp)
+    n, R, tau, eps, cw = 256, 1.5, 0.01, 0.1, 10.0
+    xc, xp, w = orc.model_circle(n, c, R, 0.1, cw, tau, eps, uniform=True)
+    ang = np.arctan2(xc[1:, 1] - c[1], xc[1:, 0] - c[0]) % (2 * np.pi)
+    np.testing.assert_allclose(ang, (2 * np.pi * np.arange(1, n) / n) % (2 * np.pi), atol=2e-6)
+    # every star rotated back by the one chord angle of the centre star's pull
+    a = R * cw / (R ** 3 + eps)
+    dang = np.arccos(1 - a * tau * tau / R / 2)
+    angp = np.arctan2(xp[1:, 1] - c[1], xp[1:, 0] - c[0]) % (2 * np.pi)
+    np.testing.assert_allclose((ang - angp) % (2 * np.pi), dang, atol=3e-6)
+    assert w[0] == np.float32(cw) and np.all(w[1:] == np.float32(0.1))
+
+
+def test_model_rectangle():
+    xc, xp, w = orc.model_rectangle(4000, (1.0, -1.0), 3.0, 2.0, 0.5, seed=9)
+    assert np.array_equal(xc, xp)
+    assert xc[:, 0].min() >= -0.5 and xc[:, 0].max() <= 2.5
+    assert xc[:, 1].min() >= -2.0 and xc[:, 1].max() <= 0.0
+    assert w.min() > 0 and w.max() <= 0.5
+
+
+@pytest.mark.parametrize("ecc", [0.0, 0.5])
+def test_double_disk_offsets_are_the_orbit_step(ecc):
+    """double_disk.cu:61-86: each disk's previous positions shift by the step
+    of a circular orbit about the common centre of mass, perpendicular to the
+    other disk's pull a_k (|o| = sqrt(r_k a_k) tau), scaled by 2^-e, and
+    pulled back by a_k tau^2 e."""
+    c1, c2 = np.array([-1.0, 0.5]), np.array([1.5, -0.25])
+    m1, m2, tau, eps = 3.0, 1.0, 0.01, 0.1
+    o1, o2 = orc.double_disk_offsets(c1, c2, m1, m2, m1, m2, ecc, tau, eps)
+    d = c2 - c1
+    r = np.linalg.norm(d)
+    for o, m_other, sign, rk in ((o1, m2, 1, r * m2 / (m1 + m2)), (o2, m1, -1, r * m1 / (m1 + m2))):
+        e = sign * d * m_other / (r ** 3 + eps)
+        a = np.linalg.norm(e)
+        perp = np.array([-e[1], e[0]]) / a
+        expect = perp * np.sqrt(rk * a) * tau / 2 ** ecc - e * tau * tau * ecc
+        np.testing.assert_allclose(o, expect, rtol=1e-5, atol=1e-9)
