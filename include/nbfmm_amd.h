@@ -37,6 +37,7 @@ extern "C" {
 #define NBFMM_ERR_HIP 3              /* a HIP runtime call failed */
 #define NBFMM_ERR_CAPACITY 4         /* num_particle exceeds the solver's capacity */
 #define NBFMM_ERR_NO_DEVICE 5
+#define NBFMM_ERR_IO 6                /* a file could not be opened or written */
 
 /* Status flags raised by the device (read with nbfmm_solver_flags). */
 #define NBFMM_FLAG_OUT_OF_BOX 1u /* a particle was outside the limits or NaN; its cell index was clamped */
@@ -199,6 +200,30 @@ int nbfmm_model_double_disk(int num_particle1, int num_particle2, const float ce
                             float radius1, float radius2, float weight, float center_weight1, float center_weight2,
                             float eccentricity, float tick, float eps, int centered, unsigned seed,
                             float* d_position_cur, float* d_position_pre, float* d_weight, void* stream);
+
+/* ---------------------------------------------------------------- video
+ * Frames are planar YUV 4:2:0 in device memory: width*height luma bytes, then
+ * width*height/2 chroma bytes (main/main.cu:234, FRAME_SIZE).  width and
+ * height must be even. */
+
+/* Stars::display (source/nbfmm/display/stars/display.cu:219-226) without its
+ * trailing Stars::prune: clears the frame (luma 0, chroma 128) and lights each
+ * star with a luma-255 stamp sized floor(sqrt(weight) / size_scale), clipped
+ * to the frame.  display_limits = {x_min, y_min, x_max, y_max} (host); row 0
+ * is y_max.  Asynchronous on `stream`; no allocation. */
+int nbfmm_display(int num_star, int width, int height, float size_scale, const float display_limits[4],
+                  const float* d_position, const float* d_weight, unsigned char* d_frame, void* stream);
+
+/* The YUV4MPEG2 file of main/main.cu:234-250.  open writes the stream header
+ * "YUV4MPEG2 W<w> H<h> F<fps>:1 Ip A1:1 C420"; write_frame appends "FRAME\n"
+ * and one frame copied from device memory on `stream` (through pinned
+ * staging: the file write of a frame happens during the next call or in
+ * close, so it overlaps the device work queued meanwhile); close writes what
+ * is pending, reports the number of frames written and frees the writer. */
+typedef struct nbfmm_y4m nbfmm_y4m;
+int nbfmm_y4m_open(const char* path, int width, int height, int fps, nbfmm_y4m** writer);
+int nbfmm_y4m_write_frame(nbfmm_y4m* writer, const unsigned char* d_frame, void* stream);
+int nbfmm_y4m_close(nbfmm_y4m* writer, long* num_frames);
 
 /* ---------------------------------------------------------------- tuning
  * The P2P kernel comes in variants (LDS staging size, unroll, waves per

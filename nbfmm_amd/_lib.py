@@ -64,6 +64,10 @@ SIGNATURES = {
     "nbfmm_model_rectangle": [c_int, c_float, c_float, c_float, c_float, c_float, ctypes.c_uint, P, P, P, P],
     "nbfmm_model_double_disk": [c_int, c_int, P, P, c_float, c_float, c_float, c_float, c_float, c_float, c_float,
                                 c_float, c_int, ctypes.c_uint, P, P, P, P],
+    "nbfmm_display": [c_int, c_int, c_int, c_float, ctypes.POINTER(c_float), P, P, P, P],
+    "nbfmm_y4m_open": [ctypes.c_char_p, c_int, c_int, c_int, ctypes.POINTER(c_void_p)],
+    "nbfmm_y4m_write_frame": [P, P, P],
+    "nbfmm_y4m_close": [P, ctypes.POINTER(ctypes.c_long)],
     "nbfmm_get_p2p_config": [P, ctypes.POINTER(c_int), ctypes.POINTER(c_int)],
     "nbfmm_autotune": [P, c_int, P, P, c_int, ctypes.POINTER(c_int), ctypes.POINTER(c_int),
                        ctypes.POINTER(c_float), P],

@@ -3,7 +3,9 @@
     python -m nbfmm_amd.build          # or nbfmm_amd.build.build()
 
 Each csrc/*.hip translation unit is compiled to an object in
-nbfmm_amd/_build/ and linked into nbfmm_amd/_build/libnbfmm_amd.so.
+nbfmm_amd/_build/ and linked into nbfmm_amd/_build/libnbfmm_amd.so; the
+command-line program csrc/nbfmm_main.cpp is linked against it as
+nbfmm_amd/_build/nbfmm_main.
 """
 from __future__ import annotations
 
@@ -17,6 +19,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT, "libnbfmm_amd.so")
+MAIN = os.path.join(OUT, "nbfmm_main")
 ROOT = os.path.dirname(HERE)
 
 ARCH = os.environ.get("NBFMM_ARCH", "gfx950")
@@ -30,7 +33,7 @@ FLAGS = [
     "-I", os.path.join(ROOT, "include"),
 ]
 
-SOURCES = ["sort.hip", "tree.hip", "p2p.hip", "adjoint.hip", "prune.hip", "model.hip", "capi.hip"]
+SOURCES = ["sort.hip", "tree.hip", "p2p.hip", "adjoint.hip", "prune.hip", "model.hip", "display.hip", "capi.hip"]
 
 
 def _compile(src: str) -> str:
@@ -52,8 +55,15 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    tmp = MAIN + ".tmp"
+    cmd = [HIPCC, *FLAGS, os.path.join(CSRC, "nbfmm_main.cpp"), "-o", tmp, "-L", OUT, "-lnbfmm_amd",
+           "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"nbfmm_main failed to build:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, MAIN)
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB} and {MAIN}")
     return LIB
 
 

@@ -39,6 +39,8 @@ int cdiv(int a, int b) { return (a + b - 1) / b; }
 
 }  // namespace
 
+int nbfmm_amd::report_error(int code, const std::string& msg) { return fail(code, msg); }
+
 struct nbfmm_solver {
   int L = 0, B = 0, max_n = 0;
   float limits[4] = {0, 0, 0, 0};
@@ -205,7 +207,7 @@ int run_pipeline(nbfmm_solver* s, int n, float tick, int mode, const CallIO& io,
 
 extern "C" {
 
-int nbfmm_version(void) { return 10100; }
+int nbfmm_version(void) { return 10200; }
 
 const char* nbfmm_last_error(void) { return g_last_error.c_str(); }
 

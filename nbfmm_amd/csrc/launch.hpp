@@ -1,9 +1,14 @@
 // Host-side launchers of the nbfmm_amd kernels (one translation unit each).
 #pragma once
 
+#include <string>
+
 #include "common.hpp"
 
 namespace nbfmm_amd {
+
+// Records `msg` as the calling thread's nbfmm_last_error() text; returns `code`.
+int report_error(int code, const std::string& msg);
 
 // Launch map of one kernel over a batch: `blocks` blocks in all, ManySys
 // telling each block its system.

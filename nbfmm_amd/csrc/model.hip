@@ -13,7 +13,7 @@
 // double-disk models shift the second disk's previous positions to set the
 // pair's orbit (double_disk.cu:61-86).  The kernel function's softening is a
 // parameter (the reference's kernelFunction uses 0.1).
-#include "common.hpp"
+#include "launch.hpp"
 #include "../../include/nbfmm_amd.h"
 
 namespace nbfmm_amd {
@@ -162,7 +162,16 @@ int model_check(int n, const float* xcur, const float* xpre, const float* w) {
   if (!xcur || !xpre || !w) return NBFMM_ERR_INVALID_ARGUMENT;
   return NBFMM_OK;
 }
-int launched() { return hipGetLastError() == hipSuccess ? NBFMM_OK : NBFMM_ERR_HIP; }
+int invalid(const char* model) {
+  return report_error(NBFMM_ERR_INVALID_ARGUMENT,
+                      std::string(model) + ": invalid argument (counts > 0, non-null arrays, positive sizes, "
+                                           "weights and eps, eccentricity >= 0)");
+}
+int launched() {
+  const hipError_t e = hipGetLastError();
+  if (e == hipSuccess) return NBFMM_OK;
+  return report_error(NBFMM_ERR_HIP, std::string("model kernel launch: ") + hipGetErrorString(e));
+}
 }  // namespace
 
 extern "C" {
@@ -170,7 +179,7 @@ extern "C" {
 int nbfmm_model_disk(int n, float cx, float cy, float radius, float weight, float tick, float eps, unsigned seed,
                      float* xcur, float* xpre, float* w, void* stream) {
   if (model_check(n, xcur, xpre, w) || !(radius > 0.0f) || !(weight > 0.0f) || !(eps > 0.0f))
-    return NBFMM_ERR_INVALID_ARGUMENT;
+    return invalid("nbfmm_model_disk");
   hipLaunchKernelGGL(k_model_disk, dim3(cdiv(n, 256)), dim3(256), 0, (hipStream_t)stream, n, make_float2(cx, cy),
                      radius, weight, 0.0f, 0, 1, tick, eps, seed, (float2*)xcur, (float2*)xpre, w);
   return launched();
@@ -178,7 +187,7 @@ int nbfmm_model_disk(int n, float cx, float cy, float radius, float weight, floa
 
 int nbfmm_model_disk_static(int n, float cx, float cy, float radius, float weight, unsigned seed, float* xcur,
                             float* xpre, float* w, void* stream) {
-  if (model_check(n, xcur, xpre, w) || !(radius > 0.0f) || !(weight > 0.0f)) return NBFMM_ERR_INVALID_ARGUMENT;
+  if (model_check(n, xcur, xpre, w) || !(radius > 0.0f) || !(weight > 0.0f)) return invalid("nbfmm_model_disk_static");
   hipLaunchKernelGGL(k_model_disk, dim3(cdiv(n, 256)), dim3(256), 0, (hipStream_t)stream, n, make_float2(cx, cy),
                      radius, weight, 0.0f, 0, 0, 0.0f, 1.0f, seed, (float2*)xcur, (float2*)xpre, w);
   return launched();
@@ -187,7 +196,7 @@ int nbfmm_model_disk_static(int n, float cx, float cy, float radius, float weigh
 int nbfmm_model_disk_center(int n, float cx, float cy, float radius, float weight, float center_weight, float tick,
                             float eps, unsigned seed, float* xcur, float* xpre, float* w, void* stream) {
   if (model_check(n, xcur, xpre, w) || !(radius > 0.0f) || !(weight > 0.0f) || !(eps > 0.0f))
-    return NBFMM_ERR_INVALID_ARGUMENT;
+    return invalid("nbfmm_model_disk_center");
   hipLaunchKernelGGL(k_model_disk, dim3(cdiv(n, 256)), dim3(256), 0, (hipStream_t)stream, n, make_float2(cx, cy),
                      radius, weight, center_weight, 1, 1, tick, eps, seed, (float2*)xcur, (float2*)xpre, w);
   return launched();
@@ -197,7 +206,7 @@ int nbfmm_model_circle_center(int n, float cx, float cy, float radius, float wei
                               float eps, int uniform_angles, unsigned seed, float* xcur, float* xpre, float* w,
                               void* stream) {
   if (model_check(n, xcur, xpre, w) || !(radius > 0.0f) || !(weight > 0.0f) || !(eps > 0.0f))
-    return NBFMM_ERR_INVALID_ARGUMENT;
+    return invalid("nbfmm_model_circle_center");
   // circle_center.cu:70-72: one chord angle for the whole circle, pulled by the centre star
   const float dang = chord_angle(radial_effect(radius, center_weight, eps), radius, tick);
   hipLaunchKernelGGL(k_model_circle, dim3(cdiv(n, 256)), dim3(256), 0, (hipStream_t)stream, n, make_float2(cx, cy),
@@ -209,7 +218,7 @@ int nbfmm_model_circle_center(int n, float cx, float cy, float radius, float wei
 int nbfmm_model_rectangle(int n, float cx, float cy, float width, float height, float max_weight, unsigned seed,
                           float* xcur, float* xpre, float* w, void* stream) {
   if (model_check(n, xcur, xpre, w) || !(width > 0.0f) || !(height > 0.0f) || !(max_weight > 0.0f))
-    return NBFMM_ERR_INVALID_ARGUMENT;
+    return invalid("nbfmm_model_rectangle");
   hipLaunchKernelGGL(k_model_rectangle, dim3(cdiv(n, 256)), dim3(256), 0, (hipStream_t)stream, n,
                      make_float2(cx, cy), width, height, max_weight, seed, (float2*)xcur, (float2*)xpre, w);
   return launched();
@@ -220,7 +229,7 @@ int nbfmm_model_double_disk(int n1, int n2, const float c1[2], const float c2[2]
                             float eps, int centered, unsigned seed, float* xcur, float* xpre, float* w,
                             void* stream) {
   if (n1 <= 0 || n2 <= 0 || !c1 || !c2 || !xcur || !xpre || !w || !(eccentricity >= 0.0f) || !(eps > 0.0f))
-    return NBFMM_ERR_INVALID_ARGUMENT;
+    return invalid("nbfmm_model_double_disk");
   hipStream_t st = (hipStream_t)stream;
   int rc;
   if (centered) {

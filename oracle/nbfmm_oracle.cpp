@@ -497,6 +497,50 @@ int orc_prune_f32(int n, const float limits[4], float* xcur, float* xpre, float*
   return prune(n, limits, xcur, xpre, w);
 }
 
+// ---- video frames (source/nbfmm/display/stars/display.cu) ----
+//
+// Stars::display (display.cu:219-226): luma cleared to 0, chroma (the last
+// width*height/2 bytes) to 128, then each star lights its stamp.  The stamp
+// grows by one ring of offsets per size level (display.cu:54-211); each ring
+// is listed by its generators (a, b), a >= b >= 0, and takes every sign and
+// order of them: level 1 (1,0); 2 (1,1); 3 (2,0); 4 (2,1); 5 (3,0) (2,2);
+// 6 (3,1); 7 (4,0) (3,2).  Intended semantics: every stamp pixel is clipped
+// to the frame (the reference's y <= height at :53 and unguarded upward arms
+// at :62, :90, :130, :182 write outside the luma plane).
+// Pinned by hand-drawn stamps of every level, clipping at the edges and
+// corners, and the pixel rule at cell boundaries (tests/test_oracle.py).
+void orc_display(int n, int width, int height, float size_scale, const float lim[4], const float* pos,
+                 const float* weight, uint8_t* frame) {
+  const size_t luma = (size_t)width * height;
+  std::fill(frame, frame + luma, (uint8_t)0);
+  std::fill(frame + luma, frame + luma + luma / 2, (uint8_t)128);
+  static const int gen[][3] = {{1, 1, 0}, {2, 1, 1}, {3, 2, 0}, {4, 2, 1}, {5, 3, 0},
+                               {5, 2, 2}, {6, 3, 1}, {7, 4, 0}, {7, 3, 2}};  // {level, a, b}
+  const float unit_w = (lim[2] - lim[0]) / (float)width;    // display.cu:45
+  const float unit_h = (lim[3] - lim[1]) / (float)height;   // display.cu:46
+  for (int i = 0; i < n; ++i) {
+    const float fx = std::floor((pos[2 * i] - lim[0]) / unit_w);       // display.cu:47
+    const float fy = std::floor((lim[3] - pos[2 * i + 1]) / unit_h);   // display.cu:48
+    if (!(fx >= 0.0f && fx < (float)width && fy >= 0.0f && fy < (float)height)) continue;
+    const int x = (int)fx, y = (int)fy;
+    const float fs = std::floor(std::sqrt(weight[i]) / size_scale);    // display.cu:49
+    const int level = fs >= 1.0f ? (int)std::min(fs, 7.0f) : 0;
+    auto light = [&](int dx, int dy) {
+      const int px = x + dx, py = y + dy;
+      if (px >= 0 && px < width && py >= 0 && py < height) frame[(size_t)py * width + px] = 255;
+    };
+    light(0, 0);
+    for (const auto& g : gen) {
+      if (g[0] > level) continue;
+      for (int sa : {-1, 1})
+        for (int sb : {-1, 1}) {
+          light(sa * g[1], sb * g[2]);
+          light(sb * g[2], sa * g[1]);
+        }
+    }
+  }
+}
+
 int orc_max_threads(void) {
 #ifdef _OPENMP
   return omp_get_max_threads();

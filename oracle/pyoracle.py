@@ -74,6 +74,9 @@ def lib():
         L.orc_model_rectangle.argtypes = [c_int, c_float, c_float, c_float, c_float, c_float, c_uint, _f32p, _f32p,
                                           _f32p]
         L.orc_double_disk_offsets.argtypes = [c_float] * 11 + [_f32p]
+        L.orc_display.argtypes = [c_int, c_int, c_int, c_float, _f32p, _f32p, _f32p,
+                                  np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")]
+        L.orc_display.restype = None
         _lib = L
     return _lib
 
@@ -296,3 +299,13 @@ def model_double_disk(n1, n2, c1, c2, r1, r2, weight, ecc, tick, eps=0.1, seed=0
     o1, o2 = double_disk_offsets(c1, c2, m1, m2, q1, q2, ecc, tick, eps)
     xp = np.concatenate([a[1] + o1, b[1] + o2]).astype(np.float32)
     return np.concatenate([a[0], b[0]]), xp, np.concatenate([a[2], b[2]])
+
+
+# ---- video frames (source/nbfmm/display/stars/display.cu) ----
+
+def display(width, height, size_scale, limits, pos, w):
+    """Stars::display's frame: uint8[height * width * 3 // 2], planar YUV 4:2:0."""
+    pos = _f32(pos).reshape(-1, 2)
+    frame = np.empty(width * height * 3 // 2, np.uint8)
+    lib().orc_display(pos.shape[0], int(width), int(height), float(size_scale), _f32(limits), pos, _f32(w), frame)
+    return frame

@@ -396,3 +396,94 @@ def test_double_disk_offsets_are_the_orbit_step(ecc):
         perp = np.array([-e[1], e[0]]) / a
         expect = perp * np.sqrt(rk * a) * tau / 2 ** ecc - e * tau * tau * ecc
         np.testing.assert_allclose(o, expect, rtol=1e-5, atol=1e-9)
+
+
+# ---- video frames (source/nbfmm/display/stars/display.cu) ----
+
+# The offsets (dx, dy) each size level adds, read off display.cu:54-211
+# (board[dx + dy * width]; dy > 0 is down).
+STAMP_LISTING = {
+    0: [(0, 0)],
+    1: [(-1, 0), (1, 0), (0, -1), (0, 1)],
+    2: [(-1, -1), (-1, 1), (1, -1), (1, 1)],
+    3: [(-2, 0), (2, 0), (0, -2), (0, 2)],
+    4: [(-2, -1), (-1, -2), (-2, 1), (-1, 2), (2, -1), (1, -2), (2, 1), (1, 2)],
+    5: [(-3, 0), (3, 0), (0, -3), (0, 3), (-2, -2), (-2, 2), (2, -2), (2, 2)],
+    6: [(-3, -1), (-1, -3), (-3, 1), (-1, 3), (3, -1), (1, -3), (3, 1), (1, 3)],
+    7: [(-4, 0), (4, 0), (0, -4), (0, 4), (-3, -2), (-2, -3), (-3, 2), (-2, 3), (3, -2), (2, -3), (3, 2), (2, 3)],
+}
+
+
+def expected_luma(width, height, stars):
+    """stars: (x, y, level) pixel centres; the listing's offsets, clipped."""
+    img = np.zeros((height, width), np.uint8)
+    for x, y, level in stars:
+        for s in range(min(level, 7) + 1):
+            for dx, dy in STAMP_LISTING[s]:
+                if 0 <= x + dx < width and 0 <= y + dy < height:
+                    img[y + dy, x + dx] = 255
+    return img
+
+
+def pixel_centre(x, y, width, height, lim):
+    uw = (lim[2] - lim[0]) / width
+    uh = (lim[3] - lim[1]) / height
+    return lim[0] + (x + 0.5) * uw, lim[3] - (y + 0.5) * uh
+
+
+@pytest.mark.parametrize("level", range(9))
+def test_display_stamp_is_the_reference_listing(level):
+    W, H, lim = 20, 14, (-5.0, -3.5, 5.0, 3.5)
+    px, py = pixel_centre(9, 6, W, H, lim)
+    wt = float(level) ** 2 * 0.25 + 0.01   # sqrt(w) / 0.5 = level + a little
+    f = orc.display(W, H, 0.5, lim, np.array([[px, py]], np.float32), np.array([wt], np.float32))
+    assert f.size == W * H * 3 // 2
+    assert np.array_equal(f[:W * H].reshape(H, W), expected_luma(W, H, [(9, 6, level)]))
+    assert np.all(f[W * H:] == 128)
+    assert (f[:W * H] == 255).sum() == [1, 5, 9, 13, 21, 29, 37, 49, 49][level]
+
+
+def test_display_clips_at_edges_and_corners():
+    W, H, lim = 24, 16, (0.0, 0.0, 24.0, 16.0)
+    rng = np.random.default_rng(0)
+    for x, y in [(0, 0), (W - 1, 0), (0, H - 1), (W - 1, H - 1), (1, 1), (W - 2, H - 2), (2, 7), (W - 3, 8),
+                 (11, 0), (11, 1), (11, H - 1), (3, H - 3)]:
+        level = int(rng.integers(0, 8))
+        px, py = pixel_centre(x, y, W, H, lim)
+        f = orc.display(W, H, 1.0, lim, np.array([[px, py]], np.float32), np.array([level ** 2 + 0.5], np.float32))
+        assert np.array_equal(f[:W * H].reshape(H, W), expected_luma(W, H, [(x, y, level)])), (x, y, level)
+
+
+def test_display_pixel_rule():
+    """x = floor((p.x - x_min) / unit), y = floor((y_max - p.y) / unit) with
+    unit 1 here: cell edges belong to the cell right of / below them, the top
+    and left edges of the box are in, the right and bottom edges are out,
+    and NaN positions are skipped."""
+    W, H, lim = 16, 12, (0.0, 0.0, 16.0, 12.0)
+    cases = [((3.0, 5.0), (3, 7)), ((0.0, 12.0), (0, 0)), ((15.999, 0.001), (15, 11)), ((7.5, 6.0), (7, 6))]
+    for (px, py), (x, y) in cases:
+        f = orc.display(W, H, 1.0, lim, np.array([[px, py]], np.float32), np.array([0.5], np.float32))
+        lit = np.argwhere(f[:W * H].reshape(H, W) == 255)
+        assert lit.tolist() == [[y, x]], (px, py)
+    for px, py in [(16.0, 5.0), (5.0, 0.0), (-1e-6, 5.0), (5.0, 12.0001), (np.nan, 3.0), (3.0, np.nan), (1e30, 1.0)]:
+        f = orc.display(W, H, 1.0, lim, np.array([[px, py]], np.float32), np.array([4.0], np.float32))
+        assert not np.any(f[:W * H]), (px, py)
+    # a NaN weight lights the point only
+    f = orc.display(W, H, 1.0, lim, np.array([[3.5, 3.5]], np.float32), np.array([np.nan], np.float32))
+    assert (f[:W * H] == 255).sum() == 1
+
+
+def test_display_is_a_union_of_stamps():
+    """Stars light pixels independently: the frame of many stars is the
+    pixelwise maximum of their single-star frames, whatever their order."""
+    W, H, lim = 40, 30, (-2.0, -1.5, 2.0, 1.5)
+    rng = np.random.default_rng(3)
+    pos = rng.uniform(-2.2, 2.2, (60, 2)).astype(np.float32)
+    w = rng.uniform(0.0, 80.0, 60).astype(np.float32)
+    full = orc.display(W, H, 1.0, lim, pos, w)
+    acc = np.zeros(W * H, np.uint8)
+    for i in range(60):
+        acc = np.maximum(acc, orc.display(W, H, 1.0, lim, pos[i:i + 1], w[i:i + 1])[:W * H])
+    assert np.array_equal(full[:W * H], acc)
+    perm = rng.permutation(60)
+    assert np.array_equal(orc.display(W, H, 1.0, lim, pos[perm], w[perm]), full)
