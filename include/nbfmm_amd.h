@@ -310,9 +310,15 @@ int nbfmm_batch_run(nbfmm_batch* batch, int num_steps, float* d_position_cur, fl
 int nbfmm_batch_flags(nbfmm_batch* batch, unsigned* flags, int clear);
 /* P2P configuration of the batch (see nbfmm_set_p2p_config): the grid of
  * blocks_per_cu blocks per CU is shared among the systems in proportion to
- * their particle counts. */
+ * their P2P work (particle counts until the batch is balanced). */
 int nbfmm_batch_set_p2p_config(nbfmm_batch* batch, int variant, int blocks_per_cu);
 int nbfmm_batch_get_p2p_config(const nbfmm_batch* batch, int* variant, int* blocks_per_cu);
+/* Sorts the given positions and shares the P2P grids (forward and adjoint)
+ * among the systems in proportion to their P2P pairs plus 32 per particle.
+ * nbfmm_batch_run does this by itself on its first graph run.  Results do
+ * not depend on the shares, only the time does.  Synchronises.  No
+ * reference counterpart. */
+int nbfmm_batch_balance(nbfmm_batch* batch, const float* d_position, const float* d_weight, void* stream);
 /* Warm-started sort of every system (see nbfmm_set_warm_start); off by default
  * for batches (many systems already keep the cold sort busy). */
 int nbfmm_batch_set_warm_start(nbfmm_batch* batch, int enable);

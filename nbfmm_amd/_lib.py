@@ -49,6 +49,7 @@ SIGNATURES = {
     "nbfmm_batch_flags": [P, P, c_int],
     "nbfmm_batch_set_p2p_config": [P, c_int, c_int],
     "nbfmm_batch_set_warm_start": [P, c_int],
+    "nbfmm_batch_balance": [P, P, P, P],
     "nbfmm_batch_get_p2p_config": [P, ctypes.POINTER(c_int), ctypes.POINTER(c_int)],
     "nbfmm_p2p_variant_count": [],
     "nbfmm_p2p_variant_info": [c_int, ctypes.POINTER(c_int), ctypes.POINTER(c_int), ctypes.POINTER(c_int)],

@@ -69,6 +69,11 @@ class Batch:
         results are identical either way."""
         check(lib().nbfmm_batch_set_warm_start(self._h, int(bool(enable))))
 
+    def balance(self, position, weight, stream=None):
+        """Share the P2P grids by each system's pair count at these positions
+        (run() does it by itself on its first graph run); synchronises."""
+        check(lib().nbfmm_batch_balance(self._h, _ptr(position), _ptr(weight), _stream(stream)))
+
     def p2p_config(self):
         v, b = ctypes.c_int(), ctypes.c_int()
         check(lib().nbfmm_batch_get_p2p_config(self._h, ctypes.byref(v), ctypes.byref(b)))

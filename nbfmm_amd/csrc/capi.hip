@@ -827,6 +827,12 @@ struct nbfmm_batch {
   int p2p_variant = kDefaultBatchP2PVariant;
   int p2p_bpc = kDefaultBatchP2PBlocksPerCU;
   std::map<std::pair<int, int>, KernelMap> p2p_maps;  // (waves per block, blocks per CU) -> map
+  // P2P work of each system, the weight of its share of the P2P grids: n
+  // until nbfmm_batch_balance measures pairs + kBalancePerParticle * n
+  std::vector<double> work;
+  double total_work = 0.0;
+  bool balanced = false;
+  unsigned long long* d_pairs = nullptr;  // S pair counts
   std::vector<void*> allocs;
   hipGraphExec_t graph_exec = nullptr;
   hipGraph_t graph = nullptr;
@@ -860,6 +866,13 @@ void batch_free(nbfmm_batch* b) {
   b->allocs.clear();
 }
 
+// A captured step graph holds the kernels and launch maps of its capture.
+void drop_batch_graph(nbfmm_batch* b) {
+  if (b->graph_exec) { hipGraphExecDestroy(b->graph_exec); b->graph_exec = nullptr; }
+  if (b->graph) { hipGraphDestroy(b->graph); b->graph = nullptr; }
+  b->gkey.valid = false;
+}
+
 // Builds a launch map from the per-system block counts (over the device
 // views vs, by default the plain ones).
 int make_map(nbfmm_batch* b, const std::vector<int>& blocks, KernelMap* km, const SysView* vs = nullptr) {
@@ -882,16 +895,48 @@ int make_map(nbfmm_batch* b, const std::vector<int>& blocks, KernelMap* km, cons
 }
 
 // P2P blocks of `waves` waves, `total_blocks` in all, shared among the
-// systems in proportion to n and never more than a system's leaves can use.
+// systems in proportion to their P2P work (b->work) and never more than a
+// system's leaves can use.  Each block works only on its own system's runs,
+// so a system given fewer blocks than its work needs finishes last.
 std::vector<int> p2p_share(const nbfmm_batch* b, int waves, int total_blocks) {
   std::vector<int> blocks(b->S);
   for (int s = 0; s < b->S; ++s) {
     const SysView& v = b->hv[s];
     const int cap = cdiv(std::min(v.n, v.B * v.B), waves);
-    const int share = (int)((double)total_blocks * v.n / (double)b->total_n + 0.5);
+    const int share = (int)((double)total_blocks * b->work[s] / b->total_work + 0.5);
     blocks[s] = std::max(1, std::min(cap, share));
   }
   return blocks;
+}
+
+int batch_use_p2p(nbfmm_batch* b, int variant, int bpc);
+
+// Pairs a particle's P2P cost is worth beyond its pair evaluations (the
+// epilogue, the run set-up): keeps systems with few pairs from being starved.
+constexpr double kBalancePerParticle = 32.0;
+
+void drop_batch_graph(nbfmm_batch* b);
+
+// Re-weights the P2P shares by the pairs of the current sort (which must be
+// valid for every system), rebuilds the P2P and adjoint P2P launch maps and
+// drops a captured step graph.  Synchronises.
+int batch_balance(nbfmm_batch* b, hipStream_t st) {
+  NB_HIP(hipMemsetAsync(b->d_pairs, 0, sizeof(unsigned long long) * b->S, st));
+  for (int s = 0; s < b->S; ++s) NB_HIP(launch_pair_count(b->hv[s], b->d_pairs + s, st));
+  std::vector<unsigned long long> pairs(b->S);
+  NB_HIP(hipMemcpyAsync(pairs.data(), b->d_pairs, sizeof(unsigned long long) * b->S, hipMemcpyDeviceToHost, st));
+  NB_HIP(hipStreamSynchronize(st));
+  b->total_work = 0.0;
+  for (int s = 0; s < b->S; ++s) {
+    b->work[s] = (double)pairs[s] + kBalancePerParticle * b->hv[s].n;
+    b->total_work += b->work[s];
+  }
+  b->p2p_maps.clear();  // their device tables stay in b->allocs until the batch is destroyed
+  if (int rc = make_map(b, p2p_share(b, kAdjP2PWaves, b->cus * kAdjP2PBlocksPerCU), &b->bl.adj_p2p)) return rc;
+  if (int rc = batch_use_p2p(b, b->p2p_variant, b->p2p_bpc)) return rc;
+  drop_batch_graph(b);
+  b->balanced = true;
+  return NBFMM_OK;
 }
 
 // Points bl.p2p at the launch map of (variant, blocks per CU), built on
@@ -966,6 +1011,9 @@ int nbfmm_batch_create(int S, const int* num_particle, const int* num_level, con
   b->total_n = (int)total;
   b->part_off.assign(S + 1, 0);
   b->hv.resize(S);
+  b->work.resize(S);
+  for (int s = 0; s < S; ++s) b->work[s] = (double)num_particle[s];
+  b->total_work = (double)total;
   // sizes of the concatenated scratch arrays
   size_t n_all = 0, head_all = 0, row_all = 0, cell_all = 0, hist_all = 0, tree_all = 0;
   std::vector<size_t> head_off(S), row_off(S), cell_off(S), hist_off(S), tree_off(S);
@@ -1111,6 +1159,7 @@ int nbfmm_batch_create(int S, const int* num_particle, const int* num_level, con
       (rc = make_map(b, asort, &b->bl.adj_sort)) || (rc = make_map(b, azero, &b->bl.adj_zero)) ||
       (rc = make_map(b, tiles, &b->bl.tiles_g, b->dv_guard)) || (rc = make_map(b, rows, &b->bl.rows_g, b->dv_guard)) ||
       (rc = make_map(b, wmerge, &b->bl.warm_merge)) || (rc = make_map(b, one, &b->bl.one)) ||
+      (rc = batch_alloc(b, &b->d_pairs, (size_t)S)) ||
       (rc = make_map(b, p2p_share(b, kAdjP2PWaves, cus * kAdjP2PBlocksPerCU), &b->bl.adj_p2p)) ||
       (rc = batch_use_p2p(b, b->p2p_variant, b->p2p_bpc))) {
     batch_free(b);
@@ -1179,6 +1228,9 @@ int nbfmm_batch_run(nbfmm_batch* b, int steps, float* xcur, float* xpre, const f
   CallIO io = call_io(xcur, w, acc, xcur, xpre);
   io.xs_out = b->xs;
   if ((rc = batch_pipeline(b, kP2PStep, io, st))) return rc;
+  // the first graph run of a batch shares the P2P grid by the pairs of the
+  // sort just made (one synchronisation, before anything is captured)
+  if (!b->balanced && (rc = batch_balance(b, st))) return rc;
   if (--steps == 0) return NBFMM_OK;
   const bool warm = b->warm_enabled;
   auto& g = b->gkey;
@@ -1206,9 +1258,7 @@ int nbfmm_batch_run(nbfmm_batch* b, int steps, float* xcur, float* xpre, const f
 int nbfmm_batch_set_warm_start(nbfmm_batch* b, int enable) {
   if (!b) return fail(NBFMM_ERR_INVALID_ARGUMENT, "null batch");
   b->warm_enabled = enable != 0;
-  if (b->graph_exec) { hipGraphExecDestroy(b->graph_exec); b->graph_exec = nullptr; }
-  if (b->graph) { hipGraphDestroy(b->graph); b->graph = nullptr; }
-  b->gkey.valid = false;
+  drop_batch_graph(b);
   return NBFMM_OK;
 }
 
@@ -1249,11 +1299,15 @@ int nbfmm_batch_set_p2p_config(nbfmm_batch* b, int variant, int blocks_per_cu) {
   if (blocks_per_cu < 1 || blocks_per_cu > 64) return fail(NBFMM_ERR_INVALID_ARGUMENT, "blocks_per_cu in [1, 64]");
   const int rc = batch_use_p2p(b, variant, blocks_per_cu);
   if (rc) return rc;
-  // a captured step graph holds the old kernel
-  if (b->graph_exec) { hipGraphExecDestroy(b->graph_exec); b->graph_exec = nullptr; }
-  if (b->graph) { hipGraphDestroy(b->graph); b->graph = nullptr; }
-  b->gkey.valid = false;
+  drop_batch_graph(b);
   return NBFMM_OK;
+}
+
+int nbfmm_batch_balance(nbfmm_batch* b, const float* pos0, const float* w0, void* stream) {
+  if (!b || !pos0 || !w0) return fail(NBFMM_ERR_INVALID_ARGUMENT, "null argument");
+  hipStream_t st = as_stream(stream);
+  if (int rc = batch_sort(b, call_io(pos0, w0), st)) return rc;
+  return batch_balance(b, st);
 }
 
 int nbfmm_batch_get_p2p_config(const nbfmm_batch* b, int* variant, int* blocks_per_cu) {

@@ -95,7 +95,12 @@ constexpr int kUpTile = 16;  // leaves per side of one M2M tile block
 
 // P2M of the occupied leaves only (the fused solve's predo already cleared the
 // empty ones).  One thread per occupied leaf, particles summed in sorted
-// order exactly as p2m_leaf, loads issued four at a time.
+// order exactly as p2m_leaf.  The kernel's time is the add chain of its
+// heaviest leaf, so each thread keeps the next kP2MDepth particles in flight
+// while it adds the current ones: one load latency per kP2MDepth particles
+// instead of one per particle group.
+constexpr int kP2MDepth = 16;
+
 template <class Sys>
 __global__ __launch_bounds__(256) void k_p2m_occ(Sys sys) {
 #pragma clang fp contract(off)
@@ -105,20 +110,29 @@ __global__ __launch_bounds__(256) void k_p2m_occ(Sys sys) {
   if (j >= *v.nocc) return;
   const int c = v.occ[j];
   const int a = v.head[c], b = v.head[c + 1];
+  // loads are unconditional (indices clamped to the leaf's last particle, an
+  // occupied leaf has b > a) so that none waits behind a branch; only the
+  // adds are predicated
+  const int last = b - 1;
+  float4 next[kP2MDepth];
+#pragma unroll
+  for (int k = 0; k < kP2MDepth; ++k) next[k] = v.part[min(a + k, last)];
   float sx = 0.0f, sy = 0.0f, sw = 0.0f;
-  int s = a;
-  for (; s + 4 <= b; s += 4) {
-    const float4 p0 = v.part[s], p1 = v.part[s + 1], p2 = v.part[s + 2], p3 = v.part[s + 3];
-    sx += p0.x * p0.z; sy += p0.y * p0.z; sw += p0.z;
-    sx += p1.x * p1.z; sy += p1.y * p1.z; sw += p1.z;
-    sx += p2.x * p2.z; sy += p2.y * p2.z; sw += p2.z;
-    sx += p3.x * p3.z; sy += p3.y * p3.z; sw += p3.z;
-  }
-  for (; s < b; ++s) {
-    const float4 p = v.part[s];
-    sx += p.x * p.z;
-    sy += p.y * p.z;
-    sw += p.z;
+  for (int s = a; s < b; s += kP2MDepth) {
+    float4 cur[kP2MDepth];
+#pragma unroll
+    for (int k = 0; k < kP2MDepth; ++k) cur[k] = next[k];
+#pragma unroll
+    for (int k = 0; k < kP2MDepth; ++k) next[k] = v.part[min(s + kP2MDepth + k, last)];
+    const int m = b - s;
+#pragma unroll
+    for (int k = 0; k < kP2MDepth; ++k) {
+      if (k < m) {
+        sx += cur[k].x * cur[k].z;
+        sy += cur[k].y * cur[k].z;
+        sw += cur[k].z;
+      }
+    }
   }
   v.mpole[c] = make_float4(sx / sw, sy / sw, sw, 0.0f);
 }

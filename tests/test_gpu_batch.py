@@ -79,6 +79,42 @@ def test_batch_run_graph_equals_single_steps(small_batch):
     b.close()
 
 
+def test_batch_balance_keeps_results(small_batch):
+    """Sharing the P2P grid by pair counts moves work between blocks only:
+    solve gives the same bits before and after balance(), and a graph run
+    (which balances on its first call, then captures) equals single solvers."""
+    from nbfmm_amd import Batch, Solver
+    bw = small_batch
+    x, p, w = bw.concat()
+    ticks = [s.tick for s in bw.systems]
+    b = Batch([s.n for s in bw.systems], [s.num_level for s in bw.systems], [s.limits for s in bw.systems],
+              [s.eps for s in bw.systems], ticks=ticks)
+    X, P, W = dev(x), dev(p), dev(w)
+    A0, A1 = torch.zeros_like(X), torch.zeros_like(X)
+    b.solve(X, W, A0)
+    b.balance(X, W)
+    b.solve(X, W, A1)
+    torch.cuda.synchronize()
+    assert torch.equal(A0, A1)
+    stream = torch.cuda.Stream()
+    with torch.cuda.stream(stream):
+        b.run(4, X, P, W, A1, use_graph=True)
+        stream.synchronize()
+    off = b.offsets
+    for k, s in enumerate(bw.systems):
+        one = Solver(s.num_level, s.n, s.limits, s.eps)
+        xs, ps, ws = dev(s.position), dev(s.position_pre), dev(s.weight)
+        a = torch.zeros_like(xs)
+        for _ in range(4):
+            one.step(s.n, s.tick, xs, ps, ws, a)
+        torch.cuda.synchronize()
+        assert torch.equal(X[off[k]:off[k + 1]], xs), k
+        assert torch.equal(A1[off[k]:off[k + 1]], a), k
+        one.close()
+    assert not b.flags().any()
+    b.close()
+
+
 def test_batch_rejects_bad_systems():
     from nbfmm_amd import Batch, NbfmmError
     with pytest.raises(NbfmmError):
