@@ -815,7 +815,6 @@ struct nbfmm_batch {
   std::vector<int> part_off;      // S+1
   std::vector<SysView> hv;        // host copies of the views
   SysView* dv = nullptr;          // device copies
-  SysView* dv_guard = nullptr;    // device copies with cold_if set (guarded cold sort passes)
   BatchLaunch bl{};
   // Off by default for batches: measured on 256 systems (7.1M particles) the
   // warm passes cost as much as the cold sort (about 0.34 ms a step each),
@@ -1078,7 +1077,6 @@ int nbfmm_batch_create(int S, const int* num_particle, const int* num_level, con
   BALLOC(gmp, tree_all);
   BALLOC(flags, (size_t)S);
   BALLOC(b->dv, (size_t)S);
-  BALLOC(b->dv_guard, (size_t)S);
   int *wstate, *wrel, *wmgrp, *wcin, *wcout, *wtouch, *wtlist, *wcnew, *winoff, *wfill, *wrowtot;
   int2* wtold;
   BALLOC(b->xs, n_all);
@@ -1130,12 +1128,6 @@ int nbfmm_batch_create(int S, const int* num_particle, const int* num_level, con
   }
   hipError_t e = hipMemcpy(b->dv, b->hv.data(), S * sizeof(SysView), hipMemcpyHostToDevice);
   if (e != hipSuccess) { batch_free(b); delete b; return hip_fail(e, "hipMemcpy views"); }
-  {
-    std::vector<SysView> guard(b->hv);
-    for (auto& v : guard) v.cold_if = v.wstate + kWarmFallback;
-    e = hipMemcpy(b->dv_guard, guard.data(), S * sizeof(SysView), hipMemcpyHostToDevice);
-    if (e != hipSuccess) { batch_free(b); delete b; return hip_fail(e, "hipMemcpy views"); }
-  }
   // launch maps
   std::vector<int> tiles(S), rows(S), p2m(S), mtile(S), mtop(S), m2l(S), asort(S), azero(S), wmerge(S), one(S, 1);
   for (int s = 0; s < S; ++s) {
@@ -1157,7 +1149,6 @@ int nbfmm_batch_create(int S, const int* num_particle, const int* num_level, con
       (rc = make_map(b, p2m, &b->bl.p2m)) || (rc = make_map(b, mtile, &b->bl.m2m_tile)) ||
       (rc = make_map(b, mtop, &b->bl.m2m_top)) || (rc = make_map(b, m2l, &b->bl.m2l)) ||
       (rc = make_map(b, asort, &b->bl.adj_sort)) || (rc = make_map(b, azero, &b->bl.adj_zero)) ||
-      (rc = make_map(b, tiles, &b->bl.tiles_g, b->dv_guard)) || (rc = make_map(b, rows, &b->bl.rows_g, b->dv_guard)) ||
       (rc = make_map(b, wmerge, &b->bl.warm_merge)) || (rc = make_map(b, one, &b->bl.one)) ||
       (rc = batch_alloc(b, &b->d_pairs, (size_t)S)) ||
       (rc = make_map(b, p2p_share(b, kAdjP2PWaves, cus * kAdjP2PBlocksPerCU), &b->bl.adj_p2p)) ||

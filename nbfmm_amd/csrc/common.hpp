@@ -60,7 +60,7 @@ struct SysView {
   uint32_t* flags;  // NBFMM_FLAG_* bits
   // warm-started sort (the previous sort's key/part/head as the starting
   // order); every array is left cleared by the pass that last uses it
-  int* wstate;      // kWarmInts: fallback flag, touched-leaf count
+  int* wstate;      // kWarmInts: [kWarmTouched] = touched-leaf count
   int* wrel;        // N: rank of a staying particle inside its old leaf
   int* wmgrp;       // N: old slots of the movers grouped by new leaf
   int* wcin;        // C: movers into each leaf
@@ -72,13 +72,11 @@ struct SysView {
   int* winoff;      // C: first slot of each leaf's movers in wmgrp
   int* wfill;       // C: movers placed into each leaf's group
   int* wrowtot;     // 2 (B + 1): row totals of the new counts and of the movers
-  const int* cold_if;  // when set, the row/column sort kernels run only if *cold_if != 0
 };
 
 // Warm-started sort.
 constexpr uint32_t kMoverBit = 0x80000000u;  // tkey flag: the particle changed leaf
-constexpr int kWarmMaxIn = 64;               // movers into one leaf the warm pass places
-constexpr int kWarmFallback = 0, kWarmTouched = 1, kWarmInts = 4;
+constexpr int kWarmTouched = 1, kWarmInts = 4;
 constexpr int kWarmMergeWaves = 16384;       // merge grid: one wave per touched leaf, grid-stride beyond
 
 // Caller arrays of one call, original particle order; a system's particles

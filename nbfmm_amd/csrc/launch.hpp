@@ -29,8 +29,6 @@ struct BatchLaunch {
   KernelMap adj_sort;  // adjoint: upstream gradient to sorted order, cdiv(n_s, 256)
   KernelMap adj_zero;  // adjoint: clear the level-0 subtree sums
   // warm-started sort: the particle kernels use adj_sort, the row kernels rows
-  KernelMap tiles_g;   // as tiles / rows, over views whose cold_if points at the
-  KernelMap rows_g;    //   system's fallback flag (the guarded cold passes)
   KernelMap warm_merge;  // one wave per touched leaf (grid-stride): its share of waves
   KernelMap one;       // one block per system
 };

@@ -73,7 +73,6 @@ __global__ __launch_bounds__(kSortThreads) void k_row_up(Sys sys, CallIO io) {
   __shared__ uint32_t h[512];
   int tile, ntiles;
   const SysView v = sys.get(tile, ntiles);
-  if (v.cold_if && *v.cold_if == 0) return;
   const float2* pos0 = io.pos0 + v.part_off;
   for (int d = threadIdx.x; d < v.B; d += kSortThreads) h[d] = 0;
   __syncthreads();
@@ -103,7 +102,6 @@ __global__ __launch_bounds__(kScanThreads) void k_row_scan(Sys sys) {
   __shared__ uint32_t carry;
   int d, nrows;
   const SysView v = sys.get(d, nrows);
-  if (v.cold_if && *v.cold_if == 0) return;
   const int ntiles = (v.n + kSortTile - 1) / kSortTile;
   uint32_t* row = v.hist + (size_t)d * ntiles;
   if (threadIdx.x == 0) {
@@ -136,7 +134,6 @@ __global__ __launch_bounds__(kSortThreads) void k_row_down(Sys sys, CallIO io) {
   __shared__ uint32_t sh[kSortThreads / kWave];
   int tile, ntiles;
   const SysView v = sys.get(tile, ntiles);
-  if (v.cold_if && *v.cold_if == 0) return;
   const float2* pos0 = io.pos0 + v.part_off;
   const float* w0 = io.w0 + v.part_off;
   const int R = v.B;
@@ -229,7 +226,6 @@ __global__ __launch_bounds__(kRowThreads) void k_row_sort(Sys sys, int zero_empt
   __shared__ uint32_t s_occ_base;
   int y, nrows;
   const SysView v = sys.get(y, nrows);
-  if (v.cold_if && *v.cold_if == 0) return;
   const int B = v.B;
   const int lane = lane_id();
   const int wave = threadIdx.x / kWave;
@@ -375,13 +371,11 @@ __global__ __launch_bounds__(256) void k_empty(SysView v) {
 //   k_warm_place   stayers of untouched leaves go straight to their slot;
 //                  movers are grouped by new leaf
 //   k_warm_merge   one wave per touched leaf merges its stayers and movers
-//   k_warm_end     clears the counters
-// Per-particle atomics are per leaf; an atomic that the whole grid shares
-// serialises across the chip, so the only shared counters are bumped once
-// per row block.  More than kWarmMaxIn movers into one leaf sets
-// the fallback flag: the warm passes stop after cleaning their scratch and
-// the cold row/column sort, launched after them with cold_if pointing at
-// that flag, sorts from scratch.
+// Each pass clears the counters it is the last to read.  Per-particle
+// atomics are per leaf; an atomic that the whole grid shares serialises
+// across the chip, so the only shared counters are bumped once per row
+// block.  Any number of movers may enter a leaf, so the warm passes never
+// need the cold sort.
 
 // First touch of leaf c records its old slot range.
 __device__ __forceinline__ void warm_touch(const SysView& v, uint32_t c) {
@@ -415,17 +409,20 @@ __global__ __launch_bounds__(256) void k_warm_gather(Sys sys, CallIO io) {
     } else {
       v.tkey[i] = k | kMoverBit;
       atomicAdd(&v.wcout[kold], 1);
-      if (atomicAdd(&v.wcin[k], 1) == kWarmMaxIn) atomicExch(&v.wstate[kWarmFallback], 1);
+      atomicAdd(&v.wcin[k], 1);
       warm_touch(v, kold);
       warm_touch(v, k);
     }
   }
+  // the touched-leaf count k_warm_finish fills (the previous step's merge,
+  // its last reader, has finished)
+  if (blk == 0 && threadIdx.x == 0) v.wstate[kWarmTouched] = 0;
   if (__any(bad) && lane_id() == 0) atomicOr(v.flags, 1u);
 }
 
 // One block per row: new count of every leaf (old count - out + in), its
 // row-local exclusive offset, and the row-local offset of its movers.
-// Clears wcin/wcout (also on fallback).
+// Clears wcin/wcout.
 template <class Sys>
 __global__ __launch_bounds__(kRowThreads) void k_warm_rows(Sys sys) {
   __shared__ uint32_t sh[kRowWaves];
@@ -443,10 +440,6 @@ __global__ __launch_bounds__(kRowThreads) void k_warm_rows(Sys sys) {
     }
     cnt = (uint32_t)(v.head[c + 1] - v.head[c] - co + ci);
     cin = (uint32_t)ci;
-  }
-  if (v.wstate[kWarmFallback]) {  // block-uniform: clear the touch flags too, then stop
-    if (x < B && v.wtouch[c]) v.wtouch[c] = 0;
-    return;
   }
   if (y == 0 && x == 0) *v.nocc = 0;  // the occupied-leaf counter k_warm_finish fills
   uint32_t tot, itot;
@@ -471,7 +464,6 @@ __global__ __launch_bounds__(kRowThreads) void k_warm_rowscan(Sys sys) {
   __shared__ uint32_t sh[kRowWaves];
   int blk, nblk;
   const SysView v = sys.get(blk, nblk);
-  if (v.wstate[kWarmFallback]) return;
   const int B = v.B;
   const int x = threadIdx.x;
   const uint32_t t = x < B ? (uint32_t)v.wrowtot[x] : 0u;
@@ -500,7 +492,6 @@ __global__ __launch_bounds__(kRowThreads) void k_warm_finish(Sys sys, int zero_e
   __shared__ uint32_t s_base, s_ibase, s_occ_base, s_touch_base;
   int y, nrows;
   const SysView v = sys.get(y, nrows);
-  if (v.wstate[kWarmFallback]) return;
   const int B = v.B;
   const int x = threadIdx.x;
   if (prescanned) {
@@ -556,7 +547,6 @@ template <class Sys>
 __global__ __launch_bounds__(256) void k_warm_place(Sys sys) {
   int blk, nblk;
   const SysView v = sys.get(blk, nblk);
-  if (v.wstate[kWarmFallback]) return;
   const int i = blk * 256 + threadIdx.x;
   if (i >= v.n) return;
   const uint32_t t = v.tkey[i];
@@ -570,15 +560,20 @@ __global__ __launch_bounds__(256) void k_warm_place(Sys sys) {
   }
 }
 
+// Original index of the particle a tpart entry holds.
+__device__ __forceinline__ uint32_t orig_of(const float4& p) { return __float_as_uint(p.w); }
+
 // One wave per touched leaf (grid-stride over the list): the leaf's stayers
-// (old slots [first, first+count) whose tkey is still this leaf) and its
-// incoming movers (at most kWarmMaxIn, one per lane) are merged by original
-// index.  Clears the leaf's touch flag and group count.
+// (old slots [first, first+count) whose tkey is still this leaf, already in
+// original-index order) and its incoming movers (any number, grouped at
+// wmgrp[winoff]) are merged by original index; indices are unique, so
+// counting the smaller ones gives each particle's exact slot.  Up to 64
+// movers (one per lane) are ranked in one pass over the stayers; more are
+// taken 64 at a time.  Clears the leaf's touch flag and group count.
 template <class Sys>
 __global__ __launch_bounds__(256) void k_warm_merge(Sys sys) {
   int blk, nblk;
   const SysView v = sys.get(blk, nblk);
-  if (v.wstate[kWarmFallback]) return;
   const int lane = lane_id();
   const int waves = nblk * (256 / kWave);
   const int nt = v.wstate[kWarmTouched];
@@ -588,47 +583,93 @@ __global__ __launch_bounds__(256) void k_warm_merge(Sys sys) {
     const int2 od = v.wtold[c];
     const int h = v.head[c];
     const int ni = v.wfill[c];
-    const int ms = lane < ni ? v.wmgrp[v.winoff[c] + lane] : 0;
-    const uint32_t om = lane < ni ? __float_as_uint(v.tpart[ms].w) : 0xffffffffu;
-    int rank_m = 0, cnt_m = 0;
-    for (int l = 0; l < ni; ++l) rank_m += __shfl(om, l, kWave) < om ? 1 : 0;
-    int before = 0;
-    for (int b = 0; b < od.y; b += kWave) {
-      const int slot = od.x + b + lane;
-      const bool stay = b + lane < od.y && v.tkey[slot] == c;
-      const uint32_t os = stay ? __float_as_uint(v.tpart[slot].w) : 0u;
-      int cnt_s = 0;
-      for (int l = 0; l < ni; ++l) {
-        const uint32_t o = __shfl(om, l, kWave);
-        cnt_s += o < os ? 1 : 0;
-        const int lt = __popcll(__ballot(stay && os < o));
-        if (lane == l) cnt_m += lt;
+    const int* grp = v.wmgrp + v.winoff[c];
+    if (ni <= kWave) {
+      const int ms = lane < ni ? grp[lane] : 0;
+      const uint32_t om = lane < ni ? orig_of(v.tpart[ms]) : 0xffffffffu;
+      int rank_m = 0, cnt_m = 0;
+      for (int l = 0; l < ni; ++l) rank_m += __shfl(om, l, kWave) < om ? 1 : 0;
+      int before = 0;
+      for (int b = 0; b < od.y; b += kWave) {
+        const int slot = od.x + b + lane;
+        const bool stay = b + lane < od.y && v.tkey[slot] == c;
+        const uint32_t os = stay ? orig_of(v.tpart[slot]) : 0u;
+        int cnt_s = 0;
+        for (int l = 0; l < ni; ++l) {
+          const uint32_t o = __shfl(om, l, kWave);
+          cnt_s += o < os ? 1 : 0;
+          const int lt = __popcll(__ballot(stay && os < o));
+          if (lane == l) cnt_m += lt;
+        }
+        const uint64_t sm = __ballot(stay);
+        if (stay) {
+          const int dst = h + before + __popcll(sm & below) + cnt_s;
+          v.key[dst] = c;
+          v.part[dst] = v.tpart[slot];
+        }
+        before += __popcll(sm);
       }
-      const uint64_t sm = __ballot(stay);
-      if (stay) {
-        const int dst = h + before + __popcll(sm & below) + cnt_s;
+      if (lane < ni) {
+        const int dst = h + rank_m + cnt_m;
         v.key[dst] = c;
-        v.part[dst] = v.tpart[slot];
+        v.part[dst] = v.tpart[ms];
       }
-      before += __popcll(sm);
-    }
-    if (lane < ni) {
-      const int dst = h + rank_m + cnt_m;
-      v.key[dst] = c;
-      v.part[dst] = v.tpart[ms];
+    } else {
+      // stayers: their rank among the stayers plus the movers below them
+      int before = 0;
+      for (int b = 0; b < od.y; b += kWave) {
+        const int slot = od.x + b + lane;
+        const bool stay = b + lane < od.y && v.tkey[slot] == c;
+        const uint32_t os = stay ? orig_of(v.tpart[slot]) : 0u;
+        int cnt_s = 0;
+        for (int m1 = 0; m1 < ni; m1 += kWave) {
+          const uint32_t o2 = m1 + lane < ni ? orig_of(v.tpart[grp[m1 + lane]]) : 0xffffffffu;
+          const int cnt = min(kWave, ni - m1);
+          for (int l = 0; l < cnt; ++l) cnt_s += __shfl(o2, l, kWave) < os ? 1 : 0;
+        }
+        const uint64_t sm = __ballot(stay);
+        if (stay) {
+          const int dst = h + before + __popcll(sm & below) + cnt_s;
+          v.key[dst] = c;
+          v.part[dst] = v.tpart[slot];
+        }
+        before += __popcll(sm);
+      }
+      // movers, 64 at a time: the movers and the stayers below each
+      for (int m0 = 0; m0 < ni; m0 += kWave) {
+        const bool mv = m0 + lane < ni;
+        const int ms = mv ? grp[m0 + lane] : 0;
+        const uint32_t om = mv ? orig_of(v.tpart[ms]) : 0xffffffffu;
+        const int cm = min(kWave, ni - m0);
+        int rank = 0;
+        for (int m1 = 0; m1 < ni; m1 += kWave) {
+          const uint32_t o2 = m1 + lane < ni ? orig_of(v.tpart[grp[m1 + lane]]) : 0xffffffffu;
+          const int cnt = min(kWave, ni - m1);
+          for (int l = 0; l < cnt; ++l) rank += __shfl(o2, l, kWave) < om ? 1 : 0;
+        }
+        for (int b = 0; b < od.y; b += kWave) {
+          const int slot = od.x + b + lane;
+          const bool stay = b + lane < od.y && v.tkey[slot] == c;
+          const uint32_t os = stay ? orig_of(v.tpart[slot]) : 0u;
+          for (int l = 0; l < cm; ++l) {
+            // every lane takes part in the shuffle: lane l need not be a stayer
+            const uint32_t o = __shfl(om, l, kWave);
+            const int lt = __popcll(__ballot(stay && os < o));
+            if (lane == l) rank += lt;
+          }
+        }
+        if (mv) {
+          const int dst = h + rank;
+          v.key[dst] = c;
+          v.part[dst] = v.tpart[ms];
+        }
+      }
     }
     if (lane == 0) {
       v.wtouch[c] = 0;
       v.wfill[c] = 0;
     }
   }
-}
-
-template <class Sys>
-__global__ void k_warm_end(Sys sys) {
-  int blk, nblk;
-  const SysView v = sys.get(blk, nblk);
-  if (threadIdx.x < kWarmInts) v.wstate[threadIdx.x] = 0;
 }
 
 // ---------------------------------------------------------------- launchers
@@ -655,8 +696,7 @@ hipError_t launch_predo_batch(const BatchLaunch& b, const CallIO& io, hipStream_
   return hipGetLastError();
 }
 
-// Warm-started predo (key/part/head hold an earlier sort of the same n),
-// followed by the cold passes, which run only if the warm passes fell back.
+// Warm-started predo (key/part/head hold an earlier sort of the same n).
 hipError_t launch_predo_warm(const SysView& v, const CallIO& io, int zero_empty, hipStream_t s) {
   if (v.n <= 0) return launch_predo(v, io, zero_empty, s);
   const OneSys one{v};
@@ -667,11 +707,6 @@ hipError_t launch_predo_warm(const SysView& v, const CallIO& io, int zero_empty,
   hipLaunchKernelGGL(k_warm_place<OneSys>, dim3(nb), dim3(256), 0, s, one);
   hipLaunchKernelGGL(k_warm_merge<OneSys>, dim3(cdiv(std::min(kWarmMergeWaves, v.B * v.B), 256 / kWave)),
                      dim3(256), 0, s, one);
-  SysView cv = v;
-  cv.cold_if = v.wstate + kWarmFallback;
-  const hipError_t e = launch_predo(cv, io, zero_empty, s);
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_warm_end<OneSys>, dim3(1), dim3(64), 0, s, one);
   return hipGetLastError();
 }
 
@@ -682,11 +717,6 @@ hipError_t launch_predo_warm_batch(const BatchLaunch& b, const CallIO& io, hipSt
   hipLaunchKernelGGL(k_warm_finish<ManySys>, dim3(b.rows.blocks), dim3(kRowThreads), 0, s, b.rows.sys, 1, 1);
   hipLaunchKernelGGL(k_warm_place<ManySys>, dim3(b.adj_sort.blocks), dim3(256), 0, s, b.adj_sort.sys);
   hipLaunchKernelGGL(k_warm_merge<ManySys>, dim3(b.warm_merge.blocks), dim3(256), 0, s, b.warm_merge.sys);
-  hipLaunchKernelGGL(k_row_up<ManySys>, dim3(b.tiles_g.blocks), dim3(kSortThreads), 0, s, b.tiles_g.sys, io);
-  hipLaunchKernelGGL(k_row_scan<ManySys>, dim3(b.rows_g.blocks), dim3(kScanThreads), 0, s, b.rows_g.sys);
-  hipLaunchKernelGGL(k_row_down<ManySys>, dim3(b.tiles_g.blocks), dim3(kSortThreads), 0, s, b.tiles_g.sys, io);
-  hipLaunchKernelGGL(k_row_sort<ManySys>, dim3(b.rows_g.blocks), dim3(kRowThreads), 0, s, b.rows_g.sys, 1);
-  hipLaunchKernelGGL(k_warm_end<ManySys>, dim3(b.one.blocks), dim3(64), 0, s, b.one.sys);
   return hipGetLastError();
 }
 

@@ -2,8 +2,9 @@
 previous sort of the same N.  Everything they produce (forces, states, the
 sorted buffers) must equal, bit for bit, a solver that sorts from scratch
 every call, whatever the particles did in between: small drifts (the warm
-passes place the few movers), large jumps and crowded leaves (the device
-falls back to the cold sort), buffer writes and changes of N (cold again)."""
+passes place the few movers), large jumps, crowded leaves (more than 64
+movers into one leaf, merged 64 at a time), buffer writes and changes of N
+(cold again)."""
 import numpy as np
 import pytest
 import torch
@@ -75,8 +76,8 @@ def test_graph_run_bitwise_equal_cold():
 @pytest.mark.parametrize("jump", ["drift", "shuffle", "crowd", "weights"])
 def test_solve_sequences(jump):
     """solve at x1, then at x2: small drift (warm placement), unrelated
-    positions (fallback on the mover count), half the particles into one
-    leaf (fallback on a leaf's movers), new masses only (no movers)."""
+    positions (most particles move), half the particles into one leaf (the
+    merge of a leaf with thousands of movers), new masses only (no movers)."""
     from nbfmm_amd.workloads import disk
     wl = disk(n=20000)
     warm, cold = pair(wl)
@@ -193,7 +194,7 @@ def _batch_pair(bw):
 
 def test_batch_warm_equals_cold():
     """Every system of a batch starts its sort from the batch's previous sort:
-    solves with drift and with shuffled positions (fallback in that system),
+    solves with drift and with shuffled positions (most particles move),
     then graph runs, all bitwise equal to a batch that sorts cold."""
     from nbfmm_amd.workloads import many_disks
     bw = many_disks(num_systems=6, n_min=500, n_max=30000, seed=13, steps=4)
