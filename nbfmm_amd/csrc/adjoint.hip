@@ -126,7 +126,7 @@ __global__ __launch_bounds__(kAdjTile * kAdjTile) void k_adj_up_tile(Sys sys) {
       const float2 c = ch[(2 * y + 1) * Dc + 2 * x], d = ch[(2 * y + 1) * Dc + 2 * x + 1];
       const float2 m = make_float2(a.x + b.x + c.x + d.x, a.y + b.y + c.y + d.y);
       lv[cur ^ 1][t] = m;
-      v.gup[v.lvl_off[l] + ((Y0 >> l) + y) * (B >> l) + (X0 >> l) + x] = m;
+      v.gup[v.lvl_off(l) + ((Y0 >> l) + y) * (B >> l) + (X0 >> l) + x] = m;
     }
     cur ^= 1;
     __syncthreads();
@@ -141,12 +141,12 @@ __global__ __launch_bounds__(1024) void k_adj_up_top(Sys sys) {
   while ((1 << (first - 1)) < (v.B < kAdjTile ? v.B : kAdjTile)) ++first;
   for (int l = first; l < v.L; ++l) {
     const int D = v.B >> l, Dc = D * 2;
-    const float2* ch = v.gup + v.lvl_off[l - 1];
+    const float2* ch = v.gup + v.lvl_off(l - 1);
     for (int c = threadIdx.x; c < D * D; c += blockDim.x) {
       const int x = c % D, y = c / D;
       const float2 a = ch[(2 * y) * Dc + 2 * x], b = ch[(2 * y) * Dc + 2 * x + 1];
       const float2 cc = ch[(2 * y + 1) * Dc + 2 * x], d = ch[(2 * y + 1) * Dc + 2 * x + 1];
-      v.gup[v.lvl_off[l] + c] = make_float2(a.x + b.x + cc.x + d.x, a.y + b.y + cc.y + d.y);
+      v.gup[v.lvl_off(l) + c] = make_float2(a.x + b.x + cc.x + d.x, a.y + b.y + cc.y + d.y);
     }
     __syncthreads();
   }
@@ -162,14 +162,14 @@ __global__ __launch_bounds__(256) void k_adj_m2l(Sys sys) {
   int blk, nblk;
   const SysView v = sys.get(blk, nblk);
   const int t = blk * blockDim.x + threadIdx.x;
-  if (t >= v.lvl_off[v.L]) return;
+  if (t >= v.lvl_off(v.L)) return;
   int l = 0;
-  while (t >= v.lvl_off[l + 1]) ++l;
+  while (t >= v.lvl_off(l + 1)) ++l;
   const int D = v.B >> l;
-  const int c = t - v.lvl_off[l];
+  const int c = t - v.lvl_off(l);
   const int tx = c % D, ty = c / D;
-  const float4* mp = v.mpole + v.lvl_off[l];
-  const float2* gu = v.gup + v.lvl_off[l];
+  const float4* mp = v.mpole + v.lvl_off(l);
+  const float2* gu = v.gup + v.lvl_off(l);
   const float4 tc = mp[c];
   const float2 tg = gu[c];
   const int px = tx & ~1, py = ty & ~1;
@@ -202,10 +202,10 @@ __global__ __launch_bounds__(256) void k_adj_m2l(Sys sys) {
 // per lane (lanes of one leaf load the same cells).
 __device__ __forceinline__ float4 leaf_multipole_grad(const SysView& v, int cx, int cy) {
   int l = v.L - 1;
-  float4 T = v.gmp[v.lvl_off[l] + (cy >> l) * (v.B >> l) + (cx >> l)];
+  float4 T = v.gmp[v.lvl_off(l) + (cy >> l) * (v.B >> l) + (cx >> l)];
   for (l = v.L - 2; l >= 0; --l) {
-    const int ci = v.lvl_off[l] + (cy >> l) * (v.B >> l) + (cx >> l);
-    const int pi = v.lvl_off[l + 1] + (cy >> (l + 1)) * (v.B >> (l + 1)) + (cx >> (l + 1));
+    const int ci = v.lvl_off(l) + (cy >> l) * (v.B >> l) + (cx >> l);
+    const int pi = v.lvl_off(l + 1) + (cy >> (l + 1)) * (v.B >> (l + 1)) + (cx >> (l + 1));
     const float4 mc = v.mpole[ci], mp = v.mpole[pi];
     const float4 d = v.gmp[ci];
     float4 n = make_float4(d.x, d.y, d.z + T.z, 0.0f);

@@ -252,7 +252,10 @@ int nbfmm_solver_create(int num_level, int max_num_particle, const float positio
   const int levels = s->L < 1 ? 1 : s->L;
   int off = 0;
   for (int l = 0; l <= kMaxLevel; ++l) {
-    v.lvl_off[l] = off;
+    if (v.lvl_off(l) != off) {  // the closed form against the running sum; cannot differ
+      delete s;
+      return fail(NBFMM_ERR_INVALID_ARGUMENT, "level offsets");
+    }
     if (l < levels) off += (s->B >> l) * (s->B >> l);
   }
   const int ncells_tree = off;
@@ -1034,7 +1037,11 @@ int nbfmm_batch_create(int S, const int* num_particle, const int* num_level, con
     const int levels = v.L < 1 ? 1 : v.L;
     int off = 0;
     for (int l = 0; l <= kMaxLevel; ++l) {
-      v.lvl_off[l] = off;
+      if (v.lvl_off(l) != off) {  // the closed form against the running sum; cannot differ
+        batch_free(b);
+        delete b;
+        return fail(NBFMM_ERR_INVALID_ARGUMENT, "level offsets");
+      }
       if (l < levels) off += (v.B >> l) * (v.B >> l);
     }
     const size_t C = (size_t)v.B * v.B;

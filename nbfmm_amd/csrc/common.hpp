@@ -56,7 +56,6 @@ struct SysView {
   float2* gsort;    // upstream gradient of the effects, sorted order
   float2* gup;      // per level: sum of the upstream gradient over the cell's particles
   float4* gmp;      // per level: gradient wrt the cell's COM (x, y) and mass (z) from M2L
-  int lvl_off[kMaxLevel + 1];  // start of level l in mpole/local; lvl_off[L] = total
   uint32_t* flags;  // NBFMM_FLAG_* bits
   // warm-started sort (the previous sort's key/part/head as the starting
   // order); every array is left cleared by the pass that last uses it
@@ -72,6 +71,18 @@ struct SysView {
   int* winoff;      // C: first slot of each leaf's movers in wmgrp
   int* wfill;       // C: movers placed into each leaf's group
   int* wrowtot;     // 2 (B + 1): row totals of the new counts and of the movers
+
+  // Start of level l in mpole/local; lvl_off(L) = total.  The dense levels
+  // k < max(L, 1) hold (B >> k)^2 = 4^(lgB - k) cells each, so
+  //   lvl_off(l) = sum_{k < l} 4^(lgB - k) = (4^(lgB + 1) - 4^(lgB - l + 1)) / 3
+  // (exact: 4^a - 4^b is a multiple of 3), with l clamped to max(L, 1).  A
+  // function rather than a table, so that no kernel indexes into the view at
+  // run time and a view read from device memory stays in registers.
+  __host__ __device__ __forceinline__ int lvl_off(int l) const {
+    const int levels = L < 1 ? 1 : L;
+    const int m = l < levels ? l : levels;
+    return (int)((((uint32_t)1 << (2 * lgB + 2)) - ((uint32_t)1 << (2 * (lgB - m) + 2))) / 3u);
+  }
 };
 
 // Warm-started sort.

@@ -27,9 +27,9 @@ static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 __device__ __forceinline__ float2 leaf_local(const SysView& v, int cx, int cy) {
 #pragma clang fp contract(off)
   int l = v.L - 1;
-  float2 acc = v.local[v.lvl_off[l] + (cy >> l) * (v.B >> l) + (cx >> l)];
+  float2 acc = v.local[v.lvl_off(l) + (cy >> l) * (v.B >> l) + (cx >> l)];
   for (l = v.L - 2; l >= 0; --l) {
-    const float2 e = v.local[v.lvl_off[l] + (cy >> l) * (v.B >> l) + (cx >> l)];
+    const float2 e = v.local[v.lvl_off(l) + (cy >> l) * (v.B >> l) + (cx >> l)];
     acc.x = e.x + acc.x;
     acc.y = e.y + acc.y;
   }

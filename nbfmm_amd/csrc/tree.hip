@@ -49,12 +49,12 @@ __global__ __launch_bounds__(256) void k_m2m(SysView v, int l) {
   if (c >= D * D) return;
   const int X = c % D, Y = c / D;
   const int Dc = D * 2;
-  const float4* ch = v.mpole + v.lvl_off[l - 1];
+  const float4* ch = v.mpole + v.lvl_off(l - 1);
   const float4 c00 = ch[(2 * Y) * Dc + 2 * X];
   const float4 c10 = ch[(2 * Y) * Dc + 2 * X + 1];
   const float4 c01 = ch[(2 * Y + 1) * Dc + 2 * X];
   const float4 c11 = ch[(2 * Y + 1) * Dc + 2 * X + 1];
-  v.mpole[v.lvl_off[l] + c] = m2m_combine(c00, c10, c01, c11);
+  v.mpole[v.lvl_off(l) + c] = m2m_combine(c00, c10, c01, c11);
 }
 
 // Parent multipole from its four children (m2m.cu:40-53), the expression
@@ -163,7 +163,7 @@ __global__ __launch_bounds__(kUpTile * kUpTile) void k_m2m_tile(Sys sys) {
       const float4 m = m2m_combine(ch[(2 * y) * Dc + 2 * x], ch[(2 * y) * Dc + 2 * x + 1],
                                    ch[(2 * y + 1) * Dc + 2 * x], ch[(2 * y + 1) * Dc + 2 * x + 1]);
       lv[cur ^ 1][t] = m;
-      v.mpole[v.lvl_off[l] + ((Y0 >> l) + y) * (B >> l) + (X0 >> l) + x] = m;
+      v.mpole[v.lvl_off(l) + ((Y0 >> l) + y) * (B >> l) + (X0 >> l) + x] = m;
     }
     cur ^= 1;
     __syncthreads();
@@ -179,10 +179,10 @@ __global__ __launch_bounds__(1024) void k_m2m_top(Sys sys) {
   while ((1 << (first - 1)) < (v.B < kUpTile ? v.B : kUpTile)) ++first;
   for (int l = first; l < v.L; ++l) {
     const int D = v.B >> l, Dc = D * 2;
-    const float4* ch = v.mpole + v.lvl_off[l - 1];
+    const float4* ch = v.mpole + v.lvl_off(l - 1);
     for (int c = threadIdx.x; c < D * D; c += blockDim.x) {
       const int x = c % D, y = c / D;
-      v.mpole[v.lvl_off[l] + c] = m2m_combine(ch[(2 * y) * Dc + 2 * x], ch[(2 * y) * Dc + 2 * x + 1],
+      v.mpole[v.lvl_off(l) + c] = m2m_combine(ch[(2 * y) * Dc + 2 * x], ch[(2 * y) * Dc + 2 * x + 1],
                                               ch[(2 * y + 1) * Dc + 2 * x], ch[(2 * y + 1) * Dc + 2 * x + 1]);
     }
     __syncthreads();
@@ -201,13 +201,13 @@ __global__ __launch_bounds__(256) void k_m2l(Sys sys) {
   int blk, nblk;
   const SysView v = sys.get(blk, nblk);
   const int t = blk * blockDim.x + threadIdx.x;
-  if (t >= v.lvl_off[v.L]) return;
+  if (t >= v.lvl_off(v.L)) return;
   int l = 0;
-  while (t >= v.lvl_off[l + 1]) ++l;
+  while (t >= v.lvl_off(l + 1)) ++l;
   const int D = v.B >> l;
-  const int c = t - v.lvl_off[l];
+  const int c = t - v.lvl_off(l);
   const int tx = c % D, ty = c / D;
-  const float4* mp = v.mpole + v.lvl_off[l];
+  const float4* mp = v.mpole + v.lvl_off(l);
   const float4 tc = mp[c];
   const int px = tx & ~1, py = ty & ~1;
   const float eps = v.eps;
@@ -237,11 +237,11 @@ __global__ __launch_bounds__(256) void k_l2l(SysView v, int l) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= D * D) return;
   const int X = c % D, Y = c / D;
-  const float2 p = v.local[v.lvl_off[l + 1] + (Y >> 1) * (D >> 1) + (X >> 1)];
-  float2 e = v.local[v.lvl_off[l] + c];
+  const float2 p = v.local[v.lvl_off(l + 1) + (Y >> 1) * (D >> 1) + (X >> 1)];
+  float2 e = v.local[v.lvl_off(l) + c];
   e.x = e.x + p.x;
   e.y = e.y + p.y;
-  v.local[v.lvl_off[l] + c] = e;
+  v.local[v.lvl_off(l) + c] = e;
 }
 
 // Reference cube [levels][B][B] <-> dense levels.
@@ -255,7 +255,7 @@ __global__ __launch_bounds__(256) void k_cube_read(SysView v, int which, void* d
   const int s = 1 << l;
   const bool slot = l < v.L && (row % s) == 0 && (col % s) == 0;
   const int D = B >> l;
-  const int d = slot ? v.lvl_off[l] + (row / s) * D + (col / s) : 0;
+  const int d = slot ? v.lvl_off(l) + (row / s) * D + (col / s) : 0;
   if (which == 0) {
     const float4 m = slot ? v.mpole[d] : make_float4(0, 0, 0, 0);
     ((float2*)dst)[i] = make_float2(m.x, m.y);
@@ -276,7 +276,7 @@ __global__ __launch_bounds__(256) void k_cube_write(SysView v, int which, const 
   const int s = 1 << l;
   if (!(l < v.L && (row % s) == 0 && (col % s) == 0)) return;
   const int D = B >> l;
-  const int d = v.lvl_off[l] + (row / s) * D + (col / s);
+  const int d = v.lvl_off(l) + (row / s) * D + (col / s);
   if (which == 0) {
     const float2 p = ((const float2*)src)[i];
     v.mpole[d].x = p.x;
@@ -307,7 +307,7 @@ int upward_top_blocks(int L, int B) {
   return (L - 1 > lgT) ? 1 : 0;
 }
 
-int m2l_blocks(const SysView& v) { return v.L > 0 ? cdiv(v.lvl_off[v.L], 256) : 0; }
+int m2l_blocks(const SysView& v) { return v.L > 0 ? cdiv(v.lvl_off(v.L), 256) : 0; }
 
 hipError_t launch_upward(const SysView& v, int max_occ, hipStream_t s) {
   if (v.L <= 0) return hipSuccess;
