@@ -319,8 +319,7 @@ int nbfmm_batch_get_p2p_config(const nbfmm_batch* batch, int* variant, int* bloc
  * not depend on the shares, only the time does.  Synchronises.  No
  * reference counterpart. */
 int nbfmm_batch_balance(nbfmm_batch* batch, const float* d_position, const float* d_weight, void* stream);
-/* Warm-started sort of every system (see nbfmm_set_warm_start); off by default
- * for batches (many systems already keep the cold sort busy). */
+/* Warm-started sort of every system (see nbfmm_set_warm_start); on by default. */
 int nbfmm_batch_set_warm_start(nbfmm_batch* batch, int enable);
 /* nbfmm_solve_adjoint of every system (concatenated arrays). */
 int nbfmm_batch_solve_adjoint(nbfmm_batch* batch, const float* d_position_origin, const float* d_weight_origin,

@@ -65,8 +65,8 @@ class Batch:
         check(lib().nbfmm_batch_set_p2p_config(self._h, int(variant), int(blocks_per_cu)))
 
     def warm_start(self, enable=True):
-        """Start each sort from the previous one (default off for batches);
-        results are identical either way."""
+        """Start each sort from the previous one (on by default); results are
+        identical either way."""
         check(lib().nbfmm_batch_set_warm_start(self._h, int(bool(enable))))
 
     def balance(self, position, weight, stream=None):
