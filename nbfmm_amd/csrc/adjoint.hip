@@ -276,21 +276,45 @@ __global__ __launch_bounds__(kAdjP2PWaves * kWave) void k_adj_p2p(Sys sys, CallI
       const int b = valid ? hr[(xt < B - 1 ? xt + 1 : B - 1) + 1] - us : 0;
       for (int w0 = 0; w0 < ulen; w0 += kP2PStage) {
         const int cnt = min(kP2PStage, ulen - w0);
+        // as the forward P2P: every load of the chunk in flight at once
+        // (indices clamped to its last source, cnt >= 1), then unconditional
+        // LDS writes; slots from cnt on are never read
+        {
+          float4 stp[kP2PStage / kWave];
+          float2 stg[kP2PStage / kWave];
 #pragma unroll
-        for (int q = 0; q < kP2PStage / kWave; ++q) {
-          const int slot = q * kWave + lane;
-          if (slot < cnt) {
-            sp[slot] = v.part[us + w0 + slot];
-            sg[slot] = v.gsort[us + w0 + slot];
+          for (int q = 0; q < kP2PStage / kWave; ++q) {
+            const int k = us + w0 + min(q * kWave + lane, cnt - 1);
+            stp[q] = v.part[k];
+            stg[q] = v.gsort[k];
+          }
+#pragma unroll
+          for (int q = 0; q < kP2PStage / kWave; ++q) {
+            sp[q * kWave + lane] = stp[q];
+            sg[q * kWave + lane] = stg[q];
           }
         }
         wave_lds_order();
         const int lo = min(max(a - w0, 0), cnt);
         const int hi = min(max(b - w0, 0), cnt);
-        for (int k = lo; k < hi; ++k) {
-          const float4 s = sp[k];
-          const float2 g = sg[k];
-          pair_adjoint(pt.x, pt.y, pt.z, gt.x, gt.y, s.x, s.y, s.z, g.x, g.y, eps, gx, gy, gm);
+        // four sources per iteration, added in the same order as one at a time
+        int k = lo;
+        for (; k + 4 <= hi; k += 4) {
+          float4 s4[4];
+          float2 g4[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            s4[u] = sp[k + u];
+            g4[u] = sg[k + u];
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+            pair_adjoint(pt.x, pt.y, pt.z, gt.x, gt.y, s4[u].x, s4[u].y, s4[u].z, g4[u].x, g4[u].y, eps, gx, gy, gm);
+        }
+        for (; k < hi; ++k) {
+          const float4 s1 = sp[k];
+          const float2 g1 = sg[k];
+          pair_adjoint(pt.x, pt.y, pt.z, gt.x, gt.y, s1.x, s1.y, s1.z, g1.x, g1.y, eps, gx, gy, gm);
         }
         wave_lds_order();
       }

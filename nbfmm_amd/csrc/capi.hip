@@ -817,7 +817,8 @@ struct nbfmm_batch {
   int total_n = 0;
   std::vector<int> part_off;      // S+1
   std::vector<SysView> hv;        // host copies of the views
-  SysView* dv = nullptr;          // device copies
+  std::vector<SysDesc> hdesc;     // per-system sizes and array offsets
+  SysDesc* ddesc = nullptr;       // device copy
   BatchLaunch bl{};
   // On by default: measured on 256 systems (7.1M particles) a batched step
   // takes 1.23 ms with the warm-started sort and 1.26 ms with the cold one.
@@ -874,9 +875,8 @@ void drop_batch_graph(nbfmm_batch* b) {
   b->gkey.valid = false;
 }
 
-// Builds a launch map from the per-system block counts (over the device
-// views vs, by default the plain ones).
-int make_map(nbfmm_batch* b, const std::vector<int>& blocks, KernelMap* km, const SysView* vs = nullptr) {
+// Builds a launch map from the per-system block counts.
+int make_map(nbfmm_batch* b, const std::vector<int>& blocks, KernelMap* km) {
   const int S = (int)blocks.size();
   std::vector<int> start(S + 1, 0);
   for (int s = 0; s < S; ++s) start[s + 1] = start[s] + blocks[s];
@@ -888,7 +888,8 @@ int make_map(nbfmm_batch* b, const std::vector<int>& blocks, KernelMap* km, cons
   if (rc) return rc;
   NB_HIP(hipMemcpy(d, start.data(), (S + 1) * sizeof(int), hipMemcpyHostToDevice));
   NB_HIP(hipMemcpy(d + S + 1, sys.data(), sys.size() * sizeof(int), hipMemcpyHostToDevice));
-  km->sys.vs = vs ? vs : b->dv;
+  km->sys.base = b->hv[0];  // system 0's arrays start at every array's base
+  km->sys.desc = b->ddesc;
   km->sys.bstart = d;
   km->sys.bsys = d + S + 1;
   km->blocks = start[S];
@@ -1082,7 +1083,7 @@ int nbfmm_batch_create(int S, const int* num_particle, const int* num_level, con
   BALLOC(gup, tree_all);
   BALLOC(gmp, tree_all);
   BALLOC(flags, (size_t)S);
-  BALLOC(b->dv, (size_t)S);
+  BALLOC(b->ddesc, (size_t)S);
   int *wstate, *wrel, *wmgrp, *wcin, *wcout, *wtouch, *wtlist, *wcnew, *winoff, *wfill, *wrowtot;
   int2* wtold;
   BALLOC(b->xs, n_all);
@@ -1132,8 +1133,29 @@ int nbfmm_batch_create(int S, const int* num_particle, const int* num_level, con
     v.wtold = wtold + cell_off[s];
     v.wrowtot = wrowtot + 2 * row_off[s];
   }
-  hipError_t e = hipMemcpy(b->dv, b->hv.data(), S * sizeof(SysView), hipMemcpyHostToDevice);
-  if (e != hipSuccess) { batch_free(b); delete b; return hip_fail(e, "hipMemcpy views"); }
+  b->hdesc.resize(S);
+  for (int s = 0; s < S; ++s) {
+    const SysView& v = b->hv[s];
+    SysDesc& d = b->hdesc[s];
+    d.n = v.n;
+    d.L = v.L;
+    d.B = v.B;
+    d.lgB = v.lgB;
+    d.xmin = v.xmin;
+    d.ymin = v.ymin;
+    d.cw = v.cw;
+    d.ch = v.ch;
+    d.eps = v.eps;
+    d.tick = v.tick;
+    d.part_off = v.part_off;
+    d.head_off = (int)head_off[s];
+    d.row_off = (int)row_off[s];
+    d.cell_off = (int)cell_off[s];
+    d.hist_off = (int)hist_off[s];
+    d.tree_off = (int)tree_off[s];
+  }
+  hipError_t e = hipMemcpy(b->ddesc, b->hdesc.data(), S * sizeof(SysDesc), hipMemcpyHostToDevice);
+  if (e != hipSuccess) { batch_free(b); delete b; return hip_fail(e, "hipMemcpy system descriptors"); }
   // launch maps
   std::vector<int> tiles(S), rows(S), p2m(S), mtile(S), mtop(S), m2l(S), asort(S), azero(S), wmerge(S), one(S, 1);
   for (int s = 0; s < S; ++s) {
@@ -1190,9 +1212,9 @@ int nbfmm_batch_offsets(const nbfmm_batch* b, int* part_off) {
 
 int nbfmm_batch_set_ticks(nbfmm_batch* b, const float* ticks) {
   if (!b || !ticks) return fail(NBFMM_ERR_INVALID_ARGUMENT, "null argument");
-  for (int s = 0; s < b->S; ++s) b->hv[s].tick = ticks[s];
+  for (int s = 0; s < b->S; ++s) b->hv[s].tick = b->hdesc[s].tick = ticks[s];
   NB_HIP(hipDeviceSynchronize());
-  NB_HIP(hipMemcpy(b->dv, b->hv.data(), b->S * sizeof(SysView), hipMemcpyHostToDevice));
+  NB_HIP(hipMemcpy(b->ddesc, b->hdesc.data(), b->S * sizeof(SysDesc), hipMemcpyHostToDevice));
   return NBFMM_OK;
 }
 

@@ -127,8 +127,27 @@ struct OneSys {
   }
 };
 
+// One system of a batch: its sizes, its box and where its part of each of
+// the batch's concatenated arrays starts.
+struct SysDesc {
+  int n, L, B, lgB;
+  float xmin, ymin, cw, ch, eps, tick;
+  int part_off;  // particle arrays
+  int head_off;  // head (B*B + 1 per system)
+  int row_off;   // rowstart, dtotal (B + 1 per system); wrowtot at twice it
+  int cell_off;  // leaf arrays (B*B per system)
+  int hist_off;  // row-pass tile histograms
+  int tree_off;  // per-level cell arrays
+};
+
+// The kernels of a batch see one system per block.  Its view is the batch's
+// base view (the kernel argument, whose arrays start at system 0) moved to
+// the system's offsets: derived from kernel-argument pointers, every array
+// stays in the global address space, and the view in registers (a view read
+// whole from device memory would carry generic pointers, i.e. flat accesses).
 struct ManySys {
-  const SysView* vs;
+  SysView base;
+  const SysDesc* desc;
   const int* bsys;
   const int* bstart;
   __device__ __forceinline__ SysView get(int& blk, int& nblk) const {
@@ -137,7 +156,49 @@ struct ManySys {
     const int b0 = bstart[s];
     blk = b - b0;
     nblk = bstart[s + 1] - b0;
-    return vs[s];
+    const SysDesc d = desc[s];
+    SysView v = base;
+    v.n = d.n;
+    v.L = d.L;
+    v.B = d.B;
+    v.lgB = d.lgB;
+    v.xmin = d.xmin;
+    v.ymin = d.ymin;
+    v.cw = d.cw;
+    v.ch = d.ch;
+    v.eps = d.eps;
+    v.tick = d.tick;
+    v.part_off = d.part_off;
+    v.tkey += d.part_off;
+    v.tpart += d.part_off;
+    v.key += d.part_off;
+    v.part += d.part_off;
+    v.eff += d.part_off;
+    v.gsort += d.part_off;
+    v.wrel += d.part_off;
+    v.wmgrp += d.part_off;
+    v.head += d.head_off;
+    v.rowstart += d.row_off;
+    v.dtotal += d.row_off;
+    v.wrowtot += 2 * d.row_off;
+    v.occ += d.cell_off;
+    v.wcin += d.cell_off;
+    v.wcout += d.cell_off;
+    v.wtouch += d.cell_off;
+    v.wtlist += d.cell_off;
+    v.wcnew += d.cell_off;
+    v.winoff += d.cell_off;
+    v.wfill += d.cell_off;
+    v.wtold += d.cell_off;
+    v.hist += d.hist_off;
+    v.mpole += d.tree_off;
+    v.local += d.tree_off;
+    v.gup += d.tree_off;
+    v.gmp += d.tree_off;
+    v.nocc += s;
+    v.flags += s;
+    v.wstate += s * kWarmInts;
+    return v;
   }
 };
 

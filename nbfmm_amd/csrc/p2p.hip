@@ -93,10 +93,18 @@ __global__ __launch_bounds__(WAVES * kWave) void k_p2p(Sys sys, CallIO io) {
       const int b = valid ? hr[(xt < B - 1 ? xt + 1 : B - 1) + 1] - us : 0;
       for (int w0 = 0; w0 < ulen; w0 += STAGE) {
         const int cnt = min(STAGE, ulen - w0);
+        // all loads of the chunk in flight at once, then the LDS writes, with
+        // no branch for the compiler to sink a load into (a load under a
+        // branch waits before the next one is issued): indices are clamped
+        // to the chunk's last source (cnt >= 1), and the slots from cnt on,
+        // which hold copies of it, are never read
+        {
+          const float4* const gsrc = v.part + us + w0;
+          float4 stage[STAGE / kWave];
 #pragma unroll
-        for (int q = 0; q < STAGE / kWave; ++q) {
-          const int slot = q * kWave + lane;
-          if (slot < cnt) src[slot] = v.part[us + w0 + slot];
+          for (int q = 0; q < STAGE / kWave; ++q) stage[q] = gsrc[min(q * kWave + lane, cnt - 1)];
+#pragma unroll
+          for (int q = 0; q < STAGE / kWave; ++q) src[q * kWave + lane] = stage[q];
         }
         wave_lds_order();
         const int lo = min(max(a - w0, 0), cnt);
