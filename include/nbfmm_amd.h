@@ -315,7 +315,9 @@ int nbfmm_batch_set_p2p_config(nbfmm_batch* batch, int variant, int blocks_per_c
 int nbfmm_batch_get_p2p_config(const nbfmm_batch* batch, int* variant, int* blocks_per_cu);
 /* Sorts the given positions and shares the P2P grids (forward and adjoint)
  * among the systems in proportion to their P2P pairs plus 32 per particle.
- * nbfmm_batch_run does this by itself on its first graph run.  Results do
+ * The first solve, step, run or adjoint call of a batch does this by itself
+ * after its sort (one synchronisation; skipped while the stream is being
+ * captured), so an explicit call is needed only to re-balance.  Results do
  * not depend on the shares, only the time does.  Synchronises.  No
  * reference counterpart. */
 int nbfmm_batch_balance(nbfmm_batch* batch, const float* d_position, const float* d_weight, void* stream);

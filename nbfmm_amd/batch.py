@@ -71,7 +71,7 @@ class Batch:
 
     def balance(self, position, weight, stream=None):
         """Share the P2P grids by each system's pair count at these positions
-        (run() does it by itself on its first graph run); synchronises."""
+        (the batch's first call does it by itself); synchronises."""
         check(lib().nbfmm_batch_balance(self._h, _ptr(position), _ptr(weight), _stream(stream)))
 
     def p2p_config(self):

@@ -969,8 +969,18 @@ int batch_sort(nbfmm_batch* b, const CallIO& io, hipStream_t st) {
   return NBFMM_OK;
 }
 
+// The first call of a batch shares its P2P grids by the pairs of its first
+// sort (one synchronisation), unless the stream is being captured.
+int batch_balance_once(nbfmm_batch* b, hipStream_t st) {
+  if (b->balanced) return NBFMM_OK;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return NBFMM_OK;
+  return batch_balance(b, st);
+}
+
 int batch_pipeline(nbfmm_batch* b, int mode, const CallIO& io, hipStream_t st) {
   if (int rc = batch_sort(b, io, st)) return rc;
+  if (int rc = batch_balance_once(b, st)) return rc;
   NB_HIP(launch_tree_batch(b->bl, st));
   NB_HIP(launch_p2p_batch(b->bl, mode, io, b->p2p_variant, st));
   return NBFMM_OK;
@@ -1247,9 +1257,6 @@ int nbfmm_batch_run(nbfmm_batch* b, int steps, float* xcur, float* xpre, const f
   CallIO io = call_io(xcur, w, acc, xcur, xpre);
   io.xs_out = b->xs;
   if ((rc = batch_pipeline(b, kP2PStep, io, st))) return rc;
-  // the first graph run of a batch shares the P2P grid by the pairs of the
-  // sort just made (one synchronisation, before anything is captured)
-  if (!b->balanced && (rc = batch_balance(b, st))) return rc;
   if (--steps == 0) return NBFMM_OK;
   const bool warm = b->warm_enabled;
   auto& g = b->gkey;
@@ -1291,6 +1298,7 @@ int nbfmm_batch_solve_adjoint(nbfmm_batch* b, const float* pos0, const float* w0
   io.gw = grad_w;
   io.gscale = 1.0f;
   if (int rc = batch_sort(b, io, st)) return rc;
+  if (int rc = batch_balance_once(b, st)) return rc;
   NB_HIP(launch_tree_batch(b->bl, st));
   NB_HIP(launch_adjoint_batch(b->bl, io, 0, st));
   return NBFMM_OK;
@@ -1307,6 +1315,7 @@ int nbfmm_batch_step_adjoint(nbfmm_batch* b, const float* x, const float* w, flo
   io.gw = grad_w;
   io.gscale = -1.0f;  // each system's tick^2
   if (int rc = batch_sort(b, io, st)) return rc;
+  if (int rc = batch_balance_once(b, st)) return rc;
   NB_HIP(launch_tree_batch(b->bl, st));
   NB_HIP(launch_adjoint_batch(b->bl, io, 1, st));
   return NBFMM_OK;
