@@ -28,9 +28,11 @@ namespace nbfmm_amd {
 static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 
 // One pair of the gather-form adjoint; target (t, mt, gt), source (s, ms, gs).
+// The 3 r f^2 (d.h) d term is summed apart, in (cx, cy), and folded in once
+// per target by adjoint_fold: one multiply fewer per pair.
 __device__ __forceinline__ void pair_adjoint(float tx, float ty, float mt, float gtx, float gty, float sx, float sy,
                                              float ms, float gsx, float gsy, float eps, float& gx, float& gy,
-                                             float& gm) {
+                                             float& cx, float& cy, float& gm) {
 #pragma clang fp contract(off)
   const float dx = sx - tx;
   const float dy = sy - ty;
@@ -40,13 +42,19 @@ __device__ __forceinline__ void pair_adjoint(float tx, float ty, float mt, float
   const float hx = mt * gsx - ms * gtx;
   const float hy = mt * gsy - ms * gty;
   const float dh = __builtin_fmaf(dx, hx, dy * hy);
-  const float c2 = 3.0f * r * f * f * dh;
+  const float c = (r * f) * (f * dh);
   gx = __builtin_fmaf(f, hx, gx);
-  gx = __builtin_fmaf(-c2, dx, gx);
   gy = __builtin_fmaf(f, hy, gy);
-  gy = __builtin_fmaf(-c2, dy, gy);
+  cx = __builtin_fmaf(c, dx, cx);
+  cy = __builtin_fmaf(c, dy, cy);
   const float dg = __builtin_fmaf(dx, gsx, dy * gsy);
   gm = __builtin_fmaf(-f, dg, gm);
+}
+
+// gx -= 3 cx, gy -= 3 cy: the term pair_adjoint sums apart.
+__device__ __forceinline__ void adjoint_fold(float& gx, float& gy, float cx, float cy) {
+  gx = __builtin_fmaf(-3.0f, cx, gx);
+  gy = __builtin_fmaf(-3.0f, cy, gy);
 }
 
 // Level-0 subtree sums start at zero for every leaf.
@@ -174,7 +182,7 @@ __global__ __launch_bounds__(256) void k_adj_m2l(Sys sys) {
   const float2 tg = gu[c];
   const int px = tx & ~1, py = ty & ~1;
   const float eps = v.eps;
-  float gx = 0.0f, gy = 0.0f, gm = 0.0f;
+  float gx = 0.0f, gy = 0.0f, gm = 0.0f, cx = 0.0f, cy = 0.0f;
 #pragma clang loop unroll(full)
   for (int wy = -2; wy < 4; ++wy) {
     const int y = py + wy;
@@ -192,9 +200,11 @@ __global__ __launch_bounds__(256) void k_adj_m2l(Sys sys) {
     for (int k = 0; k < 6; ++k) {
       const int x = px - 2 + k;
       if (row_in && x >= 0 && x < D && (abs(x - tx) > 1 || abs(y - ty) > 1))
-        pair_adjoint(tc.x, tc.y, tc.z, tg.x, tg.y, sc[k].x, sc[k].y, sc[k].z, sg[k].x, sg[k].y, eps, gx, gy, gm);
+        pair_adjoint(tc.x, tc.y, tc.z, tg.x, tg.y, sc[k].x, sc[k].y, sc[k].z, sg[k].x, sg[k].y, eps, gx, gy, cx, cy,
+                     gm);
     }
   }
+  adjoint_fold(gx, gy, cx, cy);
   v.gmp[t] = make_float4(gx, gy, gm, 0.0f);
 }
 
@@ -264,7 +274,7 @@ __global__ __launch_bounds__(kAdjP2PWaves * kWave) void k_adj_p2p(Sys sys, CallI
     const int xt = (int)(v.key[ti] & (uint32_t)(B - 1));
     const int xf = __shfl(xt, 0, kWave);
     const int xl = __shfl(xt, nt - 1, kWave);
-    float gx = 0.0f, gy = 0.0f, gm = 0.0f;
+    float gx = 0.0f, gy = 0.0f, gm = 0.0f, cx = 0.0f, cy = 0.0f;
 #pragma unroll
     for (int dr = -1; dr <= 1; ++dr) {
       const int r = y + dr;
@@ -297,28 +307,35 @@ __global__ __launch_bounds__(kAdjP2PWaves * kWave) void k_adj_p2p(Sys sys, CallI
         wave_lds_order();
         const int lo = min(max(a - w0, 0), cnt);
         const int hi = min(max(b - w0, 0), cnt);
-        // four sources per iteration, added in the same order as one at a time
-        int k = lo;
-        for (; k + 4 <= hi; k += 4) {
+        // four sources per iteration, added in the same order as one at a
+        // time; loop control on the LDS address alone (pfull >= p, so the
+        // unsigned address compare never sees a bound below the buffer)
+        const float4* p = sp + lo;
+        const float2* q = sg + lo;
+        const float4* const pend = sp + hi;
+        const float4* const pfull = sp + max(hi - 3, lo);
+        for (; p < pfull; p += 4, q += 4) {
           float4 s4[4];
           float2 g4[4];
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
-            s4[u] = sp[k + u];
-            g4[u] = sg[k + u];
+            s4[u] = p[u];
+            g4[u] = q[u];
           }
 #pragma unroll
           for (int u = 0; u < 4; ++u)
-            pair_adjoint(pt.x, pt.y, pt.z, gt.x, gt.y, s4[u].x, s4[u].y, s4[u].z, g4[u].x, g4[u].y, eps, gx, gy, gm);
+            pair_adjoint(pt.x, pt.y, pt.z, gt.x, gt.y, s4[u].x, s4[u].y, s4[u].z, g4[u].x, g4[u].y, eps, gx, gy, cx,
+                         cy, gm);
         }
-        for (; k < hi; ++k) {
-          const float4 s1 = sp[k];
-          const float2 g1 = sg[k];
-          pair_adjoint(pt.x, pt.y, pt.z, gt.x, gt.y, s1.x, s1.y, s1.z, g1.x, g1.y, eps, gx, gy, gm);
+        for (; p < pend; ++p, ++q) {
+          const float4 s1 = *p;
+          const float2 g1 = *q;
+          pair_adjoint(pt.x, pt.y, pt.z, gt.x, gt.y, s1.x, s1.y, s1.z, g1.x, g1.y, eps, gx, gy, cx, cy, gm);
         }
         wave_lds_order();
       }
     }
+    adjoint_fold(gx, gy, cx, cy);
     if (valid) {
       if (v.L > 0) {
         // P2M^T: X_c = sum m x / M_c,  M_c = sum m

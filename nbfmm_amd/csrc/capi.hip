@@ -934,7 +934,7 @@ int batch_balance(nbfmm_batch* b, hipStream_t st) {
     b->total_work += b->work[s];
   }
   b->p2p_maps.clear();  // their device tables stay in b->allocs until the batch is destroyed
-  if (int rc = make_map(b, p2p_share(b, kAdjP2PWaves, b->cus * kAdjP2PBlocksPerCU), &b->bl.adj_p2p)) return rc;
+  if (int rc = make_map(b, p2p_share(b, kAdjP2PWaves, b->cus * kAdjP2PBatchBlocksPerCU), &b->bl.adj_p2p)) return rc;
   if (int rc = batch_use_p2p(b, b->p2p_variant, b->p2p_bpc)) return rc;
   drop_batch_graph(b);
   b->balanced = true;
@@ -1189,7 +1189,7 @@ int nbfmm_batch_create(int S, const int* num_particle, const int* num_level, con
       (rc = make_map(b, asort, &b->bl.adj_sort)) || (rc = make_map(b, azero, &b->bl.adj_zero)) ||
       (rc = make_map(b, wmerge, &b->bl.warm_merge)) || (rc = make_map(b, one, &b->bl.one)) ||
       (rc = batch_alloc(b, &b->d_pairs, (size_t)S)) ||
-      (rc = make_map(b, p2p_share(b, kAdjP2PWaves, cus * kAdjP2PBlocksPerCU), &b->bl.adj_p2p)) ||
+      (rc = make_map(b, p2p_share(b, kAdjP2PWaves, cus * kAdjP2PBatchBlocksPerCU), &b->bl.adj_p2p)) ||
       (rc = batch_use_p2p(b, b->p2p_variant, b->p2p_bpc))) {
     batch_free(b);
     delete b;

@@ -17,6 +17,9 @@ constexpr int kScanChunk = kScanThreads * kScanItems;     // 2048 cells
 constexpr int kP2PStage = 256;        // adjoint P2P: sources staged in LDS per wave and pass
 constexpr int kAdjP2PWaves = 8;       // adjoint P2P: waves per block (48 KiB of LDS)
 constexpr int kAdjP2PBlocksPerCU = 3; // adjoint P2P: resident blocks per CU (LDS bound)
+// Batched adjoint P2P: several rounds of residency, so that blocks ending
+// early are replaced while others still work through heavy runs.
+constexpr int kAdjP2PBatchBlocksPerCU = 16;
 
 // P2P output modes.
 enum P2PMode : int {
