@@ -12,7 +12,8 @@ present, construction raises.
 from __future__ import annotations
 
 from ._lib import NbfmmError, lib, lib_path  # noqa: F401
-from .solver import (BUFFERS, Solver, update, FLAG_OUT_OF_BOX)  # noqa: F401
+from .solver import (BUFFERS, Solver, update, FLAG_OUT_OF_BOX, FLAG_P2P_WAIT)  # noqa: F401
 from .batch import Batch  # noqa: F401
 
-__all__ = ["Solver", "Batch", "update", "NbfmmError", "lib", "lib_path", "BUFFERS", "FLAG_OUT_OF_BOX"]
+__all__ = ["Solver", "Batch", "update", "NbfmmError", "lib", "lib_path", "BUFFERS", "FLAG_OUT_OF_BOX",
+           "FLAG_P2P_WAIT"]

@@ -135,10 +135,13 @@ int variant_waves(int variant) {
 
 // P2P grid of `variant` for `bpc` blocks per CU, never more blocks than
 // occupied leaves can use.
+// P2P grid: at most bpc blocks per CU, and no more than there can be runs
+// (one per 64 sorted particles, plus a partial one per row), so that on a
+// small grid every run (a split one with it, helped by its block's waves)
+// can start on a block of its own.
 int p2p_grid_for(const nbfmm_solver* s, int variant, int bpc) {
-  const int C = s->B * s->B;
-  const int max_occ = C < s->max_n ? C : s->max_n;
-  const int want = cdiv(max_occ, variant_waves(variant));
+  const int want = std::max(cdiv(std::min(s->B * s->B, s->max_n), variant_waves(variant)),
+                            cdiv(s->max_n, kWave) + s->B);
   int g = want < s->cus * bpc ? want : s->cus * bpc;
   return g < 1 ? 1 : g;
 }
@@ -904,7 +907,7 @@ std::vector<int> p2p_share(const nbfmm_batch* b, int waves, int total_blocks) {
   std::vector<int> blocks(b->S);
   for (int s = 0; s < b->S; ++s) {
     const SysView& v = b->hv[s];
-    const int cap = cdiv(std::min(v.n, v.B * v.B), waves);
+    const int cap = std::max(cdiv(std::min(v.n, v.B * v.B), waves), cdiv(v.n, kWave) + v.B);  // as p2p_grid_for
     const int share = (int)((double)total_blocks * b->work[s] / b->total_work + 0.5);
     blocks[s] = std::max(1, std::min(cap, share));
   }

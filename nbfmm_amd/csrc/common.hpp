@@ -91,6 +91,17 @@ struct SysView {
 // Warm-started sort.
 constexpr uint32_t kMoverBit = 0x80000000u;  // tkey flag: the particle changed leaf
 constexpr int kWarmTouched = 1, kWarmInts = 4;
+
+// Split of heavy P2P runs: a run whose staged union holds more than
+// kP2PSplitMin sources is summed in K = min(kP2PMaxSeg, ceil(U / kP2PSegLen))
+// segments of the union, by whichever waves of the block are free, and the
+// segment sums are added in segment order.  K and the segment bounds depend
+// on U alone, so every kernel configuration gives the same bits.
+constexpr int kP2PSplitMin = 2048;
+constexpr int kP2PSegLen = 1024;
+constexpr int kP2PMaxSeg = 16;
+constexpr int kP2PWaitPolls = 1 << 20;  // polls before a wave stops waiting for the segment before its own
+constexpr uint32_t kFlagP2PWait = 2u;   // NBFMM_FLAG_P2P_WAIT
 constexpr int kWarmMergeWaves = 16384;       // merge grid: one wave per touched leaf, grid-stride beyond
 
 // Caller arrays of one call, original particle order; a system's particles

@@ -12,6 +12,7 @@ import torch
 from ._lib import check, lib
 
 FLAG_OUT_OF_BOX = 1
+FLAG_P2P_WAIT = 2
 
 # buffer name -> (id, dtype, components); names follow the Solver members
 # gpuptr_position_, gpuptr_effect_, ... (solver.hpp:45-76)
