@@ -41,8 +41,11 @@ extern "C" {
 
 /* Status flags raised by the device (read with nbfmm_solver_flags). */
 #define NBFMM_FLAG_OUT_OF_BOX 1u /* a particle was outside the limits or NaN; its cell index was clamped */
-#define NBFMM_FLAG_P2P_WAIT 2u   /* a P2P wave gave up waiting for the segment before its own in a split
-                                  * run; that run's effects are wrong (not expected to happen) */
+#define NBFMM_FLAG_P2P_WAIT 2u   /* a P2P wave gave up a wait on another wave of its block: either for
+                                  * the segment before its own in a split run (that run's results are
+                                  * wrong), or for the block's split-run slot (the run was then summed by
+                                  * one wave: right, but its bits may differ from the split sum's).
+                                  * Not expected to happen. */
 
 /* Buffer ids for nbfmm_buffer_read / nbfmm_buffer_write.  They name the
  * reference Solver members that NBFMM_CHECK makes public

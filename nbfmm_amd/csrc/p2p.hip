@@ -15,6 +15,7 @@
 // original order (postdo), and in the step mode applies the Verlet update.
 #include "common.hpp"
 #include "launch.hpp"
+#include "runs.hpp"
 #include "../../include/nbfmm_amd.h"
 
 namespace nbfmm_amd {
@@ -36,57 +37,6 @@ __device__ __forceinline__ float2 leaf_local(const SysView& v, int cx, int cy) {
   return acc;
 }
 
-
-// Geometry of run j: its targets (one per lane) and, per neighbour row
-// q = 0, 1, 2 (rows y-1, y, y+1), the run's union of source ranges
-// [us, us + ulen) in the sorted particles and this lane's part [a, b) of it
-// (relative to us; empty for idle lanes and rows outside the grid).
-struct RunGeom {
-  int y, i, xt;
-  bool valid;
-  float4 pt;
-  int us[3], ulen[3];  // wave-uniform
-  int a[3], b[3];      // this lane's range in each row's union (relative to us)
-  int total;           // sum of ulen: the union's size
-};
-
-__device__ __forceinline__ void run_geom(const SysView& v, const int* s_runs, int j, int lane, RunGeom& g) {
-  const int B = v.B;
-  // everything but the lane's own target is wave-uniform: scalar registers
-  g.y = __builtin_amdgcn_readfirstlane(run_row(s_runs, B, j));
-  const int T0 = __builtin_amdgcn_readfirstlane(v.head[g.y * B] + kWave * (j - s_runs[g.y]));
-  const int nt = __builtin_amdgcn_readfirstlane(min(kWave, v.head[(g.y + 1) * B] - T0));
-  g.valid = lane < nt;
-  g.i = T0 + (g.valid ? lane : 0);
-  g.pt = v.part[g.i];
-  g.xt = (int)(v.key[g.i] & (uint32_t)(B - 1));
-  const int xf = __builtin_amdgcn_readfirstlane(__shfl(g.xt, 0, kWave));
-  const int xl = __builtin_amdgcn_readfirstlane(__shfl(g.xt, nt - 1, kWave));
-  g.total = 0;
-#pragma unroll
-  for (int q = 0; q < 3; ++q) {
-    const int r = g.y + q - 1;
-    if (r < 0 || r >= B) {
-      g.us[q] = g.ulen[q] = g.a[q] = g.b[q] = 0;
-      continue;
-    }
-    const int* hr = v.head + r * B;
-    g.us[q] = __builtin_amdgcn_readfirstlane(hr[xf > 0 ? xf - 1 : 0]);
-    g.ulen[q] = __builtin_amdgcn_readfirstlane(hr[(xl < B - 1 ? xl + 1 : B - 1) + 1]) - g.us[q];
-    g.a[q] = g.valid ? hr[g.xt > 0 ? g.xt - 1 : 0] - g.us[q] : 0;
-    g.b[q] = g.valid ? hr[(g.xt < B - 1 ? g.xt + 1 : B - 1) + 1] - g.us[q] : 0;
-    g.total += g.ulen[q];
-  }
-}
-
-// Segments of a union of `total` sources: K = 1 up to kP2PSplitMin, else
-// min(kP2PMaxSeg, ceil(total / kP2PSegLen)) segments of S sources (a
-// multiple of 64), the last one shorter.
-__device__ __forceinline__ int p2p_segments(int total) {
-  if (total <= kP2PSplitMin) return 1;
-  return min(kP2PMaxSeg, (total + kP2PSegLen - 1) / kP2PSegLen);
-}
-__device__ __forceinline__ int p2p_segment_len(int total, int K) { return (((total + K - 1) / K) + 63) & ~63; }
 
 // Sum of this lane's sources among union positions [sa, sb) (rows in order,
 // then sorted order), staged in the wave's LDS buffer STAGE sources at a time.
@@ -177,221 +127,37 @@ __device__ __forceinline__ void p2p_finish(const SysView& v, const CallIO& io, c
 }
 
 // One wavefront works one run of up to 64 targets of row y, one target per
-// lane.  Target t in leaf x_t needs, in each neighbour row r, the sorted
-// range [head[r, x_t - 1], head[r, x_t + 2]) (the row-range idea of
-// p2p.cu:105-118).  Those ranges lie inside the run's union
-// [head[r, x_first - 1], head[r, x_last + 2]), which the wave stages in its
-// LDS buffer STAGE sources at a time; each lane then loops over its own part
-// of the window, U sources per iteration.  A lane adds its sources in sorted
-// order, rows y-1, y, y+1, whatever STAGE and U.
-//
-// A run whose union exceeds kP2PSplitMin (a cluster of dense leaves) would
-// be one long serial chain on one wave, so it is split into K segments of
-// the union.  The wave that takes it publishes it in the block's slot; every
-// wave of the block that is free takes a segment (a ticket from an LDS
-// counter, generation and K in its high bits), sums it, waits until the
-// segments before it have been added into the slot's running sum, adds its
-// own and passes on; the last one runs the epilogue and frees the slot.
-// The wait always ends (tickets go out in order and segment 0 waits for
-// nothing; a poll limit raises NBFMM_FLAG_P2P_WAIT instead of hanging).  A
-// wave whose heavy run finds the slot taken keeps it pending, helps with the
-// published run and publishes its own once the slot is free (after a poll
-// limit it sums all K segments itself, in the same order).  All of this
-// state is in LDS, and the hardware executes each wave's LDS instructions in
-// program order, so the atomics are relaxed and only the compiler is kept
-// from reordering (wave_lds_order); no wave reads another's global writes
-// here, so no wave waits for its global stores.  A wave that finds
-// the run queue empty stays to help while another wave holds a run that may
-// still be published (the low bits of the run queue count runs from their
-// take until they are known light, published, or summed alone) or a split
-// run is published.
-// Every variant (stage, unroll, waves) gives the same bits.
+// lane (runs.hpp: the run queue and the split of heavy runs).  Target t in
+// leaf x_t needs, in each neighbour row r, the sorted range
+// [head[r, x_t - 1], head[r, x_t + 2]) (the row-range idea of
+// p2p.cu:105-118).  Those ranges lie inside the run's union, which the wave
+// stages in its LDS buffer STAGE sources at a time; each lane then loops over
+// its own part of the window, U sources per iteration.  A lane adds its
+// sources in sorted order, rows y-1, y, y+1, and the segments of a split run
+// in segment order, so every variant (stage, unroll, waves) gives the same
+// bits.
 template <int MODE, class Sys, int STAGE, int U, int WAVES>
 __global__ __launch_bounds__(WAVES * kWave) void k_p2p(Sys sys, CallIO io) {
   static_assert(STAGE % kWave == 0, "staging size must be a multiple of 64");
   __shared__ float4 s_src[WAVES][STAGE];
   __shared__ int s_runs[(1 << (kMaxLevel + 1)) + 1];
   __shared__ uint32_t sh[WAVES];
-  // the block's run queue: runs taken (bits 31..4) and runs taken but not
-  // yet known light, published or summed (bits 3..0, at most one per wave),
-  // bumped together so that taking a run costs one LDS atomic
-  __shared__ uint32_t s_queue;
-  // the block's slot for a split run
-  __shared__ float2 s_acc[kWave];  // running sum of the segments added so far
-  __shared__ uint32_t s_ticket;    // generation << 22 | K << 16 | next segment
-  __shared__ int s_open, s_busy, s_done, s_run;
+  __shared__ RunSlot<2> slot;
   int blk, nblk;
   const SysView v = sys.get(blk, nblk);
   const int lane = lane_id();
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   float4* src = s_src[wave];
-  const int B = v.B;
   const float eps = v.eps;
   row_runs<WAVES * kWave>(v, s_runs, sh);
-  if (threadIdx.x == 0) {
-    s_queue = 0;
-    s_ticket = 0;
-    s_open = 0;
-    s_busy = 0;
-    s_done = 0;
-    s_run = 0;
-  }
+  if (threadIdx.x == 0) run_slot_init(slot);
   __syncthreads();
-  const int total = s_runs[B];
-  RunGeom g;
-  bool exhausted = false;  // this wave found the block's run queue empty
-  int idle_polls = 0;
-  int pend = -1, pend_k = 1, pend_polls = 0;  // a split run waiting for the slot (wave-uniform)
-  // publishes split run jr of kr segments if the slot is free (lane 0's
-  // view, made wave-uniform); the run stops counting as active once published
-  auto try_publish = [&](int jr, int kr) -> bool {
-    int got = 0;
-    if (lane == 0) {
-      int expected = 0;
-      if (__hip_atomic_compare_exchange_strong(&s_busy, &expected, 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_WORKGROUP)) {
-        got = 1;
-        s_run = jr;
-        s_done = 0;
-        const uint32_t gen = ((s_ticket >> 22) + 1u) & 0x3ffu;
-        wave_lds_order();
-        __hip_atomic_exchange(&s_ticket, (gen << 22) | ((uint32_t)kr << 16), __ATOMIC_RELAXED,
-                              __HIP_MEMORY_SCOPE_WORKGROUP);
-        wave_lds_order();
-        __hip_atomic_store(&s_open, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        wave_lds_order();
-        __hip_atomic_fetch_add(&s_queue, 0xffffffffu, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-      }
-    }
-    return __builtin_amdgcn_readfirstlane(got) != 0;
-  };
-  for (;;) {
-    // what this iteration sums: segments [k0, k1) of run j out of K; a
-    // ticket (one segment of the published run) has k1 = k0 + 1 <= K
-    int j = -1, K = 1, k0 = 0, k1 = 1;
-    bool ticket = false;
-    // 1. a segment of the published split run, if any is left
-    if (__hip_atomic_load(&s_open, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) {
-      uint32_t t = 0;
-      if (lane == 0) t = __hip_atomic_fetch_add(&s_ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-      wave_lds_order();
-      t = __builtin_amdgcn_readfirstlane(t);
-      const int k = (int)(t & 0xffffu), Kt = (int)((t >> 16) & 0x3fu);
-      if (k < Kt) {
-        if (k == Kt - 1 && lane == 0) __hip_atomic_store(&s_open, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        ticket = true;
-        j = __builtin_amdgcn_readfirstlane(s_run);
-        K = Kt;
-        k0 = k;
-        k1 = k + 1;
-        run_geom(v, s_runs, j, lane, g);
-      }
-    }
-    if (!ticket) {
-      if (pend >= 0) {
-        // 2. this wave's split run waits for the slot, helping meanwhile;
-        // after the poll limit it is summed here alone (same bits)
-        if (try_publish(pend, pend_k)) {
-          pend = -1;
-          continue;
-        }
-        if (++pend_polls < kP2PWaitPolls) {
-          __builtin_amdgcn_s_sleep(1);
-          continue;
-        }
-        j = pend;
-        K = pend_k;
-        k1 = K;
-        pend = -1;
-        run_geom(v, s_runs, j, lane, g);
-      } else {
-        // 3. a new run; counted active from before it is taken until it is
-        // finished or published, so that a wave that finds the queue empty
-        // knows whether a split run may still be published
-        if (exhausted) {
-          // leave once no run is in progress and no split run is published;
-          // new segments can appear only from a run in progress (a wave that
-          // stops waiting only stops helping, so the poll limit is no error)
-          wave_lds_order();
-          if ((__hip_atomic_load(&s_queue, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) & 0xfu) == 0 &&
-              __hip_atomic_load(&s_busy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == 0)
-            break;
-          if (++idle_polls == kP2PWaitPolls) break;
-          __builtin_amdgcn_s_sleep(2);
-          continue;
-        }
-        uint32_t q = 0;
-        if (lane == 0) q = __hip_atomic_fetch_add(&s_queue, 17u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        wave_lds_order();
-        j = blk + (int)(__builtin_amdgcn_readfirstlane(q) >> 4) * nblk;
-        if (j >= total) {
-          if (lane == 0) __hip_atomic_fetch_add(&s_queue, 0xffffffffu, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-          exhausted = true;
-          continue;
-        }
-        run_geom(v, s_runs, j, lane, g);
-        K = p2p_segments(g.total);
-        k1 = K;
-        // a light run can publish nothing: it stops counting at once, so
-        // that waves without work need not stay for its end
-        if (K == 1 && lane == 0)
-          __hip_atomic_fetch_add(&s_queue, 0xffffffffu, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        if (K > 1) {
-          if (!try_publish(j, K)) {
-            pend = j;
-            pend_k = K;
-            pend_polls = 0;
-          }
-          continue;  // this wave takes segments like any other
-        }
-      }
-    }
-    // the segments [k0, k1), added in order
-    const int S = p2p_segment_len(g.total, K);
-    float ax = 0.0f, ay = 0.0f;
-    for (int k = k0; k < k1; ++k) {
-      float px, py;
-      seg_sum<STAGE, U>(v, src, g, lane, k * S, min(k * S + S, g.total), eps, px, py);
-      if (k == k0) {
-        ax = px;
-        ay = py;
-      } else {
-        ax = ax + px;
-        ay = ay + py;
-      }
-    }
-    if (!ticket) {
-      p2p_finish<MODE>(v, io, g, ax, ay);
-      // a split run summed here alone (poll limit) counted until now
-      if (K > 1 && lane == 0)
-        __hip_atomic_fetch_add(&s_queue, 0xffffffffu, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-      continue;
-    }
-    // a ticket: wait for segments 0 .. k0-1, then add this one
-    for (int polls = 0; __hip_atomic_load(&s_done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) != k0;
-         ++polls) {
-      if (polls == kP2PWaitPolls) {
-        if (lane == 0) atomicOr(v.flags, kFlagP2PWait);
-        break;
-      }
-      __builtin_amdgcn_s_sleep(1);
-    }
-    wave_lds_order();
-    if (k0 > 0) {
-      const float2 o = s_acc[lane];
-      ax = o.x + ax;
-      ay = o.y + ay;
-    }
-    if (k0 < K - 1) {
-      s_acc[lane] = make_float2(ax, ay);
-      wave_lds_order();
-      if (lane == 0) __hip_atomic_store(&s_done, k0 + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    } else {
-      wave_lds_order();
-      if (lane == 0) __hip_atomic_store(&s_busy, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-      p2p_finish<MODE>(v, io, g, ax, ay);
-    }
-  }
+  run_queue<2>(
+      v, s_runs, slot, blk, nblk, s_runs[v.B], lane,
+      [&](const RunGeom& g, int sa, int sb, float* acc) {
+        seg_sum<STAGE, U>(v, src, g, lane, sa, sb, eps, acc[0], acc[1]);
+      },
+      [&](const RunGeom& g, const float* acc) { p2p_finish<MODE>(v, io, g, acc[0], acc[1]); });
 }
 
 // L2P stage (l2p.cu:23-35): effect[i] += leaf effect of its cell.

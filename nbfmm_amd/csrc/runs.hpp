@@ -1,0 +1,265 @@
+// Runs: the unit of work of the P2P kernel.
+//
+// The sorted particles of every row of leaves are cut into runs of 64
+// consecutive targets, one wavefront per run and one target per lane.  A
+// target in leaf x of row y needs, in each neighbour row r, the sorted range
+// [head[r, x-1], head[r, x+2]) (the row-range idea of
+// source/nbfmm/core/solver/p2p.cu:105-118); those ranges lie inside the run's
+// union [head[r, x_first-1], head[r, x_last+2]).  A block rebuilds the run
+// table (runs before each row) from head and its waves draw runs b, b + nblk,
+// ... through an LDS counter.
+//
+// A run whose union exceeds kP2PSplitMin (a cluster of dense leaves) would be
+// one long serial chain on one wave, so it is split into K segments of the
+// union: K = min(kP2PMaxSeg, ceil(U / kP2PSegLen)) and segments of S sources
+// (a multiple of 64), both functions of the union size U alone.  The wave
+// that takes such a run publishes it in the block's slot; every wave of the
+// block that is free takes a segment (a ticket from an LDS counter,
+// generation and K in its high bits), sums it, waits until the segments
+// before it have been added into the slot's running sums, adds its own and
+// passes on; the last one runs the epilogue and frees the slot.  The wait
+// always ends (tickets go out in order and segment 0 waits for nothing; a
+// poll limit raises NBFMM_FLAG_P2P_WAIT instead of hanging).  A wave whose
+// heavy run finds the slot taken keeps it pending, helps with the published
+// run and publishes its own once the slot is free (after a poll limit it sums
+// the whole union itself in one pass and raises NBFMM_FLAG_P2P_WAIT: only
+// then may the bits differ from the split sum's).  A wave that finds the run
+// queue empty stays to help while another wave holds a run that may still be
+// published (the low bits of the queue word count runs from their take until
+// they are known light, published, or summed alone) or a split run is
+// published.  All of this state is in LDS, and the hardware executes each
+// wave's LDS instructions in program order, so the atomics are relaxed and
+// only the compiler is kept from reordering (wave_lds_order); no wave reads
+// another's global writes here, so no wave waits for its global stores.
+// Every kernel configuration gives the same bits.
+#pragma once
+
+#include "common.hpp"
+
+namespace nbfmm_amd {
+
+// Geometry of run j: its targets (one per lane) and, per neighbour row
+// q = 0, 1, 2 (rows y-1, y, y+1), the run's union of source ranges
+// [us, us + ulen) in the sorted particles and this lane's part [a, b) of it
+// (relative to us; empty for idle lanes and rows outside the grid).
+struct RunGeom {
+  int y, i, xt;
+  bool valid;
+  float4 pt;
+  int us[3], ulen[3];  // wave-uniform
+  int a[3], b[3];      // this lane's range in each row's union (relative to us)
+  int total;           // sum of ulen: the union's size
+};
+
+__device__ __forceinline__ void run_geom(const SysView& v, const int* s_runs, int j, int lane, RunGeom& g) {
+  const int B = v.B;
+  // everything but the lane's own target is wave-uniform: scalar registers
+  g.y = __builtin_amdgcn_readfirstlane(run_row(s_runs, B, j));
+  const int T0 = __builtin_amdgcn_readfirstlane(v.head[g.y * B] + kWave * (j - s_runs[g.y]));
+  const int nt = __builtin_amdgcn_readfirstlane(min(kWave, v.head[(g.y + 1) * B] - T0));
+  g.valid = lane < nt;
+  g.i = T0 + (g.valid ? lane : 0);
+  g.pt = v.part[g.i];
+  g.xt = (int)(v.key[g.i] & (uint32_t)(B - 1));
+  const int xf = __builtin_amdgcn_readfirstlane(__shfl(g.xt, 0, kWave));
+  const int xl = __builtin_amdgcn_readfirstlane(__shfl(g.xt, nt - 1, kWave));
+  g.total = 0;
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const int r = g.y + q - 1;
+    if (r < 0 || r >= B) {
+      g.us[q] = g.ulen[q] = g.a[q] = g.b[q] = 0;
+      continue;
+    }
+    const int* hr = v.head + r * B;
+    g.us[q] = __builtin_amdgcn_readfirstlane(hr[xf > 0 ? xf - 1 : 0]);
+    g.ulen[q] = __builtin_amdgcn_readfirstlane(hr[(xl < B - 1 ? xl + 1 : B - 1) + 1]) - g.us[q];
+    g.a[q] = g.valid ? hr[g.xt > 0 ? g.xt - 1 : 0] - g.us[q] : 0;
+    g.b[q] = g.valid ? hr[(g.xt < B - 1 ? g.xt + 1 : B - 1) + 1] - g.us[q] : 0;
+    g.total += g.ulen[q];
+  }
+}
+
+// Segments of a union of `total` sources: 1 up to kP2PSplitMin, else
+// min(kP2PMaxSeg, ceil(total / kP2PSegLen)) segments of S sources (a
+// multiple of 64), the last one shorter.
+__device__ __forceinline__ int p2p_segments(int total) {
+  if (total <= kP2PSplitMin) return 1;
+  return min(kP2PMaxSeg, (total + kP2PSegLen - 1) / kP2PSegLen);
+}
+__device__ __forceinline__ int p2p_segment_len(int total, int K) { return (((total + K - 1) / K) + 63) & ~63; }
+
+// The block's run queue and its slot for a split run, in LDS; NACC running
+// sums per lane.
+template <int NACC>
+struct RunSlot {
+  float acc[kWave][NACC];  // running sums of the segments added so far
+  uint32_t queue;          // runs taken (bits 31..4) and runs counted active (bits 3..0)
+  uint32_t ticket;         // generation << 22 | K << 16 | next segment
+  int open, busy, done, run;
+};
+
+template <int NACC>
+__device__ __forceinline__ void run_slot_init(RunSlot<NACC>& sl) {
+  sl.queue = 0;
+  sl.ticket = 0;
+  sl.open = 0;
+  sl.busy = 0;
+  sl.done = 0;
+  sl.run = 0;
+}
+
+// The block's waves work through its runs.  seg(g, sa, sb, acc) sums this
+// lane's sources among union positions [sa, sb) of run g into acc[NACC]
+// (from zero); fin(g, acc) is the epilogue of run g.  The slot must have been
+// initialised and the block synchronised.
+template <int NACC, class Seg, class Fin>
+__device__ __forceinline__ void run_queue(const SysView& v, const int* s_runs, RunSlot<NACC>& sl, int blk, int nblk,
+                                          int total, int lane, Seg&& seg, Fin&& fin) {
+  RunGeom g;
+  bool exhausted = false;  // this wave found the block's run queue empty
+  int idle_polls = 0;
+  int pend = -1, pend_k = 1, pend_polls = 0;  // a split run waiting for the slot (wave-uniform)
+  // publishes split run jr of kr segments if the slot is free (lane 0's
+  // view, made wave-uniform); the run stops counting as active once published
+  auto try_publish = [&](int jr, int kr) -> bool {
+    int got = 0;
+    if (lane == 0) {
+      int expected = 0;
+      if (__hip_atomic_compare_exchange_strong(&sl.busy, &expected, 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_WORKGROUP)) {
+        got = 1;
+        sl.run = jr;
+        sl.done = 0;
+        const uint32_t gen = ((sl.ticket >> 22) + 1u) & 0x3ffu;
+        wave_lds_order();
+        __hip_atomic_exchange(&sl.ticket, (gen << 22) | ((uint32_t)kr << 16), __ATOMIC_RELAXED,
+                              __HIP_MEMORY_SCOPE_WORKGROUP);
+        wave_lds_order();
+        __hip_atomic_store(&sl.open, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        wave_lds_order();
+        __hip_atomic_fetch_add(&sl.queue, 0xffffffffu, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      }
+    }
+    return __builtin_amdgcn_readfirstlane(got) != 0;
+  };
+  for (;;) {
+    // what this iteration sums: segment k0 of run j out of K (a ticket of
+    // the published run), or all of run j in one pass
+    int j = -1, K = 1, k0 = 0;
+    bool ticket = false;
+    // 1. a segment of the published split run, if any is left
+    if (__hip_atomic_load(&sl.open, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) {
+      uint32_t t = 0;
+      if (lane == 0) t = __hip_atomic_fetch_add(&sl.ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      wave_lds_order();
+      t = __builtin_amdgcn_readfirstlane(t);
+      const int k = (int)(t & 0xffffu), Kt = (int)((t >> 16) & 0x3fu);
+      if (k < Kt) {
+        if (k == Kt - 1 && lane == 0) __hip_atomic_store(&sl.open, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        ticket = true;
+        j = __builtin_amdgcn_readfirstlane(sl.run);
+        K = Kt;
+        k0 = k;
+        run_geom(v, s_runs, j, lane, g);
+      }
+    }
+    if (!ticket) {
+      if (pend >= 0) {
+        // 2. this wave's split run waits for the slot, helping meanwhile;
+        // after the poll limit it is summed here alone, in one pass
+        if (try_publish(pend, pend_k)) {
+          pend = -1;
+          continue;
+        }
+        if (++pend_polls < kP2PWaitPolls) {
+          __builtin_amdgcn_s_sleep(1);
+          continue;
+        }
+        j = pend;
+        K = pend_k;
+        pend = -1;
+        if (lane == 0) atomicOr(v.flags, kFlagP2PWait);
+        run_geom(v, s_runs, j, lane, g);
+      } else {
+        // 3. a new run, counted active from its take until it is known light,
+        // published or summed alone
+        if (exhausted) {
+          // leave once no run is counted active and no split run is
+          // published; new segments can appear only from those (a wave that
+          // stops waiting only stops helping, so the poll limit is no error)
+          wave_lds_order();
+          if ((__hip_atomic_load(&sl.queue, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) & 0xfu) == 0 &&
+              __hip_atomic_load(&sl.busy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == 0)
+            break;
+          if (++idle_polls == kP2PWaitPolls) break;
+          __builtin_amdgcn_s_sleep(2);
+          continue;
+        }
+        uint32_t q = 0;
+        if (lane == 0) q = __hip_atomic_fetch_add(&sl.queue, 17u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        wave_lds_order();
+        j = blk + (int)(__builtin_amdgcn_readfirstlane(q) >> 4) * nblk;
+        if (j >= total) {
+          if (lane == 0) __hip_atomic_fetch_add(&sl.queue, 0xffffffffu, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          exhausted = true;
+          continue;
+        }
+        run_geom(v, s_runs, j, lane, g);
+        K = p2p_segments(g.total);
+        // a light run can publish nothing: it stops counting at once, so
+        // that waves without work need not stay for its end
+        if (K == 1 && lane == 0)
+          __hip_atomic_fetch_add(&sl.queue, 0xffffffffu, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (K > 1) {
+          if (!try_publish(j, K)) {
+            pend = j;
+            pend_k = K;
+            pend_polls = 0;
+          }
+          continue;  // this wave takes segments like any other
+        }
+      }
+    }
+    int sa = 0, sb = g.total;
+    if (ticket) {
+      const int S = p2p_segment_len(g.total, K);
+      sa = k0 * S;
+      sb = min(sa + S, g.total);
+    }
+    float acc[NACC];
+    seg(g, sa, sb, acc);
+    if (!ticket) {
+      fin(g, acc);
+      // a split run summed here alone (poll limit) counted until now
+      if (K > 1 && lane == 0)
+        __hip_atomic_fetch_add(&sl.queue, 0xffffffffu, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      continue;
+    }
+    // a ticket: wait for segments 0 .. k0-1, then add this one
+    for (int polls = 0; __hip_atomic_load(&sl.done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) != k0; ++polls) {
+      if (polls == kP2PWaitPolls) {
+        if (lane == 0) atomicOr(v.flags, kFlagP2PWait);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    wave_lds_order();
+    if (k0 > 0) {
+#pragma unroll
+      for (int c = 0; c < NACC; ++c) acc[c] = sl.acc[lane][c] + acc[c];
+    }
+    if (k0 < K - 1) {
+#pragma unroll
+      for (int c = 0; c < NACC; ++c) sl.acc[lane][c] = acc[c];
+      wave_lds_order();
+      if (lane == 0) __hip_atomic_store(&sl.done, k0 + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    } else {
+      wave_lds_order();
+      if (lane == 0) __hip_atomic_store(&sl.busy, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      fin(g, acc);
+    }
+  }
+}
+
+}  // namespace nbfmm_amd

@@ -1,9 +1,10 @@
 """P2P runs whose source union exceeds 2048 (clusters of dense leaves) are
 summed in segments by several waves and combined in segment order.  The
-result must match the oracle, be the same bits for every kernel
-configuration (which decides how many waves help and when a run is summed
-by one wave alone), equal graph replay and batches, and never raise the
-wait flag."""
+forces must match the oracle, be the same bits for every kernel
+configuration (which decides how many waves help), equal graph replay and
+batches, and never raise the wait flag.  The adjoint P2P sums such runs
+whole, one wave each; on the same clustered inputs it must match central
+differences of the fp64 oracle and equal batches."""
 import numpy as np
 import pytest
 import torch
@@ -138,3 +139,95 @@ def test_clustered_system_in_a_batch():
         s.close()
     assert not b.flags().any()
     b.close()
+
+
+def inside(x, lim, L, margin=1e-3):
+    """Moves every point at least `margin` of a leaf width inside its leaf
+    (fp64), so that fp32 and fp64 bin it alike and small steps keep it."""
+    B = 2 ** (L + 1)
+    cell = (lim[2:].astype(np.float64) - lim[:2]) / B
+    f = (x.astype(np.float64) - lim[:2]) / cell
+    k = np.floor(f)
+    f = k + np.clip(f - k, margin, 1 - margin)
+    return (lim[:2] + f * cell).astype(np.float32)
+
+
+def test_clustered_adjoint_matches_fd_directions():
+    """The adjoint on runs with thousands of sources.  Pinned by central
+    differences of the fp64 oracle solve (cells held fixed) along
+    the computed gradient itself and along a random direction, for the
+    positions and for the masses."""
+    from nbfmm_amd import Solver
+    x, w, L, lim, eps = clustered(n=12000, seed=6)
+    x = inside(x, lim, L)
+    assert max_union(x, lim, L) > 4 * 2048
+    n = x.shape[0]
+    rng = np.random.default_rng(9)
+    g = rng.normal(0, 1, (n, 2)).astype(np.float32)
+    s = Solver(L, n, lim, eps)
+    gx = torch.zeros((n, 2), device="cuda")
+    gm = torch.zeros(n, device="cuda")
+    s.solve_adjoint(n, dev(x), dev(w), dev(g), gx, gm)
+    torch.cuda.synchronize()
+    assert s.flags() == 0
+    s.close()
+    gx = gx.double().cpu().numpy()
+    gm = gm.double().cpu().numpy()
+    x64, w64, g64 = x.astype(np.float64), w.astype(np.float64), g.astype(np.float64)
+    lim64 = lim.astype(np.float64)
+
+    def loss(p, m):
+        return float((g64 * orc.solve_f64(L, p, m, lim64, eps)).sum())
+
+    h = 1e-5
+    for v in (gx / np.linalg.norm(gx), rng.normal(0, 1, (n, 2)) / np.sqrt(2 * n)):
+        fd = (loss(x64 + h * v, w64) - loss(x64 - h * v, w64)) / (2 * h)
+        assert abs(fd - float((gx * v).sum())) < 1e-4 * np.linalg.norm(gx), (fd, float((gx * v).sum()))
+    for u in (gm / np.linalg.norm(gm), rng.normal(0, 1, n) / np.sqrt(n)):
+        fd = (loss(x64, w64 + h * u) - loss(x64, w64 - h * u)) / (2 * h)
+        assert abs(fd - float((gm * u).sum())) < 1e-4 * np.linalg.norm(gm), (fd, float((gm * u).sum()))
+
+
+def test_clustered_adjoint_identities_and_batch():
+    """Translation invariance and mass homogeneity of the adjoint, and a
+    clustered system in a batch: the same bits as alone, for the solve and
+    the Verlet-step adjoints."""
+    from nbfmm_amd import Batch, Solver
+    from nbfmm_amd.workloads import disk
+    xc, wc, Lc, limc, epsc = clustered(n=10000, seed=3)
+    n = xc.shape[0]
+    rng = np.random.default_rng(2)
+    g = rng.normal(0, 1, (n, 2)).astype(np.float32)
+    s = Solver(Lc, n, limc, epsc)
+    a = torch.zeros((n, 2), device="cuda")
+    gx = torch.zeros((n, 2), device="cuda")
+    gm = torch.zeros(n, device="cuda")
+    s.solve(n, dev(xc), dev(wc), a)
+    s.solve_adjoint(n, dev(xc), dev(wc), dev(g), gx, gm)
+    lam, pi, gw = dev(g), torch.zeros((n, 2), device="cuda"), torch.zeros(n, device="cuda")
+    s.step_adjoint(n, 1e-3, dev(xc), dev(wc), lam, pi, gw)
+    torch.cuda.synchronize()
+    assert s.flags() == 0
+    gxn, gmn, an = gx.double().cpu().numpy(), gm.double().cpu().numpy(), a.double().cpu().numpy()
+    assert np.all(np.abs(gxn.sum(0)) < 1e-4 * np.abs(gxn).sum(0))
+    lhs = float((wc.astype(np.float64) * gmn).sum())
+    rhs = float((g.astype(np.float64) * an).sum())
+    assert abs(lhs - rhs) < 1e-4 * np.abs(wc * gmn).sum()
+
+    light = disk(n=5000)
+    gl = rng.normal(0, 1, (light.n, 2)).astype(np.float32)
+    b = Batch([n, light.n], [Lc, light.num_level], [tuple(limc), light.limits], [epsc, light.eps],
+              ticks=[1e-3, light.tick])
+    X = dev(np.concatenate([xc, light.position]))
+    W = dev(np.concatenate([wc, light.weight]))
+    G = dev(np.concatenate([g, gl]))
+    GX, GM = torch.zeros_like(X), torch.zeros_like(W)
+    b.solve_adjoint(X, W, G, GX, GM)
+    LAM, PI, GW = G.clone(), torch.zeros_like(X), torch.zeros_like(W)
+    b.step_adjoint(X, W, LAM, PI, GW)
+    torch.cuda.synchronize()
+    assert torch.equal(GX[:n], gx) and torch.equal(GM[:n], gm)
+    assert torch.equal(LAM[:n], lam) and torch.equal(PI[:n], pi) and torch.equal(GW[:n], gw)
+    assert not b.flags().any()
+    b.close()
+    s.close()
