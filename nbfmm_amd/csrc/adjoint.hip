@@ -19,7 +19,8 @@
 //           gX_c += (M_c/M_p) gX_p,  gM_c += gM_p + (X_c - X_p).gX_p / M_p
 //   P2M^T:  gx_i += (m_i/M_c) gX_c,  gm_i += gM_c + (x_i - X_c).gX_c / M_c
 // A self pair contributes exactly 0 (d = 0 and h = m_i g_i - m_i g_i = 0 with
-// both products rounded).
+// both products rounded).  In P2P^T a target of nonzero mass factors h as
+// m_i (g_j - m_j g_i / m_i): fewer operations per pair, m_i applied once.
 #include "common.hpp"
 #include "launch.hpp"
 
@@ -49,6 +50,42 @@ __device__ __forceinline__ void pair_adjoint(float tx, float ty, float mt, float
   cy = __builtin_fmaf(c, dy, cy);
   const float dg = __builtin_fmaf(dx, gsx, dy * gsy);
   gm = __builtin_fmaf(-f, dg, gm);
+}
+
+// The same pair for a target of nonzero mass mt, given b = g_t / m_t:
+// h = m_t (g_s - m_s b) = m_t h', and every term with h carries the factor
+// m_t, which the caller applies once per target; two fused multiply-adds
+// form h' where h takes six operations.  Here the self pair gives
+// f_self h'_self (h'_self = g_t - m_t b is the rounding residue of b, not
+// exactly 0), which the caller takes out again (adjoint_unself).
+__device__ __forceinline__ void pair_adjoint_scaled(float tx, float ty, float bx, float by, float sx, float sy,
+                                                    float ms, float gsx, float gsy, float eps, float& gx, float& gy,
+                                                    float& cx, float& cy, float& gm) {
+#pragma clang fp contract(off)
+  const float dx = sx - tx;
+  const float dy = sy - ty;
+  const float r2 = __builtin_fmaf(dx, dx, dy * dy);
+  const float r = __builtin_amdgcn_sqrtf(r2);
+  const float f = __builtin_amdgcn_rcpf(__builtin_fmaf(r2, r, eps));
+  const float hx = __builtin_fmaf(-ms, bx, gsx);
+  const float hy = __builtin_fmaf(-ms, by, gsy);
+  const float dh = __builtin_fmaf(dx, hx, dy * hy);
+  const float c = (r * f) * (f * dh);
+  gx = __builtin_fmaf(f, hx, gx);
+  gy = __builtin_fmaf(f, hy, gy);
+  cx = __builtin_fmaf(c, dx, cx);
+  cy = __builtin_fmaf(c, dy, cy);
+  const float dg = __builtin_fmaf(dx, gsx, dy * gsy);
+  gm = __builtin_fmaf(-f, dg, gm);
+}
+
+// Takes the self pair's term out of the sums of pair_adjoint_scaled: the
+// same operations as the pair with d = 0 (f = 1/eps, c = 0, dg = 0).
+__device__ __forceinline__ void adjoint_unself(float mt, float gtx, float gty, float bx, float by, float eps, float& gx,
+                                               float& gy) {
+  const float f = __builtin_amdgcn_rcpf(__builtin_fmaf(0.0f, 0.0f, eps));
+  gx = __builtin_fmaf(-f, __builtin_fmaf(-mt, bx, gtx), gx);
+  gy = __builtin_fmaf(-f, __builtin_fmaf(-mt, by, gty), gy);
 }
 
 // gx -= 3 cx, gy -= 3 cy: the term pair_adjoint sums apart.
@@ -241,8 +278,11 @@ __device__ __forceinline__ float4 leaf_multipole_grad(const SysView& v, int cx, 
 //              (the tick^2 scale was applied to the sorted upstream).
 enum AdjMode : int { kAdjSolve = 0, kAdjStep = 1 };
 
+// Three blocks of kAdjP2PWaves waves fit a CU's LDS, so registers are held
+// to what six waves per SIMD allow (80 VGPRs; unbounded it takes 83).
 template <int MODE, class Sys>
-__global__ __launch_bounds__(kAdjP2PWaves * kWave) void k_adj_p2p(Sys sys, CallIO io) {
+__global__ __launch_bounds__(kAdjP2PWaves * kWave) __attribute__((amdgpu_waves_per_eu(6))) void k_adj_p2p(Sys sys,
+                                                                                                       CallIO io) {
   __shared__ float4 s_pos[kAdjP2PWaves][kP2PStage];
   __shared__ float2 s_g[kAdjP2PWaves][kP2PStage];
   __shared__ int s_runs[(1 << (kMaxLevel + 1)) + 1];
@@ -274,6 +314,11 @@ __global__ __launch_bounds__(kAdjP2PWaves * kWave) void k_adj_p2p(Sys sys, CallI
     const int xt = (int)(v.key[ti] & (uint32_t)(B - 1));
     const int xf = __shfl(xt, 0, kWave);
     const int xl = __shfl(xt, nt - 1, kWave);
+    // a wave whose targets all have nonzero mass takes the scaled pair
+    // (b = g_t / m_t, the factor m_t applied below); otherwise every lane
+    // takes the plain one
+    const bool scaled = __builtin_amdgcn_ballot_w64(valid && !(pt.z != 0.0f)) == 0;
+    const float2 bt = scaled && valid ? make_float2(gt.x / pt.z, gt.y / pt.z) : make_float2(0.0f, 0.0f);
     float gx = 0.0f, gy = 0.0f, gm = 0.0f, cx = 0.0f, cy = 0.0f;
 #pragma unroll
     for (int dr = -1; dr <= 1; ++dr) {
@@ -314,28 +359,54 @@ __global__ __launch_bounds__(kAdjP2PWaves * kWave) void k_adj_p2p(Sys sys, CallI
         const float2* q = sg + lo;
         const float4* const pend = sp + hi;
         const float4* const pfull = sp + max(hi - 3, lo);
-        for (; p < pfull; p += 4, q += 4) {
-          float4 s4[4];
-          float2 g4[4];
+        if (scaled) {
+          for (; p < pfull; p += 4, q += 4) {
+            float4 s4[4];
+            float2 g4[4];
 #pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            s4[u] = p[u];
-            g4[u] = q[u];
+            for (int u = 0; u < 4; ++u) {
+              s4[u] = p[u];
+              g4[u] = q[u];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+              pair_adjoint_scaled(pt.x, pt.y, bt.x, bt.y, s4[u].x, s4[u].y, s4[u].z, g4[u].x, g4[u].y, eps, gx, gy,
+                                  cx, cy, gm);
           }
+          for (; p < pend; ++p, ++q) {
+            const float4 s1 = *p;
+            const float2 g1 = *q;
+            pair_adjoint_scaled(pt.x, pt.y, bt.x, bt.y, s1.x, s1.y, s1.z, g1.x, g1.y, eps, gx, gy, cx, cy, gm);
+          }
+        } else {
+          for (; p < pfull; p += 4, q += 4) {
+            float4 s4[4];
+            float2 g4[4];
 #pragma unroll
-          for (int u = 0; u < 4; ++u)
-            pair_adjoint(pt.x, pt.y, pt.z, gt.x, gt.y, s4[u].x, s4[u].y, s4[u].z, g4[u].x, g4[u].y, eps, gx, gy, cx,
-                         cy, gm);
-        }
-        for (; p < pend; ++p, ++q) {
-          const float4 s1 = *p;
-          const float2 g1 = *q;
-          pair_adjoint(pt.x, pt.y, pt.z, gt.x, gt.y, s1.x, s1.y, s1.z, g1.x, g1.y, eps, gx, gy, cx, cy, gm);
+            for (int u = 0; u < 4; ++u) {
+              s4[u] = p[u];
+              g4[u] = q[u];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+              pair_adjoint(pt.x, pt.y, pt.z, gt.x, gt.y, s4[u].x, s4[u].y, s4[u].z, g4[u].x, g4[u].y, eps, gx, gy,
+                           cx, cy, gm);
+          }
+          for (; p < pend; ++p, ++q) {
+            const float4 s1 = *p;
+            const float2 g1 = *q;
+            pair_adjoint(pt.x, pt.y, pt.z, gt.x, gt.y, s1.x, s1.y, s1.z, g1.x, g1.y, eps, gx, gy, cx, cy, gm);
+          }
         }
         wave_lds_order();
       }
     }
+    if (scaled) adjoint_unself(pt.z, gt.x, gt.y, bt.x, bt.y, eps, gx, gy);
     adjoint_fold(gx, gy, cx, cy);
+    if (scaled) {
+      gx = pt.z * gx;
+      gy = pt.z * gy;
+    }
     if (valid) {
       if (v.L > 0) {
         // P2M^T: X_c = sum m x / M_c,  M_c = sum m

@@ -221,14 +221,15 @@ __device__ __forceinline__ void run_queue(const SysView& v, const int* s_runs, R
         }
       }
     }
-    int sa = 0, sb = g.total;
+    // (two call sites: one segment, or the whole union; the compiler keeps
+    // fewer registers live this way than with computed bounds)
+    float acc[NACC];
     if (ticket) {
       const int S = p2p_segment_len(g.total, K);
-      sa = k0 * S;
-      sb = min(sa + S, g.total);
+      seg(g, k0 * S, min(k0 * S + S, g.total), acc);
+    } else {
+      seg(g, 0, g.total, acc);
     }
-    float acc[NACC];
-    seg(g, sa, sb, acc);
     if (!ticket) {
       fin(g, acc);
       // a split run summed here alone (poll limit) counted until now

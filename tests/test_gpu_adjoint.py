@@ -11,6 +11,8 @@ Pins:
   * size-independent identities at 65536 particles: translation invariance
     (sum_i dloss/dx_i = 0) and degree-1 homogeneity in the masses
     (sum_j m_j dloss/dm_j = sum_i g_i . a_i);
+  * massless particles (the plain pair) beside massive ones (the pair with
+    the target's gradient divided by its mass), against the same differences;
   * checkpointing: K = 1, K = 4 and K = steps give the same bits;
   * batches: system by system the same bits as separate solvers.
 """
@@ -91,6 +93,30 @@ def test_solve_adjoint_matches_fd(L):
     n = 256
     pos = inside_cells(rng, n, L, lim).astype(np.float32)
     w = rng.uniform(0.5, 1.5, n).astype(np.float32)
+    g = rng.normal(0, 1, (n, 2))
+    eps = 1e-3
+    gx, gm = gpu_adjoint(L, pos, w, lim, eps, g)
+    fx, fm = fd_solve(L, pos, w, lim, eps, g)
+    assert rel(gx, fx) < 1e-3, rel(gx, fx)
+    assert rel(gm, fm) < 1e-3, rel(gm, fm)
+
+
+def test_solve_adjoint_zero_masses_matches_fd():
+    """Waves whose targets all have nonzero mass take the pair with the
+    target's gradient divided by its mass; a wave with a massless target
+    takes the plain pair.  Both against central differences."""
+    L = 3
+    rng = np.random.default_rng(77)
+    lim = (-1.0, -1.0, 1.0, 1.0)
+    pos = inside_cells(rng, 256, L, lim)
+    w = rng.uniform(0.5, 1.5, 256)
+    # massless particles beside massive ones in leaves of the top rows only
+    # (a leaf of massless particles alone has no centre of mass)
+    top = rng.choice(np.flatnonzero(pos[:, 1] > 0.6), 8, replace=False)
+    cw = (lim[2] - lim[0]) / 2 ** (L + 1)
+    pos = np.concatenate([pos, pos[top] + 0.05 * cw]).astype(np.float32)
+    w = np.concatenate([w, np.zeros(8)]).astype(np.float32)
+    n = pos.shape[0]
     g = rng.normal(0, 1, (n, 2))
     eps = 1e-3
     gx, gm = gpu_adjoint(L, pos, w, lim, eps, g)
