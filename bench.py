@@ -23,7 +23,8 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-FP32_PEAK_TFLOPS = 157.3   # MI355X published FP32 vector peak
+FP32_PEAK_TFLOPS = 157.3   # MI355X published FP32 vector peak (packed FP32)
+FP32_PEAK_NONPACKED_TFLOPS = 78.6  # the same without packed FP32, which this build does not use
 HBM_PEAK_GBS = 8000.0
 P2P_FLOPS_PER_PAIR = 14    # sub 2, r^2 3, sqrt 1, r^3 2, +eps 1, div 1, scale 2, accumulate 2
 # Pair-rate bound of the P2P arithmetic on MI355X: a throughput probe running
@@ -67,6 +68,7 @@ def main():
     ap.add_argument("--batch-systems", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint legs")
+    ap.add_argument("--no-small", action="store_true", help="skip the 16384-particle disk leg")
     ap.add_argument("--no-tune", action="store_true", help="keep the default P2P configuration")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one 256 MiB device-to-device copy (scale reference for HBM counters)")
@@ -196,6 +198,7 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+    small = None if args.no_small else small_leg()
     batched = None if args.no_batch else batch_leg(args)
     adjoint = None if args.no_adjoint else adjoint_leg()
     batched_adjoint = None if (args.no_adjoint or args.no_batch) else batch_adjoint_leg()
@@ -218,6 +221,9 @@ def main():
                    "parallelism": "single GPU", "p2p_config": tuned},
         "roofline": {"kernel": "k_p2p<kP2PStep>", "bound": "valu", "achieved": achieved,
                      "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
+                     "peak_nonpacked": FP32_PEAK_NONPACKED_TFLOPS,
+                     "frac_nonpacked": achieved / FP32_PEAK_NONPACKED_TFLOPS,
+                     "interactions_per_s": pairs / (p2p_avg_ms * 1e-3),
                      "traffic": (measured_traffic() if wl.name == "two_disk_collision" and n == 1 << 20
                                  else None),
                      "avg_launch_us": p2p_avg_ms * 1e3, "launches": launches,
@@ -226,6 +232,7 @@ def main():
                      "occupied_leaves": int(occ0)},
         "cpu_baseline": cpu,
         "accuracy": accuracy,
+        "small_disk": small,
         "batched": batched,
         "adjoint": adjoint,
         "batched_adjoint": batched_adjoint,
@@ -292,6 +299,60 @@ def batch_leg(args):
     return out
 
 
+def small_leg(reps=20):
+    """Plain FMM on the 16384-particle uniform disk (32x32 leaves over
+    [-1.1,1.1]^2, eps 1e-3, dt 1e-4): its 10 Verlet steps replayed as a
+    hipGraph, each of `reps` timed runs starting again from the initial state
+    (two 128 KiB copies on the same stream, inside the timed region); force
+    parity against the oracle and the FMM error against the full fp64 direct
+    sum at the initial state.  At this size a step is a few microseconds of
+    work per launch, so the figure mostly measures launch latency."""
+    import torch
+
+    from nbfmm_amd import Solver
+    from nbfmm_amd.workloads import disk
+    from oracle import pyoracle as orc
+
+    wl = disk()
+    n = wl.n
+    dev = torch.device("cuda:0")
+    s = Solver(wl.num_level, n, wl.limits, wl.eps)
+    x = torch.as_tensor(wl.position, device=dev).contiguous()
+    p = torch.as_tensor(wl.position_pre, device=dev).contiguous()
+    w = torch.as_tensor(wl.weight, device=dev).contiguous()
+    a = torch.zeros_like(x)
+    stream = torch.cuda.Stream()
+    with torch.cuda.stream(stream):
+        s.solve(n, x, w, a)
+    stream.synchronize()
+    g = a.cpu().numpy().astype(np.float64)
+    x0, p0 = x.clone(), p.clone()
+    with torch.cuda.stream(stream):
+        s.run(n, wl.tick, wl.steps, x, p, w, a, use_graph=True)  # warm-up (captures the graph)
+    stream.synchronize()
+    t0 = time.perf_counter()
+    with torch.cuda.stream(stream):
+        for _ in range(reps):
+            x.copy_(x0)
+            p.copy_(p0)
+            s.run(n, wl.tick, wl.steps, x, p, w, a, use_graph=True)
+    stream.synchronize()
+    dt = time.perf_counter() - t0
+    flags = s.flags()
+    s.close()
+    lim = np.array(wl.limits, np.float32)
+    ref, _ = orc.solve(wl.num_level, wl.position, wl.weight, lim, wl.eps)
+    direct = orc.direct_f64(wl.position, wl.weight, wl.eps)
+    out = {"workload": wl.name, "n": n, "num_level": wl.num_level, "steps_per_run": wl.steps, "runs": reps,
+           "value": n * wl.steps * reps / dt, "unit": "particle-steps/s", "ms_per_step": dt * 1e3 / (wl.steps * reps),
+           "gpu_vs_oracle_rel_l2": float(np.linalg.norm(g - ref) / np.linalg.norm(ref)),
+           "gpu_vs_direct_fp64_rel_l2": float(np.linalg.norm(g - direct) / np.linalg.norm(direct)),
+           "oracle_vs_direct_fp64_rel_l2": float(np.linalg.norm(ref - direct) / np.linalg.norm(direct)),
+           "flags": int(flags)}
+    log(f"[bench] small disk {out}")
+    return out
+
+
 def adjoint_leg():
     """Differentiable trajectory: 65536-particle Gaussian disk, 128x128
     leaves, 50 Verlet steps checkpointed every 10; loss = mean squared
@@ -352,12 +413,13 @@ def adjoint_leg():
     return out
 
 
-def batch_adjoint_leg():
+def batch_adjoint_leg(reps=3):
     """Batched mass inference: 64 two-disk collisions (N log-uniform in
     [4096, 262144]), 100 steps checkpointed every 10, loss = position
     mismatch at steps 25/50/75/100 against a run with the true masses,
     starting masses = true x U[0.8,1.2]; gradients wrt the masses of every
-    system in one batched backward."""
+    system in one batched backward.  Forward and backward times are means
+    over `reps` repetitions (after the untimed run that makes the target)."""
     import torch
 
     from nbfmm_amd import Batch
@@ -375,21 +437,28 @@ def batch_adjoint_leg():
     with torch.no_grad():
         target = fmm_batch_trajectory(b, steps, K, obs, X, P, W)
     torch.cuda.synchronize()
-    m = (W * torch.as_tensor(rng.uniform(0.8, 1.2, W.shape[0]).astype(np.float32), device=dev)).requires_grad_(True)
-    t0 = time.perf_counter()
-    out = fmm_batch_trajectory(b, steps, K, obs, X, P, m)
-    loss = ((out - target) ** 2).sum(-1).mean()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    loss.backward()
-    torch.cuda.synchronize()
-    t2 = time.perf_counter()
+    m0 = W * torch.as_tensor(rng.uniform(0.8, 1.2, W.shape[0]).astype(np.float32), device=dev)
+    tf = tb = 0.0
+    for _ in range(reps):
+        m = m0.clone().requires_grad_(True)
+        t0 = time.perf_counter()
+        out = fmm_batch_trajectory(b, steps, K, obs, X, P, m)
+        loss = ((out - target) ** 2).sum(-1).mean()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        loss.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        tf += t1 - t0
+        tb += t2 - t1
+    tf /= reps
+    tb /= reps
     b.close()
     total = bw.n
     res = {"workload": bw.name, "systems": len(bw.systems), "particles": total, "steps": steps,
-           "checkpoint_every": K, "forward_ms": (t1 - t0) * 1e3, "backward_ms": (t2 - t1) * 1e3,
-           "backward_over_forward": (t2 - t1) / (t1 - t0),
-           "forward_backward_particle_steps_per_s": total * steps / (t2 - t0), "loss": float(loss.detach()),
+           "checkpoint_every": K, "repeats": reps, "forward_ms": tf * 1e3, "backward_ms": tb * 1e3,
+           "backward_over_forward": tb / tf,
+           "forward_backward_particle_steps_per_s": total * steps / (tf + tb), "loss": float(loss.detach()),
            "grad_mass_finite": bool(torch.isfinite(m.grad).all())}
     log(f"[bench] batched adjoint {res}")
     return res
