@@ -65,10 +65,10 @@ struct nbfmm_solver {
   // (-1: none) that the next sort of the same n starts from
   bool warm_enabled = true;
   int warm_n = -1;
-  float2* xs = nullptr;  // N: new positions in sorted order, handed from one step of a run to the next
-  // graph of one step
-  hipGraphExec_t graph_exec = nullptr;
-  hipGraph_t graph = nullptr;
+  // graphs of one step of a run: [0] a step followed by another (its P2P
+  // makes the next sort's first pass), [1] the last step
+  hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
+  hipGraph_t graph[2] = {nullptr, nullptr};
   struct {
     int n = -1;
     float tick = 0;
@@ -94,8 +94,10 @@ int dev_alloc(nbfmm_solver* s, T** p, size_t count) {
 }
 
 void free_all(nbfmm_solver* s) {
-  if (s->graph_exec) hipGraphExecDestroy(s->graph_exec);
-  if (s->graph) hipGraphDestroy(s->graph);
+  for (int k = 0; k < 2; ++k) {
+    if (s->graph_exec[k]) hipGraphExecDestroy(s->graph_exec[k]);
+    if (s->graph[k]) hipGraphDestroy(s->graph[k]);
+  }
   for (auto& e : s->ev) hipEventDestroy(e);
   for (void* p : s->allocs) hipFree(p);
   s->allocs.clear();
@@ -156,8 +158,10 @@ int adj_grid_for(const nbfmm_solver* s) {
 
 // A captured step graph holds the launch configuration it was made with.
 void drop_graph(nbfmm_solver* s) {
-  if (s->graph_exec) { hipGraphExecDestroy(s->graph_exec); s->graph_exec = nullptr; }
-  if (s->graph) { hipGraphDestroy(s->graph); s->graph = nullptr; }
+  for (int k = 0; k < 2; ++k) {
+    if (s->graph_exec[k]) { hipGraphExecDestroy(s->graph_exec[k]); s->graph_exec[k] = nullptr; }
+    if (s->graph[k]) { hipGraphDestroy(s->graph[k]); s->graph[k] = nullptr; }
+  }
   s->gkey.n = -1;
 }
 
@@ -183,10 +187,11 @@ int p2p_timed(nbfmm_solver* s, const SysView& v, int mode, const CallIO& io, hip
 
 // Sort stage of the fused paths: warm-started from the previous sort when
 // that was of the same n, else the cold sort.  Either way key/part/head then
-// hold the sort of these n particles.
-int sort_stage(nbfmm_solver* s, const SysView& v, const CallIO& io, hipStream_t st) {
+// hold the sort of these n particles.  gathered: the previous step's P2P
+// (kP2PStepSort, inside a run) made the warm sort's first pass.
+int sort_stage(nbfmm_solver* s, const SysView& v, const CallIO& io, hipStream_t st, bool gathered = false) {
   if (s->warm_enabled && v.n > 0 && s->warm_n == v.n)
-    NB_HIP(launch_predo_warm(v, io, 1, st));
+    NB_HIP(launch_predo_warm(v, io, 1, st, gathered));
   else
     NB_HIP(launch_predo(v, io, 1, st));
   s->warm_n = v.n > 0 ? v.n : -1;
@@ -194,10 +199,11 @@ int sort_stage(nbfmm_solver* s, const SysView& v, const CallIO& io, hipStream_t 
 }
 
 // The fused pipeline shared by solve and step.
-int run_pipeline(nbfmm_solver* s, int n, float tick, int mode, const CallIO& io, hipStream_t st) {
+int run_pipeline(nbfmm_solver* s, int n, float tick, int mode, const CallIO& io, hipStream_t st,
+                 bool gathered = false) {
   SysView v = view(s, n);
   v.tick = tick;
-  int rc = sort_stage(s, v, io, st);
+  int rc = sort_stage(s, v, io, st, gathered);
   if (rc) return rc;
   if (n == 0) return NBFMM_OK;
   NB_HIP(launch_upward(v, std::min(n, s->B * s->B), st));  // P2M + M2M
@@ -286,7 +292,6 @@ int nbfmm_solver_create(int num_level, int max_num_particle, const float positio
   ALLOC(v.flags, 1);
   ALLOC(s->d_pairs, 1);
   ALLOC(s->d_kept, 1);
-  ALLOC(s->xs, N);
   ALLOC(v.wstate, (size_t)kWarmInts);
   ALLOC(v.wrel, N);
   ALLOC(v.wmgrp, N);
@@ -516,33 +521,36 @@ int nbfmm_run(nbfmm_solver* s, int n, float tick, int steps, float* xcur, float*
   if (!st) return fail(NBFMM_ERR_INVALID_ARGUMENT, "graph replay needs a non-null stream");
   if (steps == 0) return NBFMM_OK;
   // The first step is eager: it sorts from the caller's arrays (warm from an
-  // earlier sort of n, else cold) and leaves the new positions in sorted
-  // order in xs.  The other steps replay one captured step; with the warm
-  // start on, its sort reads positions from xs and masses from the previous
-  // sort (w is constant during the call), which the previous step left.
+  // earlier sort of n, else cold).  The other steps replay captured steps.
+  // With the warm start on, the P2P of every step but the last also makes
+  // the first pass of the next step's sort (binning each new position in the
+  // old sorted order, with the mass from the current sort: w is constant
+  // during the call), so those sorts start at their second pass; the last
+  // step leaves nothing pending, and the next call sorts from the caller's
+  // arrays as usual.
   CallIO io = call_io(xcur, w, acc, xcur, xpre);
-  io.xs_out = s->xs;
-  if ((rc = run_pipeline(s, n, tick, kP2PStep, io, st))) return rc;
+  const bool chain = s->warm_enabled;  // after the first step's sort, warm_n == n
+  if ((rc = run_pipeline(s, n, tick, chain && steps > 1 ? kP2PStepSort : kP2PStep, io, st))) return rc;
   if (--steps == 0) return NBFMM_OK;
-  const bool warm = s->warm_enabled && s->warm_n == n;
   auto& g = s->gkey;
-  const bool same = s->graph_exec && g.n == n && g.tick == tick && g.xcur == xcur && g.xpre == xpre &&
-                    g.w == w && g.acc == acc && g.stream == st && g.warm == warm;
+  const bool same = s->graph_exec[1] && g.n == n && g.tick == tick && g.xcur == xcur && g.xpre == xpre &&
+                    g.w == w && g.acc == acc && g.stream == st && g.warm == chain;
   if (!same) {
-    if (s->graph_exec) { hipGraphExecDestroy(s->graph_exec); s->graph_exec = nullptr; }
-    if (s->graph) { hipGraphDestroy(s->graph); s->graph = nullptr; }
-    if (warm) io.xs_in = s->xs;
-    NB_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    rc = run_pipeline(s, n, tick, kP2PStep, io, st);
-    hipGraph_t graph = nullptr;
-    hipError_t e = hipStreamEndCapture(st, &graph);
-    if (rc) { if (graph) hipGraphDestroy(graph); return rc; }
-    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
-    s->graph = graph;
-    NB_HIP(hipGraphInstantiate(&s->graph_exec, graph, nullptr, nullptr, 0));
-    g.n = n; g.tick = tick; g.xcur = xcur; g.xpre = xpre; g.w = w; g.acc = acc; g.stream = st; g.warm = warm;
+    drop_graph(s);
+    for (int k = 0; k < 2; ++k) {
+      if (k == 0 && !chain) continue;  // without the warm start every step is the last kind
+      NB_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      rc = run_pipeline(s, n, tick, k == 0 ? kP2PStepSort : kP2PStep, io, st, chain);
+      hipGraph_t graph = nullptr;
+      hipError_t e = hipStreamEndCapture(st, &graph);
+      if (rc) { if (graph) hipGraphDestroy(graph); return rc; }
+      if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+      s->graph[k] = graph;
+      NB_HIP(hipGraphInstantiate(&s->graph_exec[k], graph, nullptr, nullptr, 0));
+    }
+    g.n = n; g.tick = tick; g.xcur = xcur; g.xpre = xpre; g.w = w; g.acc = acc; g.stream = st; g.warm = chain;
   }
-  for (int k = 0; k < steps; ++k) NB_HIP(hipGraphLaunch(s->graph_exec, st));
+  for (int k = 0; k < steps; ++k) NB_HIP(hipGraphLaunch(s->graph_exec[chain && k + 1 < steps ? 0 : 1], st));
   return NBFMM_OK;
 }
 

@@ -26,6 +26,7 @@ enum P2PMode : int {
   kP2POnly = 0,     // sorted effect = P2P sum (stage p2p)
   kP2PFinal = 1,    // + leaf local (L2P), scattered to original order (postdo)
   kP2PStep = 2,     // as kP2PFinal, then the Verlet update of that particle
+  kP2PStepSort = 3, // as kP2PStep, and the first pass of the next step's warm-started sort
 };
 
 // Everything one system's kernels need: sizes, box and the device arrays.
@@ -223,6 +224,47 @@ struct ManySys {
 // result differs from the reference's correctly rounded sqrt and division by
 // about one ulp per pair.  A pair with d = 0 (a particle with itself)
 // contributes exactly 0 because eps > 0.
+// predo.cu:37-38: floorf((p - lo) / cell) with IEEE division.  Particles
+// outside the box (or NaN) are clamped into the grid and reported through
+// `bad`; the reference has no clamp (a particle at x_max indexes cell B).
+__device__ __forceinline__ uint32_t bin_key(const SysView& v, float2 p, bool& bad) {
+#pragma clang fp contract(off)
+  const float fx = floorf((p.x - v.xmin) / v.cw);
+  const float fy = floorf((p.y - v.ymin) / v.ch);
+  int ix, iy;
+  bad = false;
+  if (!(fx >= 0.0f)) { ix = 0; bad = true; }
+  else if (fx >= (float)v.B) { ix = v.B - 1; bad = true; }
+  else ix = (int)fx;
+  if (!(fy >= 0.0f)) { iy = 0; bad = true; }
+  else if (fy >= (float)v.B) { iy = v.B - 1; bad = true; }
+  else iy = (int)fy;
+  return (uint32_t)(iy * v.B + ix);
+}
+
+// First pass of the warm-started sort for the particle at sorted slot i of
+// the current sort (key kold), now at position p with mass m: its entry in
+// the new order's input (tpart, tkey, wrel), and for a particle that changes
+// leaf (a mover) the per-leaf in/out counts and the first touch of both
+// leaves, which records the leaf's old slot range.  Returns whether p was
+// outside the box.
+__device__ __forceinline__ bool warm_first_pass(const SysView& v, int i, uint32_t kold, float2 p, float m, float w) {
+  bool bad;
+  const uint32_t k = bin_key(v, p, bad);
+  v.tpart[i] = make_float4(p.x, p.y, m, w);
+  if (k == kold) {
+    v.tkey[i] = k;
+    v.wrel[i] = i - v.head[kold];
+  } else {
+    v.tkey[i] = k | kMoverBit;
+    atomicAdd(&v.wcout[kold], 1);
+    atomicAdd(&v.wcin[k], 1);
+    if (atomicExch(&v.wtouch[kold], 1) == 0) v.wtold[kold] = make_int2(v.head[kold], v.head[kold + 1] - v.head[kold]);
+    if (atomicExch(&v.wtouch[k], 1) == 0) v.wtold[k] = make_int2(v.head[k], v.head[k + 1] - v.head[k]);
+  }
+  return bad;
+}
+
 __device__ __forceinline__ void kernel_accum(float tx, float ty, float sx, float sy, float m, float eps,
                                              float& ax, float& ay) {
   const float dx = sx - tx;

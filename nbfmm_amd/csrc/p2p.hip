@@ -96,7 +96,9 @@ __device__ __forceinline__ void seg_sum(const SysView& v, float4* src, const Run
 
 // The targets' epilogue.  kP2POnly stores the sorted effect; kP2PFinal adds
 // the leaf local (L2L folded in, L2P) and scatters to the original order
-// (postdo); kP2PStep also applies the Verlet update.
+// (postdo); kP2PStep also applies the Verlet update; kP2PStepSort then makes
+// the first pass of the next step's warm-started sort from the new position
+// (what k_warm_gather would do at the start of that step, same bits).
 template <int MODE>
 __device__ __forceinline__ void p2p_finish(const SysView& v, const CallIO& io, const RunGeom& g, float ax, float ay) {
 #pragma clang fp contract(off)
@@ -111,7 +113,7 @@ __device__ __forceinline__ void p2p_finish(const SysView& v, const CallIO& io, c
   const float ey = ay + loc.y;
   const uint32_t o = __float_as_uint(g.pt.w);
   io.acc0[v.part_off + o] = make_float2(ex, ey);
-  if (MODE == kP2PStep) {
+  if (MODE == kP2PStep || MODE == kP2PStepSort) {
     // update.cu:39-41
     float2* xcur = io.xcur + v.part_off;
     float2* xpre = io.xpre + v.part_off;
@@ -122,7 +124,11 @@ __device__ __forceinline__ void p2p_finish(const SysView& v, const CallIO& io, c
     nx.y = 2.0f * x.y - xp.y + ey * v.tick * v.tick;
     xcur[o] = nx;
     xpre[o] = x;
-    if (io.xs_out) io.xs_out[v.part_off + g.i] = nx;
+    if (MODE == kP2PStepSort) {
+      if (warm_first_pass(v, g.i, (uint32_t)(g.y * v.B + g.xt), nx, g.pt.z, g.pt.w)) atomicOr(v.flags, 1u);
+    } else if (io.xs_out) {
+      io.xs_out[v.part_off + g.i] = nx;
+    }
   }
 }
 
@@ -151,6 +157,9 @@ __global__ __launch_bounds__(WAVES * kWave) void k_p2p(Sys sys, CallIO io) {
   const float eps = v.eps;
   row_runs<WAVES * kWave>(v, s_runs, sh);
   if (threadIdx.x == 0) run_slot_init(slot);
+  // the touched-leaf count the next sort fills (this step's merge, its last
+  // reader, has finished)
+  if (MODE == kP2PStepSort && blk == 0 && threadIdx.x == 0) v.wstate[kWarmTouched] = 0;
   __syncthreads();
   run_queue<2>(
       v, s_runs, slot, blk, nblk, s_runs[v.B], lane,
@@ -291,6 +300,7 @@ hipError_t launch_p2p(const SysView& v, int mode, const CallIO& io, int grid_blo
   switch (mode) {
     case kP2POnly: launch_p2p_variant<kP2POnly, OneSys>(variant, g, one, io, s); break;
     case kP2PFinal: launch_p2p_variant<kP2PFinal, OneSys>(variant, g, one, io, s); break;
+    case kP2PStepSort: launch_p2p_variant<kP2PStepSort, OneSys>(variant, g, one, io, s); break;
     default: launch_p2p_variant<kP2PStep, OneSys>(variant, g, one, io, s); break;
   }
   return hipGetLastError();

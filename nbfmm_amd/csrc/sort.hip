@@ -30,24 +30,6 @@ namespace nbfmm_amd {
 
 static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 
-// predo.cu:37-38: floorf((p - lo) / cell) with IEEE division.  Particles
-// outside the box (or NaN) are clamped into the grid and reported through
-// `bad`; the reference has no clamp (a particle at x_max indexes cell B).
-__device__ __forceinline__ uint32_t bin_key(const SysView& v, float2 p, bool& bad) {
-#pragma clang fp contract(off)
-  const float fx = floorf((p.x - v.xmin) / v.cw);
-  const float fy = floorf((p.y - v.ymin) / v.ch);
-  int ix, iy;
-  bad = false;
-  if (!(fx >= 0.0f)) { ix = 0; bad = true; }
-  else if (fx >= (float)v.B) { ix = v.B - 1; bad = true; }
-  else ix = (int)fx;
-  if (!(fy >= 0.0f)) { iy = 0; bad = true; }
-  else if (fy >= (float)v.B) { iy = v.B - 1; bad = true; }
-  else iy = (int)fy;
-  return (uint32_t)(iy * v.B + ix);
-}
-
 // Lanes of the wave whose `d` (nbits wide) equals this lane's, among the
 // lanes where `valid` holds.
 __device__ __forceinline__ uint64_t match_mask(uint32_t d, bool valid, int nbits) {
@@ -377,11 +359,6 @@ __global__ __launch_bounds__(256) void k_empty(SysView v) {
 // block.  Any number of movers may enter a leaf, so the warm passes never
 // need the cold sort.
 
-// First touch of leaf c records its old slot range.
-__device__ __forceinline__ void warm_touch(const SysView& v, uint32_t c) {
-  if (atomicExch(&v.wtouch[c], 1) == 0) v.wtold[c] = make_int2(v.head[c], v.head[c + 1] - v.head[c]);
-}
-
 template <class Sys>
 __global__ __launch_bounds__(256) void k_warm_gather(Sys sys, CallIO io) {
   int blk, nblk;
@@ -400,19 +377,7 @@ __global__ __launch_bounds__(256) void k_warm_gather(Sys sys, CallIO io) {
       p = io.pos0[v.part_off + o];
       m = io.w0[v.part_off + o];
     }
-    const uint32_t k = bin_key(v, p, bad);
-    const uint32_t kold = v.key[i];
-    v.tpart[i] = make_float4(p.x, p.y, m, old.w);
-    if (k == kold) {
-      v.tkey[i] = k;
-      v.wrel[i] = i - v.head[kold];
-    } else {
-      v.tkey[i] = k | kMoverBit;
-      atomicAdd(&v.wcout[kold], 1);
-      atomicAdd(&v.wcin[k], 1);
-      warm_touch(v, kold);
-      warm_touch(v, k);
-    }
+    bad = warm_first_pass(v, i, v.key[i], p, m, old.w);
   }
   // the touched-leaf count k_warm_finish fills (the previous step's merge,
   // its last reader, has finished)
@@ -697,11 +662,11 @@ hipError_t launch_predo_batch(const BatchLaunch& b, const CallIO& io, hipStream_
 }
 
 // Warm-started predo (key/part/head hold an earlier sort of the same n).
-hipError_t launch_predo_warm(const SysView& v, const CallIO& io, int zero_empty, hipStream_t s) {
+hipError_t launch_predo_warm(const SysView& v, const CallIO& io, int zero_empty, hipStream_t s, bool gathered) {
   if (v.n <= 0) return launch_predo(v, io, zero_empty, s);
   const OneSys one{v};
   const int nb = cdiv(v.n, 256);
-  hipLaunchKernelGGL(k_warm_gather<OneSys>, dim3(nb), dim3(256), 0, s, one, io);
+  if (!gathered) hipLaunchKernelGGL(k_warm_gather<OneSys>, dim3(nb), dim3(256), 0, s, one, io);
   hipLaunchKernelGGL(k_warm_rows<OneSys>, dim3(v.B), dim3(kRowThreads), 0, s, one);
   hipLaunchKernelGGL(k_warm_finish<OneSys>, dim3(v.B), dim3(kRowThreads), 0, s, one, zero_empty, 0);
   hipLaunchKernelGGL(k_warm_place<OneSys>, dim3(nb), dim3(256), 0, s, one);
