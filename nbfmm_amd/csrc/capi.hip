@@ -835,7 +835,6 @@ struct nbfmm_batch {
   // takes 1.23 ms with the warm-started sort and 1.26 ms with the cold one.
   bool warm_enabled = true;
   bool warm_valid = false;        // key/part/head hold a sort of every system
-  float2* xs = nullptr;           // sorted positions handed from one step of a run to the next
   int cus = 256;
   int p2p_variant = kDefaultBatchP2PVariant;
   int p2p_bpc = kDefaultBatchP2PBlocksPerCU;
@@ -847,8 +846,10 @@ struct nbfmm_batch {
   bool balanced = false;
   unsigned long long* d_pairs = nullptr;  // S pair counts
   std::vector<void*> allocs;
-  hipGraphExec_t graph_exec = nullptr;
-  hipGraph_t graph = nullptr;
+  // graphs of one step of a run, as for a solver: [0] a step followed by
+  // another, [1] the last step
+  hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
+  hipGraph_t graph[2] = {nullptr, nullptr};
   struct {
     const void *xcur = nullptr, *xpre = nullptr, *w = nullptr, *acc = nullptr;
     hipStream_t stream = nullptr;
@@ -873,16 +874,20 @@ int batch_alloc(nbfmm_batch* b, T** p, size_t count) {
 }
 
 void batch_free(nbfmm_batch* b) {
-  if (b->graph_exec) hipGraphExecDestroy(b->graph_exec);
-  if (b->graph) hipGraphDestroy(b->graph);
+  for (int k = 0; k < 2; ++k) {
+    if (b->graph_exec[k]) hipGraphExecDestroy(b->graph_exec[k]);
+    if (b->graph[k]) hipGraphDestroy(b->graph[k]);
+  }
   for (void* p : b->allocs) hipFree(p);
   b->allocs.clear();
 }
 
 // A captured step graph holds the kernels and launch maps of its capture.
 void drop_batch_graph(nbfmm_batch* b) {
-  if (b->graph_exec) { hipGraphExecDestroy(b->graph_exec); b->graph_exec = nullptr; }
-  if (b->graph) { hipGraphDestroy(b->graph); b->graph = nullptr; }
+  for (int k = 0; k < 2; ++k) {
+    if (b->graph_exec[k]) { hipGraphExecDestroy(b->graph_exec[k]); b->graph_exec[k] = nullptr; }
+    if (b->graph[k]) { hipGraphDestroy(b->graph[k]); b->graph[k] = nullptr; }
+  }
   b->gkey.valid = false;
 }
 
@@ -971,9 +976,9 @@ int batch_use_p2p(nbfmm_batch* b, int variant, int bpc) {
 
 // Sort stage of a batch: warm-started from its previous sort when there is
 // one (every system keeps its N), else cold.
-int batch_sort(nbfmm_batch* b, const CallIO& io, hipStream_t st) {
+int batch_sort(nbfmm_batch* b, const CallIO& io, hipStream_t st, bool gathered = false) {
   if (b->warm_enabled && b->warm_valid)
-    NB_HIP(launch_predo_warm_batch(b->bl, io, st));
+    NB_HIP(launch_predo_warm_batch(b->bl, io, st, gathered));
   else
     NB_HIP(launch_predo_batch(b->bl, io, st));
   b->warm_valid = true;
@@ -989,8 +994,8 @@ int batch_balance_once(nbfmm_batch* b, hipStream_t st) {
   return batch_balance(b, st);
 }
 
-int batch_pipeline(nbfmm_batch* b, int mode, const CallIO& io, hipStream_t st) {
-  if (int rc = batch_sort(b, io, st)) return rc;
+int batch_pipeline(nbfmm_batch* b, int mode, const CallIO& io, hipStream_t st, bool gathered = false) {
+  if (int rc = batch_sort(b, io, st, gathered)) return rc;
   if (int rc = batch_balance_once(b, st)) return rc;
   NB_HIP(launch_tree_batch(b->bl, st));
   NB_HIP(launch_p2p_batch(b->bl, mode, io, b->p2p_variant, st));
@@ -1107,7 +1112,6 @@ int nbfmm_batch_create(int S, const int* num_particle, const int* num_level, con
   BALLOC(b->ddesc, (size_t)S);
   int *wstate, *wrel, *wmgrp, *wcin, *wcout, *wtouch, *wtlist, *wcnew, *winoff, *wfill, *wrowtot;
   int2* wtold;
-  BALLOC(b->xs, n_all);
   BALLOC(wstate, (size_t)S * kWarmInts);
   BALLOC(wrel, n_all);
   BALLOC(wmgrp, n_all);
@@ -1262,33 +1266,31 @@ int nbfmm_batch_run(nbfmm_batch* b, int steps, float* xcur, float* xpre, const f
   }
   if (!st) return fail(NBFMM_ERR_INVALID_ARGUMENT, "graph replay needs a non-null stream");
   if (steps == 0) return NBFMM_OK;
-  // As nbfmm_run: the first step is eager and leaves the new positions in
-  // sorted order; the captured step replayed for the others reads them
-  // (with the warm start on) instead of gathering from the caller's arrays.
+  // As nbfmm_run: the first step is eager; with the warm start on, the P2P
+  // of every step but the last makes the next step's first sort pass.
   CallIO io = call_io(xcur, w, acc, xcur, xpre);
-  io.xs_out = b->xs;
-  if ((rc = batch_pipeline(b, kP2PStep, io, st))) return rc;
+  const bool chain = b->warm_enabled;  // after the first step's sort every system has a sort
+  if ((rc = batch_pipeline(b, chain && steps > 1 ? kP2PStepSort : kP2PStep, io, st))) return rc;
   if (--steps == 0) return NBFMM_OK;
-  const bool warm = b->warm_enabled;
   auto& g = b->gkey;
-  const bool same = g.valid && g.xcur == xcur && g.xpre == xpre && g.w == w && g.acc == acc && g.stream == st &&
-                    g.warm == warm;
+  const bool same = g.valid && b->graph_exec[1] && g.xcur == xcur && g.xpre == xpre && g.w == w && g.acc == acc &&
+                    g.stream == st && g.warm == chain;
   if (!same) {
-    if (b->graph_exec) { hipGraphExecDestroy(b->graph_exec); b->graph_exec = nullptr; }
-    if (b->graph) { hipGraphDestroy(b->graph); b->graph = nullptr; }
-    g.valid = false;
-    if (warm) io.xs_in = b->xs;
-    NB_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    rc = batch_pipeline(b, kP2PStep, io, st);
-    hipGraph_t graph = nullptr;
-    hipError_t e = hipStreamEndCapture(st, &graph);
-    if (rc) { if (graph) hipGraphDestroy(graph); return rc; }
-    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
-    b->graph = graph;
-    NB_HIP(hipGraphInstantiate(&b->graph_exec, graph, nullptr, nullptr, 0));
-    g.xcur = xcur; g.xpre = xpre; g.w = w; g.acc = acc; g.stream = st; g.valid = true; g.warm = warm;
+    drop_batch_graph(b);
+    for (int k = 0; k < 2; ++k) {
+      if (k == 0 && !chain) continue;  // without the warm start every step is the last kind
+      NB_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      rc = batch_pipeline(b, k == 0 ? kP2PStepSort : kP2PStep, io, st, chain);
+      hipGraph_t graph = nullptr;
+      hipError_t e = hipStreamEndCapture(st, &graph);
+      if (rc) { if (graph) hipGraphDestroy(graph); return rc; }
+      if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+      b->graph[k] = graph;
+      NB_HIP(hipGraphInstantiate(&b->graph_exec[k], graph, nullptr, nullptr, 0));
+    }
+    g.xcur = xcur; g.xpre = xpre; g.w = w; g.acc = acc; g.stream = st; g.valid = true; g.warm = chain;
   }
-  for (int k = 0; k < steps; ++k) NB_HIP(hipGraphLaunch(b->graph_exec, st));
+  for (int k = 0; k < steps; ++k) NB_HIP(hipGraphLaunch(b->graph_exec[chain && k + 1 < steps ? 0 : 1], st));
   return NBFMM_OK;
 }
 

@@ -121,13 +121,6 @@ struct CallIO {
   float2* gpre = nullptr;        // step adjoint: pi (in/out)
   float* gw = nullptr;           // d loss / d mass (solve adjoint: out; step adjoint: += )
   float gscale = 0.0f;           // upstream scale (step adjoint: tick^2)
-  // inside one multi-step run (the caller's arrays are not touched between
-  // steps): the step kernel also writes each new position at its sorted slot
-  // (xs_out), and the next warm-started sort reads positions from there in
-  // the old sorted order (xs_in) and masses from the previous sort, instead
-  // of gathering both from the caller's arrays
-  float2* xs_out = nullptr;
-  const float2* xs_in = nullptr;
 };
 
 // Which system a block works on, and its block index / block count inside

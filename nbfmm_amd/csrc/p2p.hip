@@ -124,11 +124,8 @@ __device__ __forceinline__ void p2p_finish(const SysView& v, const CallIO& io, c
     nx.y = 2.0f * x.y - xp.y + ey * v.tick * v.tick;
     xcur[o] = nx;
     xpre[o] = x;
-    if (MODE == kP2PStepSort) {
-      if (warm_first_pass(v, g.i, (uint32_t)(g.y * v.B + g.xt), nx, g.pt.z, g.pt.w)) atomicOr(v.flags, 1u);
-    } else if (io.xs_out) {
-      io.xs_out[v.part_off + g.i] = nx;
-    }
+    if (MODE == kP2PStepSort && warm_first_pass(v, g.i, (uint32_t)(g.y * v.B + g.xt), nx, g.pt.z, g.pt.w))
+      atomicOr(v.flags, 1u);
   }
 }
 
@@ -310,6 +307,8 @@ hipError_t launch_p2p_batch(const BatchLaunch& bl, int mode, const CallIO& io, i
   const dim3 g(bl.p2p.blocks);
   if (mode == kP2PFinal)
     launch_p2p_variant<kP2PFinal, ManySys>(variant, g, bl.p2p.sys, io, s);
+  else if (mode == kP2PStepSort)
+    launch_p2p_variant<kP2PStepSort, ManySys>(variant, g, bl.p2p.sys, io, s);
   else
     launch_p2p_variant<kP2PStep, ManySys>(variant, g, bl.p2p.sys, io, s);
   return hipGetLastError();

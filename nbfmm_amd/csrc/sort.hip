@@ -346,7 +346,9 @@ __global__ __launch_bounds__(256) void k_empty(SysView v) {
 // placing: a leaf's new content is its stayers merged with its incoming
 // movers by original index.  The result is the same stable argsort as the
 // cold path, bit for bit.
-//   k_warm_gather  gather, bin, mark movers and the leaves they touch
+//   k_warm_gather  gather, bin, mark movers and the leaves they touch (inside
+//                  a run, the previous step's P2P does this pass instead:
+//                  warm_first_pass)
 //   k_warm_rows    new leaf counts, row-local offsets (all rows, one block each)
 //   k_warm_finish  head, occupied-leaf list, movers' group offsets, list
 //                  of touched leaves
@@ -368,16 +370,7 @@ __global__ __launch_bounds__(256) void k_warm_gather(Sys sys, CallIO io) {
   if (i < v.n) {
     const float4 old = v.part[i];
     const uint32_t o = __float_as_uint(old.w);
-    float2 p;
-    float m;
-    if (io.xs_in) {
-      p = io.xs_in[v.part_off + i];
-      m = old.z;
-    } else {
-      p = io.pos0[v.part_off + o];
-      m = io.w0[v.part_off + o];
-    }
-    bad = warm_first_pass(v, i, v.key[i], p, m, old.w);
+    bad = warm_first_pass(v, i, v.key[i], io.pos0[v.part_off + o], io.w0[v.part_off + o], old.w);
   }
   // the touched-leaf count k_warm_finish fills (the previous step's merge,
   // its last reader, has finished)
@@ -675,8 +668,9 @@ hipError_t launch_predo_warm(const SysView& v, const CallIO& io, int zero_empty,
   return hipGetLastError();
 }
 
-hipError_t launch_predo_warm_batch(const BatchLaunch& b, const CallIO& io, hipStream_t s) {
-  hipLaunchKernelGGL(k_warm_gather<ManySys>, dim3(b.adj_sort.blocks), dim3(256), 0, s, b.adj_sort.sys, io);
+hipError_t launch_predo_warm_batch(const BatchLaunch& b, const CallIO& io, hipStream_t s, bool gathered) {
+  if (!gathered)
+    hipLaunchKernelGGL(k_warm_gather<ManySys>, dim3(b.adj_sort.blocks), dim3(256), 0, s, b.adj_sort.sys, io);
   hipLaunchKernelGGL(k_warm_rows<ManySys>, dim3(b.rows.blocks), dim3(kRowThreads), 0, s, b.rows.sys);
   hipLaunchKernelGGL(k_warm_rowscan<ManySys>, dim3(b.one.blocks), dim3(kRowThreads), 0, s, b.one.sys);
   hipLaunchKernelGGL(k_warm_finish<ManySys>, dim3(b.rows.blocks), dim3(kRowThreads), 0, s, b.rows.sys, 1, 1);
