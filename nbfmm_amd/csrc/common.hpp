@@ -238,9 +238,9 @@ __device__ __forceinline__ uint32_t bin_key(const SysView& v, float2 p, bool& ba
 // First pass of the warm-started sort for the particle at sorted slot i of
 // the current sort (key kold), now at position p with mass m: its entry in
 // the new order's input (tpart, tkey, wrel), and for a particle that changes
-// leaf (a mover) the per-leaf in/out counts and the first touch of both
-// leaves, which records the leaf's old slot range.  Returns whether p was
-// outside the box.
+// leaf (a mover) the per-leaf in/out counts (no value returned, so the wave
+// does not wait for them; the next pass marks the leaves they touch).
+// Returns whether p was outside the box.
 __device__ __forceinline__ bool warm_first_pass(const SysView& v, int i, uint32_t kold, float2 p, float m, float w) {
   bool bad;
   const uint32_t k = bin_key(v, p, bad);
@@ -252,8 +252,6 @@ __device__ __forceinline__ bool warm_first_pass(const SysView& v, int i, uint32_
     v.tkey[i] = k | kMoverBit;
     atomicAdd(&v.wcout[kold], 1);
     atomicAdd(&v.wcin[k], 1);
-    if (atomicExch(&v.wtouch[kold], 1) == 0) v.wtold[kold] = make_int2(v.head[kold], v.head[kold + 1] - v.head[kold]);
-    if (atomicExch(&v.wtouch[k], 1) == 0) v.wtold[k] = make_int2(v.head[k], v.head[k + 1] - v.head[k]);
   }
   return bad;
 }

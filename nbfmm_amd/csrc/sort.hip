@@ -379,8 +379,9 @@ __global__ __launch_bounds__(256) void k_warm_gather(Sys sys, CallIO io) {
 }
 
 // One block per row: new count of every leaf (old count - out + in), its
-// row-local exclusive offset, and the row-local offset of its movers.
-// Clears wcin/wcout.
+// row-local exclusive offset, and the row-local offset of its movers.  A
+// leaf that movers enter or leave is marked touched, with its old slot
+// range.  Clears wcin/wcout.
 template <class Sys>
 __global__ __launch_bounds__(kRowThreads) void k_warm_rows(Sys sys) {
   __shared__ uint32_t sh[kRowWaves];
@@ -392,11 +393,14 @@ __global__ __launch_bounds__(kRowThreads) void k_warm_rows(Sys sys) {
   uint32_t cnt = 0, cin = 0;
   if (x < B) {
     const int ci = v.wcin[c], co = v.wcout[c];
+    const int h0 = v.head[c], h1 = v.head[c + 1];
     if (ci | co) {
       v.wcin[c] = 0;
       v.wcout[c] = 0;
+      v.wtouch[c] = 1;
+      v.wtold[c] = make_int2(h0, h1 - h0);
     }
-    cnt = (uint32_t)(v.head[c + 1] - v.head[c] - co + ci);
+    cnt = (uint32_t)(h1 - h0 - co + ci);
     cin = (uint32_t)ci;
   }
   if (y == 0 && x == 0) *v.nocc = 0;  // the occupied-leaf counter k_warm_finish fills

@@ -154,8 +154,11 @@ def test_adjoint_after_warm_sort():
 
 def test_run_after_caller_edits_arrays():
     """Between two runs the caller moves every particle and changes masses:
-    the next run's first step reads the caller's arrays again (its later
-    steps hand positions over in sorted order)."""
+    the next run's first step reads the caller's arrays again.  Inside a run
+    the P2P of every step but the last makes the next sort's first pass, so
+    runs of 1 (nothing pending), 2 (the eager step makes the pass, the last
+    graph takes it) and more steps, and an eager step after a run, must all
+    leave nothing pending."""
     from nbfmm_amd.workloads import two_disk_collision
     wl = two_disk_collision(n=1 << 14, num_level=5, steps=6)
     warm, cold = pair(wl)
@@ -164,11 +167,13 @@ def test_run_after_caller_edits_arrays():
     Xc, Pc, Wc = Xw.clone(), Pw.clone(), Ww.clone()
     Aw, Ac = torch.zeros_like(Xw), torch.zeros_like(Xw)
     st = torch.cuda.Stream()
-    for rnd in range(3):
+    for rnd, steps in enumerate((3, 1, 2, 4)):
         with torch.cuda.stream(st):
-            warm.run(wl.n, wl.tick, 3, Xw, Pw, Ww, Aw, use_graph=True)
+            warm.run(wl.n, wl.tick, steps, Xw, Pw, Ww, Aw, use_graph=True)
+            if rnd == 2:
+                warm.step(wl.n, wl.tick, Xw, Pw, Ww, Aw)
         st.synchronize()
-        for _ in range(3):
+        for _ in range(steps + (rnd == 2)):
             cold.step(wl.n, wl.tick, Xc, Pc, Wc, Ac)
         torch.cuda.synchronize()
         assert torch.equal(Aw, Ac) and torch.equal(Xw, Xc) and torch.equal(Pw, Pc), rnd
@@ -222,8 +227,10 @@ def test_batch_warm_equals_cold():
     st = torch.cuda.Stream()
     with torch.cuda.stream(st):
         warm.run(3, Xw, Pw, M, Aw, use_graph=True)
+        warm.run(1, Xw, Pw, M, Aw, use_graph=True)
+        warm.run(2, Xw, Pw, M, Aw, use_graph=True)
         warm.run(4, Xw, Pw, M, Aw, use_graph=True)
-        cold.run(7, Xc, Pc, M, Ac, use_graph=True)
+        cold.run(10, Xc, Pc, M, Ac, use_graph=True)
     st.synchronize()
     assert torch.equal(Aw, Ac) and torch.equal(Xw, Xc) and torch.equal(Pw, Pc)
     warm.close()
