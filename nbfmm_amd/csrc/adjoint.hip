@@ -116,28 +116,41 @@ __global__ __launch_bounds__(256) void k_adj_sort(Sys sys, CallIO io) {
   v.gsort[i] = make_float2(sc * g0.x, sc * g0.y);
 }
 
-// Sum of the sorted upstream gradient over each occupied leaf (L2P^T).
+// Sum of the sorted upstream gradient over each occupied leaf (L2P^T), in
+// sorted order.  As in P2M, the time is the chain of the heaviest leaf, so
+// each thread keeps the next kAdjLeafDepth values in flight while it adds
+// the current ones; loads are unconditional (indices clamped to the leaf's
+// last particle), only the adds are predicated.
+constexpr int kAdjLeafDepth = 16;
+
 template <class Sys>
 __global__ __launch_bounds__(256) void k_adj_leaf(Sys sys) {
+#pragma clang fp contract(off)
   int blk, nblk;
   const SysView v = sys.get(blk, nblk);
   const int j = blk * blockDim.x + threadIdx.x;
   if (j >= *v.nocc) return;
   const int c = v.occ[j];
   const int a = v.head[c], b = v.head[c + 1];
+  const int last = b - 1;
+  float2 next[kAdjLeafDepth];
+#pragma unroll
+  for (int k = 0; k < kAdjLeafDepth; ++k) next[k] = v.gsort[min(a + k, last)];
   float sx = 0.0f, sy = 0.0f;
-  int i = a;
-  for (; i + 4 <= b; i += 4) {
-    const float2 g0 = v.gsort[i], g1 = v.gsort[i + 1], g2 = v.gsort[i + 2], g3 = v.gsort[i + 3];
-    sx += g0.x; sy += g0.y;
-    sx += g1.x; sy += g1.y;
-    sx += g2.x; sy += g2.y;
-    sx += g3.x; sy += g3.y;
-  }
-  for (; i < b; ++i) {
-    const float2 g = v.gsort[i];
-    sx += g.x;
-    sy += g.y;
+  for (int s = a; s < b; s += kAdjLeafDepth) {
+    float2 cur[kAdjLeafDepth];
+#pragma unroll
+    for (int k = 0; k < kAdjLeafDepth; ++k) cur[k] = next[k];
+#pragma unroll
+    for (int k = 0; k < kAdjLeafDepth; ++k) next[k] = v.gsort[min(s + kAdjLeafDepth + k, last)];
+    const int m = b - s;
+#pragma unroll
+    for (int k = 0; k < kAdjLeafDepth; ++k) {
+      if (k < m) {
+        sx += cur[k].x;
+        sy += cur[k].y;
+      }
+    }
   }
   v.gup[c] = make_float2(sx, sy);
 }
