@@ -347,6 +347,21 @@ __device__ __forceinline__ int run_row(const int* s_runs, int B, int j) {
   return lo;
 }
 
+// run_row for a whole wave (j wave-uniform): every lane tests one row of a
+// stride-S sample (S = ceil(B / 64)), then the first S lanes test the rows
+// of the sampled interval that holds the answer; two LDS round trips instead
+// of log2(B).  s_runs is nondecreasing, so each test's true lanes form a
+// prefix and the answer is the highest of them.
+__device__ __forceinline__ int run_row_wave(const int* s_runs, int B, int j, int lane) {
+  const int S = (B + kWave - 1) / kWave;
+  const int rc = lane * S;
+  const uint64_t coarse = __ballot(rc < B && s_runs[rc < B ? rc : 0] <= j);
+  const int base = (63 - __builtin_clzll(coarse)) * S;  // lane 0 always holds (s_runs[0] = 0)
+  const int rf = base + lane;
+  const uint64_t fine = __ballot(lane < S && rf < B && s_runs[rf < B ? rf : 0] <= j);
+  return base + 63 - __builtin_clzll(fine);
+}
+
 inline int level_dim(int B, int l) { return B >> l; }
 
 }  // namespace nbfmm_amd

@@ -54,7 +54,7 @@ struct RunGeom {
 __device__ __forceinline__ void run_geom(const SysView& v, const int* s_runs, int j, int lane, RunGeom& g) {
   const int B = v.B;
   // everything but the lane's own target is wave-uniform: scalar registers
-  g.y = __builtin_amdgcn_readfirstlane(run_row(s_runs, B, j));
+  g.y = __builtin_amdgcn_readfirstlane(run_row_wave(s_runs, B, j, lane));
   const int T0 = __builtin_amdgcn_readfirstlane(v.head[g.y * B] + kWave * (j - s_runs[g.y]));
   const int nt = __builtin_amdgcn_readfirstlane(min(kWave, v.head[(g.y + 1) * B] - T0));
   g.valid = lane < nt;
