@@ -476,7 +476,8 @@ hipError_t launch_adjoint(const SysView& v, const CallIO& io, int mode, int grid
   if (v.L > 0) {
     const int C = v.B * v.B;
     hipLaunchKernelGGL(k_adj_zero<OneSys>, dim3(adj_zero_blocks(v)), dim3(256), 0, s, one);
-    hipLaunchKernelGGL(k_adj_leaf<OneSys>, dim3(cdiv(v.n < C ? v.n : C, 256)), dim3(256), 0, s, one);
+    // the P2M grid (its occupied-leaf blocks of 256 threads)
+    hipLaunchKernelGGL(k_adj_leaf<OneSys>, dim3(p2m_occ_blocks(v.n < C ? v.n : C)), dim3(256), 0, s, one);
     const int tb = upward_tile_blocks(v.L, v.B);
     if (tb) hipLaunchKernelGGL(k_adj_up_tile<OneSys>, dim3(tb), dim3(kAdjTile * kAdjTile), 0, s, one);
     if (upward_top_blocks(v.L, v.B)) hipLaunchKernelGGL(k_adj_up_top<OneSys>, dim3(1), dim3(1024), 0, s, one);

@@ -89,6 +89,10 @@ struct SysView {
   }
 };
 
+// P2M: a leaf's sorted range is cut at the multiples of kP2MSeg (its
+// segments); segment sums, then the leaf's sum of them, both in order.
+constexpr int kP2MSeg = 16;
+
 // Warm-started sort.
 constexpr uint32_t kMoverBit = 0x80000000u;  // tkey flag: the particle changed leaf
 constexpr int kWarmTouched = 1, kWarmInts = 4;

@@ -530,23 +530,91 @@ __device__ __forceinline__ uint32_t orig_of(const float4& p) { return __float_as
 // original-index order) and its incoming movers (any number, grouped at
 // wmgrp[winoff]) are merged by original index; indices are unique, so
 // counting the smaller ones gives each particle's exact slot.  Up to 64
-// movers (one per lane) are ranked in one pass over the stayers; more are
-// taken 64 at a time.  Clears the leaf's touch flag and group count.
+// movers (one per lane) are ranked in one pass over the stayers.  Up to
+// kMergeLdsMovers movers are sorted by original index in the wave's LDS
+// (bitonic); each stayer then finds the movers below it by binary search
+// (p = their count) and counts itself in H[p], so the stayers below the
+// mover of rank r are H[0] + ... + H[r].  More movers are taken 64 at a
+// time against all the others.  Clears the leaf's touch flag and group
+// count.
+constexpr int kMergeLdsMovers = 512;
+
 template <class Sys>
 __global__ __launch_bounds__(256) void k_warm_merge(Sys sys) {
+  __shared__ uint2 s_mv[256 / kWave][kMergeLdsMovers];    // (original index, tpart slot), sorted
+  __shared__ uint32_t s_h[256 / kWave][kMergeLdsMovers + 1];
   int blk, nblk;
   const SysView v = sys.get(blk, nblk);
   const int lane = lane_id();
+  const int wave = (int)(threadIdx.x / kWave);
   const int waves = nblk * (256 / kWave);
   const int nt = v.wstate[kWarmTouched];
   const uint64_t below = lanes_below();
-  for (int d = blk * (256 / kWave) + (int)(threadIdx.x / kWave); d < nt; d += waves) {
+  for (int d = blk * (256 / kWave) + wave; d < nt; d += waves) {
     const uint32_t c = (uint32_t)v.wtlist[d];
     const int2 od = v.wtold[c];
     const int h = v.head[c];
     const int ni = v.wfill[c];
     const int* grp = v.wmgrp + v.winoff[c];
-    if (ni <= kWave) {
+    if (ni > kWave && ni <= kMergeLdsMovers) {
+      uint2* mv = s_mv[wave];
+      uint32_t* H = s_h[wave];
+      int P = kWave;
+      while (P < ni) P <<= 1;
+      for (int k = lane; k < P; k += kWave) {
+        const int ms = k < ni ? grp[k] : 0;
+        mv[k] = make_uint2(k < ni ? orig_of(v.tpart[ms]) : 0xffffffffu, (uint32_t)ms);
+      }
+      for (int k = lane; k <= ni; k += kWave) H[k] = 0u;
+      wave_lds_order();
+      // bitonic sort of P entries by original index (padding sorts last)
+      for (int kk = 2; kk <= P; kk <<= 1) {
+        for (int j = kk >> 1; j > 0; j >>= 1) {
+          for (int t = lane; t < P / 2; t += kWave) {
+            const int i = ((t & ~(j - 1)) << 1) | (t & (j - 1));
+            const int l = i | j;
+            const uint2 a = mv[i], b = mv[l];
+            if ((a.x > b.x) == ((i & kk) == 0)) {
+              mv[i] = b;
+              mv[l] = a;
+            }
+          }
+          wave_lds_order();
+        }
+      }
+      // stayers: rank among the stayers plus the movers below
+      int before = 0;
+      for (int b = 0; b < od.y; b += kWave) {
+        const int slot = od.x + b + lane;
+        const bool stay = b + lane < od.y && v.tkey[slot] == c;
+        const uint32_t os = stay ? orig_of(v.tpart[slot]) : 0u;
+        int p = 0;
+        for (int step = P; step >= 1; step >>= 1)
+          if (p + step <= ni && mv[p + step - 1].x < os) p += step;
+        const uint64_t sm = __ballot(stay);
+        if (stay) {
+          atomicAdd(&H[p], 1u);
+          const int dst = h + before + __popcll(sm & below) + p;
+          v.key[dst] = c;
+          v.part[dst] = v.tpart[slot];
+        }
+        before += __popcll(sm);
+      }
+      wave_lds_order();
+      // movers in rank order: rank plus the stayers below (prefix of H)
+      uint32_t carry = 0;
+      for (int r0 = 0; r0 < ni; r0 += kWave) {
+        const int r = r0 + lane;
+        const uint32_t inc = wave_inclusive_scan(r < ni ? H[r] : 0u) + carry;
+        if (r < ni) {
+          const int dst = h + r + (int)inc;
+          v.key[dst] = c;
+          v.part[dst] = v.tpart[mv[r].y];
+        }
+        carry = __shfl(inc, kWave - 1, kWave);
+      }
+      wave_lds_order();
+    } else if (ni <= kWave) {
       const int ms = lane < ni ? grp[lane] : 0;
       const uint32_t om = lane < ni ? orig_of(v.tpart[ms]) : 0xffffffffu;
       int rank_m = 0, cnt_m = 0;

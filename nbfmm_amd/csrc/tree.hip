@@ -72,16 +72,27 @@ __device__ __forceinline__ float4 m2m_combine(float4 c00, float4 c10, float4 c01
   return out;
 }
 
-// Leaf multipole from its sorted particle range (p2m.cu:24-74, intended).
+// Leaf multipole from its sorted particle range (p2m.cu:24-74, intended):
+// the range is cut at the multiples of kP2MSeg, each segment summed in order
+// from 0, the segment sums added in order from 0 (k_p2m_occ's order; a leaf
+// within one segment is summed plainly).
 __device__ __forceinline__ float4 p2m_leaf(const SysView& v, int c) {
 #pragma clang fp contract(off)
   const int a = v.head[c], b = v.head[c + 1];
   float sx = 0.0f, sy = 0.0f, sw = 0.0f;
-  for (int s = a; s < b; ++s) {
-    const float4 p = v.part[s];
-    sx += p.x * p.z;
-    sy += p.y * p.z;
-    sw += p.z;
+  for (int s = a; s < b;) {
+    const int e = min(b, (s / kP2MSeg + 1) * kP2MSeg);
+    float px = 0.0f, py = 0.0f, pw = 0.0f;
+    for (int i = s; i < e; ++i) {
+      const float4 p = v.part[i];
+      px += p.x * p.z;
+      py += p.y * p.z;
+      pw += p.z;
+    }
+    sx += px;
+    sy += py;
+    sw += pw;
+    s = e;
   }
   float4 out = make_float4(0.0f, 0.0f, sw, 0.0f);
   if (b != a) {
@@ -93,48 +104,108 @@ __device__ __forceinline__ float4 p2m_leaf(const SysView& v, int c) {
 
 constexpr int kUpTile = 16;  // leaves per side of one M2M tile block
 
-// P2M of the occupied leaves only (the fused solve's predo already cleared the
-// empty ones).  One thread per occupied leaf, particles summed in sorted
-// order exactly as p2m_leaf.  The kernel's time is the add chain of its
-// heaviest leaf, so each thread keeps the next kP2MDepth particles in flight
-// while it adds the current ones: one load latency per kP2MDepth particles
-// instead of one per particle group.
-constexpr int kP2MDepth = 16;
+// P2M of the occupied leaves (the sort already cleared the empty ones), in
+// p2m_leaf's order.  One lane per leaf walks its segments in order, one
+// segment per step with the next segment's loads already in flight (indices
+// clamped inside the segment, only the adds predicated).  A leaf of more
+// than kP2MCoop particles (a dense cluster, whose serial walk would be the
+// kernel's time) is listed in LDS instead, and the block's waves take the
+// listed leaves one each: lane t sums segment t and the wave adds the
+// segment sums in order.  Which wave takes a leaf does not change its bits.
+constexpr int kP2MCoop = 256;
+constexpr int kP2MThreads = 256;  // the adjoint's leaf kernel shares this grid (3 blocks per CU at 140 VGPRs)
 
-template <class Sys>
-__global__ __launch_bounds__(256) void k_p2m_occ(Sys sys) {
+__device__ __forceinline__ void p2m_segment(const SysView& v, int s, int e, float& px, float& py, float& pw) {
 #pragma clang fp contract(off)
-  int blk, nblk;
-  const SysView v = sys.get(blk, nblk);
-  const int j = blk * blockDim.x + threadIdx.x;
-  if (j >= *v.nocc) return;
-  const int c = v.occ[j];
-  const int a = v.head[c], b = v.head[c + 1];
-  // loads are unconditional (indices clamped to the leaf's last particle, an
-  // occupied leaf has b > a) so that none waits behind a branch; only the
-  // adds are predicated
-  const int last = b - 1;
-  float4 next[kP2MDepth];
+  float4 p[kP2MSeg];
 #pragma unroll
-  for (int k = 0; k < kP2MDepth; ++k) next[k] = v.part[min(a + k, last)];
-  float sx = 0.0f, sy = 0.0f, sw = 0.0f;
-  for (int s = a; s < b; s += kP2MDepth) {
-    float4 cur[kP2MDepth];
+  for (int k = 0; k < kP2MSeg; ++k) p[k] = v.part[min(s + k, e - 1)];
+  px = 0.0f;
+  py = 0.0f;
+  pw = 0.0f;
 #pragma unroll
-    for (int k = 0; k < kP2MDepth; ++k) cur[k] = next[k];
-#pragma unroll
-    for (int k = 0; k < kP2MDepth; ++k) next[k] = v.part[min(s + kP2MDepth + k, last)];
-    const int m = b - s;
-#pragma unroll
-    for (int k = 0; k < kP2MDepth; ++k) {
-      if (k < m) {
-        sx += cur[k].x * cur[k].z;
-        sy += cur[k].y * cur[k].z;
-        sw += cur[k].z;
-      }
+  for (int k = 0; k < kP2MSeg; ++k) {
+    if (s + k < e) {
+      px += p[k].x * p[k].z;
+      py += p[k].y * p[k].z;
+      pw += p[k].z;
     }
   }
-  v.mpole[c] = make_float4(sx / sw, sy / sw, sw, 0.0f);
+}
+
+template <class Sys>
+__global__ __launch_bounds__(kP2MThreads) void k_p2m_occ(Sys sys) {
+#pragma clang fp contract(off)
+  __shared__ int s_coop[kP2MThreads];
+  __shared__ int s_ncoop;
+  int blk, nblk;
+  const SysView v = sys.get(blk, nblk);
+  const int lane = lane_id();
+  if (threadIdx.x == 0) s_ncoop = 0;
+  __syncthreads();
+  const int j = blk * kP2MThreads + threadIdx.x;
+  const bool valid = j < *v.nocc;
+  const int c = valid ? v.occ[j] : 0;
+  const int a = valid ? v.head[c] : 0, b = valid ? v.head[c + 1] : 0;
+  const bool coop = valid && b - a > kP2MCoop;
+  // the segments of [a, b): [a, first), then kP2MSeg at a time
+  const int first = (a / kP2MSeg + 1) * kP2MSeg;
+  if (valid && !coop) {
+    float sx = 0.0f, sy = 0.0f, sw = 0.0f;
+    int s = a, e = min(b, first);
+    float4 next[kP2MSeg];
+#pragma unroll
+    for (int k = 0; k < kP2MSeg; ++k) next[k] = v.part[min(s + k, e - 1)];
+    while (s < b) {
+      float4 cur[kP2MSeg];
+#pragma unroll
+      for (int k = 0; k < kP2MSeg; ++k) cur[k] = next[k];
+      const int s2 = e, e2 = min(b, e + kP2MSeg);
+#pragma unroll
+      for (int k = 0; k < kP2MSeg; ++k) next[k] = v.part[min(s2 + k, e2 - 1)];  // past the leaf: b - 1
+      float px = 0.0f, py = 0.0f, pw = 0.0f;
+#pragma unroll
+      for (int k = 0; k < kP2MSeg; ++k) {
+        if (s + k < e) {
+          px += cur[k].x * cur[k].z;
+          py += cur[k].y * cur[k].z;
+          pw += cur[k].z;
+        }
+      }
+      sx += px;
+      sy += py;
+      sw += pw;
+      s = s2;
+      e = e2;
+    }
+    v.mpole[c] = make_float4(sx / sw, sy / sw, sw, 0.0f);
+  }
+  if (coop) s_coop[atomicAdd(&s_ncoop, 1)] = c;
+  __syncthreads();
+  // the block's large leaves, one per wave at a time, every lane taking a
+  // segment
+  const int ncoop = s_ncoop;
+  for (int q = (int)(threadIdx.x / kWave); q < ncoop; q += kP2MThreads / kWave) {
+    const int hc = __builtin_amdgcn_readfirstlane(s_coop[q]);
+    const int ha = __builtin_amdgcn_readfirstlane(v.head[hc]), hb = __builtin_amdgcn_readfirstlane(v.head[hc + 1]);
+    const int hf = (ha / kP2MSeg + 1) * kP2MSeg;
+    const int nseg = 1 + (hb - hf + kP2MSeg - 1) / kP2MSeg;  // hb > hf: more than kP2MSeg particles
+    float sx = 0.0f, sy = 0.0f, sw = 0.0f;
+    for (int t0 = 0; t0 < nseg; t0 += kWave) {
+      const int t = t0 + lane;
+      const int s = t == 0 ? ha : hf + (t - 1) * kP2MSeg;
+      const int e = t == 0 ? hf : min(hb, s + kP2MSeg);
+      float px = 0.0f, py = 0.0f, pw = 0.0f;
+      if (t < nseg) p2m_segment(v, s, e, px, py, pw);
+      const int cnt = min(kWave, nseg - t0);
+      for (int u = 0; u < cnt; ++u) {
+        sx += __shfl(px, u, kWave);
+        sy += __shfl(py, u, kWave);
+        sw += __shfl(pw, u, kWave);
+      }
+    }
+    if (lane == 0) v.mpole[hc] = make_float4(sx / sw, sy / sw, sw, 0.0f);
+  }
 }
 
 // M2M levels 1..min(L-1, 4) for one kUpTile x kUpTile tile of leaves, built
@@ -309,10 +380,12 @@ int upward_top_blocks(int L, int B) {
 
 int m2l_blocks(const SysView& v) { return v.L > 0 ? cdiv(v.lvl_off(v.L), 256) : 0; }
 
+int p2m_occ_blocks(int max_occ) { return cdiv(max_occ, kP2MThreads); }
+
 hipError_t launch_upward(const SysView& v, int max_occ, hipStream_t s) {
   if (v.L <= 0) return hipSuccess;
   const OneSys one{v};
-  hipLaunchKernelGGL(k_p2m_occ<OneSys>, dim3(cdiv(max_occ, 256)), dim3(256), 0, s, one);
+  hipLaunchKernelGGL(k_p2m_occ<OneSys>, dim3(p2m_occ_blocks(max_occ)), dim3(kP2MThreads), 0, s, one);
   const int tb = upward_tile_blocks(v.L, v.B);
   if (tb) hipLaunchKernelGGL(k_m2m_tile<OneSys>, dim3(tb), dim3(kUpTile * kUpTile), 0, s, one);
   if (upward_top_blocks(v.L, v.B)) hipLaunchKernelGGL(k_m2m_top<OneSys>, dim3(1), dim3(1024), 0, s, one);
@@ -320,7 +393,7 @@ hipError_t launch_upward(const SysView& v, int max_occ, hipStream_t s) {
 }
 
 hipError_t launch_tree_batch(const BatchLaunch& b, hipStream_t s) {
-  if (b.p2m.blocks) hipLaunchKernelGGL(k_p2m_occ<ManySys>, dim3(b.p2m.blocks), dim3(256), 0, s, b.p2m.sys);
+  if (b.p2m.blocks) hipLaunchKernelGGL(k_p2m_occ<ManySys>, dim3(b.p2m.blocks), dim3(kP2MThreads), 0, s, b.p2m.sys);
   if (b.m2m_tile.blocks)
     hipLaunchKernelGGL(k_m2m_tile<ManySys>, dim3(b.m2m_tile.blocks), dim3(kUpTile * kUpTile), 0, s, b.m2m_tile.sys);
   if (b.m2m_top.blocks)

@@ -157,8 +157,14 @@ void p2p(int L, int n, const T* pos, const T* w, const int* index, const int* he
 // source/nbfmm/core/solver/p2m.cu:24-110, intended semantics (every leaf is
 // zeroed, then occupied leaves get centre of mass = sum(m*x)/sum(m) and mass
 // sum(m); the reference's zero/scatter race p2m.cu:62-73 is not reproduced).
-// Sums run in sorted order.
+// The reference's summation order is Thrust's reduce_by_key, which nothing
+// pins; the order here is the GPU's: a leaf's sorted range is cut at the
+// multiples of kP2MSeg (its segments), each segment is summed in sorted
+// order from 0, and the segment sums are added in order from 0.  A leaf that
+// lies within one segment is summed plainly in sorted order.
 // Pinned by: check/core/solver/p2m.cu:36-46 restated in numpy (fp64).
+constexpr int kP2MSeg = 16;
+
 template <class T>
 void p2m(int L, int n, const T* pos, const T* w, const int* head, T* cell_pos, T* cell_w) {
   (void)n;
@@ -166,10 +172,18 @@ void p2m(int L, int n, const T* pos, const T* w, const int* head, T* cell_pos, T
   const int B = 1 << (L + 1);
   for (int c = 0; c < B * B; ++c) {
     T sx = T(0), sy = T(0), sw = T(0);
-    for (int s = head[c]; s < head[c + 1]; ++s) {
-      sx += pos[2 * s] * w[s];
-      sy += pos[2 * s + 1] * w[s];
-      sw += w[s];
+    for (int s = head[c]; s < head[c + 1];) {
+      const int e = std::min(head[c + 1], (s / kP2MSeg + 1) * kP2MSeg);
+      T px = T(0), py = T(0), pw = T(0);
+      for (int i = s; i < e; ++i) {
+        px += pos[2 * i] * w[i];
+        py += pos[2 * i + 1] * w[i];
+        pw += w[i];
+      }
+      sx += px;
+      sy += py;
+      sw += pw;
+      s = e;
     }
     if (head[c] != head[c + 1]) {
       cell_pos[2 * c] = sx / sw;

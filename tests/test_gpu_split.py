@@ -231,3 +231,33 @@ def test_clustered_adjoint_identities_and_batch():
     assert not b.flags().any()
     b.close()
     s.close()
+
+
+def test_clustered_p2m_bit_exact():
+    """Leaves of hundreds to thousands of stars: the fused solve's P2M (a
+    whole wave per leaf above 256 particles, segment sums added in order) and
+    the stage P2M equal the oracle's segment-order P2M bit for bit, on the
+    solver's own sorted buffers."""
+    from nbfmm_amd import Solver
+    x, w, L, lim, eps = clustered(n=12000, seed=11)
+    n = x.shape[0]
+    B = 2 ** (L + 1)
+    s = Solver(L, n, lim, eps)
+    a = torch.zeros((n, 2), device="cuda")
+    s.solve(n, dev(x), dev(w), a)
+    torch.cuda.synchronize()
+    pos = s.read_buffer("position", n).cpu().numpy()
+    wt = s.read_buffer("weight", n).cpu().numpy()
+    head = s.read_buffer("head").cpu().numpy()
+    counts = np.diff(head)
+    assert counts.max() > 1000 and ((counts > 16) & (counts <= 256)).any()
+    rp, rw = orc.p2m(L, pos, wt, head)
+    cp = s.read_buffer("cell_position").cpu().numpy()[:B * B]
+    cw = s.read_buffer("cell_weight").cpu().numpy()[:B * B]
+    np.testing.assert_array_equal(cw, rw[:B * B])
+    np.testing.assert_array_equal(cp, rp[:B * B])
+    s.p2m(n)  # the stage kernel, one thread per leaf
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(s.read_buffer("cell_weight").cpu().numpy()[:B * B], rw[:B * B])
+    np.testing.assert_array_equal(s.read_buffer("cell_position").cpu().numpy()[:B * B], rp[:B * B])
+    s.close()

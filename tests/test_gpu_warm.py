@@ -73,11 +73,12 @@ def test_graph_run_bitwise_equal_cold():
     cold.close()
 
 
-@pytest.mark.parametrize("jump", ["drift", "shuffle", "crowd", "weights"])
+@pytest.mark.parametrize("jump", ["drift", "shuffle", "cluster", "crowd", "weights"])
 def test_solve_sequences(jump):
     """solve at x1, then at x2: small drift (warm placement), unrelated
-    positions (most particles move), half the particles into one leaf (the
-    merge of a leaf with thousands of movers), new masses only (no movers)."""
+    positions (most particles move), 8 groups of 500 particles into 8 leaves
+    (leaves with 65..1024 movers: sorted in LDS), half the particles into one
+    leaf (thousands of movers), new masses only (no movers)."""
     from nbfmm_amd.workloads import disk
     wl = disk(n=20000)
     warm, cold = pair(wl)
@@ -88,6 +89,11 @@ def test_solve_sequences(jump):
         m2 = m1
     elif jump == "shuffle":
         x2 = x1[rng.permutation(wl.n)]
+        m2 = m1
+    elif jump == "cluster":
+        x2 = x1.copy()
+        for k in range(8):
+            x2[500 * k: 500 * (k + 1)] = x1[1000 + 1234 * k] + 1e-6 * rng.random((500, 2)).astype(np.float32)
         m2 = m1
     elif jump == "crowd":
         x2 = x1.copy()
