@@ -245,13 +245,15 @@ __device__ __forceinline__ uint32_t bin_key(const SysView& v, float2 p, bool& ba
 // leaf (a mover) the per-leaf in/out counts (no value returned, so the wave
 // does not wait for them; the next pass marks the leaves they touch).
 // Returns whether p was outside the box.
-__device__ __forceinline__ bool warm_first_pass(const SysView& v, int i, uint32_t kold, float2 p, float m, float w) {
+// hold = head[kold], the old leaf's first slot.
+__device__ __forceinline__ bool warm_first_pass(const SysView& v, int i, uint32_t kold, int hold, float2 p, float m,
+                                                float w) {
   bool bad;
   const uint32_t k = bin_key(v, p, bad);
   v.tpart[i] = make_float4(p.x, p.y, m, w);
   if (k == kold) {
     v.tkey[i] = k;
-    v.wrel[i] = i - v.head[kold];
+    v.wrel[i] = i - hold;
   } else {
     v.tkey[i] = k | kMoverBit;
     atomicAdd(&v.wcout[kold], 1);

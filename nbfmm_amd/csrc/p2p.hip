@@ -124,7 +124,7 @@ __device__ __forceinline__ void p2p_finish(const SysView& v, const CallIO& io, c
     nx.y = 2.0f * x.y - xp.y + ey * v.tick * v.tick;
     xcur[o] = nx;
     xpre[o] = x;
-    if (MODE == kP2PStepSort && warm_first_pass(v, g.i, (uint32_t)(g.y * v.B + g.xt), nx, g.pt.z, g.pt.w))
+    if (MODE == kP2PStepSort && warm_first_pass(v, g.i, (uint32_t)(g.y * v.B + g.xt), g.hk, nx, g.pt.z, g.pt.w))
       atomicOr(v.flags, 1u);
   }
 }
@@ -158,7 +158,7 @@ __global__ __launch_bounds__(WAVES * kWave) void k_p2p(Sys sys, CallIO io) {
   // reader, has finished)
   if (MODE == kP2PStepSort && blk == 0 && threadIdx.x == 0) v.wstate[kWarmTouched] = 0;
   __syncthreads();
-  run_queue<2>(
+  run_queue<2, MODE == kP2PStepSort>(
       v, s_runs, slot, blk, nblk, s_runs[v.B], lane,
       [&](const RunGeom& g, int sa, int sb, float* acc) {
         seg_sum<STAGE, U>(v, src, g, lane, sa, sb, eps, acc[0], acc[1]);

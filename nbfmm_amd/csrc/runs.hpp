@@ -49,8 +49,10 @@ struct RunGeom {
   int us[3], ulen[3];  // wave-uniform
   int a[3], b[3];      // this lane's range in each row's union (relative to us)
   int total;           // sum of ulen: the union's size
+  int hk;              // head of the target's own leaf (loaded only when asked for)
 };
 
+template <bool HK>
 __device__ __forceinline__ void run_geom(const SysView& v, const int* s_runs, int j, int lane, RunGeom& g) {
   const int B = v.B;
   // everything but the lane's own target is wave-uniform: scalar registers
@@ -61,6 +63,8 @@ __device__ __forceinline__ void run_geom(const SysView& v, const int* s_runs, in
   g.i = T0 + (g.valid ? lane : 0);
   g.pt = v.part[g.i];
   g.xt = (int)(v.key[g.i] & (uint32_t)(B - 1));
+  // in flight during the run's sums, for the epilogue
+  g.hk = HK ? v.head[g.y * B + g.xt] : 0;
   const int xf = __builtin_amdgcn_readfirstlane(__shfl(g.xt, 0, kWave));
   const int xl = __builtin_amdgcn_readfirstlane(__shfl(g.xt, nt - 1, kWave));
   g.total = 0;
@@ -111,9 +115,9 @@ __device__ __forceinline__ void run_slot_init(RunSlot<NACC>& sl) {
 
 // The block's waves work through its runs.  seg(g, sa, sb, acc) sums this
 // lane's sources among union positions [sa, sb) of run g into acc[NACC]
-// (from zero); fin(g, acc) is the epilogue of run g.  The slot must have been
+// (from zero); fin(g, acc) is the epilogue of run g (HK: g.hk is loaded).  The slot must have been
 // initialised and the block synchronised.
-template <int NACC, class Seg, class Fin>
+template <int NACC, bool HK, class Seg, class Fin>
 __device__ __forceinline__ void run_queue(const SysView& v, const int* s_runs, RunSlot<NACC>& sl, int blk, int nblk,
                                           int total, int lane, Seg&& seg, Fin&& fin) {
   RunGeom g;
@@ -161,7 +165,7 @@ __device__ __forceinline__ void run_queue(const SysView& v, const int* s_runs, R
         j = __builtin_amdgcn_readfirstlane(sl.run);
         K = Kt;
         k0 = k;
-        run_geom(v, s_runs, j, lane, g);
+        run_geom<HK>(v, s_runs, j, lane, g);
       }
     }
     if (!ticket) {
@@ -180,7 +184,7 @@ __device__ __forceinline__ void run_queue(const SysView& v, const int* s_runs, R
         K = pend_k;
         pend = -1;
         if (lane == 0) atomicOr(v.flags, kFlagP2PWait);
-        run_geom(v, s_runs, j, lane, g);
+        run_geom<HK>(v, s_runs, j, lane, g);
       } else {
         // 3. a new run, counted active from its take until it is known light,
         // published or summed alone
@@ -205,7 +209,7 @@ __device__ __forceinline__ void run_queue(const SysView& v, const int* s_runs, R
           exhausted = true;
           continue;
         }
-        run_geom(v, s_runs, j, lane, g);
+        run_geom<HK>(v, s_runs, j, lane, g);
         K = p2p_segments(g.total);
         // a light run can publish nothing: it stops counting at once, so
         // that waves without work need not stay for its end

@@ -370,7 +370,8 @@ __global__ __launch_bounds__(256) void k_warm_gather(Sys sys, CallIO io) {
   if (i < v.n) {
     const float4 old = v.part[i];
     const uint32_t o = __float_as_uint(old.w);
-    bad = warm_first_pass(v, i, v.key[i], io.pos0[v.part_off + o], io.w0[v.part_off + o], old.w);
+    const uint32_t kold = v.key[i];
+    bad = warm_first_pass(v, i, kold, v.head[kold], io.pos0[v.part_off + o], io.w0[v.part_off + o], old.w);
   }
   // the touched-leaf count k_warm_finish fills (the previous step's merge,
   // its last reader, has finished)
