@@ -583,12 +583,24 @@ __global__ __launch_bounds__(256) void k_warm_merge(Sys sys) {
           wave_lds_order();
         }
       }
-      // stayers: rank among the stayers plus the movers below
+      // stayers: rank among the stayers plus the movers below; the next
+      // chunk's loads are in flight while this one is placed (indices
+      // clamped to the old range's last slot, od.y > 0 inside the loop)
       int before = 0;
+      const int olast = od.x + od.y - 1;
+      uint32_t tk = 0u;
+      float4 tp = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      if (od.y > 0) {
+        tk = v.tkey[min(od.x + lane, olast)];
+        tp = v.tpart[min(od.x + lane, olast)];
+      }
       for (int b = 0; b < od.y; b += kWave) {
-        const int slot = od.x + b + lane;
-        const bool stay = b + lane < od.y && v.tkey[slot] == c;
-        const uint32_t os = stay ? orig_of(v.tpart[slot]) : 0u;
+        const uint32_t tkc = tk;
+        const float4 tpc = tp;
+        tk = v.tkey[min(od.x + b + kWave + lane, olast)];
+        tp = v.tpart[min(od.x + b + kWave + lane, olast)];
+        const bool stay = b + lane < od.y && tkc == c;
+        const uint32_t os = stay ? orig_of(tpc) : 0u;
         int p = 0;
         for (int step = P; step >= 1; step >>= 1)
           if (p + step <= ni && mv[p + step - 1].x < os) p += step;
@@ -597,7 +609,7 @@ __global__ __launch_bounds__(256) void k_warm_merge(Sys sys) {
           atomicAdd(&H[p], 1u);
           const int dst = h + before + __popcll(sm & below) + p;
           v.key[dst] = c;
-          v.part[dst] = v.tpart[slot];
+          v.part[dst] = tpc;
         }
         before += __popcll(sm);
       }

@@ -143,7 +143,7 @@ __global__ __launch_bounds__(kP2MThreads) void k_p2m_occ(Sys sys) {
   const int lane = lane_id();
   if (threadIdx.x == 0) s_ncoop = 0;
   __syncthreads();
-  const int j = blk * kP2MThreads + threadIdx.x;
+  const int j = blk * kP2MThreads + (int)threadIdx.x;
   const bool valid = j < *v.nocc;
   const int c = valid ? v.occ[j] : 0;
   const int a = valid ? v.head[c] : 0, b = valid ? v.head[c + 1] : 0;
