@@ -1197,6 +1197,7 @@ int nbfmm_batch_create(int S, const int* num_particle, const int* num_level, con
     // one wave per 64 particles (touched leaves are about twice the movers;
     // the waves grid-stride over the list)
     wmerge[s] = cdiv(std::min(C, v.n / 64 + 64), 256 / kWave);
+    b->bl.max_B = std::max(b->bl.max_B, v.B);
   }
   if ((rc = make_map(b, tiles, &b->bl.tiles)) || (rc = make_map(b, rows, &b->bl.rows)) ||
       (rc = make_map(b, p2m, &b->bl.p2m)) || (rc = make_map(b, mtile, &b->bl.m2m_tile)) ||

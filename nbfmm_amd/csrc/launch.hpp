@@ -31,6 +31,7 @@ struct BatchLaunch {
   // warm-started sort: the particle kernels use adj_sort, the row kernels rows
   KernelMap warm_merge;  // one wave per touched leaf (grid-stride): its share of waves
   KernelMap one;       // one block per system
+  int max_B = 0;       // widest leaf row among the systems
 };
 
 // sort.hip

@@ -383,9 +383,9 @@ __global__ __launch_bounds__(256) void k_warm_gather(Sys sys, CallIO io) {
 // row-local exclusive offset, and the row-local offset of its movers.  A
 // leaf that movers enter or leave is marked touched, with its old slot
 // range.  Clears wcin/wcout.
-template <class Sys>
-__global__ __launch_bounds__(kRowThreads) void k_warm_rows(Sys sys) {
-  __shared__ uint32_t sh[kRowWaves];
+template <int NT, class Sys>
+__global__ __launch_bounds__(NT) void k_warm_rows(Sys sys) {
+  __shared__ uint32_t sh[NT / kWave];
   int y, nrows;
   const SysView v = sys.get(y, nrows);
   const int B = v.B;
@@ -406,8 +406,8 @@ __global__ __launch_bounds__(kRowThreads) void k_warm_rows(Sys sys) {
   }
   if (y == 0 && x == 0) *v.nocc = 0;  // the occupied-leaf counter k_warm_finish fills
   uint32_t tot, itot;
-  const uint32_t e = block_exclusive_scan<kRowThreads>(cnt, sh, &tot);
-  const uint32_t ei = block_exclusive_scan<kRowThreads>(cin, sh, &itot);
+  const uint32_t e = block_exclusive_scan<NT>(cnt, sh, &tot);
+  const uint32_t ei = block_exclusive_scan<NT>(cin, sh, &itot);
   if (x < B) {
     v.wcnew[c] = (int)e;
     v.winoff[c] = (int)ei;
@@ -422,9 +422,9 @@ __global__ __launch_bounds__(kRowThreads) void k_warm_rows(Sys sys) {
 // counts (-> rowstart, head[B*B]) and of the movers (-> wrowtot[B+1+y]),
 // when k_warm_finish does not scan them itself (batches: one scan per system
 // instead of one per row block).
-template <class Sys>
-__global__ __launch_bounds__(kRowThreads) void k_warm_rowscan(Sys sys) {
-  __shared__ uint32_t sh[kRowWaves];
+template <int NT, class Sys>
+__global__ __launch_bounds__(NT) void k_warm_rowscan(Sys sys) {
+  __shared__ uint32_t sh[NT / kWave];
   int blk, nblk;
   const SysView v = sys.get(blk, nblk);
   const int B = v.B;
@@ -432,8 +432,8 @@ __global__ __launch_bounds__(kRowThreads) void k_warm_rowscan(Sys sys) {
   const uint32_t t = x < B ? (uint32_t)v.wrowtot[x] : 0u;
   const uint32_t ti = x < B ? (uint32_t)v.wrowtot[B + 1 + x] : 0u;
   uint32_t tot, itot;
-  const uint32_t e = block_exclusive_scan<kRowThreads>(t, sh, &tot);
-  const uint32_t ei = block_exclusive_scan<kRowThreads>(ti, sh, &itot);
+  const uint32_t e = block_exclusive_scan<NT>(t, sh, &tot);
+  const uint32_t ei = block_exclusive_scan<NT>(ti, sh, &itot);
   if (x < B) {
     v.rowstart[x] = (int)e;
     v.wrowtot[B + 1 + x] = (int)ei;
@@ -449,9 +449,9 @@ __global__ __launch_bounds__(kRowThreads) void k_warm_rowscan(Sys sys) {
 // group offsets for the row; appends the row's occupied leaves to occ (as
 // the column pass does).  zero_empty: also clear
 // the level-0 multipole of the row's empty leaves.
-template <class Sys>
-__global__ __launch_bounds__(kRowThreads) void k_warm_finish(Sys sys, int zero_empty, int prescanned) {
-  __shared__ uint32_t sh[kRowWaves];
+template <int NT, class Sys>
+__global__ __launch_bounds__(NT) void k_warm_finish(Sys sys, int zero_empty, int prescanned) {
+  __shared__ uint32_t sh[NT / kWave];
   __shared__ uint32_t s_base, s_ibase, s_occ_base, s_touch_base;
   int y, nrows;
   const SysView v = sys.get(y, nrows);
@@ -467,8 +467,8 @@ __global__ __launch_bounds__(kRowThreads) void k_warm_finish(Sys sys, int zero_e
     const uint32_t t = x < B ? (uint32_t)v.wrowtot[x] : 0u;
     const uint32_t ti = x < B ? (uint32_t)v.wrowtot[B + 1 + x] : 0u;
     uint32_t tot, itot;
-    const uint32_t e = block_exclusive_scan<kRowThreads>(t, sh, &tot);
-    const uint32_t ei = block_exclusive_scan<kRowThreads>(ti, sh, &itot);
+    const uint32_t e = block_exclusive_scan<NT>(t, sh, &tot);
+    const uint32_t ei = block_exclusive_scan<NT>(ti, sh, &itot);
     if (x == y) {
       s_base = e;
       s_ibase = ei;
@@ -493,8 +493,8 @@ __global__ __launch_bounds__(kRowThreads) void k_warm_finish(Sys sys, int zero_e
   const uint32_t occ = cnt > 0 ? 1u : 0u;
   const uint32_t touched = (x < B && v.wtouch[c]) ? 1u : 0u;
   uint32_t nocc, ntouch;
-  const uint32_t eo = block_exclusive_scan<kRowThreads>(occ, sh, &nocc);
-  const uint32_t et = block_exclusive_scan<kRowThreads>(touched, sh, &ntouch);
+  const uint32_t eo = block_exclusive_scan<NT>(occ, sh, &nocc);
+  const uint32_t et = block_exclusive_scan<NT>(touched, sh, &ntouch);
   if (x == 0) {
     s_occ_base = nocc ? atomicAdd((uint32_t*)v.nocc, nocc) : 0u;
     s_touch_base = ntouch ? atomicAdd((uint32_t*)&v.wstate[kWarmTouched], ntouch) : 0u;
@@ -739,14 +739,52 @@ hipError_t launch_predo_batch(const BatchLaunch& b, const CallIO& io, hipStream_
   return hipGetLastError();
 }
 
+// The warm row kernels take one thread per leaf of a row: blocks of the
+// smallest of 64..512 threads that covers the widest row launched.
+static int row_threads(int maxB) {
+  int t = kWave;
+  while (t < maxB) t <<= 1;
+  return t;
+}
+
+template <class Sys>
+static void launch_warm_rows(int nt, dim3 g, const Sys& sys, hipStream_t s) {
+  switch (nt) {
+    case 64: hipLaunchKernelGGL((k_warm_rows<64, Sys>), g, dim3(64), 0, s, sys); break;
+    case 128: hipLaunchKernelGGL((k_warm_rows<128, Sys>), g, dim3(128), 0, s, sys); break;
+    case 256: hipLaunchKernelGGL((k_warm_rows<256, Sys>), g, dim3(256), 0, s, sys); break;
+    default: hipLaunchKernelGGL((k_warm_rows<512, Sys>), g, dim3(512), 0, s, sys); break;
+  }
+}
+
+template <class Sys>
+static void launch_warm_rowscan(int nt, dim3 g, const Sys& sys, hipStream_t s) {
+  switch (nt) {
+    case 64: hipLaunchKernelGGL((k_warm_rowscan<64, Sys>), g, dim3(64), 0, s, sys); break;
+    case 128: hipLaunchKernelGGL((k_warm_rowscan<128, Sys>), g, dim3(128), 0, s, sys); break;
+    case 256: hipLaunchKernelGGL((k_warm_rowscan<256, Sys>), g, dim3(256), 0, s, sys); break;
+    default: hipLaunchKernelGGL((k_warm_rowscan<512, Sys>), g, dim3(512), 0, s, sys); break;
+  }
+}
+
+template <class Sys>
+static void launch_warm_finish(int nt, dim3 g, const Sys& sys, int zero_empty, int prescanned, hipStream_t s) {
+  switch (nt) {
+    case 64: hipLaunchKernelGGL((k_warm_finish<64, Sys>), g, dim3(64), 0, s, sys, zero_empty, prescanned); break;
+    case 128: hipLaunchKernelGGL((k_warm_finish<128, Sys>), g, dim3(128), 0, s, sys, zero_empty, prescanned); break;
+    case 256: hipLaunchKernelGGL((k_warm_finish<256, Sys>), g, dim3(256), 0, s, sys, zero_empty, prescanned); break;
+    default: hipLaunchKernelGGL((k_warm_finish<512, Sys>), g, dim3(512), 0, s, sys, zero_empty, prescanned); break;
+  }
+}
+
 // Warm-started predo (key/part/head hold an earlier sort of the same n).
 hipError_t launch_predo_warm(const SysView& v, const CallIO& io, int zero_empty, hipStream_t s, bool gathered) {
   if (v.n <= 0) return launch_predo(v, io, zero_empty, s);
   const OneSys one{v};
   const int nb = cdiv(v.n, 256);
   if (!gathered) hipLaunchKernelGGL(k_warm_gather<OneSys>, dim3(nb), dim3(256), 0, s, one, io);
-  hipLaunchKernelGGL(k_warm_rows<OneSys>, dim3(v.B), dim3(kRowThreads), 0, s, one);
-  hipLaunchKernelGGL(k_warm_finish<OneSys>, dim3(v.B), dim3(kRowThreads), 0, s, one, zero_empty, 0);
+  launch_warm_rows(row_threads(v.B), dim3(v.B), one, s);
+  launch_warm_finish(row_threads(v.B), dim3(v.B), one, zero_empty, 0, s);
   hipLaunchKernelGGL(k_warm_place<OneSys>, dim3(nb), dim3(256), 0, s, one);
   hipLaunchKernelGGL(k_warm_merge<OneSys>, dim3(cdiv(std::min(kWarmMergeWaves, v.B * v.B), 256 / kWave)),
                      dim3(256), 0, s, one);
@@ -756,9 +794,10 @@ hipError_t launch_predo_warm(const SysView& v, const CallIO& io, int zero_empty,
 hipError_t launch_predo_warm_batch(const BatchLaunch& b, const CallIO& io, hipStream_t s, bool gathered) {
   if (!gathered)
     hipLaunchKernelGGL(k_warm_gather<ManySys>, dim3(b.adj_sort.blocks), dim3(256), 0, s, b.adj_sort.sys, io);
-  hipLaunchKernelGGL(k_warm_rows<ManySys>, dim3(b.rows.blocks), dim3(kRowThreads), 0, s, b.rows.sys);
-  hipLaunchKernelGGL(k_warm_rowscan<ManySys>, dim3(b.one.blocks), dim3(kRowThreads), 0, s, b.one.sys);
-  hipLaunchKernelGGL(k_warm_finish<ManySys>, dim3(b.rows.blocks), dim3(kRowThreads), 0, s, b.rows.sys, 1, 1);
+  const int nt = row_threads(b.max_B);
+  launch_warm_rows(nt, dim3(b.rows.blocks), b.rows.sys, s);
+  launch_warm_rowscan(nt, dim3(b.one.blocks), b.one.sys, s);
+  launch_warm_finish(nt, dim3(b.rows.blocks), b.rows.sys, 1, 1, s);
   hipLaunchKernelGGL(k_warm_place<ManySys>, dim3(b.adj_sort.blocks), dim3(256), 0, s, b.adj_sort.sys);
   hipLaunchKernelGGL(k_warm_merge<ManySys>, dim3(b.warm_merge.blocks), dim3(256), 0, s, b.warm_merge.sys);
   return hipGetLastError();
