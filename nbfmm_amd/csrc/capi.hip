@@ -61,6 +61,17 @@ struct nbfmm_solver {
   int p2p_launches = 0;
   unsigned long long* d_pairs = nullptr;
   int* d_kept = nullptr;  // star count after a prune
+  // sorted states a trajectory backward keeps for the steps it recomputes
+  // (one per recomputed step of a segment; made by the first backward that
+  // needs them, held until the solver is destroyed)
+  struct SortSlot {
+    float4* part;
+    uint32_t* key;
+    int *head, *occ, *nocc;
+    float4* mpole;
+  };
+  std::vector<SortSlot> slots;
+  int ncells_tree = 0;
   // warm-started sort: key/part/head hold a stable sort of warm_n particles
   // (-1: none) that the next sort of the same n starts from
   bool warm_enabled = true;
@@ -268,6 +279,7 @@ int nbfmm_solver_create(int num_level, int max_num_particle, const float positio
     if (l < levels) off += (s->B >> l) * (s->B >> l);
   }
   const int ncells_tree = off;
+  s->ncells_tree = ncells_tree;
   const int ntiles = cdiv(max_num_particle, kSortTile);
   const size_t N = (size_t)max_num_particle;
   int rc = NBFMM_OK;
@@ -636,6 +648,38 @@ int nbfmm_trajectory_forward(nbfmm_solver* s, int n, float tick, int num_steps, 
   return NBFMM_OK;
 }
 
+namespace {
+
+// At least `count` sorted-state slots, each for max_n particles.
+int ensure_slots(nbfmm_solver* s, int count) {
+  const size_t N = (size_t)s->max_n, C = (size_t)s->B * s->B;
+  while ((int)s->slots.size() < count) {
+    nbfmm_solver::SortSlot sl{};
+    int rc;
+    if ((rc = dev_alloc(s, &sl.part, N)) || (rc = dev_alloc(s, &sl.key, N)) || (rc = dev_alloc(s, &sl.head, C + 1)) ||
+        (rc = dev_alloc(s, &sl.occ, C)) || (rc = dev_alloc(s, &sl.nocc, 1)) ||
+        (rc = dev_alloc(s, &sl.mpole, (size_t)s->ncells_tree)))
+      return rc;
+    s->slots.push_back(sl);
+  }
+  return NBFMM_OK;
+}
+
+// The solver's sorted state (of its last sort, of n particles) into a slot.
+int save_sort(nbfmm_solver* s, int n, const nbfmm_solver::SortSlot& sl, hipStream_t st) {
+  const SysView& v = s->v;
+  const size_t C = (size_t)s->B * s->B;
+  NB_HIP(hipMemcpyAsync(sl.part, v.part, (size_t)n * sizeof(float4), hipMemcpyDeviceToDevice, st));
+  NB_HIP(hipMemcpyAsync(sl.key, v.key, (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
+  NB_HIP(hipMemcpyAsync(sl.head, v.head, (C + 1) * sizeof(int), hipMemcpyDeviceToDevice, st));
+  NB_HIP(hipMemcpyAsync(sl.occ, v.occ, std::min(C, (size_t)n) * sizeof(int), hipMemcpyDeviceToDevice, st));
+  NB_HIP(hipMemcpyAsync(sl.nocc, v.nocc, sizeof(int), hipMemcpyDeviceToDevice, st));
+  NB_HIP(hipMemcpyAsync(sl.mpole, v.mpole, (size_t)s->ncells_tree * sizeof(float4), hipMemcpyDeviceToDevice, st));
+  return NBFMM_OK;
+}
+
+}  // namespace
+
 // Reverse pass of a checkpointed trajectory.  In: the checkpoints written by
 // nbfmm_trajectory_forward, the loss gradient at each observation step
 // (obs_grad, n_obs x n float2).  Out: gradients wrt the initial x, x_pre and
@@ -667,6 +711,7 @@ int nbfmm_trajectory_backward(nbfmm_solver* s, int n, float tick, int num_steps,
     return NBFMM_OK;
   };
   if ((rc = add_obs(num_steps))) return rc;
+  if ((rc = ensure_slots(s, K - 1))) return rc;
   const int nseg = (num_steps + K - 1) / K;
   for (int sg = nseg - 1; sg >= 0; --sg) {
     const int k0 = sg * K;
@@ -676,11 +721,33 @@ int nbfmm_trajectory_backward(nbfmm_solver* s, int n, float tick, int num_steps,
     NB_HIP(hipMemcpyAsync(xp, c + (size_t)n * 2, sb, hipMemcpyDeviceToDevice, st));
     for (int k = k0; k < k1; ++k) {
       NB_HIP(hipMemcpyAsync(seg + (size_t)(k - k0) * n * 2, xc, sb, hipMemcpyDeviceToDevice, st));
-      if (k + 1 < k1 && (rc = nbfmm_step(s, n, tick, xc, xp, w, ac, stream))) return rc;
+      if (k + 1 < k1) {
+        // the step sorts x_k and builds its tree: kept for the adjoint at x_k
+        if ((rc = nbfmm_step(s, n, tick, xc, xp, w, ac, stream))) return rc;
+        if (n > 0 && (rc = save_sort(s, n, s->slots[k - k0], st))) return rc;
+      }
     }
     for (int k = k1 - 1; k >= k0; --k) {
-      if ((rc = nbfmm_step_adjoint(s, n, tick, seg + (size_t)(k - k0) * n * 2, w, lam, pi, grad_w, stream)))
+      if (k + 1 < k1 && n > 0) {
+        // the same sort and tree as nbfmm_step_adjoint would build at x_k
+        const nbfmm_solver::SortSlot& sl = s->slots[k - k0];
+        SysView v = view(s, n);
+        v.part = sl.part;
+        v.key = sl.key;
+        v.head = sl.head;
+        v.occ = sl.occ;
+        v.nocc = sl.nocc;
+        v.mpole = sl.mpole;
+        CallIO io = call_io(seg + (size_t)(k - k0) * n * 2, w);
+        io.gin = (const float2*)lam;
+        io.gpos = (float2*)lam;
+        io.gpre = (float2*)pi;
+        io.gw = grad_w;
+        io.gscale = tick * tick;
+        NB_HIP(launch_adjoint(v, io, 1, s->adj_grid, st));
+      } else if ((rc = nbfmm_step_adjoint(s, n, tick, seg + (size_t)(k - k0) * n * 2, w, lam, pi, grad_w, stream))) {
         return rc;
+      }
       if ((rc = add_obs(k))) return rc;
     }
   }
