@@ -278,7 +278,12 @@ int nbfmm_trajectory_forward(nbfmm_solver* solver, int num_particle, float tick,
                              const float* d_weight, float* d_acceleration, float* d_checkpoints, int n_obs,
                              const int* obs_steps, float* d_obs_position, void* stream);
 /* d_obs_grad: n_obs x N float2 = dloss/d(observed positions).  Writes the
- * gradients wrt the initial positions, initial x_pre and the weights. */
+ * gradients wrt the initial positions, initial x_pre and the weights.  The
+ * step adjoint at a recomputed step uses the sort and tree that step built
+ * (the same bits as sorting again): the solver keeps checkpoint_every - 1
+ * such states, allocated by the first call that needs them (a synchronous
+ * allocation) and freed with the solver, each 20 B per particle of its
+ * capacity plus its leaf grid and tree. */
 int nbfmm_trajectory_backward(nbfmm_solver* solver, int num_particle, float tick, int num_steps,
                               int checkpoint_every, const float* d_weight, const float* d_checkpoints, int n_obs,
                               const int* obs_steps, const float* d_obs_grad, float* d_grad_position0,
