@@ -340,6 +340,15 @@ int nbfmm_batch_solve_adjoint(nbfmm_batch* batch, const float* d_position_origin
 /* nbfmm_step_adjoint of every system, each with its own tick. */
 int nbfmm_batch_step_adjoint(nbfmm_batch* batch, const float* d_position, const float* d_weight, float* d_lambda,
                              float* d_pi, float* d_grad_weight, void* stream);
+/* For a checkpointed trajectory's recompute (no reference counterpart):
+ * keep the batch's current sort and tree (of the positions of its last
+ * step or solve) in slot `slot` (0..4096; the first use of a slot allocates
+ * it, synchronously, as large as the batch's own sorted arrays and trees),
+ * and run nbfmm_batch_step_adjoint on a kept state without sorting again:
+ * the same bits as nbfmm_batch_step_adjoint at those positions. */
+int nbfmm_batch_keep_sort(nbfmm_batch* batch, int slot, void* stream);
+int nbfmm_batch_step_adjoint_kept(nbfmm_batch* batch, int slot, float* d_lambda, float* d_pi, float* d_grad_weight,
+                                  void* stream);
 
 #ifdef __cplusplus
 }

@@ -131,8 +131,12 @@ class _BatchTrajectory(torch.autograd.Function):
                 seg.append(x.clone())
                 if k + 1 < k1:
                     b.step(x, p, m, acc)
+                    b.keep_sort(k - k0)  # the sort and tree of x_k, for its adjoint
             for k in range(k1 - 1, k0 - 1, -1):
-                b.step_adjoint(seg[k - k0], m, lam, pi, gm)
+                if k + 1 < k1:
+                    b.step_adjoint_kept(k - k0, lam, pi, gm)
+                else:
+                    b.step_adjoint(seg[k - k0], m, lam, pi, gm)
                 add_obs(k)
         return None, None, None, None, lam, pi, gm
 

@@ -99,6 +99,15 @@ class Batch:
         check(lib().nbfmm_batch_step_adjoint(self._h, _ptr(position), _ptr(weight), _ptr(lam), _ptr(pi),
                                              _ptr(grad_weight), _stream(stream)))
 
+    def keep_sort(self, slot, stream=None):
+        """Keep the current sort and tree (of the last step's positions) in `slot`."""
+        check(lib().nbfmm_batch_keep_sort(self._h, int(slot), _stream(stream)))
+
+    def step_adjoint_kept(self, slot, lam, pi, grad_weight, stream=None):
+        """step_adjoint at the positions whose sort `slot` kept, without sorting again."""
+        check(lib().nbfmm_batch_step_adjoint_kept(self._h, int(slot), _ptr(lam), _ptr(pi), _ptr(grad_weight),
+                                                  _stream(stream)))
+
     def flags(self, clear=False):
         f = np.zeros(self.num_systems, np.uint32)
         check(lib().nbfmm_batch_flags(self._h, f.ctypes.data, int(bool(clear))))

@@ -893,6 +893,16 @@ int nbfmm_pair_count(nbfmm_solver* s, int n, double* pairs, int* occupied) {
 struct nbfmm_batch {
   int S = 0;
   int total_n = 0;
+  size_t n_all = 0, head_all = 0, cell_all = 0, tree_all = 0;  // sizes of the concatenated arrays
+  // sorted states kept for the step adjoint (nbfmm_batch_keep_sort), each
+  // laid out as the batch's own arrays
+  struct SortSlot {
+    float4* part;
+    uint32_t* key;
+    int *head, *occ, *nocc;
+    float4* mpole;
+  };
+  std::vector<SortSlot> slots;
   std::vector<int> part_off;      // S+1
   std::vector<SysView> hv;        // host copies of the views
   std::vector<SysDesc> hdesc;     // per-system sizes and array offsets
@@ -1151,6 +1161,10 @@ int nbfmm_batch_create(int S, const int* num_particle, const int* num_level, con
     tree_all += off;
   }
   b->part_off[S] = (int)n_all;
+  b->n_all = n_all;
+  b->head_all = head_all;
+  b->cell_all = cell_all;
+  b->tree_all = tree_all;
   uint32_t *tkey, *key, *hist, *dtotal, *flags;
   float4 *tpart, *part, *mpole;
   float2 *eff, *local, *gsort, *gup;
@@ -1399,6 +1413,56 @@ int nbfmm_batch_step_adjoint(nbfmm_batch* b, const float* x, const float* w, flo
   if (int rc = batch_balance_once(b, st)) return rc;
   NB_HIP(launch_tree_batch(b->bl, st));
   NB_HIP(launch_adjoint_batch(b->bl, io, 1, st));
+  return NBFMM_OK;
+}
+
+int nbfmm_batch_keep_sort(nbfmm_batch* b, int slot, void* stream) {
+  if (!b) return fail(NBFMM_ERR_INVALID_ARGUMENT, "null batch");
+  if (slot < 0 || slot > 4096) return fail(NBFMM_ERR_INVALID_ARGUMENT, "slot in [0, 4096]");
+  if (!b->warm_valid) return fail(NBFMM_ERR_INVALID_ARGUMENT, "no sort to keep: step or solve first");
+  while ((int)b->slots.size() <= slot) {
+    nbfmm_batch::SortSlot sl{};
+    int rc;
+    if ((rc = batch_alloc(b, &sl.part, b->n_all)) || (rc = batch_alloc(b, &sl.key, b->n_all)) ||
+        (rc = batch_alloc(b, &sl.head, b->head_all)) || (rc = batch_alloc(b, &sl.occ, b->cell_all)) ||
+        (rc = batch_alloc(b, &sl.nocc, (size_t)b->S)) || (rc = batch_alloc(b, &sl.mpole, b->tree_all)))
+      return rc;
+    b->slots.push_back(sl);
+  }
+  hipStream_t st = as_stream(stream);
+  const nbfmm_batch::SortSlot& sl = b->slots[slot];
+  const SysView& v = b->hv[0];  // system 0's view: every array's base
+  NB_HIP(hipMemcpyAsync(sl.part, v.part, b->n_all * sizeof(float4), hipMemcpyDeviceToDevice, st));
+  NB_HIP(hipMemcpyAsync(sl.key, v.key, b->n_all * sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
+  NB_HIP(hipMemcpyAsync(sl.head, v.head, b->head_all * sizeof(int), hipMemcpyDeviceToDevice, st));
+  NB_HIP(hipMemcpyAsync(sl.occ, v.occ, b->cell_all * sizeof(int), hipMemcpyDeviceToDevice, st));
+  NB_HIP(hipMemcpyAsync(sl.nocc, v.nocc, (size_t)b->S * sizeof(int), hipMemcpyDeviceToDevice, st));
+  NB_HIP(hipMemcpyAsync(sl.mpole, v.mpole, b->tree_all * sizeof(float4), hipMemcpyDeviceToDevice, st));
+  return NBFMM_OK;
+}
+
+int nbfmm_batch_step_adjoint_kept(nbfmm_batch* b, int slot, float* lam, float* pi, float* grad_w, void* stream) {
+  if (!b || !lam || !pi || !grad_w) return fail(NBFMM_ERR_INVALID_ARGUMENT, "null argument");
+  if (slot < 0 || slot >= (int)b->slots.size()) return fail(NBFMM_ERR_INVALID_ARGUMENT, "no sort kept in this slot");
+  hipStream_t st = as_stream(stream);
+  const nbfmm_batch::SortSlot& sl = b->slots[slot];
+  // the adjoint's launch maps with every system's sorted arrays in the slot
+  BatchLaunch bl = b->bl;
+  for (KernelMap* km : {&bl.adj_sort, &bl.adj_zero, &bl.p2m, &bl.m2m_tile, &bl.m2m_top, &bl.m2l, &bl.adj_p2p}) {
+    km->sys.base.part = sl.part;
+    km->sys.base.key = sl.key;
+    km->sys.base.head = sl.head;
+    km->sys.base.occ = sl.occ;
+    km->sys.base.nocc = sl.nocc;
+    km->sys.base.mpole = sl.mpole;
+  }
+  CallIO io;
+  io.gin = (const float2*)lam;
+  io.gpos = (float2*)lam;
+  io.gpre = (float2*)pi;
+  io.gw = grad_w;
+  io.gscale = -1.0f;  // each system's tick^2
+  NB_HIP(launch_adjoint_batch(bl, io, 1, st));
   return NBFMM_OK;
 }
 

@@ -344,3 +344,42 @@ def test_batch_adjoint_equals_single():
         assert torch.equal(LAM[sl], lam) and torch.equal(PI[sl], pi) and torch.equal(GW[sl], gw), k
         one.close()
     b.close()
+
+
+def test_batch_trajectory_checkpoints_and_single():
+    """fmm_batch_trajectory (its backward keeps the sort of every recomputed
+    step for that step's adjoint): gradients for K = 1 (nothing kept), 3 and
+    steps are bitwise equal, and system by system equal the single-system
+    trajectory's."""
+    from nbfmm_amd import Batch, Solver
+    from nbfmm_amd.autograd import fmm_batch_trajectory, fmm_trajectory
+    from nbfmm_amd.workloads import many_disks
+    bw = many_disks(num_systems=3, n_min=500, n_max=4000, seed=23, steps=7)
+    x, p, w = bw.concat()
+    b = Batch([s.n for s in bw.systems], [s.num_level for s in bw.systems], [s.limits for s in bw.systems],
+              [s.eps for s in bw.systems], ticks=[s.tick for s in bw.systems])
+    steps, obs = 7, [3, 7]
+    rng = np.random.default_rng(8)
+    wts = dev(rng.normal(0, 1, (len(obs), x.shape[0], 2)))
+    grads = []
+    for K in (1, 3, steps):
+        X, P, M = (dev(a).requires_grad_(True) for a in (x, p, w))
+        out = fmm_batch_trajectory(b, steps, K, obs, X, P, M)
+        (out * wts).sum().backward()
+        torch.cuda.synchronize()
+        grads.append([t.grad.clone() for t in (X, P, M)])
+    for g in grads[1:]:
+        for a, c in zip(grads[0], g):
+            assert torch.equal(a, c)
+    off = b.offsets
+    for k, sy in enumerate(bw.systems):
+        s = Solver(sy.num_level, sy.n, sy.limits, sy.eps)
+        sl = slice(off[k], off[k + 1])
+        X, P, M = (dev(a[sl]).requires_grad_(True) for a in (x, p, w))
+        out = fmm_trajectory(s, sy.tick, steps, 3, obs, X, P, M)
+        (out * wts[:, sl]).sum().backward()
+        torch.cuda.synchronize()
+        for a, t in zip(grads[0], (X, P, M)):
+            assert torch.equal(a[sl], t.grad), k
+        s.close()
+    b.close()
