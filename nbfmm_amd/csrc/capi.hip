@@ -315,7 +315,7 @@ int nbfmm_solver_create(int num_level, int max_num_particle, const float positio
   ALLOC(v.wcnew, (size_t)C);
   ALLOC(v.winoff, (size_t)C);
   ALLOC(v.wfill, (size_t)C);
-  ALLOC(v.wrowtot, 2 * ((size_t)s->B + 1));
+  ALLOC(v.wrowtot, 4 * ((size_t)s->B + 1));
 #undef ALLOC
   // P2P: enough waves to fill every CU
   s->cus = cus;
@@ -1204,7 +1204,7 @@ int nbfmm_batch_create(int S, const int* num_particle, const int* num_level, con
   BALLOC(winoff, cell_all);
   BALLOC(wfill, cell_all);
   BALLOC(wtold, cell_all);
-  BALLOC(wrowtot, 2 * row_all);
+  BALLOC(wrowtot, 4 * row_all);
 #undef BALLOC
   for (int s = 0; s < S; ++s) {
     SysView& v = b->hv[s];
@@ -1237,7 +1237,7 @@ int nbfmm_batch_create(int S, const int* num_particle, const int* num_level, con
     v.winoff = winoff + cell_off[s];
     v.wfill = wfill + cell_off[s];
     v.wtold = wtold + cell_off[s];
-    v.wrowtot = wrowtot + 2 * row_off[s];
+    v.wrowtot = wrowtot + 4 * row_off[s];
   }
   b->hdesc.resize(S);
   for (int s = 0; s < S; ++s) {

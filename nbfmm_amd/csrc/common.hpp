@@ -74,7 +74,7 @@ struct SysView {
   int* wcnew;       // C: row-local exclusive offset of the new leaf counts
   int* winoff;      // C: first slot of each leaf's movers in wmgrp
   int* wfill;       // C: movers placed into each leaf's group
-  int* wrowtot;     // 2 (B + 1): row totals of the new counts and of the movers
+  int* wrowtot;     // 4 (B + 1): row totals of the new counts, movers, occupied and touched leaves
 
   // Start of level l in mpole/local; lvl_off(L) = total.  The dense levels
   // k < max(L, 1) hold (B >> k)^2 = 4^(lgB - k) cells each, so
@@ -146,7 +146,7 @@ struct SysDesc {
   float xmin, ymin, cw, ch, eps, tick;
   int part_off;  // particle arrays
   int head_off;  // head (B*B + 1 per system)
-  int row_off;   // rowstart, dtotal (B + 1 per system); wrowtot at twice it
+  int row_off;   // rowstart, dtotal (B + 1 per system); wrowtot at four times it
   int cell_off;  // leaf arrays (B*B per system)
   int hist_off;  // row-pass tile histograms
   int tree_off;  // per-level cell arrays
@@ -192,7 +192,7 @@ struct ManySys {
     v.head += d.head_off;
     v.rowstart += d.row_off;
     v.dtotal += d.row_off;
-    v.wrowtot += 2 * d.row_off;
+    v.wrowtot += 4 * d.row_off;
     v.occ += d.cell_off;
     v.wcin += d.cell_off;
     v.wcout += d.cell_off;

@@ -154,9 +154,6 @@ __global__ __launch_bounds__(WAVES * kWave) void k_p2p(Sys sys, CallIO io) {
   const float eps = v.eps;
   row_runs<WAVES * kWave>(v, s_runs, sh);
   if (threadIdx.x == 0) run_slot_init(slot);
-  // the touched-leaf count the next sort fills (this step's merge, its last
-  // reader, has finished)
-  if (MODE == kP2PStepSort && blk == 0 && threadIdx.x == 0) v.wstate[kWarmTouched] = 0;
   __syncthreads();
   run_queue<2, MODE == kP2PStepSort>(
       v, s_runs, slot, blk, nblk, s_runs[v.B], lane,

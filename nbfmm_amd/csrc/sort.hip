@@ -357,9 +357,9 @@ __global__ __launch_bounds__(256) void k_empty(SysView v) {
 //   k_warm_merge   one wave per touched leaf merges its stayers and movers
 // Each pass clears the counters it is the last to read.  Per-particle
 // atomics are per leaf; an atomic that the whole grid shares serialises
-// across the chip, so the only shared counters are bumped once per row
-// block.  Any number of movers may enter a leaf, so the warm passes never
-// need the cold sort.
+// across the chip, so the occupied and touched leaf lists take their row
+// bases from scans of per-row counts instead of shared counters.  Any number
+// of movers may enter a leaf, so the warm passes never need the cold sort.
 
 template <class Sys>
 __global__ __launch_bounds__(256) void k_warm_gather(Sys sys, CallIO io) {
@@ -373,16 +373,14 @@ __global__ __launch_bounds__(256) void k_warm_gather(Sys sys, CallIO io) {
     const uint32_t kold = v.key[i];
     bad = warm_first_pass(v, i, kold, v.head[kold], io.pos0[v.part_off + o], io.w0[v.part_off + o], old.w);
   }
-  // the touched-leaf count k_warm_finish fills (the previous step's merge,
-  // its last reader, has finished)
-  if (blk == 0 && threadIdx.x == 0) v.wstate[kWarmTouched] = 0;
   if (__any(bad) && lane_id() == 0) atomicOr(v.flags, 1u);
 }
 
 // One block per row: new count of every leaf (old count - out + in), its
 // row-local exclusive offset, and the row-local offset of its movers.  A
 // leaf that movers enter or leave is marked touched, with its old slot
-// range.  Clears wcin/wcout.
+// range.  The row's totals (particles, movers, occupied leaves, touched
+// leaves) go to wrowtot.  Clears wcin/wcout.
 template <int NT, class Sys>
 __global__ __launch_bounds__(NT) void k_warm_rows(Sys sys) {
   __shared__ uint32_t sh[NT / kWave];
@@ -392,10 +390,12 @@ __global__ __launch_bounds__(NT) void k_warm_rows(Sys sys) {
   const int x = threadIdx.x;
   const int c = y * B + x;
   uint32_t cnt = 0, cin = 0;
+  bool touched = false;
   if (x < B) {
     const int ci = v.wcin[c], co = v.wcout[c];
     const int h0 = v.head[c], h1 = v.head[c + 1];
-    if (ci | co) {
+    touched = (ci | co) != 0;
+    if (touched) {
       v.wcin[c] = 0;
       v.wcout[c] = 0;
       v.wtouch[c] = 1;
@@ -404,7 +404,8 @@ __global__ __launch_bounds__(NT) void k_warm_rows(Sys sys) {
     cnt = (uint32_t)(h1 - h0 - co + ci);
     cin = (uint32_t)ci;
   }
-  if (y == 0 && x == 0) *v.nocc = 0;  // the occupied-leaf counter k_warm_finish fills
+  const int nocc = __syncthreads_count(cnt > 0);
+  const int ntouch = __syncthreads_count(touched);
   uint32_t tot, itot;
   const uint32_t e = block_exclusive_scan<NT>(cnt, sh, &tot);
   const uint32_t ei = block_exclusive_scan<NT>(cin, sh, &itot);
@@ -415,13 +416,16 @@ __global__ __launch_bounds__(NT) void k_warm_rows(Sys sys) {
   if (x == 0) {
     v.wrowtot[y] = (int)tot;
     v.wrowtot[B + 1 + y] = (int)itot;
+    v.wrowtot[2 * (B + 1) + y] = nocc;
+    v.wrowtot[3 * (B + 1) + y] = ntouch;
   }
 }
 
 // One block per system: exclusive scans of the row totals of the new
-// counts (-> rowstart, head[B*B]) and of the movers (-> wrowtot[B+1+y]),
-// when k_warm_finish does not scan them itself (batches: one scan per system
-// instead of one per row block).
+// counts (-> rowstart, head[B*B]), of the movers, the occupied and the
+// touched leaves (in place in wrowtot; the last two totals -> nocc and the
+// touched-leaf count), when k_warm_finish does not scan them itself
+// (batches: one scan per system instead of one per row block).
 template <int NT, class Sys>
 __global__ __launch_bounds__(NT) void k_warm_rowscan(Sys sys) {
   __shared__ uint32_t sh[NT / kWave];
@@ -431,23 +435,32 @@ __global__ __launch_bounds__(NT) void k_warm_rowscan(Sys sys) {
   const int x = threadIdx.x;
   const uint32_t t = x < B ? (uint32_t)v.wrowtot[x] : 0u;
   const uint32_t ti = x < B ? (uint32_t)v.wrowtot[B + 1 + x] : 0u;
-  uint32_t tot, itot;
+  const uint32_t to = x < B ? (uint32_t)v.wrowtot[2 * (B + 1) + x] : 0u;
+  const uint32_t tt = x < B ? (uint32_t)v.wrowtot[3 * (B + 1) + x] : 0u;
+  uint32_t tot, itot, otot, ttot;
   const uint32_t e = block_exclusive_scan<NT>(t, sh, &tot);
   const uint32_t ei = block_exclusive_scan<NT>(ti, sh, &itot);
+  const uint32_t eo = block_exclusive_scan<NT>(to, sh, &otot);
+  const uint32_t et = block_exclusive_scan<NT>(tt, sh, &ttot);
   if (x < B) {
     v.rowstart[x] = (int)e;
     v.wrowtot[B + 1 + x] = (int)ei;
+    v.wrowtot[2 * (B + 1) + x] = (int)eo;
+    v.wrowtot[3 * (B + 1) + x] = (int)et;
   }
   if (x == 0) {
     v.rowstart[B] = (int)tot;
     v.head[B * B] = (int)tot;
+    *v.nocc = (int)otot;
+    v.wstate[kWarmTouched] = (int)ttot;
   }
 }
 
-// One block per row: the row's start (scanned here from the B row totals,
-// or read from rowstart when k_warm_rowscan ran), then head and the movers'
-// group offsets for the row; appends the row's occupied leaves to occ (as
-// the column pass does).  zero_empty: also clear
+// One block per row: the row's start and its bases in the occupied and the
+// touched leaf lists (scanned here from the B row totals, or read from
+// rowstart and wrowtot when k_warm_rowscan ran), then head and the movers'
+// group offsets for the row; writes the row's occupied leaves to occ and its
+// touched leaves to wtlist, both in row-major order.  zero_empty: also clear
 // the level-0 multipole of the row's empty leaves.
 template <int NT, class Sys>
 __global__ __launch_bounds__(NT) void k_warm_finish(Sys sys, int zero_empty, int prescanned) {
@@ -461,22 +474,32 @@ __global__ __launch_bounds__(NT) void k_warm_finish(Sys sys, int zero_empty, int
     if (x == 0) {
       s_base = (uint32_t)v.rowstart[y];
       s_ibase = (uint32_t)v.wrowtot[B + 1 + y];
+      s_occ_base = (uint32_t)v.wrowtot[2 * (B + 1) + y];
+      s_touch_base = (uint32_t)v.wrowtot[3 * (B + 1) + y];
     }
   } else {
-    // row starts: exclusive scan of the row totals, up to row y
+    // exclusive scans of the row totals, up to row y
     const uint32_t t = x < B ? (uint32_t)v.wrowtot[x] : 0u;
     const uint32_t ti = x < B ? (uint32_t)v.wrowtot[B + 1 + x] : 0u;
-    uint32_t tot, itot;
+    const uint32_t to = x < B ? (uint32_t)v.wrowtot[2 * (B + 1) + x] : 0u;
+    const uint32_t tt = x < B ? (uint32_t)v.wrowtot[3 * (B + 1) + x] : 0u;
+    uint32_t tot, itot, otot, ttot;
     const uint32_t e = block_exclusive_scan<NT>(t, sh, &tot);
     const uint32_t ei = block_exclusive_scan<NT>(ti, sh, &itot);
+    const uint32_t eo = block_exclusive_scan<NT>(to, sh, &otot);
+    const uint32_t et = block_exclusive_scan<NT>(tt, sh, &ttot);
     if (x == y) {
       s_base = e;
       s_ibase = ei;
+      s_occ_base = eo;
+      s_touch_base = et;
       v.rowstart[y] = (int)e;
     }
     if (y == B - 1 && x == 0) {
       v.rowstart[B] = (int)tot;
       v.head[B * B] = (int)tot;
+      *v.nocc = (int)otot;
+      v.wstate[kWarmTouched] = (int)ttot;
     }
   }
   __syncthreads();
@@ -495,11 +518,6 @@ __global__ __launch_bounds__(NT) void k_warm_finish(Sys sys, int zero_empty, int
   uint32_t nocc, ntouch;
   const uint32_t eo = block_exclusive_scan<NT>(occ, sh, &nocc);
   const uint32_t et = block_exclusive_scan<NT>(touched, sh, &ntouch);
-  if (x == 0) {
-    s_occ_base = nocc ? atomicAdd((uint32_t*)v.nocc, nocc) : 0u;
-    s_touch_base = ntouch ? atomicAdd((uint32_t*)&v.wstate[kWarmTouched], ntouch) : 0u;
-  }
-  __syncthreads();
   if (occ) v.occ[s_occ_base + eo] = c;
   if (touched) v.wtlist[s_touch_base + et] = c;
 }
