@@ -470,6 +470,16 @@ __global__ __launch_bounds__(NT) void k_warm_finish(Sys sys, int zero_empty, int
   const SysView v = sys.get(y, nrows);
   const int B = v.B;
   const int x = threadIdx.x;
+  const int c = y * B + x;
+  // the row's own loads first, so that their latency overlaps the scans
+  int e = 0, e1 = 0, win = 0;
+  bool tch = false;
+  if (x < B) {
+    e = v.wcnew[c];
+    e1 = x + 1 < B ? v.wcnew[c + 1] : v.wrowtot[y];
+    win = v.winoff[c];
+    tch = v.wtouch[c] != 0;
+  }
   if (prescanned) {
     if (x == 0) {
       s_base = (uint32_t)v.rowstart[y];
@@ -503,18 +513,14 @@ __global__ __launch_bounds__(NT) void k_warm_finish(Sys sys, int zero_empty, int
     }
   }
   __syncthreads();
-  const int c = y * B + x;
-  uint32_t cnt = 0;
+  const uint32_t cnt = (uint32_t)(e1 - e);  // 0 past the row
   if (x < B) {
-    const int e = v.wcnew[c];
-    const int e1 = x + 1 < B ? v.wcnew[c + 1] : v.wrowtot[y];
-    cnt = (uint32_t)(e1 - e);
     v.head[c] = (int)s_base + e;
-    v.winoff[c] += (int)s_ibase;
+    v.winoff[c] = win + (int)s_ibase;
     if (zero_empty && cnt == 0) v.mpole[c] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   }
   const uint32_t occ = cnt > 0 ? 1u : 0u;
-  const uint32_t touched = (x < B && v.wtouch[c]) ? 1u : 0u;
+  const uint32_t touched = tch ? 1u : 0u;
   uint32_t nocc, ntouch;
   const uint32_t eo = block_exclusive_scan<NT>(occ, sh, &nocc);
   const uint32_t et = block_exclusive_scan<NT>(touched, sh, &ntouch);
