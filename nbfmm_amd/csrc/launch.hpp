@@ -20,7 +20,7 @@ struct KernelMap {
 struct BatchLaunch {
   KernelMap tiles;     // row pass: cdiv(n_s, kSortTile) per system
   KernelMap rows;      // row scan / column pass: B_s per system
-  KernelMap p2m;       // P2M of occupied leaves: p2m_occ_blocks(min(n_s, B_s^2))
+  KernelMap p2m;       // P2M over all leaves: p2m_leaf_blocks(B_s)
   KernelMap m2m_tile;  // M2M tiles
   KernelMap m2m_top;   // M2M top levels
   KernelMap m2l;       // M2L: cdiv(tree cells, 256)
@@ -53,6 +53,7 @@ int upward_tile_blocks(int L, int B);
 int upward_top_blocks(int L, int B);
 int m2l_blocks(const SysView& v);
 int p2m_occ_blocks(int max_occ);  // P2M grid for up to max_occ occupied leaves
+int p2m_leaf_blocks(int B);       // P2M grid for all B * B leaves
 hipError_t launch_m2m(const SysView& v, hipStream_t s);
 hipError_t launch_m2l(const SysView& v, hipStream_t s);
 hipError_t launch_l2l(const SysView& v, hipStream_t s);

@@ -105,7 +105,10 @@ __device__ __forceinline__ float4 p2m_leaf(const SysView& v, int c) {
 constexpr int kUpTile = 16;  // leaves per side of one M2M tile block
 
 // P2M of the occupied leaves (the sort already cleared the empty ones), in
-// p2m_leaf's order.  One lane per leaf walks its segments in order, one
+// p2m_leaf's order.  The grid covers every leaf and reads its range from
+// head directly (an empty leaf's lane does nothing): going through the
+// occupied-leaf list would add two dependent loads (its length, then its
+// entry) before the particle loads.  One lane per leaf walks its segments in order, one
 // segment per step with the next segment's loads already in flight (indices
 // clamped inside the segment, only the adds predicated).  A leaf of more
 // than kP2MCoop particles (a dense cluster, whose serial walk would be the
@@ -143,10 +146,14 @@ __global__ __launch_bounds__(kP2MThreads) void k_p2m_occ(Sys sys) {
   const int lane = lane_id();
   if (threadIdx.x == 0) s_ncoop = 0;
   __syncthreads();
-  const int j = blk * kP2MThreads + (int)threadIdx.x;
-  const bool valid = j < *v.nocc;
-  const int c = valid ? v.occ[j] : 0;
-  const int a = valid ? v.head[c] : 0, b = valid ? v.head[c + 1] : 0;
+  // one thread per leaf; empty leaves (cleared by the sort) are skipped
+  const int c = blk * kP2MThreads + (int)threadIdx.x;
+  int a = 0, b = 0;
+  if (c < v.B * v.B) {
+    a = v.head[c];
+    b = v.head[c + 1];
+  }
+  const bool valid = b > a;
   const bool coop = valid && b - a > kP2MCoop;
   // the segments of [a, b): [a, first), then kP2MSeg at a time
   const int first = (a / kP2MSeg + 1) * kP2MSeg;
@@ -381,11 +388,12 @@ int upward_top_blocks(int L, int B) {
 int m2l_blocks(const SysView& v) { return v.L > 0 ? cdiv(v.lvl_off(v.L), 256) : 0; }
 
 int p2m_occ_blocks(int max_occ) { return cdiv(max_occ, kP2MThreads); }
+int p2m_leaf_blocks(int B) { return cdiv(B * B, kP2MThreads); }
 
 hipError_t launch_upward(const SysView& v, int max_occ, hipStream_t s) {
   if (v.L <= 0) return hipSuccess;
   const OneSys one{v};
-  hipLaunchKernelGGL(k_p2m_occ<OneSys>, dim3(p2m_occ_blocks(max_occ)), dim3(kP2MThreads), 0, s, one);
+  hipLaunchKernelGGL(k_p2m_occ<OneSys>, dim3(p2m_leaf_blocks(v.B)), dim3(kP2MThreads), 0, s, one);
   const int tb = upward_tile_blocks(v.L, v.B);
   if (tb) hipLaunchKernelGGL(k_m2m_tile<OneSys>, dim3(tb), dim3(kUpTile * kUpTile), 0, s, one);
   if (upward_top_blocks(v.L, v.B)) hipLaunchKernelGGL(k_m2m_top<OneSys>, dim3(1), dim3(1024), 0, s, one);
