@@ -128,10 +128,11 @@ __global__ __launch_bounds__(256) void k_adj_leaf(Sys sys) {
 #pragma clang fp contract(off)
   int blk, nblk;
   const SysView v = sys.get(blk, nblk);
-  const int j = blk * blockDim.x + threadIdx.x;
-  if (j >= *v.nocc) return;
-  const int c = v.occ[j];
+  // one thread per leaf, ranges read from head (empty leaves return)
+  const int c = blk * blockDim.x + threadIdx.x;
+  if (c >= v.B * v.B) return;
   const int a = v.head[c], b = v.head[c + 1];
+  if (b <= a) return;
   const int last = b - 1;
   float2 next[kAdjLeafDepth];
 #pragma unroll
@@ -474,10 +475,9 @@ hipError_t launch_adjoint(const SysView& v, const CallIO& io, int mode, int grid
   const OneSys one{v};
   hipLaunchKernelGGL(k_adj_sort<OneSys>, dim3(cdiv(v.n, 256)), dim3(256), 0, s, one, io);
   if (v.L > 0) {
-    const int C = v.B * v.B;
     hipLaunchKernelGGL(k_adj_zero<OneSys>, dim3(adj_zero_blocks(v)), dim3(256), 0, s, one);
-    // the P2M grid (its occupied-leaf blocks of 256 threads)
-    hipLaunchKernelGGL(k_adj_leaf<OneSys>, dim3(p2m_occ_blocks(v.n < C ? v.n : C)), dim3(256), 0, s, one);
+    // the P2M grid (one thread per leaf, blocks of 256)
+    hipLaunchKernelGGL(k_adj_leaf<OneSys>, dim3(p2m_leaf_blocks(v.B)), dim3(256), 0, s, one);
     const int tb = upward_tile_blocks(v.L, v.B);
     if (tb) hipLaunchKernelGGL(k_adj_up_tile<OneSys>, dim3(tb), dim3(kAdjTile * kAdjTile), 0, s, one);
     if (upward_top_blocks(v.L, v.B)) hipLaunchKernelGGL(k_adj_up_top<OneSys>, dim3(1), dim3(1024), 0, s, one);

@@ -1269,7 +1269,7 @@ int nbfmm_batch_create(int S, const int* num_particle, const int* num_level, con
     const int C = v.B * v.B;
     tiles[s] = cdiv(v.n, kSortTile);
     rows[s] = v.B;
-    p2m[s] = v.L > 0 ? p2m_leaf_blocks(v.B) : 0;  // also the adjoint leaf kernel's grid (it guards on nocc)
+    p2m[s] = v.L > 0 ? p2m_leaf_blocks(v.B) : 0;  // also the adjoint leaf kernel's grid
     mtile[s] = upward_tile_blocks(v.L, v.B);
     mtop[s] = upward_top_blocks(v.L, v.B);
     m2l[s] = m2l_blocks(v);

@@ -575,8 +575,14 @@ __global__ __launch_bounds__(256) void k_warm_merge(Sys sys) {
   const int waves = nblk * (256 / kWave);
   const int nt = v.wstate[kWarmTouched];
   const uint64_t below = lanes_below();
-  for (int d = blk * (256 / kWave) + wave; d < nt; d += waves) {
-    const uint32_t c = (uint32_t)v.wtlist[d];
+  const int cmax = v.B * v.B - 1;  // wtlist holds at most B * B leaves
+  int d = blk * (256 / kWave) + wave;
+  // the first list entry is loaded together with the count (clamped; used
+  // only when d < nt), the next one while this leaf is merged
+  uint32_t cnext = (uint32_t)v.wtlist[min(d, cmax)];
+  for (; d < nt; d += waves) {
+    const uint32_t c = cnext;
+    cnext = (uint32_t)v.wtlist[min(d + waves, cmax)];
     const int2 od = v.wtold[c];
     const int h = v.head[c];
     const int ni = v.wfill[c];
