@@ -536,14 +536,18 @@ __global__ __launch_bounds__(256) void k_warm_place(Sys sys) {
   const SysView v = sys.get(blk, nblk);
   const int i = blk * 256 + threadIdx.x;
   if (i >= v.n) return;
+  // the particle's own entries are loaded together with its key (a stayer,
+  // nearly every particle, needs them; a mover's are simply unused)
   const uint32_t t = v.tkey[i];
+  const int rel = v.wrel[i];
+  const float4 p = v.tpart[i];
   if (t & kMoverBit) {
     const uint32_t k = t & ~kMoverBit;
     v.wmgrp[v.winoff[k] + atomicAdd(&v.wfill[k], 1)] = i;
   } else if (!v.wtouch[t]) {
-    const int dst = v.head[t] + v.wrel[i];
+    const int dst = v.head[t] + rel;
     v.key[dst] = t;
-    v.part[dst] = v.tpart[i];
+    v.part[dst] = p;
   }
 }
 
