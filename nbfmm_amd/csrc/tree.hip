@@ -290,14 +290,25 @@ __global__ __launch_bounds__(256) void k_m2l(Sys sys) {
   const int px = tx & ~1, py = ty & ~1;
   const float eps = v.eps;
   float ax = 0.0f, ay = 0.0f;
+  // window row wy + 1 is loaded while row wy is summed
+  float4 nxt[6];
+  {
+    const int yc = min(max(py - 2, 0), D - 1);
+#pragma clang loop unroll(full)
+    for (int k = 0; k < 6; ++k) nxt[k] = mp[yc * D + min(max(px - 2 + k, 0), D - 1)];
+  }
 #pragma clang loop unroll(full)
   for (int wy = -2; wy < 4; ++wy) {
     const int y = py + wy;
     const bool row_in = y >= 0 && y < D;
-    const int yc = min(max(y, 0), D - 1);
     float4 sc[6];
 #pragma clang loop unroll(full)
-    for (int k = 0; k < 6; ++k) sc[k] = mp[yc * D + min(max(px - 2 + k, 0), D - 1)];
+    for (int k = 0; k < 6; ++k) sc[k] = nxt[k];
+    if (wy + 1 < 4) {
+      const int yn = min(max(y + 1, 0), D - 1);
+#pragma clang loop unroll(full)
+      for (int k = 0; k < 6; ++k) nxt[k] = mp[yn * D + min(max(px - 2 + k, 0), D - 1)];
+    }
 #pragma clang loop unroll(full)
     for (int k = 0; k < 6; ++k) {
       const int x = px - 2 + k;
