@@ -117,7 +117,9 @@ __device__ __forceinline__ void p2p_finish(const SysView& v, const CallIO& io, c
     // update.cu:39-41
     float2* xcur = io.xcur + v.part_off;
     float2* xpre = io.xpre + v.part_off;
-    const float2 x = xcur[o];
+    // x_cur[o] is the position this step sorted (a step's forces are taken
+    // at x_cur, so the sort read it there and carried it in pt): not read again
+    const float2 x = make_float2(g.pt.x, g.pt.y);
     const float2 xp = xpre[o];
     float2 nx;
     nx.x = 2.0f * x.x - xp.x + ex * v.tick * v.tick;
