@@ -41,11 +41,14 @@ extern "C" {
 
 /* Status flags raised by the device (read with nbfmm_solver_flags). */
 #define NBFMM_FLAG_OUT_OF_BOX 1u /* a particle was outside the limits or NaN; its cell index was clamped */
-#define NBFMM_FLAG_P2P_WAIT 2u   /* a P2P wave gave up a wait on another wave of its block: either for
-                                  * the segment before its own in a split run (that run's results are
-                                  * wrong), or for the block's split-run slot (the run was then summed by
-                                  * one wave: right, but its bits may differ from the split sum's).
-                                  * Not expected to happen. */
+#define NBFMM_FLAG_P2P_WAIT 2u   /* a P2P wave gave up a wait on another wave of its block (after
+                                  * nbfmm_set_p2p_wait_polls polls): for the segments before its own in
+                                  * a split run, or for the block's split-run slot.  That run was then
+                                  * summed by one wave in one pass: its results are right, but their bits
+                                  * may differ from the split sum's.  Not expected at the default limit. */
+#define NBFMM_FLAG_P2P_LOST 4u   /* a P2P wave found the split-run protocol broken (a lock held past every
+                                  * bound) and dropped a segment: that run's results are missing.  Cannot
+                                  * happen unless the protocol has a defect; reported rather than hung on. */
 
 /* Buffer ids for nbfmm_buffer_read / nbfmm_buffer_write.  They name the
  * reference Solver members that NBFMM_CHECK makes public
@@ -148,7 +151,10 @@ int nbfmm_prune(nbfmm_solver* solver, int num_star, float* d_position_cur, float
                 float* d_weight, int* num_kept, void* stream);
 
 /* num_steps calls of nbfmm_step.  use_graph = 1 captures one step into a
- * hipGraph on the first call and replays it (stream must not be NULL). */
+ * hipGraph on the first call and replays it.  A graph cannot be captured on
+ * the NULL stream: with stream NULL the replays run on a stream the solver
+ * makes on first use, ordered after the work already queued on the NULL
+ * stream and before the work queued on it afterwards. */
 int nbfmm_run(nbfmm_solver* solver, int num_particle, float tick, int num_steps, float* d_position_cur,
               float* d_position_pre, const float* d_weight, float* d_acceleration, int use_graph, void* stream);
 
@@ -239,6 +245,12 @@ int nbfmm_p2p_variant_count(void);
 int nbfmm_p2p_variant_info(int variant, int* stage, int* unroll, int* waves_per_block);
 int nbfmm_set_p2p_config(nbfmm_solver* solver, int variant, int blocks_per_cu);
 int nbfmm_get_p2p_config(const nbfmm_solver* solver, int* variant, int* blocks_per_cu);
+/* Bound on every wait of a P2P wave for another wave of its block (each poll a
+ * short sleep; default 2^20, >= 1).  After it the wave gives up the wait and
+ * sums the run alone (NBFMM_FLAG_P2P_WAIT): results stay right, only their
+ * bits may change.  Exists so that tests can force the give-up path from the
+ * host; no reference counterpart. */
+int nbfmm_set_p2p_wait_polls(nbfmm_solver* solver, int polls);
 /* Times every configuration on this input (repeats launches each), keeps the
  * fastest in the solver and returns it.  Synchronises. */
 int nbfmm_autotune(nbfmm_solver* solver, int num_particle, const float* d_position_origin,
@@ -323,6 +335,8 @@ int nbfmm_batch_flags(nbfmm_batch* batch, unsigned* flags, int clear);
  * their P2P work (particle counts until the batch is balanced). */
 int nbfmm_batch_set_p2p_config(nbfmm_batch* batch, int variant, int blocks_per_cu);
 int nbfmm_batch_get_p2p_config(const nbfmm_batch* batch, int* variant, int* blocks_per_cu);
+/* nbfmm_set_p2p_wait_polls for every system of the batch. */
+int nbfmm_batch_set_p2p_wait_polls(nbfmm_batch* batch, int polls);
 /* Sorts the given positions and shares the P2P grids (forward and adjoint)
  * among the systems in proportion to their P2P pairs plus 32 per particle.
  * The first solve, step, run or adjoint call of a batch does this by itself

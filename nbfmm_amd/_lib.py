@@ -70,6 +70,8 @@ SIGNATURES = {
     "nbfmm_y4m_write_frame": [P, P, P],
     "nbfmm_y4m_close": [P, ctypes.POINTER(ctypes.c_long)],
     "nbfmm_get_p2p_config": [P, ctypes.POINTER(c_int), ctypes.POINTER(c_int)],
+    "nbfmm_set_p2p_wait_polls": [P, c_int],
+    "nbfmm_batch_set_p2p_wait_polls": [P, c_int],
     "nbfmm_autotune": [P, c_int, P, P, c_int, ctypes.POINTER(c_int), ctypes.POINTER(c_int),
                        ctypes.POINTER(c_float), P],
     "nbfmm_batch_solve_adjoint": [P, P, P, P, P, P, P],

@@ -64,6 +64,10 @@ class Batch:
         every launch; all configurations give identical results."""
         check(lib().nbfmm_batch_set_p2p_config(self._h, int(variant), int(blocks_per_cu)))
 
+    def set_p2p_wait_polls(self, polls):
+        """Solver.set_p2p_wait_polls for every system of the batch."""
+        check(lib().nbfmm_batch_set_p2p_wait_polls(self._h, int(polls)))
+
     def warm_start(self, enable=True):
         """Start each sort from the previous one (on by default); results are
         identical either way."""

@@ -41,6 +41,35 @@ int cdiv(int a, int b) { return (a + b - 1) / b; }
 
 int nbfmm_amd::report_error(int code, const std::string& msg) { return fail(code, msg); }
 
+// A stream of the library's own on which graph replays asked for on the
+// null stream run (a graph cannot be captured on the null stream), ordered
+// after the work queued on the null stream before the call and before the
+// work queued after it.  Made on first use.
+struct SideStream {
+  hipStream_t s = nullptr;
+  hipEvent_t in = nullptr, out = nullptr;
+  hipError_t begin() {
+    hipError_t e;
+    if (!s && (e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking)) != hipSuccess) return e;
+    if (!in && (e = hipEventCreateWithFlags(&in, hipEventDisableTiming)) != hipSuccess) return e;
+    if (!out && (e = hipEventCreateWithFlags(&out, hipEventDisableTiming)) != hipSuccess) return e;
+    if ((e = hipEventRecord(in, nullptr)) != hipSuccess) return e;
+    return hipStreamWaitEvent(s, in, 0);
+  }
+  hipError_t end() {
+    hipError_t e;
+    if ((e = hipEventRecord(out, s)) != hipSuccess) return e;
+    return hipStreamWaitEvent(nullptr, out, 0);
+  }
+  void release() {
+    if (s) hipStreamDestroy(s);
+    if (in) hipEventDestroy(in);
+    if (out) hipEventDestroy(out);
+    s = nullptr;
+    in = out = nullptr;
+  }
+};
+
 struct nbfmm_solver {
   int L = 0, B = 0, max_n = 0;
   float limits[4] = {0, 0, 0, 0};
@@ -80,6 +109,7 @@ struct nbfmm_solver {
   // makes the next sort's first pass), [1] the last step
   hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
   hipGraph_t graph[2] = {nullptr, nullptr};
+  SideStream side;  // replays asked for on the null stream run here
   struct {
     int n = -1;
     float tick = 0;
@@ -110,6 +140,7 @@ void free_all(nbfmm_solver* s) {
     if (s->graph[k]) hipGraphDestroy(s->graph[k]);
   }
   for (auto& e : s->ev) hipEventDestroy(e);
+  s->side.release();
   for (void* p : s->allocs) hipFree(p);
   s->allocs.clear();
 }
@@ -530,8 +561,17 @@ int nbfmm_run(nbfmm_solver* s, int n, float tick, int steps, float* xcur, float*
     }
     return NBFMM_OK;
   }
-  if (!st) return fail(NBFMM_ERR_INVALID_ARGUMENT, "graph replay needs a non-null stream");
   if (steps == 0) return NBFMM_OK;
+  if (!st) {
+    // graphs are captured and replayed on the library's side stream, ordered
+    // with the null stream on both sides
+    NB_HIP(s->side.begin());
+    rc = nbfmm_run(s, n, tick, steps, xcur, xpre, w, acc, use_graph, (void*)s->side.s);
+    const hipError_t e = s->side.end();
+    if (rc) return rc;
+    if (e != hipSuccess) return hip_fail(e, "side stream");
+    return NBFMM_OK;
+  }
   // The first step is eager: it sorts from the caller's arrays (warm from an
   // earlier sort of n, else cold).  The other steps replay captured steps.
   // With the warm start on, the P2P of every step but the last also makes
@@ -784,6 +824,14 @@ int nbfmm_get_p2p_config(const nbfmm_solver* s, int* variant, int* blocks_per_cu
   return NBFMM_OK;
 }
 
+int nbfmm_set_p2p_wait_polls(nbfmm_solver* s, int polls) {
+  if (!s) return fail(NBFMM_ERR_INVALID_ARGUMENT, "null solver");
+  if (polls < 1) return fail(NBFMM_ERR_INVALID_ARGUMENT, "polls must be >= 1");
+  s->v.wait_polls = polls;
+  drop_graph(s);
+  return NBFMM_OK;
+}
+
 // Times every P2P configuration (variant x blocks per CU) on this input and
 // keeps the fastest.  Synchronises.  The configurations give identical
 // results, so tuning never changes an answer.
@@ -927,6 +975,7 @@ struct nbfmm_batch {
   // another, [1] the last step
   hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
   hipGraph_t graph[2] = {nullptr, nullptr};
+  SideStream side;  // replays asked for on the null stream run here
   struct {
     const void *xcur = nullptr, *xpre = nullptr, *w = nullptr, *acc = nullptr;
     hipStream_t stream = nullptr;
@@ -955,6 +1004,7 @@ void batch_free(nbfmm_batch* b) {
     if (b->graph_exec[k]) hipGraphExecDestroy(b->graph_exec[k]);
     if (b->graph[k]) hipGraphDestroy(b->graph[k]);
   }
+  b->side.release();
   for (void* p : b->allocs) hipFree(p);
   b->allocs.clear();
 }
@@ -1346,8 +1396,15 @@ int nbfmm_batch_run(nbfmm_batch* b, int steps, float* xcur, float* xpre, const f
       if ((rc = nbfmm_batch_step(b, xcur, xpre, w, acc, stream))) return rc;
     return NBFMM_OK;
   }
-  if (!st) return fail(NBFMM_ERR_INVALID_ARGUMENT, "graph replay needs a non-null stream");
   if (steps == 0) return NBFMM_OK;
+  if (!st) {
+    NB_HIP(b->side.begin());
+    rc = nbfmm_batch_run(b, steps, xcur, xpre, w, acc, use_graph, (void*)b->side.s);
+    const hipError_t e = b->side.end();
+    if (rc) return rc;
+    if (e != hipSuccess) return hip_fail(e, "side stream");
+    return NBFMM_OK;
+  }
   // As nbfmm_run: the first step is eager; with the warm start on, the P2P
   // of every step but the last makes the next step's first sort pass.
   CallIO io = call_io(xcur, w, acc, xcur, xpre);
@@ -1472,6 +1529,17 @@ int nbfmm_batch_set_p2p_config(nbfmm_batch* b, int variant, int blocks_per_cu) {
   if (blocks_per_cu < 1 || blocks_per_cu > 64) return fail(NBFMM_ERR_INVALID_ARGUMENT, "blocks_per_cu in [1, 64]");
   const int rc = batch_use_p2p(b, variant, blocks_per_cu);
   if (rc) return rc;
+  drop_batch_graph(b);
+  return NBFMM_OK;
+}
+
+int nbfmm_batch_set_p2p_wait_polls(nbfmm_batch* b, int polls) {
+  if (!b) return fail(NBFMM_ERR_INVALID_ARGUMENT, "null batch");
+  if (polls < 1) return fail(NBFMM_ERR_INVALID_ARGUMENT, "polls must be >= 1");
+  // the views every launch map carries (maps made later copy hv[0])
+  for (auto& v : b->hv) v.wait_polls = polls;
+  for (auto& kv : b->p2p_maps) kv.second.sys.base.wait_polls = polls;
+  b->bl.p2p.sys.base.wait_polls = polls;
   drop_batch_graph(b);
   return NBFMM_OK;
 }

@@ -7,6 +7,9 @@
 namespace nbfmm_amd {
 
 constexpr int kWave = 64;
+// Polls (each a short sleep) before a P2P wave stops waiting for another wave
+// of its block; nbfmm_set_p2p_wait_polls changes it per solver or batch.
+constexpr int kP2PWaitPolls = 1 << 20;
 constexpr int kMaxLevel = 8;          // base_dim <= 512 (source/nbfmm/core/solver.cu:25)
 constexpr int kSortThreads = 256;     // 4 waves per sort tile
 constexpr int kSortWaveItems = 8;     // 64-element chunks per wave
@@ -41,6 +44,7 @@ struct SysView {
   float eps;        // kernel softening
   float tick;       // Verlet step (dt*sqrt(G)), step mode only
   int part_off;     // first particle of this system in the caller's arrays
+  int wait_polls = kP2PWaitPolls;  // P2P: bound on every wait for another wave of the block
   // particle scratch (capacity max_n)
   uint32_t* tkey;   // row-sorted keys (row pass output)
   float4* tpart;    // row-sorted particles: x, y, m, original index (bits)
@@ -105,8 +109,8 @@ constexpr int kWarmTouched = 1, kWarmInts = 4;
 constexpr int kP2PSplitMin = 2048;
 constexpr int kP2PSegLen = 1024;
 constexpr int kP2PMaxSeg = 16;
-constexpr int kP2PWaitPolls = 1 << 20;  // polls before a wave stops waiting for the segment before its own
-constexpr uint32_t kFlagP2PWait = 2u;   // NBFMM_FLAG_P2P_WAIT
+constexpr uint32_t kFlagP2PWait = 2u;      // NBFMM_FLAG_P2P_WAIT
+constexpr uint32_t kFlagP2PProtocol = 4u;  // NBFMM_FLAG_P2P_LOST
 constexpr int kWarmMergeWaves = 16384;       // merge grid: one wave per touched leaf, grid-stride beyond
 
 // Caller arrays of one call, original particle order; a system's particles

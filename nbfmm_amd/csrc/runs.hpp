@@ -18,12 +18,15 @@
 // generation and K in its high bits), sums it, waits until the segments
 // before it have been added into the slot's running sums, adds its own and
 // passes on; the last one runs the epilogue and frees the slot.  The wait
-// always ends (tickets go out in order and segment 0 waits for nothing; a
-// poll limit raises NBFMM_FLAG_P2P_WAIT instead of hanging).  A wave whose
-// heavy run finds the slot taken keeps it pending, helps with the published
-// run and publishes its own once the slot is free (after a poll limit it sums
-// the whole union itself in one pass and raises NBFMM_FLAG_P2P_WAIT: only
-// then may the bits differ from the split sum's).  A wave that finds the run
+// ends normally (tickets go out in order and segment 0 waits for nothing);
+// it is bounded all the same: after a poll limit the waiting wave abandons
+// the run, sums its whole union alone in one pass and runs the epilogue, and
+// the other waves of that run drop their segments (RunSlot below).  A wave
+// whose heavy run finds the slot taken keeps it pending, helps with the
+// published run and publishes its own once the slot is free (after a poll
+// limit it sums the whole union itself in one pass).  Either give-up raises
+// NBFMM_FLAG_P2P_WAIT: the results are right, but their bits may differ from
+// the split sum's.  A wave that finds the run
 // queue empty stays to help while another wave holds a run that may still be
 // published (the low bits of the queue word count runs from their take until
 // they are known light, published, or summed alone) or a split run is
@@ -95,32 +98,72 @@ __device__ __forceinline__ int p2p_segment_len(int total, int K) { return (((tot
 
 // The block's run queue and its slot for a split run, in LDS; NACC running
 // sums per lane.
+//
+// The slot's completion word `done` holds the generation of the published
+// run (bits 20..29, the generation of its tickets), a lock bit, an abandon
+// bit and the number of segments added so far (bits 0..15).  The wave
+// holding segment k adds it only after locking the word at count k (a
+// compare-and-swap from the unlocked value), so the running sums are written
+// only by the lock holder of the current generation; it then stores count
+// k + 1, unlocked.  A wave that has waited its poll limit for the segments
+// before its own abandons the run instead (a compare-and-swap from an
+// unlocked value of the same generation to the abandoned one): it sums the
+// whole union alone, in one pass, runs the epilogue and frees the slot.
+// Every other wave of that generation then drops its segment.  Only one of
+// "count reaches K - 1 under the last segment's lock" and "abandoned" can
+// happen, so the epilogue of a run runs exactly once, and a wave of an old
+// generation can neither lock nor abandon a run published later.
+constexpr uint32_t kDoneCount = 0xffffu;
+constexpr uint32_t kDoneLock = 1u << 16;
+constexpr uint32_t kDoneAbandon = 1u << 17;
+constexpr int kDoneGenShift = 20;
+
 template <int NACC>
 struct RunSlot {
   float acc[kWave][NACC];  // running sums of the segments added so far
   uint32_t queue;          // runs taken (bits 31..4) and runs counted active (bits 3..0)
   uint32_t ticket;         // generation << 22 | K << 16 | next segment
-  int open, busy, done, run;
+  uint32_t done;           // generation << 20 | abandon | lock | segments added
+  int busy, run;
 };
 
 template <int NACC>
 __device__ __forceinline__ void run_slot_init(RunSlot<NACC>& sl) {
   sl.queue = 0;
   sl.ticket = 0;
-  sl.open = 0;
-  sl.busy = 0;
   sl.done = 0;
+  sl.busy = 0;
   sl.run = 0;
+}
+
+// Lane 0's compare-and-swap on an LDS word, its outcome made wave-uniform.
+__device__ __forceinline__ bool wave_cas(uint32_t* w, uint32_t expected, uint32_t desired, int lane) {
+  int ok = 0;
+  if (lane == 0) {
+    uint32_t e = expected;
+    ok = __hip_atomic_compare_exchange_strong(w, &e, desired, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_WORKGROUP) ? 1 : 0;
+  }
+  wave_lds_order();
+  return __builtin_amdgcn_readfirstlane(ok) != 0;
+}
+
+__device__ __forceinline__ uint32_t wave_load(uint32_t* w) {
+  wave_lds_order();
+  const uint32_t x = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  return __builtin_amdgcn_readfirstlane(x);
 }
 
 // The block's waves work through its runs.  seg(g, sa, sb, acc) sums this
 // lane's sources among union positions [sa, sb) of run g into acc[NACC]
 // (from zero); fin(g, acc) is the epilogue of run g (HK: g.hk is loaded).  The slot must have been
-// initialised and the block synchronised.
+// initialised and the block synchronised.  v.wait_polls bounds every wait
+// on another wave.
 template <int NACC, bool HK, class Seg, class Fin>
 __device__ __forceinline__ void run_queue(const SysView& v, const int* s_runs, RunSlot<NACC>& sl, int blk, int nblk,
                                           int total, int lane, Seg&& seg, Fin&& fin) {
   RunGeom g;
+  const int limit = v.wait_polls;
   bool exhausted = false;  // this wave found the block's run queue empty
   int idle_polls = 0;
   int pend = -1, pend_k = 1, pend_polls = 0;  // a split run waiting for the slot (wave-uniform)
@@ -134,13 +177,12 @@ __device__ __forceinline__ void run_queue(const SysView& v, const int* s_runs, R
                                                __HIP_MEMORY_SCOPE_WORKGROUP)) {
         got = 1;
         sl.run = jr;
-        sl.done = 0;
         const uint32_t gen = ((sl.ticket >> 22) + 1u) & 0x3ffu;
+        __hip_atomic_store(&sl.done, gen << kDoneGenShift, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         wave_lds_order();
+        // the tickets open last: a wave that draws one sees this run and its done word
         __hip_atomic_exchange(&sl.ticket, (gen << 22) | ((uint32_t)kr << 16), __ATOMIC_RELAXED,
                               __HIP_MEMORY_SCOPE_WORKGROUP);
-        wave_lds_order();
-        __hip_atomic_store(&sl.open, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         wave_lds_order();
         __hip_atomic_fetch_add(&sl.queue, 0xffffffffu, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
       }
@@ -149,23 +191,27 @@ __device__ __forceinline__ void run_queue(const SysView& v, const int* s_runs, R
   };
   for (;;) {
     // what this iteration sums: segment k0 of run j out of K (a ticket of
-    // the published run), or all of run j in one pass
+    // the published run, generation gen), or all of run j in one pass
     int j = -1, K = 1, k0 = 0;
+    uint32_t gen = 0;
     bool ticket = false;
     // 1. a segment of the published split run, if any is left
-    if (__hip_atomic_load(&sl.open, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) {
-      uint32_t t = 0;
-      if (lane == 0) t = __hip_atomic_fetch_add(&sl.ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-      wave_lds_order();
-      t = __builtin_amdgcn_readfirstlane(t);
-      const int k = (int)(t & 0xffffu), Kt = (int)((t >> 16) & 0x3fu);
-      if (k < Kt) {
-        if (k == Kt - 1 && lane == 0) __hip_atomic_store(&sl.open, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        ticket = true;
-        j = __builtin_amdgcn_readfirstlane(sl.run);
-        K = Kt;
-        k0 = k;
-        run_geom<HK>(v, s_runs, j, lane, g);
+    {
+      const uint32_t t0 = wave_load(&sl.ticket);
+      if ((t0 & 0xffffu) < ((t0 >> 16) & 0x3fu)) {
+        uint32_t t = 0;
+        if (lane == 0) t = __hip_atomic_fetch_add(&sl.ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        wave_lds_order();
+        t = __builtin_amdgcn_readfirstlane(t);
+        const int k = (int)(t & 0xffffu), Kt = (int)((t >> 16) & 0x3fu);
+        if (k < Kt) {
+          ticket = true;
+          gen = t >> 22;
+          j = __builtin_amdgcn_readfirstlane(sl.run);
+          K = Kt;
+          k0 = k;
+          run_geom<HK>(v, s_runs, j, lane, g);
+        }
       }
     }
     if (!ticket) {
@@ -176,7 +222,7 @@ __device__ __forceinline__ void run_queue(const SysView& v, const int* s_runs, R
           pend = -1;
           continue;
         }
-        if (++pend_polls < kP2PWaitPolls) {
+        if (++pend_polls < limit) {
           __builtin_amdgcn_s_sleep(1);
           continue;
         }
@@ -191,12 +237,13 @@ __device__ __forceinline__ void run_queue(const SysView& v, const int* s_runs, R
         if (exhausted) {
           // leave once no run is counted active and no split run is
           // published; new segments can appear only from those (a wave that
-          // stops waiting only stops helping, so the poll limit is no error)
+          // stops waiting only stops helping, so this bound is no give-up and
+          // does not follow v.wait_polls)
           wave_lds_order();
           if ((__hip_atomic_load(&sl.queue, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) & 0xfu) == 0 &&
               __hip_atomic_load(&sl.busy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == 0)
             break;
-          if (++idle_polls == kP2PWaitPolls) break;
+          if (++idle_polls >= kP2PWaitPolls) break;
           __builtin_amdgcn_s_sleep(2);
           continue;
         }
@@ -241,15 +288,45 @@ __device__ __forceinline__ void run_queue(const SysView& v, const int* s_runs, R
         __hip_atomic_fetch_add(&sl.queue, 0xffffffffu, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
       continue;
     }
-    // a ticket: wait for segments 0 .. k0-1, then add this one
-    for (int polls = 0; __hip_atomic_load(&sl.done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) != k0; ++polls) {
-      if (polls == kP2PWaitPolls) {
-        if (lane == 0) atomicOr(v.flags, kFlagP2PWait);
-        break;
+    // a ticket: lock the done word at count k0 (segments 0 .. k0-1 added),
+    // or abandon the run after the poll limit
+    const uint32_t mine = (gen << kDoneGenShift) | (uint32_t)k0;
+    int outcome = 0;  // 1: locked, 2: abandoned by this wave, 3: dropped
+    for (int polls = 0; outcome == 0; ++polls) {
+      const uint32_t d = wave_load(&sl.done);
+      if ((d >> kDoneGenShift) != gen || (d & kDoneAbandon)) {
+        outcome = 3;  // abandoned by another wave, or the slot already holds a later run
+      } else if (d == mine) {
+        if (wave_cas(&sl.done, d, d | kDoneLock, lane)) outcome = 1;
+      } else if (!(d & kDoneLock) && polls >= limit) {
+        if (wave_cas(&sl.done, d, d | kDoneAbandon, lane)) outcome = 2;
+      } else if (polls >= limit + (1 << 16)) {
+        // a lock is held only across a few LDS accesses (a few microseconds
+        // at most; this is milliseconds): reaching it is a broken protocol,
+        // reported rather than waited for
+        if (lane == 0) atomicOr(v.flags, kFlagP2PWait | kFlagP2PProtocol);
+        outcome = 3;
+      } else {
+        __builtin_amdgcn_s_sleep(1);
       }
-      __builtin_amdgcn_s_sleep(1);
     }
-    wave_lds_order();
+    if (outcome == 3) continue;
+    if (outcome == 2) {
+      // this wave owns the run: no further tickets, the whole union here in
+      // one pass, its epilogue, then the slot is free
+      if (lane == 0) {
+        atomicOr(v.flags, kFlagP2PWait);
+        __hip_atomic_store(&sl.ticket, (gen << 22) | ((uint32_t)K << 16) | (uint32_t)K, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_WORKGROUP);
+      }
+      wave_lds_order();
+      seg(g, 0, g.total, acc);
+      fin(g, acc);
+      wave_lds_order();
+      if (lane == 0) __hip_atomic_store(&sl.busy, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      continue;
+    }
+    // locked: add this segment to the running sums
     if (k0 > 0) {
 #pragma unroll
       for (int c = 0; c < NACC; ++c) acc[c] = sl.acc[lane][c] + acc[c];
@@ -258,10 +335,13 @@ __device__ __forceinline__ void run_queue(const SysView& v, const int* s_runs, R
 #pragma unroll
       for (int c = 0; c < NACC; ++c) sl.acc[lane][c] = acc[c];
       wave_lds_order();
-      if (lane == 0) __hip_atomic_store(&sl.done, k0 + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      if (lane == 0) __hip_atomic_store(&sl.done, mine + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     } else {
       wave_lds_order();
-      if (lane == 0) __hip_atomic_store(&sl.busy, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      if (lane == 0) {
+        __hip_atomic_store(&sl.done, mine + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_store(&sl.busy, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      }
       fin(g, acc);
     }
   }

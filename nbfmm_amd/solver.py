@@ -219,6 +219,12 @@ class Solver:
         results are identical either way."""
         check(lib().nbfmm_set_warm_start(self._h, int(bool(enable))))
 
+    def set_p2p_wait_polls(self, polls: int) -> None:
+        """Bound on every wait of a P2P wave for another wave of its block;
+        past it the wave sums its run alone (FLAG_P2P_WAIT; same results,
+        bits may differ).  Default 2^20; tests set 1 to force that path."""
+        check(lib().nbfmm_set_p2p_wait_polls(self._h, int(polls)))
+
     # -- measurement ------------------------------------------------------
     def profile(self, enable: bool = True) -> None:
         check(lib().nbfmm_profile_enable(self._h, int(bool(enable))))

@@ -112,20 +112,33 @@ def two_disk_collision(n=1 << 20, seed=1002, steps=100, box=4.0, num_level=8, G=
     vel = np.concatenate(vels)
     w = np.concatenate(ws)
     return _finish("two_disk_collision", pos, vel, w, num_level, (-box, -box, box, box), 1e-3, 1e-4, G, steps,
-                   G_choice="1/N")
+                   G_per_N=G * n)
+
+
+# G of the mass-inference workloads, in units of 1/N.  A trajectory depends
+# on the other particles' masses through tick^2 * a, and with G = 1/N (total
+# mass about 1) a 20 % mass perturbation moves the positions of a two-disk
+# collision of 65536 particles by 7.8e-8 RMS after 100 steps: below one fp32
+# ulp at |x| ~ 2 (2.4e-7), so the fit's loss would be rounding noise.  With
+# G = 1e4 / N the same perturbation moves them by 2.0e-3 RMS (4.1e-4 after
+# 50 steps), 40 times the difference between the fp32 and fp64 trajectories
+# of the same masses (5.4e-5), while the particles stay inside the box.
+FIT_G_PER_N = 1e4
 
 
 def gaussian_disk_system(n=65536, seed=1003, steps=50, num_level=6, G=None):
     """Single Gaussian disk (sigma 0.5, truncated at 1.9), masses U[0.5,1.5],
-    7 levels over [-2.09, 2.09]^2 (1.1 x the truncation radius), dt=1e-4."""
+    7 levels over [-2.09, 2.09]^2 (1.1 x the truncation radius), dt=1e-4.
+    The mass-inference workload: G = FIT_G_PER_N / N unless given."""
     rng = np.random.default_rng(seed)
     if G is None:
-        G = 1.0 / n
+        G = FIT_G_PER_N / n
     pos = gaussian_disk(rng, n, 0.5, 1.9)
     w = rng.uniform(0.5, 1.5, n)
     vel = circular_velocity(pos, w, (0.0, 0.0), G)
     R = 1.1 * 1.9
-    return _finish("gaussian_disk", pos, vel, w, num_level, (-R, -R, R, R), 1e-3, 1e-4, G, steps, G_choice="1/N")
+    return _finish("gaussian_disk", pos, vel, w, num_level, (-R, -R, R, R), 1e-3, 1e-4, G, steps,
+                   G_choice=f"{FIT_G_PER_N:g}/N")
 
 
 @dataclass
@@ -169,16 +182,18 @@ def many_disks(num_systems=256, n_min=1024, n_max=131072, seed=1004, steps=20):
     return BatchWorkload("many_disks", systems, steps)
 
 
-def many_collisions(num_systems=64, n_min=4096, n_max=262144, seed=1005, steps=100):
+def many_collisions(num_systems=64, n_min=4096, n_max=262144, seed=1005, steps=100, g_per_n=FIT_G_PER_N):
     """Independent two-disk collisions (the two-disk geometry with N_s
-    particles): N log-uniform in [n_min, n_max], levels as in many_disks,
-    G = 1/N per system."""
+    particles): N log-uniform in [n_min, n_max], levels as in many_disks.
+    The batched mass-inference workload: G = g_per_n / N per system
+    (FIT_G_PER_N)."""
     rng = np.random.default_rng(seed)
     systems = []
     for k in range(num_systems):
         n = int(np.floor(np.exp(rng.uniform(np.log(n_min), np.log(n_max)))))
         levels = int(np.clip(np.round(np.log(n / 16.0) / np.log(4.0)), 3, 9))
-        w = two_disk_collision(n=n, seed=int(rng.integers(1 << 30)), steps=steps, num_level=levels - 1)
+        w = two_disk_collision(n=n, seed=int(rng.integers(1 << 30)), steps=steps, num_level=levels - 1,
+                               G=g_per_n / n)
         w.name = f"sys{k}"
         systems.append(w)
     return BatchWorkload("many_collisions", systems, steps)

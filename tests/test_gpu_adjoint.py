@@ -288,7 +288,8 @@ def test_forces_autograd_function():
 
 def test_mass_inference_loss_decreases():
     """Fit masses to a trajectory made with perturbed masses (small version
-    of the mass-inference workload): a few gradient steps reduce the loss."""
+    of the mass-inference workload, at its own G and tick): a few gradient
+    steps reduce the loss."""
     from nbfmm_amd import Solver
     from nbfmm_amd.autograd import fmm_trajectory
     from nbfmm_amd.workloads import gaussian_disk_system
@@ -297,7 +298,7 @@ def test_mass_inference_loss_decreases():
     s = Solver(wl.num_level, wl.n, wl.limits, wl.eps)
     true_m = dev(wl.weight)
     obs = [10, 20]
-    tick = wl.dt * np.sqrt(wl.G) * 30  # longer steps so the masses matter
+    tick = wl.tick
     with torch.no_grad():
         target = fmm_trajectory(s, tick, 20, 5, obs, dev(wl.position), dev(wl.position_pre), true_m)
     m = (true_m * dev(rng.uniform(0.8, 1.2, wl.n))).requires_grad_(True)
@@ -305,7 +306,7 @@ def test_mass_inference_loss_decreases():
     for _ in range(4):
         out = fmm_trajectory(s, tick, 20, 5, obs, dev(wl.position), dev(wl.position_pre), m)
         loss = ((out - target) ** 2).mean()
-        losses.append(float(loss))
+        losses.append(float(loss.detach()))
         m.grad = None
         loss.backward()
         with torch.no_grad():

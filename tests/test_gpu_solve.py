@@ -305,3 +305,25 @@ def test_leaf_size_classes(case):
     assert rel_l2(got, ref) <= RTOL_L2, (case, rel_l2(got, ref))
     assert max_rel(got, ref) <= RTOL_MAX, (case, max_rel(got, ref))
     s.close()
+
+
+def test_graph_run_on_the_null_stream():
+    """run(use_graph=True) on torch's default (null) stream: the replays go to
+    the solver's own stream, ordered with the null stream, and equal eager
+    steps bit for bit."""
+    from nbfmm_amd import Solver
+    from nbfmm_amd.workloads import disk
+    wl = disk(n=6000, steps=5)
+    a = Solver(wl.num_level, wl.n, wl.limits, wl.eps)
+    b = Solver(wl.num_level, wl.n, wl.limits, wl.eps)
+    Xa, Pa, W = dev(wl.position), dev(wl.position_pre), dev(wl.weight)
+    Xb, Pb = Xa.clone(), Pa.clone()
+    Aa, Ab = torch.zeros_like(Xa), torch.zeros_like(Xa)
+    a.run(wl.n, wl.tick, 5, Xa, Pa, W, Aa)  # default stream, default use_graph=True
+    Xa.add_(0.0)  # work queued after the call on the null stream sees its results
+    for _ in range(5):
+        b.step(wl.n, wl.tick, Xb, Pb, W, Ab)
+    torch.cuda.synchronize()
+    assert torch.equal(Xa, Xb) and torch.equal(Pa, Pb) and torch.equal(Aa, Ab)
+    a.close()
+    b.close()

@@ -261,3 +261,93 @@ def test_clustered_p2m_bit_exact():
     np.testing.assert_array_equal(s.read_buffer("cell_weight").cpu().numpy()[:B * B], rw[:B * B])
     np.testing.assert_array_equal(s.read_buffer("cell_position").cpu().numpy()[:B * B], rp[:B * B])
     s.close()
+
+
+FLAG_LOST = 4
+
+
+@pytest.mark.parametrize("polls", [1, 3])
+def test_forced_give_up_solve_matches_oracle(polls):
+    """The wait bound set to 1 or 3 polls from the host: waves holding a
+    segment give up on the segments before theirs, abandon the run and sum
+    its whole union alone; waves with a heavy run that finds the slot taken
+    sum it alone.  Every kernel still ends normally, each run's epilogue runs
+    once, and the forces match the oracle in every configuration."""
+    from nbfmm_amd import FLAG_P2P_WAIT, Solver
+    from nbfmm_amd.autotune import set_config, variants
+    # about two heavy runs per block on the smallest grid, so that a heavy run
+    # also finds its block's slot taken
+    x, w, L, lim, eps = clustered(n=30000, seed=12)
+    assert max_union(x, lim, L) > 8 * 2048
+    n = x.shape[0]
+    s = Solver(L, n, lim, eps)
+    s.set_p2p_wait_polls(polls)
+    X, W = dev(x), dev(w)
+    want, _ = orc.solve(L, x, w, lim, eps)
+    raised = 0
+    for v in range(len(variants())):
+        for bpc in (1, 16):
+            set_config(s, v, bpc)
+            a = torch.zeros_like(X)
+            s.solve(n, X, W, a)
+            torch.cuda.synchronize()
+            got = a.cpu().numpy()
+            assert np.linalg.norm(got - want) / np.linalg.norm(want) < 1e-5, (v, bpc)
+            f = s.flags(clear=True)
+            assert not f & FLAG_LOST, (v, bpc, f)
+            raised |= f
+    assert raised & FLAG_P2P_WAIT  # the give-up path was taken
+    s.close()
+
+
+def test_forced_give_up_steps_match_default():
+    """Graph-replayed Verlet steps (the epilogue updates x and x_pre in place,
+    so a run finished twice or not at all would show at once) with the wait
+    bound at 1 poll: the trajectory matches the default solver's to rounding
+    (bits may differ where a run was summed in one pass)."""
+    from nbfmm_amd import FLAG_P2P_WAIT, Solver
+    x, w, L, lim, eps = clustered(n=8000, seed=13)
+    n = x.shape[0]
+    xp = x * np.float32(0.999)
+    tick = 1e-3
+    forced = Solver(L, n, lim, eps)
+    forced.set_p2p_wait_polls(1)
+    ref = Solver(L, n, lim, eps)
+    Xa, Pa, Wa = dev(x), dev(xp), dev(w)
+    Xb, Pb, Wb = dev(x), dev(xp), dev(w)
+    Aa, Ab = torch.zeros_like(Xa), torch.zeros_like(Xb)
+    st = torch.cuda.Stream()
+    with torch.cuda.stream(st):
+        forced.run(n, tick, 6, Xa, Pa, Wa, Aa, use_graph=True)
+        ref.run(n, tick, 6, Xb, Pb, Wb, Ab, use_graph=True)
+    st.synchronize()
+    scale = float(Ab.abs().max())
+    assert float((Aa - Ab).abs().max()) < 1e-4 * scale
+    assert float((Xa - Xb).abs().max()) < 1e-6 and float((Pa - Pb).abs().max()) < 1e-6
+    f = forced.flags()
+    assert f & FLAG_P2P_WAIT and not f & FLAG_LOST
+    assert ref.flags() == 0
+    forced.close()
+    ref.close()
+
+
+def test_forced_give_up_in_a_batch():
+    from nbfmm_amd import Batch
+    from nbfmm_amd.workloads import disk
+    xc, wc, Lc, limc, epsc = clustered(n=10000, seed=14)
+    light = disk(n=5000)
+    b = Batch([xc.shape[0], light.n], [Lc, light.num_level], [tuple(limc), light.limits], [epsc, light.eps])
+    b.set_p2p_wait_polls(1)
+    X, W = dev(np.concatenate([xc, light.position])), dev(np.concatenate([wc, light.weight]))
+    A = torch.zeros_like(X)
+    b.solve(X, W, A)
+    torch.cuda.synchronize()
+    off = b.offsets
+    for k, (x, w, L, lim, eps) in enumerate([(xc, wc, Lc, limc, epsc),
+                                             (light.position, light.weight, light.num_level,
+                                              np.array(light.limits, np.float32), light.eps)]):
+        want, _ = orc.solve(L, x, w, lim, eps)
+        got = A[off[k]:off[k + 1]].cpu().numpy()
+        assert np.linalg.norm(got - want) / np.linalg.norm(want) < 1e-5, k
+    assert not (b.flags() & FLAG_LOST).any()
+    b.close()
