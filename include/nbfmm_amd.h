@@ -101,6 +101,20 @@ int nbfmm_solver_info(const nbfmm_solver* solver, int* num_level, int* base_dim,
 int nbfmm_solve(nbfmm_solver* solver, int num_particle, const float* d_position_origin,
                 const float* d_weight_origin, float* d_effect_origin, void* stream);
 
+/* No counterpart in the reference: nbfmm_solve for callers that cannot
+ * allocate device memory (code under hipGraph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No solver: the sizes come with the call,
+ * and all scratch lives in a static 16 MiB device array per GPU (held until
+ * the process ends); the call allocates nothing.  Same kernels as
+ * nbfmm_solve with a cold sort, so the same bits.  A size whose scratch
+ * exceeds 16 MiB (about 40 B per particle, 24 B per cell of the tree and
+ * 8 B per leaf: 262144 particles at num_level 7 fit) fails with
+ * NBFMM_ERR_CAPACITY.  Calls that overlap on one GPU share the array and
+ * race; out-of-box particles are clamped but no flag is reported. */
+int nbfmm_solve_noalloc(int num_level, int num_particle, const float position_limits[4], float epsilon,
+                        const float* d_position_origin, const float* d_weight_origin, float* d_effect_origin,
+                        void* stream);
+
 /* The eight stages of nbfmm::Solver (include/nbfmm/core/solver.hpp:111-207),
  * with the same pre/post conditions on the solver's buffers.  solve() is
  * their composition; the stages exist so each one can be checked in

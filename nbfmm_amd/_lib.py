@@ -20,6 +20,7 @@ SIGNATURES = {
     "nbfmm_solver_destroy": [P],
     "nbfmm_solver_info": [P, ctypes.POINTER(c_int), ctypes.POINTER(c_int), ctypes.POINTER(c_int)],
     "nbfmm_solve": [P, c_int, P, P, P, P],
+    "nbfmm_solve_noalloc": [c_int, c_int, ctypes.POINTER(c_float), c_float, P, P, P, P],
     "nbfmm_predo": [P, c_int, P, P, P],
     "nbfmm_p2p": [P, c_int, P],
     "nbfmm_p2m": [P, c_int, P],

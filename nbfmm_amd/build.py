@@ -33,7 +33,8 @@ FLAGS = [
     "-I", os.path.join(ROOT, "include"),
 ]
 
-SOURCES = ["sort.hip", "tree.hip", "p2p.hip", "adjoint.hip", "prune.hip", "model.hip", "display.hip", "capi.hip"]
+SOURCES = ["sort.hip", "tree.hip", "p2p.hip", "adjoint.hip", "prune.hip", "model.hip", "display.hip", "capi.hip",
+           "noalloc.hip"]
 
 
 def _compile(src: str) -> str:
