@@ -36,16 +36,51 @@ P2P_PAIR_BOUND_PER_S = 3.19e12
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_two_disk_N1048576.md")
 
 
-def measured_traffic():
-    """HBM bytes per P2P launch from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary of this workload (None if absent)."""
+def measured_traffic(kernel="k_p2p<kP2PStepSort>"):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary of this workload (None if absent)."""
     try:
         for line in open(TRAFFIC_PROFILE):
-            if line.startswith("traffic_bytes_per_launch k_p2p<kP2PStep>:"):
+            if line.startswith(f"traffic_bytes_per_launch {kernel}:"):
                 return float(line.split(":")[1])
     except OSError:
         return None
     return None
+
+
+# Algorithmic (compulsory) HBM bytes of the other stages of a step, per
+# particle, per leaf and per tree cell (all levels):
+#   sort (warm, from its second pass): read the binned particles of the
+#     previous order (key 4 + particle 16 + rank 4) and write the sorted key
+#     and particle (4 + 16): 44 B per particle; head and the occupied-leaf
+#     list: 8 B per leaf;
+#   upward (P2M + M2M): read x, y, m (12 B per particle); write every cell's
+#     multipole (16 B) and read it once more from its parent (16 B, all but
+#     the top level);
+#   M2L: read each cell's multipole (16 B) and write its local (8 B).
+SORT_B_PER_PARTICLE, SORT_B_PER_LEAF = 44, 8
+UP_B_PER_PARTICLE, UP_B_PER_CELL = 12, 32
+M2L_B_PER_CELL = 24
+M2L_FLOPS_PER_PAIR = 14
+
+
+def tree_cells(num_level):
+    B = 1 << (num_level + 1)
+    return sum((B >> l) ** 2 for l in range(max(num_level, 1)))
+
+
+def m2l_pairs(num_level):
+    """M2L cell pairs of one step (source/nbfmm/core/solver/m2l.cu:39-51):
+    per level and axis, the window [p-2, p+4) of the target's parent p
+    (clipped to the grid) minus its own 3-cell neighbourhood."""
+    B = 1 << (num_level + 1)
+    tot = 0
+    for l in range(num_level):
+        D = B >> l
+        W = sum(min((t & ~1) + 4, D) - max((t & ~1) - 2, 0) for t in range(D))
+        Nn = sum(min(t + 2, D) - max(t - 1, 0) for t in range(D))
+        tot += W * W - Nn * Nn
+    return tot
 
 
 def log(msg):
@@ -70,6 +105,8 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint legs")
     ap.add_argument("--no-small", action="store_true", help="skip the 16384-particle disk leg")
     ap.add_argument("--no-tune", action="store_true", help="keep the default P2P configuration")
+    ap.add_argument("--no-sensitivity", action="store_true",
+                    help="skip the step at a larger G and the cold-sort step")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one 256 MiB device-to-device copy (scale reference for HBM counters)")
     args = ap.parse_args()
@@ -131,25 +168,48 @@ def main():
     flags = s.flags()
     log(f"[bench] graph: {dt_s * 1e3 / args.steps:.4f} ms/step, {value:.4e} particle-steps/s, flags={flags}")
 
-    # instrumented pass over the same number of steps from the same state:
-    # HIP events around each P2P launch on the launch stream
+    # instrumented pass over the same number of steps from the same state,
+    # eager, with the launches the graph replays (each step's P2P is
+    # k_p2p<kP2PStepSort>, which also makes the next sort's first pass, but
+    # the last one's): HIP events on the launch stream around each stage
     x.copy_(x0)
     p.copy_(p0)
     torch.cuda.synchronize()
     s.profile(True)
-    s.p2p_time(reset=True)
+    s.stage_times(reset=True)
     te0 = time.perf_counter()
     with torch.cuda.stream(stream):
         s.run(n, wl.tick, args.steps, x, p, w, a, use_graph=False)
     stream.synchronize()
     te1 = time.perf_counter()
-    p2p_ms, launches = s.p2p_time(reset=True)
+    stage_ms, launches = s.stage_times(reset=True)
     s.profile(False)
-    p2p_avg_ms = p2p_ms / max(launches, 1)
+    avg_ms = [t / max(launches, 1) for t in stage_ms]
+    p2p_avg_ms = avg_ms[3]
     pairs = 0.5 * (pairs0 + pairs1)
     achieved = P2P_FLOPS_PER_PAIR * pairs / (p2p_avg_ms * 1e-3) / 1e12
-    log(f"[bench] eager: {(te1 - te0) * 1e3 / args.steps:.4f} ms/step; P2P avg {p2p_avg_ms * 1e3:.1f} us over "
-        f"{launches} launches; pairs/launch {pairs:.4e}; {achieved:.2f} TFLOP/s")
+    log(f"[bench] eager: {(te1 - te0) * 1e3 / args.steps:.4f} ms/step; stages (us) "
+        f"{[round(t * 1e3, 1) for t in avg_ms]}; P2P pairs/launch {pairs:.4e}; {achieved:.2f} TFLOP/s")
+    B = 1 << (wl.num_level + 1)
+    cells = tree_cells(wl.num_level)
+    npairs_m2l = m2l_pairs(wl.num_level)
+
+    def hbm(nbytes, ms):
+        gbs = nbytes / (ms * 1e-3) / 1e9
+        return {"us": ms * 1e3, "algorithmic_bytes": nbytes, "GBps": gbs, "frac_hbm": gbs / HBM_PEAK_GBS}
+
+    stages = {
+        "sort": hbm(SORT_B_PER_PARTICLE * n + SORT_B_PER_LEAF * B * B, avg_ms[0]),
+        "upward": hbm(UP_B_PER_PARTICLE * n + UP_B_PER_CELL * cells, avg_ms[1]),
+        "m2l": hbm(M2L_B_PER_CELL * cells, avg_ms[2]),
+        "p2p": {"us": p2p_avg_ms * 1e3, "flop": P2P_FLOPS_PER_PAIR * pairs, "TFLOPs": achieved,
+                "frac_fp32": achieved / FP32_PEAK_TFLOPS},
+        "note": "eager launches of the replayed step, HIP events between stages (launch gaps included); "
+                "sort = warm passes 2-5 (the first is inside the previous step's P2P)",
+    }
+    stages["m2l"]["flop"] = M2L_FLOPS_PER_PAIR * npairs_m2l
+    stages["m2l"]["TFLOPs"] = M2L_FLOPS_PER_PAIR * npairs_m2l / (avg_ms[2] * 1e-3) / 1e12
+    sens = None if args.no_sensitivity else sort_sensitivity(args, wl)
 
     accuracy = None
     cpu = None
@@ -188,7 +248,8 @@ def main():
         tcpu = time.perf_counter() - tc0
         cpu = {"value": n * ksteps / tcpu, "unit": "particle-steps/s", "cores": threads, "kind": "port",
                "sample": f"{ksteps} full steps of {wl.name} N={n} (after 1 untimed step), OpenMP, "
-                         f"{tcpu:.1f} s"}
+                         f"{tcpu:.1f} s",
+               "cpu_model": orc.cpu_model(), "isa": orc.isa(), "nproc": os.cpu_count()}
         log(f"[bench] cpu baseline {cpu}")
 
     if args.calib_copy:
@@ -219,7 +280,7 @@ def main():
         "config": {"workload": wl.name, "n": n, "num_level": wl.num_level, "leaf_grid": 1 << (wl.num_level + 1),
                    "limits": list(wl.limits), "eps": wl.eps, "dt": wl.dt, "G": wl.G, "mode": "plain, hipGraph",
                    "parallelism": "single GPU", "p2p_config": tuned},
-        "roofline": {"kernel": "k_p2p<kP2PStep>", "bound": "valu", "achieved": achieved,
+        "roofline": {"kernel": "k_p2p<kP2PStepSort>", "bound": "valu", "achieved": achieved,
                      "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
                      "peak_nonpacked": FP32_PEAK_NONPACKED_TFLOPS,
                      "frac_nonpacked": achieved / FP32_PEAK_NONPACKED_TFLOPS,
@@ -230,6 +291,8 @@ def main():
                      "pairs_per_launch": pairs, "flops_per_pair": P2P_FLOPS_PER_PAIR,
                      "valu_issue_frac": pairs / (p2p_avg_ms * 1e-3 * P2P_PAIR_BOUND_PER_S),
                      "occupied_leaves": int(occ0)},
+        "stages": stages,
+        "sort_sensitivity": sens,
         "cpu_baseline": cpu,
         "accuracy": accuracy,
         "small_disk": small,
@@ -239,6 +302,42 @@ def main():
     }
     s.close()
     print(json.dumps(out), flush=True)
+
+
+def sort_sensitivity(args, wl0):
+    """What a step costs when particles change leaf more often: the same
+    workload at G = 1e4 / N (velocities 100x, about 100x the leaf changes per
+    step, warm-started sort) and at the default G with the cold sort every
+    step (warm start off), both graph-replayed like the headline."""
+    import torch
+
+    from nbfmm_amd import Solver
+    from nbfmm_amd.workloads import two_disk_collision
+
+    out = {}
+    steps = max(10, args.steps // 2)
+    for key, G, warm in (("fast_G_1e4_over_N", 1e4 / wl0.n, True), ("cold_sort", None, False)):
+        wl = two_disk_collision(n=wl0.n, num_level=wl0.num_level, G=G)
+        s = Solver(wl.num_level, wl.n, wl.limits, wl.eps)
+        s.warm_start(warm)
+        x = torch.as_tensor(wl.position, device="cuda").contiguous()
+        p = torch.as_tensor(wl.position_pre, device="cuda").contiguous()
+        w = torch.as_tensor(wl.weight, device="cuda").contiguous()
+        a = torch.zeros_like(x)
+        st = torch.cuda.Stream()
+        with torch.cuda.stream(st):
+            s.run(wl.n, wl.tick, 3, x, p, w, a, use_graph=True)
+        st.synchronize()
+        t0 = time.perf_counter()
+        with torch.cuda.stream(st):
+            s.run(wl.n, wl.tick, steps, x, p, w, a, use_graph=True)
+        st.synchronize()
+        dt = time.perf_counter() - t0
+        out[key] = {"G": wl.G, "warm_start": warm, "steps": steps, "ms_per_step": dt * 1e3 / steps,
+                    "value": wl.n * steps / dt, "flags": int(s.flags())}
+        s.close()
+    log(f"[bench] sort sensitivity {out}")
+    return out
 
 
 def batch_leg(args):
@@ -404,13 +503,25 @@ def adjoint_leg():
     tb /= reps
     gm = leaves[2].grad
     s.close()
+    sig = signal_over_rounding(float(loss.detach()), target)
     out = {"workload": wl.name, "n": n, "num_level": wl.num_level, "steps": steps, "checkpoint_every": K,
+           "G": wl.G, "mass_signal": sig,
            "forward_ms": tf * 1e3, "backward_ms": tb * 1e3, "backward_over_forward": tb / tf,
            "forward_particle_steps_per_s": n * steps / tf,
            "forward_backward_particle_steps_per_s": n * steps / (tf + tb),
            "loss": float(loss.detach()), "grad_mass_finite": bool(torch.isfinite(gm).all())}
     log(f"[bench] adjoint {out}")
     return out
+
+
+def signal_over_rounding(loss, target):
+    """The fit's signal against fp32 rounding: the RMS position mismatch at
+    the starting masses (sqrt of the mean squared distance) over the RMS
+    fp32 spacing (one ulp) of the observed positions."""
+    t = target.detach().float().cpu().numpy()
+    ulp = float(np.sqrt(np.mean(np.spacing(np.abs(t).astype(np.float32)) ** 2)))
+    rms = float(np.sqrt(max(loss, 0.0)))
+    return {"rms_mismatch": rms, "rms_fp32_ulp": ulp, "ratio": rms / ulp if ulp > 0 else None}
 
 
 def batch_adjoint_leg(reps=3):
@@ -424,7 +535,7 @@ def batch_adjoint_leg(reps=3):
 
     from nbfmm_amd import Batch
     from nbfmm_amd.autograd import fmm_batch_trajectory
-    from nbfmm_amd.workloads import many_collisions
+    from nbfmm_amd.workloads import FIT_G_PER_N, many_collisions
 
     bw = many_collisions()
     steps, K, obs = bw.steps, 10, [25, 50, 75, 100]
@@ -455,11 +566,14 @@ def batch_adjoint_leg(reps=3):
     tb /= reps
     b.close()
     total = bw.n
+    # the loss at the starting masses (the last repetition's; all start alike)
+    sig = signal_over_rounding(float(loss.detach()), target)
     res = {"workload": bw.name, "systems": len(bw.systems), "particles": total, "steps": steps,
            "checkpoint_every": K, "repeats": reps, "forward_ms": tf * 1e3, "backward_ms": tb * 1e3,
            "backward_over_forward": tb / tf,
            "forward_backward_particle_steps_per_s": total * steps / (tf + tb), "loss": float(loss.detach()),
-           "grad_mass_finite": bool(torch.isfinite(m.grad).all())}
+           "grad_mass_finite": bool(torch.isfinite(m.grad).all()), "G_per_N": FIT_G_PER_N,
+           "mass_signal": sig}
     log(f"[bench] batched adjoint {res}")
     return res
 

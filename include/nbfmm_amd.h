@@ -164,7 +164,9 @@ int nbfmm_step(nbfmm_solver* solver, int num_particle, float tick, float* d_posi
 int nbfmm_prune(nbfmm_solver* solver, int num_star, float* d_position_cur, float* d_position_pre,
                 float* d_weight, int* num_kept, void* stream);
 
-/* num_steps calls of nbfmm_step.  use_graph = 1 captures one step into a
+/* num_steps calls of nbfmm_step (the same bits; with the warm start on, the
+ * P2P of every step but the last also makes the first pass of the next
+ * step's sort).  use_graph = 1 captures one step into a
  * hipGraph on the first call and replays it.  A graph cannot be captured on
  * the NULL stream: with stream NULL the replays run on a stream the solver
  * makes on first use, ordered after the work already queued on the NULL
@@ -172,13 +174,18 @@ int nbfmm_prune(nbfmm_solver* solver, int num_star, float* d_position_cur, float
 int nbfmm_run(nbfmm_solver* solver, int num_particle, float tick, int num_steps, float* d_position_cur,
               float* d_position_pre, const float* d_weight, float* d_acceleration, int use_graph, void* stream);
 
-/* Stage timing: when enabled, solve/step record HIP events around the P2P
- * kernel on `stream`; nbfmm_p2p_time synchronises and returns the summed
- * milliseconds and the number of timed launches since the last reset.
+/* Stage timing (no reference counterpart): when enabled, solve, step and run
+ * (run then steps eagerly, with the launches its graph would replay) record
+ * HIP events on `stream` around the four stages of each call: the sort, the
+ * upward pass (P2M + M2M), M2L, and P2P (with L2L, L2P, un-permute and the
+ * Verlet epilogue).  nbfmm_stage_times synchronises and returns the summed
+ * milliseconds of each stage (ms[4]) and the number of timed calls since the
+ * last reset; nbfmm_p2p_time returns the P2P stage's sum alone.
  * Pair counts for roofline figures come from nbfmm_pair_count (synchronises;
  * ordered pairs of the P2P stage for the current head, self pairs excluded). */
 int nbfmm_profile_enable(nbfmm_solver* solver, int enable);
 int nbfmm_p2p_time(nbfmm_solver* solver, double* total_ms, int* launches, int reset);
+int nbfmm_stage_times(nbfmm_solver* solver, double* ms, int* calls, int reset);
 int nbfmm_pair_count(nbfmm_solver* solver, int num_particle, double* pairs, int* occupied_cells);
 
 /* ---------------------------------------------------------------- warm start
@@ -187,8 +194,9 @@ int nbfmm_pair_count(nbfmm_solver* solver, int num_particle, double* pairs, int*
  * adjoints start their sort from the solver's previous sort when that was of
  * the same num_particle: particles whose leaf did not change keep their order,
  * and only the others are placed.  The result is the same stable sort, bit
- * for bit; when too many particles changed leaf the cold sort runs instead
- * (decided on the device).  Writing a buffer through nbfmm_buffer_write
+ * for bit.  When a leaf gains more than 512 particles in one call, the cold
+ * sort's passes run instead (decided on the device, inside the same launches
+ * plus one), so a call never costs much more than a cold sort.  Writing a buffer through nbfmm_buffer_write
  * makes the next sort cold.  Enabled by default; disabling it forces the
  * cold sort every call. */
 int nbfmm_set_warm_start(nbfmm_solver* solver, int enable);

@@ -38,6 +38,7 @@ SIGNATURES = {
     "nbfmm_run": [P, c_int, c_float, c_int, P, P, P, P, c_int, P],
     "nbfmm_profile_enable": [P, c_int],
     "nbfmm_p2p_time": [P, ctypes.POINTER(c_double), ctypes.POINTER(c_int), c_int],
+    "nbfmm_stage_times": [P, ctypes.POINTER(c_double), ctypes.POINTER(c_int), c_int],
     "nbfmm_pair_count": [P, c_int, ctypes.POINTER(c_double), ctypes.POINTER(c_int)],
     "nbfmm_batch_create": [c_int, P, P, P, P, ctypes.POINTER(c_void_p)],
     "nbfmm_batch_destroy": [P],

@@ -70,6 +70,11 @@ struct SideStream {
   }
 };
 
+// Stages of the fused pipeline timed in profile mode: sort, upward pass
+// (P2M + M2M), M2L, P2P (with L2L, L2P, un-permute and the Verlet epilogue).
+constexpr int kProfStages = 4;
+constexpr int kProfEvents = kProfStages + 1;
+
 struct nbfmm_solver {
   int L = 0, B = 0, max_n = 0;
   float limits[4] = {0, 0, 0, 0};
@@ -82,12 +87,13 @@ struct nbfmm_solver {
   int p2p_bpc = kDefaultP2PBlocksPerCU;     // P2P blocks per compute unit
   SysView v{};  // arrays and sizes; v.n, v.pos0, v.w0 are set per call
   std::vector<void*> allocs;
-  // profiling of the P2P launch
+  // profiling of the fused pipeline: kProfEvents events per solve/step
+  // (before the sort, after it, after the upward pass, after M2L, after P2P)
   bool profile = false;
-  std::vector<hipEvent_t> ev;  // pairs (start, stop)
+  std::vector<hipEvent_t> ev;
   size_t ev_used = 0;
-  double p2p_ms = 0.0;
-  int p2p_launches = 0;
+  double stage_ms[kProfStages] = {0.0, 0.0, 0.0, 0.0};
+  int prof_steps = 0;
   unsigned long long* d_pairs = nullptr;
   int* d_kept = nullptr;  // star count after a prune
   // sorted states a trajectory backward keeps for the steps it recomputes
@@ -207,23 +213,17 @@ void drop_graph(nbfmm_solver* s) {
   s->gkey.n = -1;
 }
 
-int p2p_timed(nbfmm_solver* s, const SysView& v, int mode, const CallIO& io, hipStream_t st) {
-  hipEvent_t e0 = nullptr, e1 = nullptr;
-  if (s->profile) {
-    if (s->ev_used + 2 > s->ev.size()) {
-      for (int k = 0; k < 64; ++k) {
-        hipEvent_t e;
-        NB_HIP(hipEventCreate(&e));
-        s->ev.push_back(e);
-      }
+// The next group of kProfEvents profiling events (made 64 groups at a time).
+int prof_events(nbfmm_solver* s, hipEvent_t** out) {
+  if (s->ev_used + kProfEvents > s->ev.size()) {
+    for (int k = 0; k < 64 * kProfEvents; ++k) {
+      hipEvent_t e;
+      NB_HIP(hipEventCreate(&e));
+      s->ev.push_back(e);
     }
-    e0 = s->ev[s->ev_used];
-    e1 = s->ev[s->ev_used + 1];
-    s->ev_used += 2;
-    NB_HIP(hipEventRecord(e0, st));
   }
-  NB_HIP(launch_p2p(v, mode, io, s->p2p_grid, s->p2p_variant, st));
-  if (s->profile) NB_HIP(hipEventRecord(e1, st));
+  *out = s->ev.data() + s->ev_used;
+  s->ev_used += kProfEvents;
   return NBFMM_OK;
 }
 
@@ -245,13 +245,23 @@ int run_pipeline(nbfmm_solver* s, int n, float tick, int mode, const CallIO& io,
                  bool gathered = false) {
   SysView v = view(s, n);
   v.tick = tick;
+  hipEvent_t* E = nullptr;
+  if (s->profile && n > 0) {
+    if (int rc = prof_events(s, &E)) return rc;
+    NB_HIP(hipEventRecord(E[0], st));
+  }
   int rc = sort_stage(s, v, io, st, gathered);
   if (rc) return rc;
   if (n == 0) return NBFMM_OK;
+  if (E) NB_HIP(hipEventRecord(E[1], st));
   NB_HIP(launch_upward(v, std::min(n, s->B * s->B), st));  // P2M + M2M
+  if (E) NB_HIP(hipEventRecord(E[2], st));
   NB_HIP(launch_m2l(v, st));
+  if (E) NB_HIP(hipEventRecord(E[3], st));
   // L2L and L2P are folded into the P2P kernel's epilogue
-  return p2p_timed(s, v, mode, io, st);
+  NB_HIP(launch_p2p(v, mode, io, s->p2p_grid, s->p2p_variant, st));
+  if (E) NB_HIP(hipEventRecord(E[4], st));
+  return NBFMM_OK;
 }
 
 }  // namespace
@@ -391,7 +401,9 @@ int nbfmm_predo(nbfmm_solver* s, int n, const float* pos0, const float* w0, void
 int nbfmm_p2p(nbfmm_solver* s, int n, void* stream) {
   int rc = check_n(s, n);
   if (rc) return rc;
-  return p2p_timed(s, view(s, n), kP2POnly, call_io(nullptr, nullptr), as_stream(stream));
+  NB_HIP(launch_p2p(view(s, n), kP2POnly, call_io(nullptr, nullptr), s->p2p_grid, s->p2p_variant,
+                    as_stream(stream)));
+  return NBFMM_OK;
 }
 
 int nbfmm_p2m(nbfmm_solver* s, int n, void* stream) {
@@ -554,10 +566,15 @@ int nbfmm_run(nbfmm_solver* s, int n, float tick, int steps, float* xcur, float*
   if (rc) return rc;
   if (steps < 0) return fail(NBFMM_ERR_INVALID_ARGUMENT, "num_steps < 0");
   hipStream_t st = as_stream(stream);
+  if (n > 0 && (!xcur || !xpre || !w || !acc)) return fail(NBFMM_ERR_INVALID_ARGUMENT, "null array");
   if (!use_graph || s->profile || n == 0) {
+    // eager: the launches of the replayed steps, one by one (with the warm
+    // start on, every step but the last makes the next sort's first pass)
+    CallIO io = call_io(xcur, w, acc, xcur, xpre);
+    const bool chain = s->warm_enabled && n > 0;
     for (int k = 0; k < steps; ++k) {
-      rc = nbfmm_step(s, n, tick, xcur, xpre, w, acc, stream);
-      if (rc) return rc;
+      const int mode = chain && k + 1 < steps ? kP2PStepSort : kP2PStep;
+      if ((rc = run_pipeline(s, n, tick, mode, io, st, chain && k > 0))) return rc;
     }
     return NBFMM_OK;
   }
@@ -903,19 +920,43 @@ int nbfmm_profile_enable(nbfmm_solver* s, int enable) {
   return NBFMM_OK;
 }
 
-int nbfmm_p2p_time(nbfmm_solver* s, double* total_ms, int* launches, int reset) {
-  if (!s) return fail(NBFMM_ERR_INVALID_ARGUMENT, "null solver");
-  for (size_t k = 0; k + 1 < s->ev_used; k += 2) {
-    NB_HIP(hipEventSynchronize(s->ev[k + 1]));
-    float ms = 0.0f;
-    NB_HIP(hipEventElapsedTime(&ms, s->ev[k], s->ev[k + 1]));
-    s->p2p_ms += ms;
-    s->p2p_launches += 1;
+// Folds the recorded event groups into the per-stage sums.
+static int prof_collect(nbfmm_solver* s) {
+  for (size_t k = 0; k + kProfEvents <= s->ev_used; k += kProfEvents) {
+    NB_HIP(hipEventSynchronize(s->ev[k + kProfStages]));
+    for (int j = 0; j < kProfStages; ++j) {
+      float ms = 0.0f;
+      NB_HIP(hipEventElapsedTime(&ms, s->ev[k + j], s->ev[k + j + 1]));
+      s->stage_ms[j] += ms;
+    }
+    s->prof_steps += 1;
   }
   s->ev_used = 0;
-  if (total_ms) *total_ms = s->p2p_ms;
-  if (launches) *launches = s->p2p_launches;
-  if (reset) { s->p2p_ms = 0.0; s->p2p_launches = 0; }
+  return NBFMM_OK;
+}
+
+int nbfmm_p2p_time(nbfmm_solver* s, double* total_ms, int* launches, int reset) {
+  if (!s) return fail(NBFMM_ERR_INVALID_ARGUMENT, "null solver");
+  if (int rc = prof_collect(s)) return rc;
+  if (total_ms) *total_ms = s->stage_ms[kProfStages - 1];
+  if (launches) *launches = s->prof_steps;
+  if (reset) {
+    for (double& m : s->stage_ms) m = 0.0;
+    s->prof_steps = 0;
+  }
+  return NBFMM_OK;
+}
+
+int nbfmm_stage_times(nbfmm_solver* s, double* ms, int* steps, int reset) {
+  if (!s) return fail(NBFMM_ERR_INVALID_ARGUMENT, "null solver");
+  if (int rc = prof_collect(s)) return rc;
+  if (ms)
+    for (int j = 0; j < kProfStages; ++j) ms[j] = s->stage_ms[j];
+  if (steps) *steps = s->prof_steps;
+  if (reset) {
+    for (double& m : s->stage_ms) m = 0.0;
+    s->prof_steps = 0;
+  }
   return NBFMM_OK;
 }
 

@@ -99,7 +99,12 @@ constexpr int kP2MSeg = 16;
 
 // Warm-started sort.
 constexpr uint32_t kMoverBit = 0x80000000u;  // tkey flag: the particle changed leaf
-constexpr int kWarmTouched = 1, kWarmInts = 4;
+// wstate words: [kWarmTouched] touched-leaf count; [kWarmCold] nonzero when
+// a leaf gains more than kMergeLdsMovers movers in this sort, which then
+// runs the cold passes instead of the merge (set by k_warm_rows, cleared by
+// the next sort's first pass)
+constexpr int kWarmTouched = 1, kWarmCold = 2, kWarmInts = 4;
+constexpr int kMergeLdsMovers = 512;  // movers of one leaf the merge sorts in LDS
 
 // Split of heavy P2P runs: a run whose staged union holds more than
 // kP2PSplitMin sources is summed in K = min(kP2PMaxSeg, ceil(U / kP2PSegLen))
@@ -243,26 +248,48 @@ __device__ __forceinline__ uint32_t bin_key(const SysView& v, float2 p, bool& ba
   return (uint32_t)(iy * v.B + ix);
 }
 
+// Adds, for every lane where `act` holds, one to cnt[k] of that lane: one
+// atomic per distinct k among the wave's active lanes (a crowd of particles
+// moving into or out of one leaf would otherwise serialise on one address).
+// All lanes of the wave call it.
+__device__ __forceinline__ void wave_count_add(int* cnt, uint32_t k, bool act) {
+  uint64_t pend = __ballot(act);
+  while (pend) {
+    const int leader = __builtin_ctzll(pend);
+    const uint32_t kl = (uint32_t)__builtin_amdgcn_readlane((int)k, leader);
+    const uint64_t same = __ballot(act && k == kl);
+    if ((threadIdx.x & 63) == (unsigned)leader) atomicAdd(&cnt[kl], (int)__popcll(same));
+    pend &= ~same;
+  }
+}
+
 // First pass of the warm-started sort for the particle at sorted slot i of
 // the current sort (key kold), now at position p with mass m: its entry in
 // the new order's input (tpart, tkey, wrel), and for a particle that changes
-// leaf (a mover) the per-leaf in/out counts (no value returned, so the wave
-// does not wait for them; the next pass marks the leaves they touch).
-// Returns whether p was outside the box.
-// hold = head[kold], the old leaf's first slot.
-__device__ __forceinline__ bool warm_first_pass(const SysView& v, int i, uint32_t kold, int hold, float2 p, float m,
-                                                float w) {
-  bool bad;
-  const uint32_t k = bin_key(v, p, bad);
-  v.tpart[i] = make_float4(p.x, p.y, m, w);
-  if (k == kold) {
-    v.tkey[i] = k;
-    v.wrel[i] = i - hold;
-  } else {
-    v.tkey[i] = k | kMoverBit;
-    atomicAdd(&v.wcout[kold], 1);
-    atomicAdd(&v.wcin[k], 1);
+// leaf (a mover) the per-leaf in/out counts.  Lanes with `valid` false take
+// part only in the counts (all lanes of the wave call it).  Returns whether
+// p was outside the box.  hold = head[kold], the old leaf's first slot.
+__device__ __forceinline__ bool warm_first_pass(const SysView& v, bool valid, int i, uint32_t kold, int hold,
+                                                float2 p, float m, float w) {
+  bool bad = false, mover = false;
+  uint32_t k = 0;
+  if (valid) {
+    k = bin_key(v, p, bad);
+    // the previous sort's cold switch has been read by all its passes
+    if (i == 0) v.wstate[kWarmCold] = 0;
+    v.tpart[i] = make_float4(p.x, p.y, m, w);
+    mover = k != kold;
+    if (!mover) {
+      v.tkey[i] = k;
+      v.wrel[i] = i - hold;
+    } else {
+      v.tkey[i] = k | kMoverBit;
+    }
   }
+  // no value returned, so the wave does not wait for them; the next pass
+  // marks the leaves they touch
+  wave_count_add(v.wcout, kold, mover);
+  wave_count_add(v.wcin, k, mover);
   return bad;
 }
 

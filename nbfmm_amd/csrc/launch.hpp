@@ -39,7 +39,8 @@ struct BatchLaunch {
 hipError_t launch_predo(const SysView& v, const CallIO& io, int zero_empty, hipStream_t s);
 hipError_t launch_predo_batch(const BatchLaunch& b, const CallIO& io, hipStream_t s);
 hipError_t launch_occ_from_head(const SysView& v, hipStream_t s);
-// warm start from the previous sort of the same n (falls back to the cold sort on the device)
+// warm start from the previous sort of the same n; a leaf gaining more than
+// kMergeLdsMovers movers switches it to the cold passes (decided on the device)
 // gathered: the previous step's P2P (kP2PStepSort) made the first pass
 hipError_t launch_predo_warm(const SysView& v, const CallIO& io, int zero_empty, hipStream_t s, bool gathered = false);
 hipError_t launch_predo_warm_batch(const BatchLaunch& b, const CallIO& io, hipStream_t s, bool gathered = false);

@@ -94,6 +94,33 @@ __device__ __forceinline__ void seg_sum(const SysView& v, float4* src, const Run
   }
 }
 
+// L2P, un-permute and the Verlet update of one valid target (step modes):
+// writes the effect, x_cur and x_pre in the caller's order and returns the
+// new position.
+__device__ __forceinline__ float2 p2p_step_update(const SysView& v, const CallIO& io, const RunGeom& g, float ax,
+                                                  float ay) {
+#pragma clang fp contract(off)
+  // l2p.cu:34: effect += cell effect; postdo.cu:32: scatter by perm
+  const float2 loc = v.L > 0 ? leaf_local(v, g.xt, g.y) : make_float2(0.0f, 0.0f);
+  const float ex = ax + loc.x;
+  const float ey = ay + loc.y;
+  const uint32_t o = __float_as_uint(g.pt.w);
+  io.acc0[v.part_off + o] = make_float2(ex, ey);
+  // update.cu:39-41
+  float2* xcur = io.xcur + v.part_off;
+  float2* xpre = io.xpre + v.part_off;
+  // x_cur[o] is the position this step sorted (a step's forces are taken
+  // at x_cur, so the sort read it there and carried it in pt): not read again
+  const float2 x = make_float2(g.pt.x, g.pt.y);
+  const float2 xp = xpre[o];
+  float2 nx;
+  nx.x = 2.0f * x.x - xp.x + ex * v.tick * v.tick;
+  nx.y = 2.0f * x.y - xp.y + ey * v.tick * v.tick;
+  xcur[o] = nx;
+  xpre[o] = x;
+  return nx;
+}
+
 // The targets' epilogue.  kP2POnly stores the sorted effect; kP2PFinal adds
 // the leaf local (L2L folded in, L2P) and scatters to the original order
 // (postdo); kP2PStep also applies the Verlet update; kP2PStepSort then makes
@@ -102,32 +129,26 @@ __device__ __forceinline__ void seg_sum(const SysView& v, float4* src, const Run
 template <int MODE>
 __device__ __forceinline__ void p2p_finish(const SysView& v, const CallIO& io, const RunGeom& g, float ax, float ay) {
 #pragma clang fp contract(off)
+  if (MODE == kP2PStepSort) {
+    // the whole wave takes part in the first pass's mover counts
+    float2 nx = make_float2(0.0f, 0.0f);
+    if (g.valid) nx = p2p_step_update(v, io, g, ax, ay);
+    if (warm_first_pass(v, g.valid, g.i, (uint32_t)(g.y * v.B + g.xt), g.hk, nx, g.pt.z, g.pt.w))
+      atomicOr(v.flags, 1u);
+    return;
+  }
   if (!g.valid) return;
   if (MODE == kP2POnly) {
     v.eff[g.i] = make_float2(ax, ay);
     return;
   }
-  // l2p.cu:34: effect += cell effect; postdo.cu:32: scatter by perm
-  const float2 loc = v.L > 0 ? leaf_local(v, g.xt, g.y) : make_float2(0.0f, 0.0f);
-  const float ex = ax + loc.x;
-  const float ey = ay + loc.y;
-  const uint32_t o = __float_as_uint(g.pt.w);
-  io.acc0[v.part_off + o] = make_float2(ex, ey);
-  if (MODE == kP2PStep || MODE == kP2PStepSort) {
-    // update.cu:39-41
-    float2* xcur = io.xcur + v.part_off;
-    float2* xpre = io.xpre + v.part_off;
-    // x_cur[o] is the position this step sorted (a step's forces are taken
-    // at x_cur, so the sort read it there and carried it in pt): not read again
-    const float2 x = make_float2(g.pt.x, g.pt.y);
-    const float2 xp = xpre[o];
-    float2 nx;
-    nx.x = 2.0f * x.x - xp.x + ex * v.tick * v.tick;
-    nx.y = 2.0f * x.y - xp.y + ey * v.tick * v.tick;
-    xcur[o] = nx;
-    xpre[o] = x;
-    if (MODE == kP2PStepSort && warm_first_pass(v, g.i, (uint32_t)(g.y * v.B + g.xt), g.hk, nx, g.pt.z, g.pt.w))
-      atomicOr(v.flags, 1u);
+  if (MODE == kP2PFinal) {
+    // l2p.cu:34: effect += cell effect; postdo.cu:32: scatter by perm
+    const float2 loc = v.L > 0 ? leaf_local(v, g.xt, g.y) : make_float2(0.0f, 0.0f);
+    const uint32_t o = __float_as_uint(g.pt.w);
+    io.acc0[v.part_off + o] = make_float2(ax + loc.x, ay + loc.y);
+  } else {
+    p2p_step_update(v, io, g, ax, ay);
   }
 }
 

@@ -50,19 +50,18 @@ __device__ __forceinline__ uint64_t lanes_below() {
 
 // ------------------------------------------------------------- row pass
 
-template <class Sys>
-__global__ __launch_bounds__(kSortThreads) void k_row_up(Sys sys, CallIO io) {
-  __shared__ uint32_t h[512];
-  int tile, ntiles;
-  const SysView v = sys.get(tile, ntiles);
+// Row histogram of one tile of 2048 particles in input order, by a block of
+// NT threads (h: 512 words of LDS).  All threads of the block call it.
+template <int NT>
+__device__ __forceinline__ void row_up_tile(const SysView& v, const CallIO& io, int tile, int ntiles, uint32_t* h) {
   const float2* pos0 = io.pos0 + v.part_off;
-  for (int d = threadIdx.x; d < v.B; d += kSortThreads) h[d] = 0;
+  for (int d = threadIdx.x; d < v.B; d += NT) h[d] = 0;
   __syncthreads();
   const int base = tile * kSortTile;
   bool any_bad = false;
 #pragma unroll
-  for (int j = 0; j < kSortWaveItems; ++j) {
-    const int i = base + j * kSortThreads + threadIdx.x;
+  for (int j = 0; j < kSortTile / NT; ++j) {
+    const int i = base + j * NT + threadIdx.x;
     if (i < v.n) {
       bool bad;
       const uint32_t key = bin_key(v, pos0[i], bad);
@@ -72,22 +71,27 @@ __global__ __launch_bounds__(kSortThreads) void k_row_up(Sys sys, CallIO io) {
   }
   if (any_bad) atomicOr(v.flags, 1u);
   __syncthreads();
-  for (int d = threadIdx.x; d < v.B; d += kSortThreads) v.hist[(size_t)d * ntiles + tile] = h[d];
+  for (int d = threadIdx.x; d < v.B; d += NT) v.hist[(size_t)d * ntiles + tile] = h[d];
+  __syncthreads();
 }
 
-// One block per row: exclusive scan of that row's tile counts, in place; the
-// row total goes to dtotal[row].  Block 0 also clears the occupied-cell
-// counter that the column pass fills.
 template <class Sys>
-__global__ __launch_bounds__(kScanThreads) void k_row_scan(Sys sys) {
-  __shared__ uint32_t sh[kScanThreads / kWave];
-  __shared__ uint32_t carry;
-  int d, nrows;
-  const SysView v = sys.get(d, nrows);
+__global__ __launch_bounds__(kSortThreads) void k_row_up(Sys sys, CallIO io) {
+  __shared__ uint32_t h[512];
+  int tile, ntiles;
+  const SysView v = sys.get(tile, ntiles);
+  row_up_tile<kSortThreads>(v, io, tile, ntiles, h);
+}
+
+// Exclusive scan of row d's tile counts, in place, by a block of
+// kScanThreads threads; the row total goes to dtotal[d].  Row 0 also clears
+// the occupied-cell counter that the column pass fills.  All threads of the
+// block call it.
+__device__ __forceinline__ void row_scan_row(const SysView& v, int d, uint32_t* sh, uint32_t* carry) {
   const int ntiles = (v.n + kSortTile - 1) / kSortTile;
   uint32_t* row = v.hist + (size_t)d * ntiles;
   if (threadIdx.x == 0) {
-    carry = 0;
+    *carry = 0;
     if (d == 0) *v.nocc = 0;
   }
   __syncthreads();
@@ -96,26 +100,42 @@ __global__ __launch_bounds__(kScanThreads) void k_row_scan(Sys sys) {
     const uint32_t c = t < ntiles ? row[t] : 0u;
     uint32_t tot;
     const uint32_t e = block_exclusive_scan<kScanThreads>(c, sh, &tot);
-    const uint32_t base = carry;
+    const uint32_t base = *carry;
     if (t < ntiles) row[t] = base + e;
     __syncthreads();
-    if (threadIdx.x == 0) carry = base + tot;
+    if (threadIdx.x == 0) *carry = base + tot;
     __syncthreads();
   }
-  if (threadIdx.x == 0) v.dtotal[d] = carry;
+  if (threadIdx.x == 0) v.dtotal[d] = *carry;
+  __syncthreads();
 }
 
-// Stable scatter of one tile by row.  Wave w owns elements [w*512, w*512+512)
-// of the tile as 8 chunks of 64 in input order; a per-wave LDS counter per row
-// carries the running rank across chunks; per-wave counts are then
-// prefix-summed over waves, so rows keep input order (stable).
+// One block per row.
 template <class Sys>
-__global__ __launch_bounds__(kSortThreads) void k_row_down(Sys sys, CallIO io) {
-  __shared__ uint32_t s_base[512];
-  __shared__ uint32_t s_wcnt[kSortThreads / kWave][512];
-  __shared__ uint32_t sh[kSortThreads / kWave];
-  int tile, ntiles;
-  const SysView v = sys.get(tile, ntiles);
+__global__ __launch_bounds__(kScanThreads) void k_row_scan(Sys sys) {
+  __shared__ uint32_t sh[kScanThreads / kWave];
+  __shared__ uint32_t carry;
+  int d, nrows;
+  const SysView v = sys.get(d, nrows);
+  row_scan_row(v, d, sh, &carry);
+}
+
+// Stable scatter of one tile by row, by a block of kSortThreads threads.
+// Wave w owns elements [w*512, w*512+512) of the tile as 8 chunks of 64 in
+// input order; a per-wave LDS counter per row carries the running rank
+// across chunks; per-wave counts are then prefix-summed over waves, so rows
+// keep input order (stable).  All threads of the block call it.
+struct RowDownLds {
+  uint32_t base[512];
+  uint32_t wcnt[kSortThreads / kWave][512];
+  uint32_t sh[kSortThreads / kWave];
+};
+
+__device__ __forceinline__ void row_down_tile(const SysView& v, const CallIO& io, int tile, int ntiles,
+                                              RowDownLds& L) {
+  uint32_t* s_base = L.base;
+  auto& s_wcnt = L.wcnt;
+  uint32_t* sh = L.sh;
   const float2* pos0 = io.pos0 + v.part_off;
   const float* w0 = io.w0 + v.part_off;
   const int R = v.B;
@@ -188,6 +208,15 @@ __global__ __launch_bounds__(kSortThreads) void k_row_down(Sys sys, CallIO io) {
     v.tkey[p] = key[j];
     v.tpart[p] = make_float4(q[j].x, q[j].y, w0[i], __uint_as_float((uint32_t)i));
   }
+  __syncthreads();
+}
+
+template <class Sys>
+__global__ __launch_bounds__(kSortThreads) void k_row_down(Sys sys, CallIO io) {
+  __shared__ RowDownLds L;
+  int tile, ntiles;
+  const SysView v = sys.get(tile, ntiles);
+  row_down_tile(v, io, tile, ntiles, L);
 }
 
 // ---------------------------------------------------------- column pass
@@ -198,8 +227,10 @@ constexpr int kRowGroup = kRowThreads * kSortWaveItems;  // 4096 particles per g
 
 // One block per row y.  zero_empty: also clear the level-0 multipole of the
 // row's empty cells (the fused solve's P2M then visits occupied cells only).
+// only_if_cold: the column pass of a warm-started sort that switched to the
+// cold passes; returns at once when it did not.
 template <class Sys>
-__global__ __launch_bounds__(kRowThreads) void k_row_sort(Sys sys, int zero_empty) {
+__global__ __launch_bounds__(kRowThreads) void k_row_sort(Sys sys, int zero_empty, int only_if_cold) {
   __shared__ uint32_t s_hist[512];
   __shared__ uint32_t s_start[512];
   __shared__ uint32_t s_run[512];
@@ -208,6 +239,7 @@ __global__ __launch_bounds__(kRowThreads) void k_row_sort(Sys sys, int zero_empt
   __shared__ uint32_t s_occ_base;
   int y, nrows;
   const SysView v = sys.get(y, nrows);
+  if (only_if_cold && v.wstate[kWarmCold] == 0) return;
   const int B = v.B;
   const int lane = lane_id();
   const int wave = threadIdx.x / kWave;
@@ -358,21 +390,34 @@ __global__ __launch_bounds__(256) void k_empty(SysView v) {
 // Each pass clears the counters it is the last to read.  Per-particle
 // atomics are per leaf; an atomic that the whole grid shares serialises
 // across the chip, so the occupied and touched leaf lists take their row
-// bases from scans of per-row counts instead of shared counters.  Any number
-// of movers may enter a leaf, so the warm passes never need the cold sort.
+// bases from scans of per-row counts instead of shared counters.  A leaf that
+// gains more than kMergeLdsMovers movers switches the sort to the cold passes,
+// decided on the device by k_warm_rows: the next kernels then make the cold
+// row pass (histograms in k_warm_finish, scan in k_warm_place, scatter in
+// k_warm_merge) and one more launch of the column pass (k_row_sort, which
+// returns at once when the switch is off), so such a step costs about one
+// cold sort, whatever the number of movers.
 
 template <class Sys>
 __global__ __launch_bounds__(256) void k_warm_gather(Sys sys, CallIO io) {
   int blk, nblk;
   const SysView v = sys.get(blk, nblk);
   const int i = blk * 256 + threadIdx.x;
-  bool bad = false;
-  if (i < v.n) {
-    const float4 old = v.part[i];
+  const bool valid = i < v.n;
+  float4 old = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  uint32_t kold = 0;
+  int hold = 0;
+  float2 p = make_float2(0.0f, 0.0f);
+  float m = 0.0f;
+  if (valid) {
+    old = v.part[i];
     const uint32_t o = __float_as_uint(old.w);
-    const uint32_t kold = v.key[i];
-    bad = warm_first_pass(v, i, kold, v.head[kold], io.pos0[v.part_off + o], io.w0[v.part_off + o], old.w);
+    kold = v.key[i];
+    hold = v.head[kold];
+    p = io.pos0[v.part_off + o];
+    m = io.w0[v.part_off + o];
   }
+  const bool bad = warm_first_pass(v, valid, i, kold, hold, p, m, old.w);
   if (__any(bad) && lane_id() == 0) atomicOr(v.flags, 1u);
 }
 
@@ -404,6 +449,9 @@ __global__ __launch_bounds__(NT) void k_warm_rows(Sys sys) {
     cnt = (uint32_t)(h1 - h0 - co + ci);
     cin = (uint32_t)ci;
   }
+  // a leaf gaining more movers than the merge sorts in LDS switches this
+  // sort to the cold passes (one atomic per row that has such a leaf)
+  if (__syncthreads_or(x < B && cin > (uint32_t)kMergeLdsMovers) && x == 0) atomicOr(&v.wstate[kWarmCold], 1);
   const int nocc = __syncthreads_count(cnt > 0);
   const int ntouch = __syncthreads_count(touched);
   uint32_t tot, itot;
@@ -463,9 +511,10 @@ __global__ __launch_bounds__(NT) void k_warm_rowscan(Sys sys) {
 // touched leaves to wtlist, both in row-major order.  zero_empty: also clear
 // the level-0 multipole of the row's empty leaves.
 template <int NT, class Sys>
-__global__ __launch_bounds__(NT) void k_warm_finish(Sys sys, int zero_empty, int prescanned) {
+__global__ __launch_bounds__(NT) void k_warm_finish(Sys sys, CallIO io, int zero_empty, int prescanned) {
   __shared__ uint32_t sh[NT / kWave];
   __shared__ uint32_t s_base, s_ibase, s_occ_base, s_touch_base;
+  __shared__ uint32_t s_hist[512];
   int y, nrows;
   const SysView v = sys.get(y, nrows);
   const int B = v.B;
@@ -526,25 +575,51 @@ __global__ __launch_bounds__(NT) void k_warm_finish(Sys sys, int zero_empty, int
   const uint32_t et = block_exclusive_scan<NT>(touched, sh, &ntouch);
   if (occ) v.occ[s_occ_base + eo] = c;
   if (touched) v.wtlist[s_touch_base + et] = c;
+  if (v.wstate[kWarmCold]) {
+    // cold switch: the row histograms of the cold row pass (k_row_up), the
+    // system's tiles shared among its row blocks
+    const int ntiles = (v.n + kSortTile - 1) / kSortTile;
+    for (int t = y; t < ntiles; t += nrows) row_up_tile<NT>(v, io, t, ntiles, s_hist);
+  }
 }
 
 // Stayers of untouched leaves go to head[leaf] + their old rank; movers are
 // put into their new leaf's group (slots of the group in any order).
 template <class Sys>
 __global__ __launch_bounds__(256) void k_warm_place(Sys sys) {
+  __shared__ uint32_t sh[kScanThreads / kWave];
+  __shared__ uint32_t carry;
   int blk, nblk;
   const SysView v = sys.get(blk, nblk);
+  if (v.wstate[kWarmCold]) {
+    // cold switch: the row scan of the cold row pass (k_row_scan), the rows
+    // shared among the system's blocks
+    for (int d = blk; d < v.B; d += nblk) row_scan_row(v, d, sh, &carry);
+    return;
+  }
   const int i = blk * 256 + threadIdx.x;
-  if (i >= v.n) return;
+  if (blk * 256 >= v.n) return;  // whole waves below stay for the movers' slots
+  const bool valid = i < v.n;
   // the particle's own entries are loaded together with its key (a stayer,
   // nearly every particle, needs them; a mover's are simply unused)
-  const uint32_t t = v.tkey[i];
-  const int rel = v.wrel[i];
-  const float4 p = v.tpart[i];
-  if (t & kMoverBit) {
-    const uint32_t k = t & ~kMoverBit;
-    v.wmgrp[v.winoff[k] + atomicAdd(&v.wfill[k], 1)] = i;
-  } else if (!v.wtouch[t]) {
+  const uint32_t t = valid ? v.tkey[i] : 0u;
+  const int rel = valid ? v.wrel[i] : 0;
+  const float4 p = valid ? v.tpart[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  const bool mover = valid && (t & kMoverBit);
+  const uint32_t k = t & ~kMoverBit;
+  // a slot in the new leaf's group: one atomic per distinct leaf of the wave
+  uint64_t pend = __ballot(mover);
+  while (pend) {
+    const int leader = __builtin_ctzll(pend);
+    const uint32_t kl = (uint32_t)__builtin_amdgcn_readlane((int)k, leader);
+    const uint64_t same = __ballot(mover && k == kl);
+    int base = 0;
+    if (lane_id() == leader) base = atomicAdd(&v.wfill[kl], (int)__popcll(same));
+    base = __builtin_amdgcn_readlane(base, leader);
+    if (mover && k == kl) v.wmgrp[v.winoff[kl] + base + (int)__popcll(same & lanes_below())] = i;
+    pend &= ~same;
+  }
+  if (valid && !mover && !v.wtouch[t]) {
     const int dst = v.head[t] + rel;
     v.key[dst] = t;
     v.part[dst] = p;
@@ -556,28 +631,48 @@ __device__ __forceinline__ uint32_t orig_of(const float4& p) { return __float_as
 
 // One wave per touched leaf (grid-stride over the list): the leaf's stayers
 // (old slots [first, first+count) whose tkey is still this leaf, already in
-// original-index order) and its incoming movers (any number, grouped at
-// wmgrp[winoff]) are merged by original index; indices are unique, so
-// counting the smaller ones gives each particle's exact slot.  Up to 64
-// movers (one per lane) are ranked in one pass over the stayers.  Up to
-// kMergeLdsMovers movers are sorted by original index in the wave's LDS
-// (bitonic); each stayer then finds the movers below it by binary search
-// (p = their count) and counts itself in H[p], so the stayers below the
-// mover of rank r are H[0] + ... + H[r].  More movers are taken 64 at a
-// time against all the others.  Clears the leaf's touch flag and group
-// count.
-constexpr int kMergeLdsMovers = 512;
+// original-index order) and its incoming movers (grouped at wmgrp[winoff])
+// are merged by original index; indices are unique, so counting the smaller
+// ones gives each particle's exact slot.  Up to 64 movers (one per lane) are
+// ranked in one pass over the stayers.  Up to kMergeLdsMovers movers are
+// sorted by original index in the wave's LDS (bitonic); each stayer then
+// finds the movers below it by binary search (p = their count) and counts
+// itself in H[p], so the stayers below the mover of rank r are
+// H[0] + ... + H[r].  A leaf with more movers never gets here: k_warm_rows
+// switched the sort to the cold passes, and then this kernel only clears the
+// touched leaves' flags and makes the cold row pass's scatter (k_row_down).
+// Clears the leaf's touch flag and group count.
+union MergeLds {
+  struct {
+    uint2 mv[256 / kWave][kMergeLdsMovers];  // (original index, tpart slot), sorted
+    uint32_t h[256 / kWave][kMergeLdsMovers + 1];
+  } m;
+  RowDownLds d;
+};
 
 template <class Sys>
-__global__ __launch_bounds__(256) void k_warm_merge(Sys sys) {
-  __shared__ uint2 s_mv[256 / kWave][kMergeLdsMovers];    // (original index, tpart slot), sorted
-  __shared__ uint32_t s_h[256 / kWave][kMergeLdsMovers + 1];
+__global__ __launch_bounds__(256) void k_warm_merge(Sys sys, CallIO io) {
+  __shared__ MergeLds lds;
+  auto& s_mv = lds.m.mv;
+  auto& s_h = lds.m.h;
   int blk, nblk;
   const SysView v = sys.get(blk, nblk);
   const int lane = lane_id();
   const int wave = (int)(threadIdx.x / kWave);
   const int waves = nblk * (256 / kWave);
   const int nt = v.wstate[kWarmTouched];
+  if (v.wstate[kWarmCold]) {
+    for (int d = blk * (256 / kWave) + wave; d < nt; d += waves) {
+      const int c = v.wtlist[d];
+      if (lane == 0) {
+        v.wtouch[c] = 0;
+        v.wfill[c] = 0;
+      }
+    }
+    const int ntiles = (v.n + kSortTile - 1) / kSortTile;
+    for (int t = blk; t < ntiles; t += nblk) row_down_tile(v, io, t, ntiles, lds.d);
+    return;
+  }
   const uint64_t below = lanes_below();
   const int cmax = v.B * v.B - 1;  // wtlist holds at most B * B leaves
   int d = blk * (256 / kWave) + wave;
@@ -591,7 +686,7 @@ __global__ __launch_bounds__(256) void k_warm_merge(Sys sys) {
     const int h = v.head[c];
     const int ni = v.wfill[c];
     const int* grp = v.wmgrp + v.winoff[c];
-    if (ni > kWave && ni <= kMergeLdsMovers) {
+    if (ni > kWave) {  // ni <= kMergeLdsMovers: a leaf with more switched the sort to cold
       uint2* mv = s_mv[wave];
       uint32_t* H = s_h[wave];
       int P = kWave;
@@ -661,7 +756,7 @@ __global__ __launch_bounds__(256) void k_warm_merge(Sys sys) {
         carry = __shfl(inc, kWave - 1, kWave);
       }
       wave_lds_order();
-    } else if (ni <= kWave) {
+    } else {
       const int ms = lane < ni ? grp[lane] : 0;
       const uint32_t om = lane < ni ? orig_of(v.tpart[ms]) : 0xffffffffu;
       int rank_m = 0, cnt_m = 0;
@@ -691,56 +786,6 @@ __global__ __launch_bounds__(256) void k_warm_merge(Sys sys) {
         v.key[dst] = c;
         v.part[dst] = v.tpart[ms];
       }
-    } else {
-      // stayers: their rank among the stayers plus the movers below them
-      int before = 0;
-      for (int b = 0; b < od.y; b += kWave) {
-        const int slot = od.x + b + lane;
-        const bool stay = b + lane < od.y && v.tkey[slot] == c;
-        const uint32_t os = stay ? orig_of(v.tpart[slot]) : 0u;
-        int cnt_s = 0;
-        for (int m1 = 0; m1 < ni; m1 += kWave) {
-          const uint32_t o2 = m1 + lane < ni ? orig_of(v.tpart[grp[m1 + lane]]) : 0xffffffffu;
-          const int cnt = min(kWave, ni - m1);
-          for (int l = 0; l < cnt; ++l) cnt_s += __shfl(o2, l, kWave) < os ? 1 : 0;
-        }
-        const uint64_t sm = __ballot(stay);
-        if (stay) {
-          const int dst = h + before + __popcll(sm & below) + cnt_s;
-          v.key[dst] = c;
-          v.part[dst] = v.tpart[slot];
-        }
-        before += __popcll(sm);
-      }
-      // movers, 64 at a time: the movers and the stayers below each
-      for (int m0 = 0; m0 < ni; m0 += kWave) {
-        const bool mv = m0 + lane < ni;
-        const int ms = mv ? grp[m0 + lane] : 0;
-        const uint32_t om = mv ? orig_of(v.tpart[ms]) : 0xffffffffu;
-        const int cm = min(kWave, ni - m0);
-        int rank = 0;
-        for (int m1 = 0; m1 < ni; m1 += kWave) {
-          const uint32_t o2 = m1 + lane < ni ? orig_of(v.tpart[grp[m1 + lane]]) : 0xffffffffu;
-          const int cnt = min(kWave, ni - m1);
-          for (int l = 0; l < cnt; ++l) rank += __shfl(o2, l, kWave) < om ? 1 : 0;
-        }
-        for (int b = 0; b < od.y; b += kWave) {
-          const int slot = od.x + b + lane;
-          const bool stay = b + lane < od.y && v.tkey[slot] == c;
-          const uint32_t os = stay ? orig_of(v.tpart[slot]) : 0u;
-          for (int l = 0; l < cm; ++l) {
-            // every lane takes part in the shuffle: lane l need not be a stayer
-            const uint32_t o = __shfl(om, l, kWave);
-            const int lt = __popcll(__ballot(stay && os < o));
-            if (lane == l) rank += lt;
-          }
-        }
-        if (mv) {
-          const int dst = h + rank;
-          v.key[dst] = c;
-          v.part[dst] = v.tpart[ms];
-        }
-      }
     }
     if (lane == 0) {
       v.wtouch[c] = 0;
@@ -761,7 +806,7 @@ hipError_t launch_predo(const SysView& v, const CallIO& io, int zero_empty, hipS
   hipLaunchKernelGGL(k_row_up<OneSys>, dim3(ntiles), dim3(kSortThreads), 0, s, one, io);
   hipLaunchKernelGGL(k_row_scan<OneSys>, dim3(v.B), dim3(kScanThreads), 0, s, one);
   hipLaunchKernelGGL(k_row_down<OneSys>, dim3(ntiles), dim3(kSortThreads), 0, s, one, io);
-  hipLaunchKernelGGL(k_row_sort<OneSys>, dim3(v.B), dim3(kRowThreads), 0, s, one, zero_empty);
+  hipLaunchKernelGGL(k_row_sort<OneSys>, dim3(v.B), dim3(kRowThreads), 0, s, one, zero_empty, 0);
   return hipGetLastError();
 }
 
@@ -769,7 +814,7 @@ hipError_t launch_predo_batch(const BatchLaunch& b, const CallIO& io, hipStream_
   hipLaunchKernelGGL(k_row_up<ManySys>, dim3(b.tiles.blocks), dim3(kSortThreads), 0, s, b.tiles.sys, io);
   hipLaunchKernelGGL(k_row_scan<ManySys>, dim3(b.rows.blocks), dim3(kScanThreads), 0, s, b.rows.sys);
   hipLaunchKernelGGL(k_row_down<ManySys>, dim3(b.tiles.blocks), dim3(kSortThreads), 0, s, b.tiles.sys, io);
-  hipLaunchKernelGGL(k_row_sort<ManySys>, dim3(b.rows.blocks), dim3(kRowThreads), 0, s, b.rows.sys, 1);
+  hipLaunchKernelGGL(k_row_sort<ManySys>, dim3(b.rows.blocks), dim3(kRowThreads), 0, s, b.rows.sys, 1, 0);
   return hipGetLastError();
 }
 
@@ -802,12 +847,19 @@ static void launch_warm_rowscan(int nt, dim3 g, const Sys& sys, hipStream_t s) {
 }
 
 template <class Sys>
-static void launch_warm_finish(int nt, dim3 g, const Sys& sys, int zero_empty, int prescanned, hipStream_t s) {
+static void launch_warm_finish(int nt, dim3 g, const Sys& sys, const CallIO& io, int zero_empty, int prescanned,
+                               hipStream_t s) {
   switch (nt) {
-    case 64: hipLaunchKernelGGL((k_warm_finish<64, Sys>), g, dim3(64), 0, s, sys, zero_empty, prescanned); break;
-    case 128: hipLaunchKernelGGL((k_warm_finish<128, Sys>), g, dim3(128), 0, s, sys, zero_empty, prescanned); break;
-    case 256: hipLaunchKernelGGL((k_warm_finish<256, Sys>), g, dim3(256), 0, s, sys, zero_empty, prescanned); break;
-    default: hipLaunchKernelGGL((k_warm_finish<512, Sys>), g, dim3(512), 0, s, sys, zero_empty, prescanned); break;
+    case 64: hipLaunchKernelGGL((k_warm_finish<64, Sys>), g, dim3(64), 0, s, sys, io, zero_empty, prescanned); break;
+    case 128:
+      hipLaunchKernelGGL((k_warm_finish<128, Sys>), g, dim3(128), 0, s, sys, io, zero_empty, prescanned);
+      break;
+    case 256:
+      hipLaunchKernelGGL((k_warm_finish<256, Sys>), g, dim3(256), 0, s, sys, io, zero_empty, prescanned);
+      break;
+    default:
+      hipLaunchKernelGGL((k_warm_finish<512, Sys>), g, dim3(512), 0, s, sys, io, zero_empty, prescanned);
+      break;
   }
 }
 
@@ -818,10 +870,12 @@ hipError_t launch_predo_warm(const SysView& v, const CallIO& io, int zero_empty,
   const int nb = cdiv(v.n, 256);
   if (!gathered) hipLaunchKernelGGL(k_warm_gather<OneSys>, dim3(nb), dim3(256), 0, s, one, io);
   launch_warm_rows(row_threads(v.B), dim3(v.B), one, s);
-  launch_warm_finish(row_threads(v.B), dim3(v.B), one, zero_empty, 0, s);
+  launch_warm_finish(row_threads(v.B), dim3(v.B), one, io, zero_empty, 0, s);
   hipLaunchKernelGGL(k_warm_place<OneSys>, dim3(nb), dim3(256), 0, s, one);
   hipLaunchKernelGGL(k_warm_merge<OneSys>, dim3(cdiv(std::min(kWarmMergeWaves, v.B * v.B), 256 / kWave)),
-                     dim3(256), 0, s, one);
+                     dim3(256), 0, s, one, io);
+  // the column pass, if the sort switched to the cold passes
+  hipLaunchKernelGGL(k_row_sort<OneSys>, dim3(v.B), dim3(kRowThreads), 0, s, one, zero_empty, 1);
   return hipGetLastError();
 }
 
@@ -831,9 +885,10 @@ hipError_t launch_predo_warm_batch(const BatchLaunch& b, const CallIO& io, hipSt
   const int nt = row_threads(b.max_B);
   launch_warm_rows(nt, dim3(b.rows.blocks), b.rows.sys, s);
   launch_warm_rowscan(nt, dim3(b.one.blocks), b.one.sys, s);
-  launch_warm_finish(nt, dim3(b.rows.blocks), b.rows.sys, 1, 1, s);
+  launch_warm_finish(nt, dim3(b.rows.blocks), b.rows.sys, io, 1, 1, s);
   hipLaunchKernelGGL(k_warm_place<ManySys>, dim3(b.adj_sort.blocks), dim3(256), 0, s, b.adj_sort.sys);
-  hipLaunchKernelGGL(k_warm_merge<ManySys>, dim3(b.warm_merge.blocks), dim3(256), 0, s, b.warm_merge.sys);
+  hipLaunchKernelGGL(k_warm_merge<ManySys>, dim3(b.warm_merge.blocks), dim3(256), 0, s, b.warm_merge.sys, io);
+  hipLaunchKernelGGL(k_row_sort<ManySys>, dim3(b.rows.blocks), dim3(kRowThreads), 0, s, b.rows.sys, 1, 1);
   return hipGetLastError();
 }
 

@@ -235,6 +235,14 @@ class Solver:
         check(lib().nbfmm_p2p_time(self._h, ctypes.byref(ms), ctypes.byref(n), int(bool(reset))))
         return ms.value, n.value
 
+    def stage_times(self, reset: bool = True):
+        """Summed milliseconds of the sort, upward (P2M+M2M), M2L and P2P
+        stages of the calls timed since the last reset, and their number."""
+        ms = (ctypes.c_double * 4)()
+        n = ctypes.c_int()
+        check(lib().nbfmm_stage_times(self._h, ms, ctypes.byref(n), int(bool(reset))))
+        return list(ms), n.value
+
     def pair_count(self, num_particle):
         pairs = ctypes.c_double()
         occ = ctypes.c_int()

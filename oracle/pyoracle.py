@@ -19,6 +19,50 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
 _lib = None
+_isa = None
+
+# x86-64 ISA levels of the builds (oracle/Makefile) and the CPU flags each needs
+_ISA_BUILDS = [
+    ("x86-64-v4", "liboracle_v4.so", {"avx512f", "avx512bw", "avx512cd", "avx512dq", "avx512vl"}),
+    ("x86-64-v3", "liboracle_v3.so", {"avx2", "fma", "bmi2", "movbe"}),
+    ("x86-64-v2", "liboracle.so", set()),
+]
+
+
+def _cpu_flags():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("flags"):
+                return set(line.split(":", 1)[1].split())
+    except OSError:
+        pass
+    return set()
+
+
+def cpu_model() -> str:
+    """The host CPU's model name (for the CPU baseline's record)."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _pick():
+    flags = _cpu_flags()
+    for isa, name, need in _ISA_BUILDS:
+        path = os.path.join(_HERE, "_build", name)
+        if need <= flags and os.path.exists(path):
+            return isa, path
+    return "x86-64-v2", _LIB_PATH
+
+
+def isa() -> str:
+    """ISA level of the loaded build."""
+    lib()
+    return _isa
 
 _f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
@@ -32,11 +76,12 @@ def build() -> str:
 
 
 def lib():
-    global _lib
+    global _lib, _isa
     if _lib is None:
         if not os.path.exists(_LIB_PATH):
             build()
-        L = ctypes.CDLL(_LIB_PATH)
+        _isa, path = _pick()
+        L = ctypes.CDLL(path)
         c_int, c_float, c_double = ctypes.c_int, ctypes.c_float, ctypes.c_double
         L.orc_max_threads.restype = c_int
         L.orc_set_threads.argtypes = [c_int]

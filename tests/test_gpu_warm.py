@@ -241,3 +241,71 @@ def test_batch_warm_equals_cold():
     assert torch.equal(Aw, Ac) and torch.equal(Xw, Xc) and torch.equal(Pw, Pc)
     warm.close()
     cold.close()
+
+
+def _jump_state(wl, count, rng):
+    """x_pre such that the first Verlet step (tiny tick, so the forces barely
+    matter) carries `count` particles into one leaf and leaves the rest."""
+    x = wl.position.astype(np.float64)
+    target = x.copy()
+    target[:count] = x[0] + 1e-6 * rng.random((count, 2))
+    pre = 2 * x - target
+    return wl.position, pre.astype(np.float32)
+
+
+@pytest.mark.parametrize("count", [600, 5000])
+def test_graph_run_switches_to_cold_inside_the_chain(count):
+    """A replayed run whose second step finds more than 512 movers entering
+    one leaf: that step's warm-started sort switches to the cold passes on the
+    device (inside the captured graph), and every step still equals a solver
+    that sorts cold, bit for bit."""
+    from nbfmm_amd.workloads import disk
+    wl = disk(n=20000)
+    rng = np.random.default_rng(31)
+    x, pre = _jump_state(wl, count, rng)
+    warm, cold = pair(wl)
+    tick = 1e-6
+    Xw, Pw, W = dev(x), dev(pre), dev(wl.weight)
+    Xc, Pc = Xw.clone(), Pw.clone()
+    Aw, Ac = torch.zeros_like(Xw), torch.zeros_like(Xw)
+    st = torch.cuda.Stream()
+    for steps in (2, 3):  # the second step of the first run sorts the crowd
+        with torch.cuda.stream(st):
+            warm.run(wl.n, tick, steps, Xw, Pw, W, Aw, use_graph=True)
+            for _ in range(steps):
+                cold.step(wl.n, tick, Xc, Pc, W, Ac)
+        st.synchronize()
+        assert torch.equal(Aw, Ac) and torch.equal(Xw, Xc) and torch.equal(Pw, Pc), steps
+        assert_same_state(warm, cold, wl.n)
+        if steps == 2:
+            # the crowded leaf holds the jumpers
+            head = warm.read_buffer("head").cpu().numpy()
+            assert np.diff(head).max() >= count
+    warm.close()
+    cold.close()
+
+
+def test_batch_switches_to_cold_for_one_system():
+    """In a batch, one system's leaf gains thousands of movers (its sort
+    switches to the cold passes) while the others stay warm: bitwise equal to
+    a batch that sorts cold."""
+    from nbfmm_amd.workloads import many_disks
+    bw = many_disks(num_systems=4, n_min=3000, n_max=30000, seed=19, steps=3)
+    warm, cold = _batch_pair(bw)
+    x, p, m = bw.concat()
+    off = warm.offsets
+    rng = np.random.default_rng(6)
+    x2 = x.copy()
+    k = 2
+    n2 = (off[k + 1] - off[k]) // 2
+    x2[off[k]:off[k] + n2] = x[off[k]] + 1e-6 * rng.random((n2, 2)).astype(np.float32)
+    M = dev(m)
+    for xi in (x, x2, x):
+        aw, ac = torch.zeros((x.shape[0], 2), device="cuda"), torch.zeros((x.shape[0], 2), device="cuda")
+        warm.solve(dev(xi), M, aw)
+        cold.solve(dev(xi), M, ac)
+        torch.cuda.synchronize()
+        assert torch.equal(aw, ac)
+    assert not warm.flags().any()
+    warm.close()
+    cold.close()
