@@ -506,7 +506,8 @@ hipError_t launch_adjoint_batch(const BatchLaunch& bl, const CallIO& io, int mod
                        bl.m2m_tile.sys);
   if (bl.m2m_top.blocks)
     hipLaunchKernelGGL(k_adj_up_top<ManySys>, dim3(bl.m2m_top.blocks), dim3(1024), 0, s, bl.m2m_top.sys);
-  if (bl.m2l.blocks) hipLaunchKernelGGL(k_adj_m2l<ManySys>, dim3(bl.m2l.blocks), dim3(256), 0, s, bl.m2l.sys);
+  if (bl.adj_m2l.blocks)
+    hipLaunchKernelGGL(k_adj_m2l<ManySys>, dim3(bl.adj_m2l.blocks), dim3(256), 0, s, bl.adj_m2l.sys);
   if (mode == kAdjSolve)
     hipLaunchKernelGGL((k_adj_p2p<kAdjSolve, ManySys>), dim3(bl.adj_p2p.blocks), dim3(kAdjP2PWaves * kWave), 0, s, bl.adj_p2p.sys,
                        io);

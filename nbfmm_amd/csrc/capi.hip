@@ -254,9 +254,9 @@ int run_pipeline(nbfmm_solver* s, int n, float tick, int mode, const CallIO& io,
   if (rc) return rc;
   if (n == 0) return NBFMM_OK;
   if (E) NB_HIP(hipEventRecord(E[1], st));
-  NB_HIP(launch_upward(v, std::min(n, s->B * s->B), st));  // P2M + M2M
+  NB_HIP(launch_upward(v, st, false));  // P2M + M2M of the tiles
   if (E) NB_HIP(hipEventRecord(E[2], st));
-  NB_HIP(launch_m2l(v, st));
+  NB_HIP(launch_m2l_fused(v, st));  // the M2M levels above the tiles + M2L
   if (E) NB_HIP(hipEventRecord(E[3], st));
   // L2L and L2P are folded into the P2P kernel's epilogue
   NB_HIP(launch_p2p(v, mode, io, s->p2p_grid, s->p2p_variant, st));
@@ -629,7 +629,7 @@ static int adjoint_prepare(nbfmm_solver* s, int n, const CallIO& io, hipStream_t
   const SysView v = view(s, n);
   const int rc = sort_stage(s, v, io, st);
   if (rc) return rc;
-  if (n > 0) NB_HIP(launch_upward(v, std::min(n, s->B * s->B), st));
+  if (n > 0) NB_HIP(launch_upward(v, st, true));
   return NBFMM_OK;
 }
 
@@ -862,8 +862,8 @@ int nbfmm_autotune(nbfmm_solver* s, int n, const float* pos0, const float* w0, i
   const SysView v = view(s, n);
   const CallIO io = call_io(pos0, w0);
   if ((rc = sort_stage(s, v, io, st))) return rc;
-  NB_HIP(launch_upward(v, std::min(n, s->B * s->B), st));
-  NB_HIP(launch_m2l(v, st));
+  NB_HIP(launch_upward(v, st, false));
+  NB_HIP(launch_m2l_fused(v, st));
   // Time the step kernel itself (L2L walk, scatter and Verlet epilogue
   // included) with its outputs pointed at scratch that the next predo
   // rewrites: accelerations into eff, x and x_pre into the two halves of the
@@ -1165,7 +1165,7 @@ int batch_balance_once(nbfmm_batch* b, hipStream_t st) {
 int batch_pipeline(nbfmm_batch* b, int mode, const CallIO& io, hipStream_t st, bool gathered = false) {
   if (int rc = batch_sort(b, io, st, gathered)) return rc;
   if (int rc = batch_balance_once(b, st)) return rc;
-  NB_HIP(launch_tree_batch(b->bl, st));
+  NB_HIP(launch_tree_batch(b->bl, st, true));
   NB_HIP(launch_p2p_batch(b->bl, mode, io, b->p2p_variant, st));
   return NBFMM_OK;
 }
@@ -1354,16 +1354,19 @@ int nbfmm_batch_create(int S, const int* num_particle, const int* num_level, con
   hipError_t e = hipMemcpy(b->ddesc, b->hdesc.data(), S * sizeof(SysDesc), hipMemcpyHostToDevice);
   if (e != hipSuccess) { batch_free(b); delete b; return hip_fail(e, "hipMemcpy system descriptors"); }
   // launch maps
-  std::vector<int> tiles(S), rows(S), p2m(S), mtile(S), mtop(S), m2l(S), asort(S), azero(S), wmerge(S), one(S, 1);
+  std::vector<int> tiles(S), rows(S), p2m(S), up(S), mtile(S), mtop(S), m2l(S), am2l(S), asort(S), azero(S),
+      wmerge(S), one(S, 1);
   for (int s = 0; s < S; ++s) {
     const SysView& v = b->hv[s];
     const int C = v.B * v.B;
     tiles[s] = cdiv(v.n, kSortTile);
     rows[s] = v.B;
     p2m[s] = v.L > 0 ? p2m_leaf_blocks(v.B) : 0;  // also the adjoint leaf kernel's grid
+    up[s] = up_blocks(v.L, v.B);
     mtile[s] = upward_tile_blocks(v.L, v.B);
     mtop[s] = upward_top_blocks(v.L, v.B);
-    m2l[s] = m2l_blocks(v);
+    m2l[s] = m2l_fused_blocks(v);
+    am2l[s] = m2l_blocks(v);
     asort[s] = cdiv(v.n, 256);
     azero[s] = adj_zero_blocks(v);
     // one wave per 64 particles (touched leaves are about twice the movers;
@@ -1373,6 +1376,7 @@ int nbfmm_batch_create(int S, const int* num_particle, const int* num_level, con
   }
   if ((rc = make_map(b, tiles, &b->bl.tiles)) || (rc = make_map(b, rows, &b->bl.rows)) ||
       (rc = make_map(b, p2m, &b->bl.p2m)) || (rc = make_map(b, mtile, &b->bl.m2m_tile)) ||
+      (rc = make_map(b, up, &b->bl.up)) || (rc = make_map(b, am2l, &b->bl.adj_m2l)) ||
       (rc = make_map(b, mtop, &b->bl.m2m_top)) || (rc = make_map(b, m2l, &b->bl.m2l)) ||
       (rc = make_map(b, asort, &b->bl.adj_sort)) || (rc = make_map(b, azero, &b->bl.adj_zero)) ||
       (rc = make_map(b, wmerge, &b->bl.warm_merge)) || (rc = make_map(b, one, &b->bl.one)) ||
@@ -1492,7 +1496,7 @@ int nbfmm_batch_solve_adjoint(nbfmm_batch* b, const float* pos0, const float* w0
   io.gscale = 1.0f;
   if (int rc = batch_sort(b, io, st)) return rc;
   if (int rc = batch_balance_once(b, st)) return rc;
-  NB_HIP(launch_tree_batch(b->bl, st));
+  NB_HIP(launch_tree_batch(b->bl, st, false));
   NB_HIP(launch_adjoint_batch(b->bl, io, 0, st));
   return NBFMM_OK;
 }
@@ -1509,7 +1513,7 @@ int nbfmm_batch_step_adjoint(nbfmm_batch* b, const float* x, const float* w, flo
   io.gscale = -1.0f;  // each system's tick^2
   if (int rc = batch_sort(b, io, st)) return rc;
   if (int rc = batch_balance_once(b, st)) return rc;
-  NB_HIP(launch_tree_batch(b->bl, st));
+  NB_HIP(launch_tree_batch(b->bl, st, false));
   NB_HIP(launch_adjoint_batch(b->bl, io, 1, st));
   return NBFMM_OK;
 }
@@ -1546,7 +1550,7 @@ int nbfmm_batch_step_adjoint_kept(nbfmm_batch* b, int slot, float* lam, float* p
   const nbfmm_batch::SortSlot& sl = b->slots[slot];
   // the adjoint's launch maps with every system's sorted arrays in the slot
   BatchLaunch bl = b->bl;
-  for (KernelMap* km : {&bl.adj_sort, &bl.adj_zero, &bl.p2m, &bl.m2m_tile, &bl.m2m_top, &bl.m2l, &bl.adj_p2p}) {
+  for (KernelMap* km : {&bl.adj_sort, &bl.adj_zero, &bl.p2m, &bl.m2m_tile, &bl.m2m_top, &bl.adj_m2l, &bl.adj_p2p}) {
     km->sys.base.part = sl.part;
     km->sys.base.key = sl.key;
     km->sys.base.head = sl.head;

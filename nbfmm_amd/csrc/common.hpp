@@ -99,11 +99,12 @@ constexpr int kP2MSeg = 16;
 
 // Warm-started sort.
 constexpr uint32_t kMoverBit = 0x80000000u;  // tkey flag: the particle changed leaf
-// wstate words: [kWarmTouched] touched-leaf count; [kWarmCold] nonzero when
-// a leaf gains more than kMergeLdsMovers movers in this sort, which then
-// runs the cold passes instead of the merge (set by k_warm_rows, cleared by
-// the next sort's first pass)
-constexpr int kWarmTouched = 1, kWarmCold = 2, kWarmInts = 4;
+// wstate words: [kWarmTouched] touched-leaf count; [kWarmColdReq] set by the
+// first pass when a leaf's mover count passes kMergeLdsMovers (cleared by
+// k_warm_finish, after k_warm_rows has read it); [kWarmCold] k_warm_rows'
+// copy of it, which the later passes of the same sort read: the sort then
+// takes the cold passes instead of the merge
+constexpr int kWarmTouched = 1, kWarmCold = 2, kWarmColdReq = 3, kWarmInts = 4;
 constexpr int kMergeLdsMovers = 512;  // movers of one leaf the merge sorts in LDS
 
 // Split of heavy P2P runs: a run whose staged union holds more than
@@ -248,18 +249,49 @@ __device__ __forceinline__ uint32_t bin_key(const SysView& v, float2 p, bool& ba
   return (uint32_t)(iy * v.B + ix);
 }
 
-// Adds, for every lane where `act` holds, one to cnt[k] of that lane: one
-// atomic per distinct k among the wave's active lanes (a crowd of particles
-// moving into or out of one leaf would otherwise serialise on one address).
-// All lanes of the wave call it.
-__device__ __forceinline__ void wave_count_add(int* cnt, uint32_t k, bool act) {
-  uint64_t pend = __ballot(act);
-  while (pend) {
-    const int leader = __builtin_ctzll(pend);
-    const uint32_t kl = (uint32_t)__builtin_amdgcn_readlane((int)k, leader);
-    const uint64_t same = __ballot(act && k == kl);
-    if ((threadIdx.x & 63) == (unsigned)leader) atomicAdd(&cnt[kl], (int)__popcll(same));
-    pend &= ~same;
+// Lanes of the wave whose `d` (nbits wide) equals this lane's, among the
+// lanes where `valid` holds.
+__device__ __forceinline__ uint64_t match_mask(uint32_t d, bool valid, int nbits) {
+  uint64_t m = __ballot(valid);
+  for (int b = 0; b < nbits; ++b) {
+    const bool bit = (d >> b) & 1u;
+    const uint64_t bal = __ballot(valid && bit);
+    m &= bit ? bal : ~bal;
+  }
+  return m;
+}
+
+__device__ __forceinline__ uint64_t lanes_below() {
+  const int lane = (int)(threadIdx.x & (kWave - 1));
+  return lane == 0 ? 0ull : (~0ull >> (kWave - lane));
+}
+
+
+// Adds, for every lane where `act` holds, one to cnt[k] of that lane (k of
+// nbits bits).  A wave with few active lanes adds one per lane; one with many
+// (a crowd of particles moving into or out of one leaf together, which
+// would otherwise serialise on one address) adds once per distinct k, all
+// leaders in one instruction.  limit >= 0: the add that takes a count past
+// `limit` also sets *over (those atomics return their old value).  All lanes
+// of the wave call it.
+__device__ __forceinline__ void wave_count_add(int* cnt, uint32_t k, bool act, int nbits, int limit = -1,
+                                               int* over = nullptr) {
+  const uint64_t any = __ballot(act);
+  if (any == 0) return;
+  int add = 1;
+  bool go = act;
+  if (__popcll(any) > 8) {
+    const uint64_t m = match_mask(k, act, nbits);  // this lane's group (empty for inactive lanes)
+    go = act && (threadIdx.x & (kWave - 1)) == (unsigned)__builtin_ctzll(m);
+    add = (int)__popcll(m);
+  }
+  if (go) {
+    if (limit >= 0) {
+      const int old = atomicAdd(&cnt[k], add);
+      if (old <= limit && old + add > limit) atomicOr(over, 1);
+    } else {
+      atomicAdd(&cnt[k], add);
+    }
   }
 }
 
@@ -275,8 +307,6 @@ __device__ __forceinline__ bool warm_first_pass(const SysView& v, bool valid, in
   uint32_t k = 0;
   if (valid) {
     k = bin_key(v, p, bad);
-    // the previous sort's cold switch has been read by all its passes
-    if (i == 0) v.wstate[kWarmCold] = 0;
     v.tpart[i] = make_float4(p.x, p.y, m, w);
     mover = k != kold;
     if (!mover) {
@@ -286,10 +316,10 @@ __device__ __forceinline__ bool warm_first_pass(const SysView& v, bool valid, in
       v.tkey[i] = k | kMoverBit;
     }
   }
-  // no value returned, so the wave does not wait for them; the next pass
-  // marks the leaves they touch
-  wave_count_add(v.wcout, kold, mover);
-  wave_count_add(v.wcin, k, mover);
+  // the next pass marks the leaves they touch; a leaf whose incoming count
+  // passes kMergeLdsMovers switches this sort to the cold passes
+  wave_count_add(v.wcout, kold, mover, 2 * v.lgB);
+  wave_count_add(v.wcin, k, mover, 2 * v.lgB, kMergeLdsMovers, &v.wstate[kWarmColdReq]);
   return bad;
 }
 

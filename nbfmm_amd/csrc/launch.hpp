@@ -20,10 +20,12 @@ struct KernelMap {
 struct BatchLaunch {
   KernelMap tiles;     // row pass: cdiv(n_s, kSortTile) per system
   KernelMap rows;      // row scan / column pass: B_s per system
-  KernelMap p2m;       // P2M over all leaves: p2m_leaf_blocks(B_s)
-  KernelMap m2m_tile;  // M2M tiles
+  KernelMap p2m;       // one thread per leaf: p2m_leaf_blocks(B_s) (the adjoint's leaf kernel)
+  KernelMap up;        // upward pass: one block per tile of leaves
+  KernelMap m2m_tile;  // adjoint upward tiles
   KernelMap m2m_top;   // M2M top levels
-  KernelMap m2l;       // M2L: cdiv(tree cells, 256)
+  KernelMap m2l;       // fused M2L: m2l_fused_blocks
+  KernelMap adj_m2l;   // adjoint M2L: cdiv(tree cells, 256)
   KernelMap p2p;       // P2P: share of the P2P grid (blocks of the chosen variant)
   KernelMap adj_p2p;   // adjoint P2P: share of its grid (kAdjP2PWaves-wave blocks)
   KernelMap adj_sort;  // adjoint: upstream gradient to sorted order, cdiv(n_s, 256)
@@ -47,14 +49,20 @@ hipError_t launch_predo_warm_batch(const BatchLaunch& b, const CallIO& io, hipSt
 
 // tree.hip
 hipError_t launch_p2m(const SysView& v, hipStream_t s);
-// fused solve: P2M of occupied leaves (empty ones cleared by predo) + all M2M levels
-hipError_t launch_upward(const SysView& v, int max_occ, hipStream_t s);
-hipError_t launch_tree_batch(const BatchLaunch& b, hipStream_t s);  // P2M, M2M, M2L
-int upward_tile_blocks(int L, int B);
+// fused paths: P2M of the occupied leaves (empty ones cleared by the sort) and
+// the M2M levels of the upward tiles; with_top: also the levels above them
+// (the fused M2L makes those itself)
+hipError_t launch_upward(const SysView& v, hipStream_t s, bool with_top);
+// fused paths' M2L: cells with particles only, and the M2M levels above the tiles
+hipError_t launch_m2l_fused(const SysView& v, hipStream_t s);
+// forward: upward pass + fused M2L; else (adjoints) upward pass + the top M2M levels
+hipError_t launch_tree_batch(const BatchLaunch& b, hipStream_t s, bool forward);
+int upward_tile_blocks(int L, int B);  // adjoint upward tiles (0 when there is no M2M level)
+int up_blocks(int L, int B);           // the upward pass's tiles
 int upward_top_blocks(int L, int B);
 int m2l_blocks(const SysView& v);
-int p2m_occ_blocks(int max_occ);  // P2M grid for up to max_occ occupied leaves
-int p2m_leaf_blocks(int B);       // P2M grid for all B * B leaves
+int m2l_fused_blocks(const SysView& v);
+int p2m_leaf_blocks(int B);       // one thread per leaf: all B * B leaves
 hipError_t launch_m2m(const SysView& v, hipStream_t s);
 hipError_t launch_m2l(const SysView& v, hipStream_t s);
 hipError_t launch_l2l(const SysView& v, hipStream_t s);

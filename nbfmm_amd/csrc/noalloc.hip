@@ -4,7 +4,7 @@
 // in one file-scope device array of kNoAllocScratchBytes per GPU, held until
 // the process ends; the caller passes only inputs and outputs.
 //
-// The kernels are the solver's (cold sort, upward pass, M2L, P2P with the
+// The kernels are the solver's (cold sort, upward pass, fused M2L, P2P with the
 // L2P/un-permute epilogue), on a view whose arrays are carved out of that
 // array, so the results equal nbfmm_solve's bit for bit.  The array keeps
 // the last call's contents; the cold sort rewrites every array it reads
@@ -143,8 +143,8 @@ extern "C" int nbfmm_solve_noalloc(int num_level, int num_particle, const float 
   hipStream_t st = (hipStream_t)stream;
   hipError_t e = launch_fill(1, 0.0f, (float*)v.flags, st);
   if (e == hipSuccess) e = launch_predo(v, io, 1, st);
-  if (e == hipSuccess) e = launch_upward(v, num_particle < v.B * v.B ? num_particle : v.B * v.B, st);
-  if (e == hipSuccess) e = launch_m2l(v, st);
+  if (e == hipSuccess) e = launch_upward(v, st, false);
+  if (e == hipSuccess) e = launch_m2l_fused(v, st);
   if (e == hipSuccess) {
     // the solver's P2P grid (p2p_grid_for) for this n
     int stage = 0, unroll = 0, waves = 0;

@@ -30,24 +30,6 @@ namespace nbfmm_amd {
 
 static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 
-// Lanes of the wave whose `d` (nbits wide) equals this lane's, among the
-// lanes where `valid` holds.
-__device__ __forceinline__ uint64_t match_mask(uint32_t d, bool valid, int nbits) {
-  uint64_t m = __ballot(valid);
-  for (int b = 0; b < nbits; ++b) {
-    const bool bit = (d >> b) & 1u;
-    const uint64_t bal = __ballot(valid && bit);
-    m &= bit ? bal : ~bal;
-  }
-  return m;
-}
-
-__device__ __forceinline__ uint64_t lanes_below() {
-  const int lane = lane_id();
-  return lane == 0 ? 0ull : (~0ull >> (kWave - lane));
-}
-
-
 // ------------------------------------------------------------- row pass
 
 // Row histogram of one tile of 2048 particles in input order, by a block of
@@ -83,10 +65,11 @@ __global__ __launch_bounds__(kSortThreads) void k_row_up(Sys sys, CallIO io) {
   row_up_tile<kSortThreads>(v, io, tile, ntiles, h);
 }
 
-// Exclusive scan of row d's tile counts, in place, by a block of
-// kScanThreads threads; the row total goes to dtotal[d].  Row 0 also clears
+// Exclusive scan of row d's tile counts, in place, by a block of NT
+// threads; the row total goes to dtotal[d].  Row 0 also clears
 // the occupied-cell counter that the column pass fills.  All threads of the
 // block call it.
+template <int NT>
 __device__ __forceinline__ void row_scan_row(const SysView& v, int d, uint32_t* sh, uint32_t* carry) {
   const int ntiles = (v.n + kSortTile - 1) / kSortTile;
   uint32_t* row = v.hist + (size_t)d * ntiles;
@@ -95,11 +78,11 @@ __device__ __forceinline__ void row_scan_row(const SysView& v, int d, uint32_t* 
     if (d == 0) *v.nocc = 0;
   }
   __syncthreads();
-  for (int t0 = 0; t0 < ntiles; t0 += kScanThreads) {
+  for (int t0 = 0; t0 < ntiles; t0 += NT) {
     const int t = t0 + threadIdx.x;
     const uint32_t c = t < ntiles ? row[t] : 0u;
     uint32_t tot;
-    const uint32_t e = block_exclusive_scan<kScanThreads>(c, sh, &tot);
+    const uint32_t e = block_exclusive_scan<NT>(c, sh, &tot);
     const uint32_t base = *carry;
     if (t < ntiles) row[t] = base + e;
     __syncthreads();
@@ -117,7 +100,7 @@ __global__ __launch_bounds__(kScanThreads) void k_row_scan(Sys sys) {
   __shared__ uint32_t carry;
   int d, nrows;
   const SysView v = sys.get(d, nrows);
-  row_scan_row(v, d, sh, &carry);
+  row_scan_row<kScanThreads>(v, d, sh, &carry);
 }
 
 // Stable scatter of one tile by row, by a block of kSortThreads threads.
@@ -211,6 +194,86 @@ __device__ __forceinline__ void row_down_tile(const SysView& v, const CallIO& io
   __syncthreads();
 }
 
+// row_down_tile with each item's key and rank kept in LDS instead of
+// registers (positions are loaded again for the scatter), so that a kernel
+// that carries it as a rarely taken branch keeps its register count.
+struct RowDownLeanLds {
+  RowDownLds d;
+  uint32_t kr[kSortTile];  // key | rank << 18 (key < 2^18, rank < 512)
+};
+
+__device__ __forceinline__ void row_down_tile_lean(const SysView& v, const CallIO& io, int tile, int ntiles,
+                                                   RowDownLeanLds& L) {
+  uint32_t* s_base = L.d.base;
+  auto& s_wcnt = L.d.wcnt;
+  const float2* pos0 = io.pos0 + v.part_off;
+  const float* w0 = io.w0 + v.part_off;
+  const int R = v.B;
+  const int lane = lane_id();
+  const int wave = threadIdx.x / kWave;
+  {
+    const int d0 = 2 * threadIdx.x, d1 = d0 + 1;
+    const uint32_t t0 = d0 < R ? v.dtotal[d0] : 0u;
+    const uint32_t t1 = d1 < R ? v.dtotal[d1] : 0u;
+    uint32_t tot;
+    const uint32_t e = block_exclusive_scan<kSortThreads>(t0 + t1, L.d.sh, &tot);
+    if (d0 < R) s_base[d0] = e + v.hist[(size_t)d0 * ntiles + tile];
+    if (d1 < R) s_base[d1] = e + t0 + v.hist[(size_t)d1 * ntiles + tile];
+    if (tile == 0) {
+      if (d0 < R) v.rowstart[d0] = (int)e;
+      if (d1 < R) v.rowstart[d1] = (int)(e + t0);
+      if (threadIdx.x == 0) v.rowstart[R] = (int)tot;
+    }
+  }
+  for (int d = threadIdx.x; d < R; d += kSortThreads) {
+#pragma unroll
+    for (int w = 0; w < kSortThreads / kWave; ++w) s_wcnt[w][d] = 0;
+  }
+  __syncthreads();
+  const int wbase = tile * kSortTile + wave * (kWave * kSortWaveItems);
+  const uint64_t below = lanes_below();
+#pragma unroll 1
+  for (int j = 0; j < kSortWaveItems; ++j) {
+    const int i = wbase + j * kWave + lane;
+    const bool valid = i < v.n;
+    bool bad;
+    const uint32_t key = valid ? bin_key(v, pos0[i], bad) : 0u;
+    const uint32_t d = key >> v.lgB;
+    const uint64_t m = match_mask(d, valid, v.lgB);
+    const uint32_t r = (uint32_t)__popcll(m & below);
+    uint32_t prior = 0;
+    if (valid) prior = s_wcnt[wave][d];
+    L.kr[wave * (kWave * kSortWaveItems) + j * kWave + lane] = key | ((prior + r) << 18);
+    wave_lds_order();
+    if (valid && r == 0) s_wcnt[wave][d] = prior + (uint32_t)__popcll(m);
+    wave_lds_order();
+  }
+  __syncthreads();
+  for (int d = threadIdx.x; d < R; d += kSortThreads) {
+    uint32_t run = 0;
+#pragma unroll
+    for (int w = 0; w < kSortThreads / kWave; ++w) {
+      const uint32_t c = s_wcnt[w][d];
+      s_wcnt[w][d] = run;
+      run += c;
+    }
+  }
+  __syncthreads();
+#pragma unroll 1
+  for (int j = 0; j < kSortWaveItems; ++j) {
+    const int i = wbase + j * kWave + lane;
+    if (i >= v.n) continue;
+    const uint32_t kr = L.kr[wave * (kWave * kSortWaveItems) + j * kWave + lane];
+    const uint32_t key = kr & 0x3ffffu;
+    const uint32_t d = key >> v.lgB;
+    const uint32_t p = s_base[d] + s_wcnt[wave][d] + (kr >> 18);
+    const float2 q = pos0[i];
+    v.tkey[p] = key;
+    v.tpart[p] = make_float4(q.x, q.y, w0[i], __uint_as_float((uint32_t)i));
+  }
+  __syncthreads();
+}
+
 template <class Sys>
 __global__ __launch_bounds__(kSortThreads) void k_row_down(Sys sys, CallIO io) {
   __shared__ RowDownLds L;
@@ -225,65 +288,88 @@ constexpr int kRowThreads = 512;                         // 8 waves per row bloc
 constexpr int kRowWaves = kRowThreads / kWave;
 constexpr int kRowGroup = kRowThreads * kSortWaveItems;  // 4096 particles per group
 
-// One block per row y.  zero_empty: also clear the level-0 multipole of the
-// row's empty cells (the fused solve's P2M then visits occupied cells only).
-// only_if_cold: the column pass of a warm-started sort that switched to the
-// cold passes; returns at once when it did not.
-template <class Sys>
-__global__ __launch_bounds__(kRowThreads) void k_row_sort(Sys sys, int zero_empty, int only_if_cold) {
-  __shared__ uint32_t s_hist[512];
-  __shared__ uint32_t s_start[512];
-  __shared__ uint32_t s_run[512];
-  __shared__ uint32_t s_wcnt[kRowWaves][512];
-  __shared__ uint32_t sh[kRowWaves];
-  __shared__ uint32_t s_occ_base;
-  int y, nrows;
-  const SysView v = sys.get(y, nrows);
-  if (only_if_cold && v.wstate[kWarmCold] == 0) return;
+// Column pass of row y by a block of NT threads (NT <= 512): stable counting
+// sort of the row's segment of the row-sorted scratch by column, writing the
+// sorted keys and particles, head of the row's cells and the row's occupied
+// cells.  zero_empty: also clear the level-0 multipole of the row's empty
+// cells (the fused solve's P2M then visits occupied cells only).  Each thread
+// takes 512 / NT consecutive cells in the scans.  All threads of the block
+// call it.
+template <int NT>
+struct RowSortLds {
+  uint32_t hist[512];
+  uint32_t start[512];
+  uint32_t run[512];
+  uint32_t wcnt[NT / kWave][512];
+  uint32_t sh[NT / kWave];
+  uint32_t occ_base;
+};
+
+template <int NT>
+__device__ __forceinline__ void row_sort_row(const SysView& v, int y, int zero_empty, RowSortLds<NT>& L) {
+  constexpr int NW = NT / kWave;
+  constexpr int LPT = 512 / NT;        // cells per thread in the scans
+  constexpr int GROUP = NT * kSortWaveItems;
   const int B = v.B;
   const int lane = lane_id();
   const int wave = threadIdx.x / kWave;
   const int rs = v.rowstart[y], re = v.rowstart[y + 1];
   const uint32_t cmask = (uint32_t)(B - 1);
 
-  for (int x = threadIdx.x; x < B; x += kRowThreads) { s_hist[x] = 0; s_run[x] = 0; }
+  for (int x = threadIdx.x; x < B; x += NT) { L.hist[x] = 0; L.run[x] = 0; }
   __syncthreads();
-  for (int p0 = rs + (int)threadIdx.x; p0 < re; p0 += kRowThreads * kSortWaveItems) {
+  for (int p0 = rs + (int)threadIdx.x; p0 < re; p0 += NT * kSortWaveItems) {
     uint32_t k[kSortWaveItems];
 #pragma unroll
     for (int j = 0; j < kSortWaveItems; ++j) {
-      const int p = p0 + j * kRowThreads;
+      const int p = p0 + j * NT;
       k[j] = p < re ? v.tkey[p] : 0xffffffffu;
     }
 #pragma unroll
     for (int j = 0; j < kSortWaveItems; ++j)
-      if (k[j] != 0xffffffffu) atomicAdd(&s_hist[k[j] & cmask], 1u);
+      if (k[j] != 0xffffffffu) atomicAdd(&L.hist[k[j] & cmask], 1u);
   }
   __syncthreads();
   {
-    // cell starts inside the row, head, occupied cells (B <= 512: one each)
-    const int x = threadIdx.x;
-    const uint32_t c = x < B ? s_hist[x] : 0u;
-    const uint32_t occ = (x < B && c > 0) ? 1u : 0u;
+    // cell starts inside the row, head, occupied cells
+    const int x0 = threadIdx.x * LPT;
+    uint32_t c[LPT], csum = 0, osum = 0;
+#pragma unroll
+    for (int j = 0; j < LPT; ++j) {
+      c[j] = x0 + j < B ? L.hist[x0 + j] : 0u;
+      csum += c[j];
+      osum += c[j] > 0 ? 1u : 0u;
+    }
     uint32_t tot;
-    const uint32_t e = block_exclusive_scan<kRowThreads>(c, sh, &tot);
+    uint32_t e = block_exclusive_scan<NT>(csum, L.sh, &tot);
     uint32_t nocc;
-    const uint32_t eo = block_exclusive_scan<kRowThreads>(occ, sh, &nocc);
-    if (threadIdx.x == 0) s_occ_base = nocc ? atomicAdd((uint32_t*)v.nocc, nocc) : 0u;
-    if (x < B) {
-      s_start[x] = e;
-      v.head[y * B + x] = rs + (int)e;
-      if (zero_empty && c == 0) v.mpole[y * B + x] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    uint32_t eo = block_exclusive_scan<NT>(osum, L.sh, &nocc);
+    if (threadIdx.x == 0) L.occ_base = nocc ? atomicAdd((uint32_t*)v.nocc, nocc) : 0u;
+#pragma unroll
+    for (int j = 0; j < LPT; ++j) {
+      const int x = x0 + j;
+      if (x < B) {
+        L.start[x] = e;
+        v.head[y * B + x] = rs + (int)e;
+        if (zero_empty && c[j] == 0) v.mpole[y * B + x] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      }
+      e += c[j];
     }
     if (y == B - 1 && threadIdx.x == 0) v.head[B * B] = re;
     __syncthreads();
-    if (occ) v.occ[s_occ_base + eo] = y * B + x;
+#pragma unroll
+    for (int j = 0; j < LPT; ++j) {
+      if (x0 + j < B && c[j] > 0) {
+        v.occ[L.occ_base + eo] = y * B + x0 + j;
+        ++eo;
+      }
+    }
   }
   const uint64_t below = lanes_below();
-  for (int g = rs; g < re; g += kRowGroup) {
-    for (int x = threadIdx.x; x < B; x += kRowThreads) {
+  for (int g = rs; g < re; g += GROUP) {
+    for (int x = threadIdx.x; x < B; x += NT) {
 #pragma unroll
-      for (int w = 0; w < kRowWaves; ++w) s_wcnt[w][x] = 0;
+      for (int w = 0; w < NW; ++w) L.wcnt[w][x] = 0;
     }
     __syncthreads();
     const int wbase = g + wave * (kWave * kSortWaveItems);
@@ -301,23 +387,23 @@ __global__ __launch_bounds__(kRowThreads) void k_row_sort(Sys sys, int zero_empt
       const uint64_t m = match_mask(d, valid, v.lgB);
       const uint32_t r = (uint32_t)__popcll(m & below);
       uint32_t prior = 0;
-      if (valid) prior = s_wcnt[wave][d];
+      if (valid) prior = L.wcnt[wave][d];
       rank[j] = prior + r;
       wave_lds_order();
-      if (valid && r == 0) s_wcnt[wave][d] = prior + (uint32_t)__popcll(m);
+      if (valid && r == 0) L.wcnt[wave][d] = prior + (uint32_t)__popcll(m);
       wave_lds_order();
     }
     __syncthreads();
     // per cell: offset of each wave = earlier groups + earlier waves
-    for (int x = threadIdx.x; x < B; x += kRowThreads) {
-      uint32_t run = s_run[x];
+    for (int x = threadIdx.x; x < B; x += NT) {
+      uint32_t run = L.run[x];
 #pragma unroll
-      for (int w = 0; w < kRowWaves; ++w) {
-        const uint32_t c = s_wcnt[w][x];
-        s_wcnt[w][x] = run;
-        run += c;
+      for (int w = 0; w < NW; ++w) {
+        const uint32_t cc = L.wcnt[w][x];
+        L.wcnt[w][x] = run;
+        run += cc;
       }
-      s_run[x] = run;
+      L.run[x] = run;
     }
     __syncthreads();
     float4 pv[kSortWaveItems];
@@ -331,12 +417,125 @@ __global__ __launch_bounds__(kRowThreads) void k_row_sort(Sys sys, int zero_empt
       const int e = wbase + j * kWave + lane;
       if (e >= re) continue;
       const uint32_t d = key[j] & cmask;
-      const int p = rs + (int)(s_start[d] + s_wcnt[wave][d] + rank[j]);
+      const int p = rs + (int)(L.start[d] + L.wcnt[wave][d] + rank[j]);
       v.key[p] = key[j];
       v.part[p] = pv[j];
     }
     __syncthreads();
   }
+}
+
+// row_sort_row for a block of NT threads with each item's key and rank kept
+// in LDS and one item in registers at a time, so that a kernel that carries
+// it as a rarely taken branch keeps its register count.
+template <int NT>
+struct RowSortLeanLds {
+  RowSortLds<NT> s;
+  uint32_t kr[NT * kSortWaveItems];  // column | rank << 16 (column < 512, rank < 512)
+};
+
+template <int NT>
+__device__ __forceinline__ void row_sort_row_lean(const SysView& v, int y, int zero_empty, RowSortLeanLds<NT>& LL) {
+  RowSortLds<NT>& L = LL.s;
+  constexpr int NW = NT / kWave;
+  constexpr int LPT = 512 / NT;
+  constexpr int GROUP = NT * kSortWaveItems;
+  const int B = v.B;
+  const int lane = lane_id();
+  const int wave = threadIdx.x / kWave;
+  const int rs = v.rowstart[y], re = v.rowstart[y + 1];
+  const uint32_t cmask = (uint32_t)(B - 1);
+  for (int x = threadIdx.x; x < B; x += NT) { L.hist[x] = 0; L.run[x] = 0; }
+  __syncthreads();
+#pragma unroll 1
+  for (int p = rs + (int)threadIdx.x; p < re; p += NT) atomicAdd(&L.hist[v.tkey[p] & cmask], 1u);
+  __syncthreads();
+  {
+    const int x0 = threadIdx.x * LPT;
+    uint32_t csum = 0, osum = 0;
+    for (int j = 0; j < LPT; ++j) {
+      const uint32_t c = x0 + j < B ? L.hist[x0 + j] : 0u;
+      csum += c;
+      osum += c > 0 ? 1u : 0u;
+    }
+    uint32_t tot;
+    uint32_t e = block_exclusive_scan<NT>(csum, L.sh, &tot);
+    uint32_t nocc;
+    uint32_t eo = block_exclusive_scan<NT>(osum, L.sh, &nocc);
+    if (threadIdx.x == 0) L.occ_base = nocc ? atomicAdd((uint32_t*)v.nocc, nocc) : 0u;
+    for (int j = 0; j < LPT; ++j) {
+      const int x = x0 + j;
+      const uint32_t c = x < B ? L.hist[x] : 0u;
+      if (x < B) {
+        L.start[x] = e;
+        v.head[y * B + x] = rs + (int)e;
+        if (zero_empty && c == 0) v.mpole[y * B + x] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      }
+      e += c;
+    }
+    if (y == B - 1 && threadIdx.x == 0) v.head[B * B] = re;
+    __syncthreads();
+    for (int j = 0; j < LPT; ++j) {
+      const int x = x0 + j;
+      if (x < B && L.hist[x] > 0) {
+        v.occ[L.occ_base + eo] = y * B + x;
+        ++eo;
+      }
+    }
+  }
+  const uint64_t below = lanes_below();
+  for (int g = rs; g < re; g += GROUP) {
+    for (int x = threadIdx.x; x < B; x += NT) {
+      for (int w = 0; w < NW; ++w) L.wcnt[w][x] = 0;
+    }
+    __syncthreads();
+    const int wbase = g + wave * (kWave * kSortWaveItems);
+#pragma unroll 1
+    for (int j = 0; j < kSortWaveItems; ++j) {
+      const int e = wbase + j * kWave + lane;
+      const bool valid = e < re;
+      const uint32_t d = valid ? v.tkey[e] & cmask : 0u;
+      const uint64_t m = match_mask(d, valid, v.lgB);
+      const uint32_t r = (uint32_t)__popcll(m & below);
+      uint32_t prior = 0;
+      if (valid) prior = L.wcnt[wave][d];
+      LL.kr[wave * (kWave * kSortWaveItems) + j * kWave + lane] = d | ((prior + r) << 16);
+      wave_lds_order();
+      if (valid && r == 0) L.wcnt[wave][d] = prior + (uint32_t)__popcll(m);
+      wave_lds_order();
+    }
+    __syncthreads();
+    for (int x = threadIdx.x; x < B; x += NT) {
+      uint32_t run = L.run[x];
+      for (int w = 0; w < NW; ++w) {
+        const uint32_t cc = L.wcnt[w][x];
+        L.wcnt[w][x] = run;
+        run += cc;
+      }
+      L.run[x] = run;
+    }
+    __syncthreads();
+#pragma unroll 1
+    for (int j = 0; j < kSortWaveItems; ++j) {
+      const int e = wbase + j * kWave + lane;
+      if (e >= re) continue;
+      const uint32_t kr = LL.kr[wave * (kWave * kSortWaveItems) + j * kWave + lane];
+      const uint32_t d = kr & 0xffffu;
+      const int p = rs + (int)(L.start[d] + L.wcnt[wave][d] + (kr >> 16));
+      v.key[p] = (uint32_t)(y * B) + d;
+      v.part[p] = v.tpart[e];
+    }
+    __syncthreads();
+  }
+}
+
+// One block per row y.
+template <class Sys>
+__global__ __launch_bounds__(kRowThreads) void k_row_sort(Sys sys, int zero_empty) {
+  __shared__ RowSortLds<kRowThreads> L;
+  int y, nrows;
+  const SysView v = sys.get(y, nrows);
+  row_sort_row<kRowThreads>(v, y, zero_empty, L);
 }
 
 // Occupied-leaf list rebuilt from head alone (after head was written through
@@ -391,12 +590,13 @@ __global__ __launch_bounds__(256) void k_empty(SysView v) {
 // atomics are per leaf; an atomic that the whole grid shares serialises
 // across the chip, so the occupied and touched leaf lists take their row
 // bases from scans of per-row counts instead of shared counters.  A leaf that
-// gains more than kMergeLdsMovers movers switches the sort to the cold passes,
-// decided on the device by k_warm_rows: the next kernels then make the cold
-// row pass (histograms in k_warm_finish, scan in k_warm_place, scatter in
-// k_warm_merge) and one more launch of the column pass (k_row_sort, which
-// returns at once when the switch is off), so such a step costs about one
-// cold sort, whatever the number of movers.
+// gains more than kMergeLdsMovers movers switches the sort to the cold passes:
+// the first pass raises the request when a leaf's incoming count passes that
+// limit (the one returning atomic of the wave's count), and the warm kernels
+// then make the cold sort's four passes in their place (row histograms in
+// k_warm_rows, row scan in k_warm_finish, row scatter in k_warm_place, column
+// pass in k_warm_merge), with no launch of their own, so such a step costs
+// about one cold sort, whatever the number of movers.
 
 template <class Sys>
 __global__ __launch_bounds__(256) void k_warm_gather(Sys sys, CallIO io) {
@@ -427,13 +627,15 @@ __global__ __launch_bounds__(256) void k_warm_gather(Sys sys, CallIO io) {
 // range.  The row's totals (particles, movers, occupied leaves, touched
 // leaves) go to wrowtot.  Clears wcin/wcout.
 template <int NT, class Sys>
-__global__ __launch_bounds__(NT) void k_warm_rows(Sys sys) {
+__global__ __launch_bounds__(NT) void k_warm_rows(Sys sys, CallIO io) {
   __shared__ uint32_t sh[NT / kWave];
+  __shared__ uint32_t s_hist[512];
   int y, nrows;
   const SysView v = sys.get(y, nrows);
   const int B = v.B;
   const int x = threadIdx.x;
   const int c = y * B + x;
+  const int cold = v.wstate[kWarmColdReq];
   uint32_t cnt = 0, cin = 0;
   bool touched = false;
   if (x < B) {
@@ -449,9 +651,6 @@ __global__ __launch_bounds__(NT) void k_warm_rows(Sys sys) {
     cnt = (uint32_t)(h1 - h0 - co + ci);
     cin = (uint32_t)ci;
   }
-  // a leaf gaining more movers than the merge sorts in LDS switches this
-  // sort to the cold passes (one atomic per row that has such a leaf)
-  if (__syncthreads_or(x < B && cin > (uint32_t)kMergeLdsMovers) && x == 0) atomicOr(&v.wstate[kWarmCold], 1);
   const int nocc = __syncthreads_count(cnt > 0);
   const int ntouch = __syncthreads_count(touched);
   uint32_t tot, itot;
@@ -466,6 +665,13 @@ __global__ __launch_bounds__(NT) void k_warm_rows(Sys sys) {
     v.wrowtot[B + 1 + y] = (int)itot;
     v.wrowtot[2 * (B + 1) + y] = nocc;
     v.wrowtot[3 * (B + 1) + y] = ntouch;
+    v.wstate[kWarmCold] = cold;  // every row block writes the same value
+  }
+  if (cold) {
+    // cold switch: the row histograms of the cold row pass (k_row_up), the
+    // system's tiles shared among its row blocks
+    const int ntiles = (v.n + kSortTile - 1) / kSortTile;
+    for (int t = y; t < ntiles; t += nrows) row_up_tile<NT>(v, io, t, ntiles, s_hist);
   }
 }
 
@@ -513,14 +719,14 @@ __global__ __launch_bounds__(NT) void k_warm_rowscan(Sys sys) {
 template <int NT, class Sys>
 __global__ __launch_bounds__(NT) void k_warm_finish(Sys sys, CallIO io, int zero_empty, int prescanned) {
   __shared__ uint32_t sh[NT / kWave];
-  __shared__ uint32_t s_base, s_ibase, s_occ_base, s_touch_base;
-  __shared__ uint32_t s_hist[512];
+  __shared__ uint32_t s_base, s_ibase, s_occ_base, s_touch_base, s_carry;
   int y, nrows;
   const SysView v = sys.get(y, nrows);
   const int B = v.B;
   const int x = threadIdx.x;
   const int c = y * B + x;
   // the row's own loads first, so that their latency overlaps the scans
+  const int cold = v.wstate[kWarmCold];
   int e = 0, e1 = 0, win = 0;
   bool tch = false;
   if (x < B) {
@@ -575,50 +781,45 @@ __global__ __launch_bounds__(NT) void k_warm_finish(Sys sys, CallIO io, int zero
   const uint32_t et = block_exclusive_scan<NT>(touched, sh, &ntouch);
   if (occ) v.occ[s_occ_base + eo] = c;
   if (touched) v.wtlist[s_touch_base + et] = c;
-  if (v.wstate[kWarmCold]) {
-    // cold switch: the row histograms of the cold row pass (k_row_up), the
-    // system's tiles shared among its row blocks
-    const int ntiles = (v.n + kSortTile - 1) / kSortTile;
-    for (int t = y; t < ntiles; t += nrows) row_up_tile<NT>(v, io, t, ntiles, s_hist);
+  // k_warm_rows has read the first pass's request: clear it for the next sort
+  if (y == 0 && x == 0) v.wstate[kWarmColdReq] = 0;
+  if (cold) {
+    // cold switch: the row scan of the cold row pass (k_row_scan), the rows
+    // shared among the system's row blocks
+    for (int d = y; d < B; d += nrows) row_scan_row<NT>(v, d, sh, &s_carry);
   }
 }
 
 // Stayers of untouched leaves go to head[leaf] + their old rank; movers are
 // put into their new leaf's group (slots of the group in any order).
 template <class Sys>
-__global__ __launch_bounds__(256) void k_warm_place(Sys sys) {
-  __shared__ uint32_t sh[kScanThreads / kWave];
-  __shared__ uint32_t carry;
+__global__ __launch_bounds__(256) void k_warm_place(Sys sys, CallIO io) {
+  __shared__ RowDownLeanLds lds;
   int blk, nblk;
   const SysView v = sys.get(blk, nblk);
-  if (v.wstate[kWarmCold]) {
-    // cold switch: the row scan of the cold row pass (k_row_scan), the rows
-    // shared among the system's blocks
-    for (int d = blk; d < v.B; d += nblk) row_scan_row(v, d, sh, &carry);
-    return;
-  }
   const int i = blk * 256 + threadIdx.x;
-  if (blk * 256 >= v.n) return;  // whole waves below stay for the movers' slots
   const bool valid = i < v.n;
   // the particle's own entries are loaded together with its key (a stayer,
-  // nearly every particle, needs them; a mover's are simply unused)
+  // nearly every particle, needs them; a mover's are simply unused) and the
+  // cold switch
+  const int cold = v.wstate[kWarmCold];
   const uint32_t t = valid ? v.tkey[i] : 0u;
   const int rel = valid ? v.wrel[i] : 0;
   const float4 p = valid ? v.tpart[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (cold) {
+    // cold switch: the scatter of the cold row pass (k_row_down), the tiles
+    // shared among the system's blocks
+    const int ntiles = (v.n + kSortTile - 1) / kSortTile;
+    for (int tt = blk; tt < ntiles; tt += nblk) row_down_tile_lean(v, io, tt, ntiles, lds);
+    return;
+  }
+  if (blk * 256 >= v.n) return;  // whole waves below stay for the movers' slots
   const bool mover = valid && (t & kMoverBit);
   const uint32_t k = t & ~kMoverBit;
-  // a slot in the new leaf's group: one atomic per distinct leaf of the wave
-  uint64_t pend = __ballot(mover);
-  while (pend) {
-    const int leader = __builtin_ctzll(pend);
-    const uint32_t kl = (uint32_t)__builtin_amdgcn_readlane((int)k, leader);
-    const uint64_t same = __ballot(mover && k == kl);
-    int base = 0;
-    if (lane_id() == leader) base = atomicAdd(&v.wfill[kl], (int)__popcll(same));
-    base = __builtin_amdgcn_readlane(base, leader);
-    if (mover && k == kl) v.wmgrp[v.winoff[kl] + base + (int)__popcll(same & lanes_below())] = i;
-    pend &= ~same;
-  }
+  // a slot in the new leaf's group (slots of a group in any order); a leaf
+  // takes at most kMergeLdsMovers movers here (more switch the sort to cold),
+  // so at most that many atomics meet on one address
+  if (mover) v.wmgrp[v.winoff[k] + atomicAdd(&v.wfill[k], 1)] = i;
   if (valid && !mover && !v.wtouch[t]) {
     const int dst = v.head[t] + rel;
     v.key[dst] = t;
@@ -638,20 +839,20 @@ __device__ __forceinline__ uint32_t orig_of(const float4& p) { return __float_as
 // sorted by original index in the wave's LDS (bitonic); each stayer then
 // finds the movers below it by binary search (p = their count) and counts
 // itself in H[p], so the stayers below the mover of rank r are
-// H[0] + ... + H[r].  A leaf with more movers never gets here: k_warm_rows
-// switched the sort to the cold passes, and then this kernel only clears the
-// touched leaves' flags and makes the cold row pass's scatter (k_row_down).
+// H[0] + ... + H[r].  A leaf with more movers never gets here: the sort
+// switched to the cold passes, and then this kernel only clears the touched
+// leaves' flags and makes the cold sort's column pass (k_row_sort).
 // Clears the leaf's touch flag and group count.
 union MergeLds {
   struct {
     uint2 mv[256 / kWave][kMergeLdsMovers];  // (original index, tpart slot), sorted
     uint32_t h[256 / kWave][kMergeLdsMovers + 1];
   } m;
-  RowDownLds d;
+  RowSortLeanLds<256> r;
 };
 
 template <class Sys>
-__global__ __launch_bounds__(256) void k_warm_merge(Sys sys, CallIO io) {
+__global__ __launch_bounds__(256) void k_warm_merge(Sys sys) {
   __shared__ MergeLds lds;
   auto& s_mv = lds.m.mv;
   auto& s_h = lds.m.h;
@@ -661,24 +862,25 @@ __global__ __launch_bounds__(256) void k_warm_merge(Sys sys, CallIO io) {
   const int wave = (int)(threadIdx.x / kWave);
   const int waves = nblk * (256 / kWave);
   const int nt = v.wstate[kWarmTouched];
-  if (v.wstate[kWarmCold]) {
-    for (int d = blk * (256 / kWave) + wave; d < nt; d += waves) {
+  const int cold = v.wstate[kWarmCold];
+  const uint64_t below = lanes_below();
+  const int cmax = v.B * v.B - 1;  // wtlist holds at most B * B leaves
+  int d = blk * (256 / kWave) + wave;
+  // the first list entry is loaded together with the count and the cold
+  // switch (clamped; used only when d < nt), the next one while this leaf is
+  // merged
+  uint32_t cnext = (uint32_t)v.wtlist[min(d, cmax)];
+  if (cold) {
+    for (; d < nt; d += waves) {
       const int c = v.wtlist[d];
       if (lane == 0) {
         v.wtouch[c] = 0;
         v.wfill[c] = 0;
       }
     }
-    const int ntiles = (v.n + kSortTile - 1) / kSortTile;
-    for (int t = blk; t < ntiles; t += nblk) row_down_tile(v, io, t, ntiles, lds.d);
+    for (int y = blk; y < v.B; y += nblk) row_sort_row_lean<256>(v, y, 1, lds.r);
     return;
   }
-  const uint64_t below = lanes_below();
-  const int cmax = v.B * v.B - 1;  // wtlist holds at most B * B leaves
-  int d = blk * (256 / kWave) + wave;
-  // the first list entry is loaded together with the count (clamped; used
-  // only when d < nt), the next one while this leaf is merged
-  uint32_t cnext = (uint32_t)v.wtlist[min(d, cmax)];
   for (; d < nt; d += waves) {
     const uint32_t c = cnext;
     cnext = (uint32_t)v.wtlist[min(d + waves, cmax)];
@@ -806,7 +1008,7 @@ hipError_t launch_predo(const SysView& v, const CallIO& io, int zero_empty, hipS
   hipLaunchKernelGGL(k_row_up<OneSys>, dim3(ntiles), dim3(kSortThreads), 0, s, one, io);
   hipLaunchKernelGGL(k_row_scan<OneSys>, dim3(v.B), dim3(kScanThreads), 0, s, one);
   hipLaunchKernelGGL(k_row_down<OneSys>, dim3(ntiles), dim3(kSortThreads), 0, s, one, io);
-  hipLaunchKernelGGL(k_row_sort<OneSys>, dim3(v.B), dim3(kRowThreads), 0, s, one, zero_empty, 0);
+  hipLaunchKernelGGL(k_row_sort<OneSys>, dim3(v.B), dim3(kRowThreads), 0, s, one, zero_empty);
   return hipGetLastError();
 }
 
@@ -814,7 +1016,7 @@ hipError_t launch_predo_batch(const BatchLaunch& b, const CallIO& io, hipStream_
   hipLaunchKernelGGL(k_row_up<ManySys>, dim3(b.tiles.blocks), dim3(kSortThreads), 0, s, b.tiles.sys, io);
   hipLaunchKernelGGL(k_row_scan<ManySys>, dim3(b.rows.blocks), dim3(kScanThreads), 0, s, b.rows.sys);
   hipLaunchKernelGGL(k_row_down<ManySys>, dim3(b.tiles.blocks), dim3(kSortThreads), 0, s, b.tiles.sys, io);
-  hipLaunchKernelGGL(k_row_sort<ManySys>, dim3(b.rows.blocks), dim3(kRowThreads), 0, s, b.rows.sys, 1, 0);
+  hipLaunchKernelGGL(k_row_sort<ManySys>, dim3(b.rows.blocks), dim3(kRowThreads), 0, s, b.rows.sys, 1);
   return hipGetLastError();
 }
 
@@ -827,12 +1029,12 @@ static int row_threads(int maxB) {
 }
 
 template <class Sys>
-static void launch_warm_rows(int nt, dim3 g, const Sys& sys, hipStream_t s) {
+static void launch_warm_rows(int nt, dim3 g, const Sys& sys, const CallIO& io, hipStream_t s) {
   switch (nt) {
-    case 64: hipLaunchKernelGGL((k_warm_rows<64, Sys>), g, dim3(64), 0, s, sys); break;
-    case 128: hipLaunchKernelGGL((k_warm_rows<128, Sys>), g, dim3(128), 0, s, sys); break;
-    case 256: hipLaunchKernelGGL((k_warm_rows<256, Sys>), g, dim3(256), 0, s, sys); break;
-    default: hipLaunchKernelGGL((k_warm_rows<512, Sys>), g, dim3(512), 0, s, sys); break;
+    case 64: hipLaunchKernelGGL((k_warm_rows<64, Sys>), g, dim3(64), 0, s, sys, io); break;
+    case 128: hipLaunchKernelGGL((k_warm_rows<128, Sys>), g, dim3(128), 0, s, sys, io); break;
+    case 256: hipLaunchKernelGGL((k_warm_rows<256, Sys>), g, dim3(256), 0, s, sys, io); break;
+    default: hipLaunchKernelGGL((k_warm_rows<512, Sys>), g, dim3(512), 0, s, sys, io); break;
   }
 }
 
@@ -869,13 +1071,11 @@ hipError_t launch_predo_warm(const SysView& v, const CallIO& io, int zero_empty,
   const OneSys one{v};
   const int nb = cdiv(v.n, 256);
   if (!gathered) hipLaunchKernelGGL(k_warm_gather<OneSys>, dim3(nb), dim3(256), 0, s, one, io);
-  launch_warm_rows(row_threads(v.B), dim3(v.B), one, s);
+  launch_warm_rows(row_threads(v.B), dim3(v.B), one, io, s);
   launch_warm_finish(row_threads(v.B), dim3(v.B), one, io, zero_empty, 0, s);
-  hipLaunchKernelGGL(k_warm_place<OneSys>, dim3(nb), dim3(256), 0, s, one);
+  hipLaunchKernelGGL(k_warm_place<OneSys>, dim3(nb), dim3(256), 0, s, one, io);
   hipLaunchKernelGGL(k_warm_merge<OneSys>, dim3(cdiv(std::min(kWarmMergeWaves, v.B * v.B), 256 / kWave)),
-                     dim3(256), 0, s, one, io);
-  // the column pass, if the sort switched to the cold passes
-  hipLaunchKernelGGL(k_row_sort<OneSys>, dim3(v.B), dim3(kRowThreads), 0, s, one, zero_empty, 1);
+                     dim3(256), 0, s, one);
   return hipGetLastError();
 }
 
@@ -883,12 +1083,11 @@ hipError_t launch_predo_warm_batch(const BatchLaunch& b, const CallIO& io, hipSt
   if (!gathered)
     hipLaunchKernelGGL(k_warm_gather<ManySys>, dim3(b.adj_sort.blocks), dim3(256), 0, s, b.adj_sort.sys, io);
   const int nt = row_threads(b.max_B);
-  launch_warm_rows(nt, dim3(b.rows.blocks), b.rows.sys, s);
+  launch_warm_rows(nt, dim3(b.rows.blocks), b.rows.sys, io, s);
   launch_warm_rowscan(nt, dim3(b.one.blocks), b.one.sys, s);
   launch_warm_finish(nt, dim3(b.rows.blocks), b.rows.sys, io, 1, 1, s);
-  hipLaunchKernelGGL(k_warm_place<ManySys>, dim3(b.adj_sort.blocks), dim3(256), 0, s, b.adj_sort.sys);
-  hipLaunchKernelGGL(k_warm_merge<ManySys>, dim3(b.warm_merge.blocks), dim3(256), 0, s, b.warm_merge.sys, io);
-  hipLaunchKernelGGL(k_row_sort<ManySys>, dim3(b.rows.blocks), dim3(kRowThreads), 0, s, b.rows.sys, 1, 1);
+  hipLaunchKernelGGL(k_warm_place<ManySys>, dim3(b.adj_sort.blocks), dim3(256), 0, s, b.adj_sort.sys, io);
+  hipLaunchKernelGGL(k_warm_merge<ManySys>, dim3(b.warm_merge.blocks), dim3(256), 0, s, b.warm_merge.sys);
   return hipGetLastError();
 }
 

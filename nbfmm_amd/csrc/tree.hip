@@ -1,5 +1,9 @@
 // Quadtree passes of the monopole FMM: P2M, M2M, M2L, L2L.
 //
+// Multipoles are float4 {COM x, COM y, mass, particles in the cell}: the
+// count (not in the reference, which keeps COM and mass) tells the fused M2L
+// which cells hold particles.
+//
 // Reference: source/nbfmm/core/solver/{p2m,m2m,m2l,l2l}.cu.  The reference
 // keeps every level in a sparse B x B x L cube; here each level l is a dense
 // D_l x D_l array (D_l = B >> l), so every pass is a coalesced stream
@@ -64,7 +68,9 @@ __device__ __forceinline__ float4 m2m_combine(float4 c00, float4 c10, float4 c01
   const float px = c00.x * c00.z + c10.x * c10.z + c01.x * c01.z + c11.x * c11.z;
   const float py = c00.y * c00.z + c10.y * c10.z + c01.y * c01.z + c11.y * c11.z;
   const float pw = c00.z + c10.z + c01.z + c11.z;
-  float4 out = make_float4(0.0f, 0.0f, pw, 0.0f);
+  // w: particles in the cell (counts are exact in float up to 2^24, and a
+  // sum of nonnegative counts is nonzero exactly when one of them is)
+  float4 out = make_float4(0.0f, 0.0f, pw, c00.w + c10.w + c01.w + c11.w);
   if (pw != 0.0f) {
     out.x = px / pw;
     out.y = py / pw;
@@ -94,7 +100,7 @@ __device__ __forceinline__ float4 p2m_leaf(const SysView& v, int c) {
     sw += pw;
     s = e;
   }
-  float4 out = make_float4(0.0f, 0.0f, sw, 0.0f);
+  float4 out = make_float4(0.0f, 0.0f, sw, (float)(b - a));
   if (b != a) {
     out.x = sx / sw;
     out.y = sy / sw;
@@ -136,90 +142,80 @@ __device__ __forceinline__ void p2m_segment(const SysView& v, int s, int e, floa
   }
 }
 
-template <class Sys>
-__global__ __launch_bounds__(kP2MThreads) void k_p2m_occ(Sys sys) {
+// Multipole of an occupied leaf [a, b) of at most kP2MCoop particles in
+// p2m_leaf's order: one lane walks its segments in order, one segment per
+// step with the next segment's loads already in flight (indices clamped
+// inside the segment, only the adds predicated).
+__device__ __forceinline__ float4 p2m_leaf_lane(const SysView& v, int a, int b) {
 #pragma clang fp contract(off)
-  __shared__ int s_coop[kP2MThreads];
-  __shared__ int s_ncoop;
-  int blk, nblk;
-  const SysView v = sys.get(blk, nblk);
-  const int lane = lane_id();
-  if (threadIdx.x == 0) s_ncoop = 0;
-  __syncthreads();
-  // one thread per leaf; empty leaves (cleared by the sort) are skipped
-  const int c = blk * kP2MThreads + (int)threadIdx.x;
-  int a = 0, b = 0;
-  if (c < v.B * v.B) {
-    a = v.head[c];
-    b = v.head[c + 1];
-  }
-  const bool valid = b > a;
-  const bool coop = valid && b - a > kP2MCoop;
-  // the segments of [a, b): [a, first), then kP2MSeg at a time
   const int first = (a / kP2MSeg + 1) * kP2MSeg;
-  if (valid && !coop) {
-    float sx = 0.0f, sy = 0.0f, sw = 0.0f;
-    int s = a, e = min(b, first);
-    float4 next[kP2MSeg];
+  float sx = 0.0f, sy = 0.0f, sw = 0.0f;
+  int s = a, e = min(b, first);
+  float4 next[kP2MSeg];
 #pragma unroll
-    for (int k = 0; k < kP2MSeg; ++k) next[k] = v.part[min(s + k, e - 1)];
-    while (s < b) {
-      float4 cur[kP2MSeg];
+  for (int k = 0; k < kP2MSeg; ++k) next[k] = v.part[min(s + k, e - 1)];
+  while (s < b) {
+    float4 cur[kP2MSeg];
 #pragma unroll
-      for (int k = 0; k < kP2MSeg; ++k) cur[k] = next[k];
-      const int s2 = e, e2 = min(b, e + kP2MSeg);
+    for (int k = 0; k < kP2MSeg; ++k) cur[k] = next[k];
+    const int s2 = e, e2 = min(b, e + kP2MSeg);
 #pragma unroll
-      for (int k = 0; k < kP2MSeg; ++k) next[k] = v.part[min(s2 + k, e2 - 1)];  // past the leaf: b - 1
-      float px = 0.0f, py = 0.0f, pw = 0.0f;
+    for (int k = 0; k < kP2MSeg; ++k) next[k] = v.part[min(s2 + k, e2 - 1)];  // past the leaf: b - 1
+    float px = 0.0f, py = 0.0f, pw = 0.0f;
 #pragma unroll
-      for (int k = 0; k < kP2MSeg; ++k) {
-        if (s + k < e) {
-          px += cur[k].x * cur[k].z;
-          py += cur[k].y * cur[k].z;
-          pw += cur[k].z;
-        }
-      }
-      sx += px;
-      sy += py;
-      sw += pw;
-      s = s2;
-      e = e2;
-    }
-    v.mpole[c] = make_float4(sx / sw, sy / sw, sw, 0.0f);
-  }
-  if (coop) s_coop[atomicAdd(&s_ncoop, 1)] = c;
-  __syncthreads();
-  // the block's large leaves, one per wave at a time, every lane taking a
-  // segment
-  const int ncoop = s_ncoop;
-  for (int q = (int)(threadIdx.x / kWave); q < ncoop; q += kP2MThreads / kWave) {
-    const int hc = __builtin_amdgcn_readfirstlane(s_coop[q]);
-    const int ha = __builtin_amdgcn_readfirstlane(v.head[hc]), hb = __builtin_amdgcn_readfirstlane(v.head[hc + 1]);
-    const int hf = (ha / kP2MSeg + 1) * kP2MSeg;
-    const int nseg = 1 + (hb - hf + kP2MSeg - 1) / kP2MSeg;  // hb > hf: more than kP2MSeg particles
-    float sx = 0.0f, sy = 0.0f, sw = 0.0f;
-    for (int t0 = 0; t0 < nseg; t0 += kWave) {
-      const int t = t0 + lane;
-      const int s = t == 0 ? ha : hf + (t - 1) * kP2MSeg;
-      const int e = t == 0 ? hf : min(hb, s + kP2MSeg);
-      float px = 0.0f, py = 0.0f, pw = 0.0f;
-      if (t < nseg) p2m_segment(v, s, e, px, py, pw);
-      const int cnt = min(kWave, nseg - t0);
-      for (int u = 0; u < cnt; ++u) {
-        sx += __shfl(px, u, kWave);
-        sy += __shfl(py, u, kWave);
-        sw += __shfl(pw, u, kWave);
+    for (int k = 0; k < kP2MSeg; ++k) {
+      if (s + k < e) {
+        px += cur[k].x * cur[k].z;
+        py += cur[k].y * cur[k].z;
+        pw += cur[k].z;
       }
     }
-    if (lane == 0) v.mpole[hc] = make_float4(sx / sw, sy / sw, sw, 0.0f);
+    sx += px;
+    sy += py;
+    sw += pw;
+    s = s2;
+    e = e2;
   }
+  return make_float4(sx / sw, sy / sw, sw, (float)(b - a));
 }
 
-// M2M levels 1..min(L-1, 4) for one kUpTile x kUpTile tile of leaves, built
-// in LDS from the tile's level-0 multipoles.
+// Multipole of a leaf [ha, hb) of more than kP2MSeg particles by a whole
+// wave (wave-uniform arguments): lane t sums segment t, the wave adds the
+// segment sums in order (p2m_leaf's order).  The result is in every lane.
+__device__ __forceinline__ float4 p2m_leaf_wave(const SysView& v, int ha, int hb, int lane) {
+#pragma clang fp contract(off)
+  const int hf = (ha / kP2MSeg + 1) * kP2MSeg;
+  const int nseg = 1 + (hb - hf + kP2MSeg - 1) / kP2MSeg;
+  float sx = 0.0f, sy = 0.0f, sw = 0.0f;
+  for (int t0 = 0; t0 < nseg; t0 += kWave) {
+    const int t = t0 + lane;
+    const int s = t == 0 ? ha : hf + (t - 1) * kP2MSeg;
+    const int e = t == 0 ? hf : min(hb, s + kP2MSeg);
+    float px = 0.0f, py = 0.0f, pw = 0.0f;
+    if (t < nseg) p2m_segment(v, s, e, px, py, pw);
+    const int cnt = min(kWave, nseg - t0);
+    for (int u = 0; u < cnt; ++u) {
+      sx += __shfl(px, u, kWave);
+      sy += __shfl(py, u, kWave);
+      sw += __shfl(pw, u, kWave);
+    }
+  }
+  return make_float4(sx / sw, sy / sw, sw, (float)(hb - ha));
+}
+
+// Upward pass of the fused paths: one block per kUpTile x kUpTile tile of
+// leaves (smaller grids: the whole grid), one thread per leaf.  P2M of the
+// tile's occupied leaves (the sort already cleared the empty ones; leaves
+// above kP2MCoop particles, a dense cluster whose serial walk would be the
+// kernel's time, are listed in LDS and taken by the block's waves one each,
+// which does not change their bits), then M2M levels 1..min(L-1, lgT) in LDS
+// from those multipoles (m2m.cu:22-54), every level written out.  The levels
+// above the tiles are made by the fused M2L (or k_m2m_top).
 template <class Sys>
-__global__ __launch_bounds__(kUpTile * kUpTile) void k_m2m_tile(Sys sys) {
+__global__ __launch_bounds__(kUpTile * kUpTile) void k_upward(Sys sys) {
   __shared__ float4 lv[2][kUpTile * kUpTile];
+  __shared__ int s_coop[kUpTile * kUpTile];
+  __shared__ int s_ncoop;
   int blk, nblk;
   const SysView v = sys.get(blk, nblk);
   const int B = v.B;
@@ -227,10 +223,40 @@ __global__ __launch_bounds__(kUpTile * kUpTile) void k_m2m_tile(Sys sys) {
   const int tiles = B / T;
   const int X0 = (blk % tiles) * T, Y0 = (blk / tiles) * T;
   const int t = threadIdx.x;
+  const int lane = lane_id();
+  if (t == 0) s_ncoop = 0;
+  __syncthreads();
+  const bool in = t < T * T;
+  const int c = (Y0 + t / T) * B + X0 + t % T;
+  int a = 0, b = 0;
+  if (in) {
+    a = v.head[c];
+    b = v.head[c + 1];
+  }
+  const bool valid = b > a;
+  const bool coop = valid && b - a > kP2MCoop;
+  float4 m = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (valid && !coop) {
+    m = p2m_leaf_lane(v, a, b);
+    v.mpole[c] = m;
+  }
+  if (in) lv[0][t] = m;
+  if (coop) s_coop[atomicAdd(&s_ncoop, 1)] = t;
+  __syncthreads();
+  const int ncoop = s_ncoop;
+  for (int q = (int)(threadIdx.x / kWave); q < ncoop; q += (kUpTile * kUpTile) / kWave) {
+    const int tq = __builtin_amdgcn_readfirstlane(s_coop[q]);
+    const int cq = (Y0 + tq / T) * B + X0 + tq % T;
+    const int ha = __builtin_amdgcn_readfirstlane(v.head[cq]), hb = __builtin_amdgcn_readfirstlane(v.head[cq + 1]);
+    const float4 mq = p2m_leaf_wave(v, ha, hb, lane);
+    if (lane == 0) {
+      v.mpole[cq] = mq;
+      lv[0][tq] = mq;
+    }
+  }
+  __syncthreads();
   int lgT = 0;
   while ((1 << lgT) < T) ++lgT;
-  if (t < T * T) lv[0][t] = v.mpole[(Y0 + t / T) * B + X0 + t % T];
-  __syncthreads();
   const int top = (v.L - 1) < lgT ? (v.L - 1) : lgT;
   int cur = 0;
   for (int l = 1; l <= top; ++l) {
@@ -238,10 +264,10 @@ __global__ __launch_bounds__(kUpTile * kUpTile) void k_m2m_tile(Sys sys) {
     if (t < Dl * Dl) {
       const int x = t % Dl, y = t / Dl;
       const float4* ch = lv[cur];
-      const float4 m = m2m_combine(ch[(2 * y) * Dc + 2 * x], ch[(2 * y) * Dc + 2 * x + 1],
-                                   ch[(2 * y + 1) * Dc + 2 * x], ch[(2 * y + 1) * Dc + 2 * x + 1]);
-      lv[cur ^ 1][t] = m;
-      v.mpole[v.lvl_off(l) + ((Y0 >> l) + y) * (B >> l) + (X0 >> l) + x] = m;
+      const float4 mm = m2m_combine(ch[(2 * y) * Dc + 2 * x], ch[(2 * y) * Dc + 2 * x + 1],
+                                    ch[(2 * y + 1) * Dc + 2 * x], ch[(2 * y + 1) * Dc + 2 * x + 1]);
+      lv[cur ^ 1][t] = mm;
+      v.mpole[v.lvl_off(l) + ((Y0 >> l) + y) * (B >> l) + (X0 >> l) + x] = mm;
     }
     cur ^= 1;
     __syncthreads();
@@ -319,6 +345,126 @@ __global__ __launch_bounds__(256) void k_m2l(Sys sys) {
   v.local[t] = make_float2(ax, ay);
 }
 
+// log2 of the upward tiles' side: the upward pass makes levels 1..lgT.
+__device__ __host__ __forceinline__ int up_tile_lg(int B) {
+  const int T = B < kUpTile ? B : kUpTile;
+  int lgT = 0;
+  while ((1 << lgT) < T) ++lgT;
+  return lgT;
+}
+
+// One target's M2L sum over its whole 6 x 6 window, branch-free: a window
+// cell that is outside the grid, adjacent to the target or empty enters as a
+// massless source at the target itself, whose term is exactly +0 (d = 0,
+// 0 / eps = 0), so the sum has k_m2l's bits (its running sum is never -0)
+// while the 36 evaluations can overlap.  Rows are loaded one ahead.
+__device__ __forceinline__ float2 m2l_window(const float4* mp, int D, int tx, int ty, float4 tc, float eps) {
+  const int px = tx & ~1, py = ty & ~1;
+  float ax = 0.0f, ay = 0.0f;
+  float4 nxt[6];
+  {
+    const int yc = min(max(py - 2, 0), D - 1);
+#pragma clang loop unroll(full)
+    for (int k = 0; k < 6; ++k) nxt[k] = mp[yc * D + min(max(px - 2 + k, 0), D - 1)];
+  }
+#pragma clang loop unroll(full)
+  for (int wy = -2; wy < 4; ++wy) {
+    const int y = py + wy;
+    const bool row_in = y >= 0 && y < D;
+    float4 sc[6];
+#pragma clang loop unroll(full)
+    for (int k = 0; k < 6; ++k) sc[k] = nxt[k];
+    if (wy + 1 < 4) {
+      const int yn = min(max(y + 1, 0), D - 1);
+#pragma clang loop unroll(full)
+      for (int k = 0; k < 6; ++k) nxt[k] = mp[yn * D + min(max(px - 2 + k, 0), D - 1)];
+    }
+#pragma clang loop unroll(full)
+    for (int k = 0; k < 6; ++k) {
+      const int x = px - 2 + k;
+      const bool use = row_in && x >= 0 && x < D && (abs(x - tx) > 1 || abs(y - ty) > 1) && sc[k].w != 0.0f;
+      kernel_accum_ieee(tc.x, tc.y, use ? sc[k].x : tc.x, use ? sc[k].y : tc.y, use ? sc[k].z : 0.0f, eps, ax, ay);
+    }
+  }
+  return make_float2(ax, ay);
+}
+
+constexpr int kTopLds = 256 + 64 + 16;  // levels lgT+1..L-1 at B = 512
+
+// M2L of the fused paths (solve, step, batches).  Cells without particles
+// get (0, 0) (the P2P reads the locals of occupied leaves and of their
+// ancestors only) and empty source cells are skipped: the other cells' bits
+// are k_m2l's.  When the tree has levels above the upward tiles, the
+// system's block 0 first makes them in LDS from the tiles' top level
+// (m2m_combine, written out as k_m2m_top would) and does their M2L; the
+// other blocks take levels 0..lgT as k_m2l does, one thread per cell.
+template <class Sys>
+__global__ __launch_bounds__(256) void k_m2l_fused(Sys sys) {
+  int blk, nblk;
+  const SysView v = sys.get(blk, nblk);
+  const int lgT = up_tile_lg(v.B);
+  const bool has_top = v.L - 1 > lgT;
+  const float eps = v.eps;
+  if (has_top && blk == 0) {
+    // LDS: level lgT + 1 at 0 (made from level lgT in global memory), each
+    // level above right after the one below
+    __shared__ float4 lv[kTopLds];
+    int D = v.B >> (lgT + 1);
+    {
+      const float4* ch = v.mpole + v.lvl_off(lgT);
+      const int Dp = 2 * D;
+      for (int c = threadIdx.x; c < D * D; c += blockDim.x) {
+        const int x = c % D, y = c / D;
+        const float4 m = m2m_combine(ch[(2 * y) * Dp + 2 * x], ch[(2 * y) * Dp + 2 * x + 1],
+                                     ch[(2 * y + 1) * Dp + 2 * x], ch[(2 * y + 1) * Dp + 2 * x + 1]);
+        lv[c] = m;
+        v.mpole[v.lvl_off(lgT + 1) + c] = m;
+      }
+    }
+    __syncthreads();
+    int po = 0;
+    for (int l = lgT + 2; l < v.L; ++l) {
+      const int Dp = D, o = po + Dp * Dp;
+      D >>= 1;
+      for (int c = threadIdx.x; c < D * D; c += blockDim.x) {
+        const int x = c % D, y = c / D;
+        const float4* ch = lv + po;
+        const float4 m = m2m_combine(ch[(2 * y) * Dp + 2 * x], ch[(2 * y) * Dp + 2 * x + 1],
+                                     ch[(2 * y + 1) * Dp + 2 * x], ch[(2 * y + 1) * Dp + 2 * x + 1]);
+        lv[o + c] = m;
+        v.mpole[v.lvl_off(l) + c] = m;
+      }
+      po = o;
+      __syncthreads();
+    }
+    po = 0;
+    D = v.B >> (lgT + 1);
+    for (int l = lgT + 1; l < v.L; ++l) {
+      const float4* mp = lv + po;
+      for (int c = threadIdx.x; c < D * D; c += blockDim.x)
+        v.local[v.lvl_off(l) + c] =
+            mp[c].w != 0.0f ? m2l_window(mp, D, c % D, c / D, mp[c], eps) : make_float2(0.0f, 0.0f);
+      po += D * D;
+      D >>= 1;
+    }
+    return;
+  }
+  const int t = (blk - (has_top ? 1 : 0)) * blockDim.x + threadIdx.x;
+  if (t >= v.lvl_off(has_top ? lgT + 1 : v.L)) return;
+  int l = 0;
+  while (t >= v.lvl_off(l + 1)) ++l;
+  const int D = v.B >> l;
+  const int c = t - v.lvl_off(l);
+  const int tx = c % D, ty = c / D;
+  const float4* mp = v.mpole + v.lvl_off(l);
+  const float4 tc = mp[c];
+  if (tc.w == 0.0f) {
+    v.local[t] = make_float2(0.0f, 0.0f);
+    return;
+  }
+  v.local[t] = m2l_window(mp, D, tx, ty, tc, eps);
+}
+
 // L2L (l2l.cu:21-36): level l cell += its parent's accumulated effect.
 __global__ __launch_bounds__(256) void k_l2l(SysView v, int l) {
 #pragma clang fp contract(off)
@@ -389,35 +535,46 @@ int upward_tile_blocks(int L, int B) {
   return (B / T) * (B / T);
 }
 
-int upward_top_blocks(int L, int B) {
+int up_blocks(int L, int B) {
+  if (L <= 0) return 0;
   const int T = B < kUpTile ? B : kUpTile;
-  int lgT = 0;
-  while ((1 << lgT) < T) ++lgT;
-  return (L - 1 > lgT) ? 1 : 0;
+  return (B / T) * (B / T);
 }
+
+int upward_top_blocks(int L, int B) { return (L - 1 > up_tile_lg(B)) ? 1 : 0; }
 
 int m2l_blocks(const SysView& v) { return v.L > 0 ? cdiv(v.lvl_off(v.L), 256) : 0; }
 
-int p2m_occ_blocks(int max_occ) { return cdiv(max_occ, kP2MThreads); }
+int m2l_fused_blocks(const SysView& v) {
+  if (v.L <= 0) return 0;
+  const int lgT = up_tile_lg(v.B);
+  if (v.L - 1 > lgT) return 1 + cdiv(v.lvl_off(lgT + 1), 256);
+  return cdiv(v.lvl_off(v.L), 256);
+}
+
 int p2m_leaf_blocks(int B) { return cdiv(B * B, kP2MThreads); }
 
-hipError_t launch_upward(const SysView& v, int max_occ, hipStream_t s) {
+hipError_t launch_upward(const SysView& v, hipStream_t s, bool with_top) {
   if (v.L <= 0) return hipSuccess;
   const OneSys one{v};
-  hipLaunchKernelGGL(k_p2m_occ<OneSys>, dim3(p2m_leaf_blocks(v.B)), dim3(kP2MThreads), 0, s, one);
-  const int tb = upward_tile_blocks(v.L, v.B);
-  if (tb) hipLaunchKernelGGL(k_m2m_tile<OneSys>, dim3(tb), dim3(kUpTile * kUpTile), 0, s, one);
-  if (upward_top_blocks(v.L, v.B)) hipLaunchKernelGGL(k_m2m_top<OneSys>, dim3(1), dim3(1024), 0, s, one);
+  hipLaunchKernelGGL(k_upward<OneSys>, dim3(up_blocks(v.L, v.B)), dim3(kUpTile * kUpTile), 0, s, one);
+  if (with_top && upward_top_blocks(v.L, v.B)) hipLaunchKernelGGL(k_m2m_top<OneSys>, dim3(1), dim3(1024), 0, s, one);
   return hipGetLastError();
 }
 
-hipError_t launch_tree_batch(const BatchLaunch& b, hipStream_t s) {
-  if (b.p2m.blocks) hipLaunchKernelGGL(k_p2m_occ<ManySys>, dim3(b.p2m.blocks), dim3(kP2MThreads), 0, s, b.p2m.sys);
-  if (b.m2m_tile.blocks)
-    hipLaunchKernelGGL(k_m2m_tile<ManySys>, dim3(b.m2m_tile.blocks), dim3(kUpTile * kUpTile), 0, s, b.m2m_tile.sys);
-  if (b.m2m_top.blocks)
+hipError_t launch_m2l_fused(const SysView& v, hipStream_t s) {
+  if (v.L <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_m2l_fused<OneSys>, dim3(m2l_fused_blocks(v)), dim3(256), 0, s, OneSys{v});
+  return hipGetLastError();
+}
+
+hipError_t launch_tree_batch(const BatchLaunch& b, hipStream_t s, bool forward) {
+  if (b.up.blocks) hipLaunchKernelGGL(k_upward<ManySys>, dim3(b.up.blocks), dim3(kUpTile * kUpTile), 0, s, b.up.sys);
+  if (forward) {
+    if (b.m2l.blocks) hipLaunchKernelGGL(k_m2l_fused<ManySys>, dim3(b.m2l.blocks), dim3(256), 0, s, b.m2l.sys);
+  } else if (b.m2m_top.blocks) {
     hipLaunchKernelGGL(k_m2m_top<ManySys>, dim3(b.m2m_top.blocks), dim3(1024), 0, s, b.m2m_top.sys);
-  if (b.m2l.blocks) hipLaunchKernelGGL(k_m2l<ManySys>, dim3(b.m2l.blocks), dim3(256), 0, s, b.m2l.sys);
+  }
   return hipGetLastError();
 }
 
