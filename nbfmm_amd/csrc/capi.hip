@@ -75,6 +75,44 @@ struct SideStream {
 constexpr int kProfStages = 4;
 constexpr int kProfEvents = kProfStages + 1;
 
+// Steps captured in the multi-step graph of a run: a replayed graph starts
+// about 8 us after the previous one ends, while the kernels inside one follow
+// each other without a gap.
+constexpr int kRunGraphSteps = 8;
+
+// Captures `reps` consecutive calls of `body` on st into a graph and
+// instantiates it.
+template <class F>
+int capture_graph(hipStream_t st, int reps, F&& body, hipGraph_t* graph, hipGraphExec_t* exec) {
+  NB_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  int rc = NBFMM_OK;
+  for (int r = 0; r < reps && rc == NBFMM_OK; ++r) rc = body();
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(st, &g);
+  if (rc) {
+    if (g) hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+  *graph = g;
+  NB_HIP(hipGraphInstantiate(exec, g, nullptr, nullptr, 0));
+  return NBFMM_OK;
+}
+
+// Replays `steps` steps: `steps - 1` of kind [0] (the last kind when not
+// chained), the multi-step graph wherever it fits, then one of kind [1].
+int replay_run(hipGraphExec_t* exec, bool chain, int steps, hipStream_t st) {
+  int inner = chain ? steps - 1 : steps;
+  const int mid = chain ? 0 : 1;
+  while (inner >= kRunGraphSteps) {
+    NB_HIP(hipGraphLaunch(exec[2], st));
+    inner -= kRunGraphSteps;
+  }
+  for (; inner > 0; --inner) NB_HIP(hipGraphLaunch(exec[mid], st));
+  if (chain) NB_HIP(hipGraphLaunch(exec[1], st));
+  return NBFMM_OK;
+}
+
 struct nbfmm_solver {
   int L = 0, B = 0, max_n = 0;
   float limits[4] = {0, 0, 0, 0};
@@ -111,10 +149,10 @@ struct nbfmm_solver {
   // (-1: none) that the next sort of the same n starts from
   bool warm_enabled = true;
   int warm_n = -1;
-  // graphs of one step of a run: [0] a step followed by another (its P2P
-  // makes the next sort's first pass), [1] the last step
-  hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
-  hipGraph_t graph[2] = {nullptr, nullptr};
+  // graphs of a run: [0] a step followed by another (its P2P makes the next
+  // sort's first pass), [1] the last step, [2] kRunGraphSteps steps of kind [0]
+  hipGraphExec_t graph_exec[3] = {nullptr, nullptr, nullptr};
+  hipGraph_t graph[3] = {nullptr, nullptr, nullptr};
   SideStream side;  // replays asked for on the null stream run here
   struct {
     int n = -1;
@@ -141,7 +179,7 @@ int dev_alloc(nbfmm_solver* s, T** p, size_t count) {
 }
 
 void free_all(nbfmm_solver* s) {
-  for (int k = 0; k < 2; ++k) {
+  for (int k = 0; k < 3; ++k) {
     if (s->graph_exec[k]) hipGraphExecDestroy(s->graph_exec[k]);
     if (s->graph[k]) hipGraphDestroy(s->graph[k]);
   }
@@ -206,7 +244,7 @@ int adj_grid_for(const nbfmm_solver* s) {
 
 // A captured step graph holds the launch configuration it was made with.
 void drop_graph(nbfmm_solver* s) {
-  for (int k = 0; k < 2; ++k) {
+  for (int k = 0; k < 3; ++k) {
     if (s->graph_exec[k]) { hipGraphExecDestroy(s->graph_exec[k]); s->graph_exec[k] = nullptr; }
     if (s->graph[k]) { hipGraphDestroy(s->graph[k]); s->graph[k] = nullptr; }
   }
@@ -606,21 +644,15 @@ int nbfmm_run(nbfmm_solver* s, int n, float tick, int steps, float* xcur, float*
                     g.w == w && g.acc == acc && g.stream == st && g.warm == chain;
   if (!same) {
     drop_graph(s);
-    for (int k = 0; k < 2; ++k) {
-      if (k == 0 && !chain) continue;  // without the warm start every step is the last kind
-      NB_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-      rc = run_pipeline(s, n, tick, k == 0 ? kP2PStepSort : kP2PStep, io, st, chain);
-      hipGraph_t graph = nullptr;
-      hipError_t e = hipStreamEndCapture(st, &graph);
-      if (rc) { if (graph) hipGraphDestroy(graph); return rc; }
-      if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
-      s->graph[k] = graph;
-      NB_HIP(hipGraphInstantiate(&s->graph_exec[k], graph, nullptr, nullptr, 0));
-    }
+    // without the warm start every step is the last kind
+    const int mid = chain ? kP2PStepSort : kP2PStep;
+    auto one = [&](int mode) { return [&, mode]() { return run_pipeline(s, n, tick, mode, io, st, chain); }; };
+    if (chain && (rc = capture_graph(st, 1, one(kP2PStepSort), &s->graph[0], &s->graph_exec[0]))) return rc;
+    if ((rc = capture_graph(st, 1, one(kP2PStep), &s->graph[1], &s->graph_exec[1]))) return rc;
+    if ((rc = capture_graph(st, kRunGraphSteps, one(mid), &s->graph[2], &s->graph_exec[2]))) return rc;
     g.n = n; g.tick = tick; g.xcur = xcur; g.xpre = xpre; g.w = w; g.acc = acc; g.stream = st; g.warm = chain;
   }
-  for (int k = 0; k < steps; ++k) NB_HIP(hipGraphLaunch(s->graph_exec[chain && k + 1 < steps ? 0 : 1], st));
-  return NBFMM_OK;
+  return replay_run(s->graph_exec, chain, steps, st);
 }
 
 // The forward state the adjoint needs (sorted particles, head, multipoles
@@ -1012,10 +1044,10 @@ struct nbfmm_batch {
   bool balanced = false;
   unsigned long long* d_pairs = nullptr;  // S pair counts
   std::vector<void*> allocs;
-  // graphs of one step of a run, as for a solver: [0] a step followed by
-  // another, [1] the last step
-  hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
-  hipGraph_t graph[2] = {nullptr, nullptr};
+  // graphs of a run, as for a solver: [0] a step followed by another, [1] the
+  // last step, [2] kRunGraphSteps steps of kind [0]
+  hipGraphExec_t graph_exec[3] = {nullptr, nullptr, nullptr};
+  hipGraph_t graph[3] = {nullptr, nullptr, nullptr};
   SideStream side;  // replays asked for on the null stream run here
   struct {
     const void *xcur = nullptr, *xpre = nullptr, *w = nullptr, *acc = nullptr;
@@ -1041,7 +1073,7 @@ int batch_alloc(nbfmm_batch* b, T** p, size_t count) {
 }
 
 void batch_free(nbfmm_batch* b) {
-  for (int k = 0; k < 2; ++k) {
+  for (int k = 0; k < 3; ++k) {
     if (b->graph_exec[k]) hipGraphExecDestroy(b->graph_exec[k]);
     if (b->graph[k]) hipGraphDestroy(b->graph[k]);
   }
@@ -1052,7 +1084,7 @@ void batch_free(nbfmm_batch* b) {
 
 // A captured step graph holds the kernels and launch maps of its capture.
 void drop_batch_graph(nbfmm_batch* b) {
-  for (int k = 0; k < 2; ++k) {
+  for (int k = 0; k < 3; ++k) {
     if (b->graph_exec[k]) { hipGraphExecDestroy(b->graph_exec[k]); b->graph_exec[k] = nullptr; }
     if (b->graph[k]) { hipGraphDestroy(b->graph[k]); b->graph[k] = nullptr; }
   }
@@ -1461,21 +1493,14 @@ int nbfmm_batch_run(nbfmm_batch* b, int steps, float* xcur, float* xpre, const f
                     g.stream == st && g.warm == chain;
   if (!same) {
     drop_batch_graph(b);
-    for (int k = 0; k < 2; ++k) {
-      if (k == 0 && !chain) continue;  // without the warm start every step is the last kind
-      NB_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-      rc = batch_pipeline(b, k == 0 ? kP2PStepSort : kP2PStep, io, st, chain);
-      hipGraph_t graph = nullptr;
-      hipError_t e = hipStreamEndCapture(st, &graph);
-      if (rc) { if (graph) hipGraphDestroy(graph); return rc; }
-      if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
-      b->graph[k] = graph;
-      NB_HIP(hipGraphInstantiate(&b->graph_exec[k], graph, nullptr, nullptr, 0));
-    }
+    const int mid = chain ? kP2PStepSort : kP2PStep;
+    auto one = [&](int mode) { return [&, mode]() { return batch_pipeline(b, mode, io, st, chain); }; };
+    if (chain && (rc = capture_graph(st, 1, one(kP2PStepSort), &b->graph[0], &b->graph_exec[0]))) return rc;
+    if ((rc = capture_graph(st, 1, one(kP2PStep), &b->graph[1], &b->graph_exec[1]))) return rc;
+    if ((rc = capture_graph(st, kRunGraphSteps, one(mid), &b->graph[2], &b->graph_exec[2]))) return rc;
     g.xcur = xcur; g.xpre = xpre; g.w = w; g.acc = acc; g.stream = st; g.valid = true; g.warm = chain;
   }
-  for (int k = 0; k < steps; ++k) NB_HIP(hipGraphLaunch(b->graph_exec[chain && k + 1 < steps ? 0 : 1], st));
-  return NBFMM_OK;
+  return replay_run(b->graph_exec, chain, steps, st);
 }
 
 int nbfmm_batch_set_warm_start(nbfmm_batch* b, int enable) {

@@ -8,7 +8,8 @@
     also at a 10^4 times larger G, where particles change leaf 100 times as
     often;
   * 256 independent disks (N log-uniform in [1024, 131072]): graph-replayed
-    batched steps equal each system's own Solver bit for bit, and ten
+    batched steps (runs of 3 and 11 steps: single-step graphs and the 8-step
+    one) equal each system's own Solver bit for bit, and ten
     sampled systems match the oracle's solve;
   * 64 two-disk collisions (N log-uniform in [4096, 262144]), 100 steps
     checkpointed every 10, observations at steps 25/50/75/100: the batched
@@ -41,7 +42,7 @@ def test_two_disk_collision_graph_chain_equals_cold_sorts(g_scale):
     from nbfmm_amd import Solver
     from nbfmm_amd.workloads import two_disk_collision
     n = 1 << 20
-    wl = two_disk_collision(n=n, steps=10, G=g_scale / n)
+    wl = two_disk_collision(n=n, steps=16, G=g_scale / n)
     assert wl.num_level == 8 and wl.n == n
     warm = Solver(wl.num_level, n, wl.limits, wl.eps)
     cold = Solver(wl.num_level, n, wl.limits, wl.eps)
@@ -52,8 +53,8 @@ def test_two_disk_collision_graph_chain_equals_cold_sorts(g_scale):
     st = torch.cuda.Stream()
     with torch.cuda.stream(st):
         warm.run(n, wl.tick, 4, Xw, Pw, W, Aw, use_graph=True)  # eager step, then chained replays
-        warm.run(n, wl.tick, 6, Xw, Pw, W, Aw, use_graph=True)  # the same graphs again
-        for _ in range(10):
+        warm.run(n, wl.tick, 12, Xw, Pw, W, Aw, use_graph=True)  # the same graphs again, and the 8-step one
+        for _ in range(16):
             cold.step(n, wl.tick, Xc, Pc, W, Ac)
     st.synchronize()
     assert torch.equal(Aw, Ac) and torch.equal(Xw, Xc) and torch.equal(Pw, Pc)
@@ -62,7 +63,7 @@ def test_two_disk_collision_graph_chain_equals_cold_sorts(g_scale):
     assert torch.equal(warm.read_buffer("head"), cold.read_buffer("head"))
     assert warm.flags() == 0 and cold.flags() == 0
     lim = np.array(wl.limits, np.float32)
-    ox, op, oa, clamped = orc.run(wl.num_level, lim, wl.eps, wl.tick, 10, wl.position, wl.position_pre, wl.weight)
+    ox, op, oa, clamped = orc.run(wl.num_level, lim, wl.eps, wl.tick, 16, wl.position, wl.position_pre, wl.weight)
     assert clamped == 0
     xg = Xw.cpu().numpy()
     disp = np.abs(ox - wl.position).max()
@@ -87,6 +88,7 @@ def test_many_disks_batch_equals_single_solvers():
     st = torch.cuda.Stream()
     with torch.cuda.stream(st):
         b.run(3, X, P, W, A, use_graph=True)
+        b.run(11, X, P, W, A, use_graph=True)  # with the 8-step graph
     st.synchronize()
     assert not b.flags().any()
     off = b.offsets
@@ -94,7 +96,7 @@ def test_many_disks_batch_equals_single_solvers():
         s = Solver(sy.num_level, sy.n, sy.limits, sy.eps)
         xs, ps, ws = dev(sy.position), dev(sy.position_pre), dev(sy.weight)
         a = torch.zeros_like(xs)
-        for _ in range(3):
+        for _ in range(14):
             s.step(sy.n, sy.tick, xs, ps, ws, a)
         torch.cuda.synchronize()
         sl = slice(off[k], off[k + 1])

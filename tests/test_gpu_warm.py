@@ -55,7 +55,7 @@ def test_steps_bitwise_equal_cold():
 
 def test_graph_run_bitwise_equal_cold():
     from nbfmm_amd.workloads import two_disk_collision
-    wl = two_disk_collision(n=1 << 15, num_level=5, steps=10)
+    wl = two_disk_collision(n=1 << 15, num_level=5, steps=29)
     warm, cold = pair(wl)
     Xw, Pw, W = dev(wl.position), dev(wl.position_pre), dev(wl.weight)
     Xc, Pc = Xw.clone(), Pw.clone()
@@ -64,8 +64,9 @@ def test_graph_run_bitwise_equal_cold():
     with torch.cuda.stream(st):
         warm.run(wl.n, wl.tick, 4, Xw, Pw, W, Aw, use_graph=True)   # cold eager step, then warm replays
         warm.run(wl.n, wl.tick, 6, Xw, Pw, W, Aw, use_graph=True)   # same graph again
+        warm.run(wl.n, wl.tick, 19, Xw, Pw, W, Aw, use_graph=True)  # two 8-step graphs, one single, the last
     st.synchronize()
-    for _ in range(10):
+    for _ in range(29):
         cold.step(wl.n, wl.tick, Xc, Pc, W, Ac)
     torch.cuda.synchronize()
     assert torch.equal(Aw, Ac) and torch.equal(Xw, Xc) and torch.equal(Pw, Pc)
