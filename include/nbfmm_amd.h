@@ -195,11 +195,22 @@ int nbfmm_pair_count(nbfmm_solver* solver, int num_particle, double* pairs, int*
  * the same num_particle: particles whose leaf did not change keep their order,
  * and only the others are placed.  The result is the same stable sort, bit
  * for bit.  When a leaf gains more than 512 particles in one call, the cold
- * sort's passes run instead (decided on the device, inside the same launches
- * plus one), so a call never costs much more than a cold sort.  Writing a buffer through nbfmm_buffer_write
- * makes the next sort cold.  Enabled by default; disabling it forces the
- * cold sort every call. */
+ * sort's passes run instead (decided on the device, inside the same
+ * launches), so a call never costs much more than a cold sort.  Writing a
+ * buffer through nbfmm_buffer_write makes the next sort cold.  Enabled by
+ * default; disabling it forces the cold sort every call. */
 int nbfmm_set_warm_start(nbfmm_solver* solver, int enable);
+
+/* ---------------------------------------------------------------- small trees
+ * No reference counterpart.  For num_level 1..4 and 1..4096 particles,
+ * solve, step and run sort the particles and build every multipole level in
+ * one kernel of one block (the reference's predo, P2M and M2M,
+ * source/nbfmm/core/solver/predo.cu:26-135, p2m.cu:24-110, m2m.cu:22-54),
+ * which at these sizes replaces five latency-bound launches; M2L and the
+ * P2P epilogue follow as usual.  Its sort is cold, so the warm start does not
+ * apply there.  Every array and result is the same, bit for bit.  Enabled
+ * by default; disabling it takes the general kernels. */
+int nbfmm_set_small_front(nbfmm_solver* solver, int enable);
 
 /* ---------------------------------------------------------------- models
  * Initial conditions of the reference's star models (the .cu files under source/nbfmm/model/,

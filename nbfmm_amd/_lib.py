@@ -57,6 +57,7 @@ SIGNATURES = {
     "nbfmm_p2p_variant_info": [c_int, ctypes.POINTER(c_int), ctypes.POINTER(c_int), ctypes.POINTER(c_int)],
     "nbfmm_set_p2p_config": [P, c_int, c_int],
     "nbfmm_set_warm_start": [P, c_int],
+    "nbfmm_set_small_front": [P, c_int],
     "nbfmm_prune": [P, c_int, P, P, P, ctypes.POINTER(c_int), P],
     "nbfmm_model_disk": [c_int, c_float, c_float, c_float, c_float, c_float, c_float, ctypes.c_uint, P, P, P, P],
     "nbfmm_model_disk_static": [c_int, c_float, c_float, c_float, c_float, ctypes.c_uint, P, P, P, P],

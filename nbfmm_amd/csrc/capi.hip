@@ -149,6 +149,8 @@ struct nbfmm_solver {
   // (-1: none) that the next sort of the same n starts from
   bool warm_enabled = true;
   int warm_n = -1;
+  // small trees: the sort, P2M and M2M of solve/step/run in one block
+  bool front_enabled = true;
   // graphs of a run: [0] a step followed by another (its P2P makes the next
   // sort's first pass), [1] the last step, [2] kRunGraphSteps steps of kind [0]
   hipGraphExec_t graph_exec[3] = {nullptr, nullptr, nullptr};
@@ -278,6 +280,11 @@ int sort_stage(nbfmm_solver* s, const SysView& v, const CallIO& io, hipStream_t 
   return NBFMM_OK;
 }
 
+// Whether solve/step/run of n particles take the one-block front (sort,
+// P2M, M2M) of small trees; its sort is cold, so their steps do not chain
+// the next sort's first pass.
+bool use_front(const nbfmm_solver* s, int n) { return s->front_enabled && small_front_fits(s->L, n); }
+
 // The fused pipeline shared by solve and step.
 int run_pipeline(nbfmm_solver* s, int n, float tick, int mode, const CallIO& io, hipStream_t st,
                  bool gathered = false) {
@@ -288,11 +295,18 @@ int run_pipeline(nbfmm_solver* s, int n, float tick, int mode, const CallIO& io,
     if (int rc = prof_events(s, &E)) return rc;
     NB_HIP(hipEventRecord(E[0], st));
   }
-  int rc = sort_stage(s, v, io, st, gathered);
-  if (rc) return rc;
-  if (n == 0) return NBFMM_OK;
-  if (E) NB_HIP(hipEventRecord(E[1], st));
-  NB_HIP(launch_upward(v, st, false));  // P2M + M2M of the tiles
+  const bool front = use_front(s, n);
+  if (front) {
+    NB_HIP(launch_small_front(v, io, st));
+    s->warm_n = n;
+    if (E) NB_HIP(hipEventRecord(E[1], st));
+  } else {
+    int rc = sort_stage(s, v, io, st, gathered);
+    if (rc) return rc;
+    if (n == 0) return NBFMM_OK;
+    if (E) NB_HIP(hipEventRecord(E[1], st));
+    NB_HIP(launch_upward(v, st, false));  // P2M + M2M of the tiles
+  }
   if (E) NB_HIP(hipEventRecord(E[2], st));
   NB_HIP(launch_m2l_fused(v, st));  // the M2M levels above the tiles + M2L
   if (E) NB_HIP(hipEventRecord(E[3], st));
@@ -575,6 +589,13 @@ int nbfmm_set_warm_start(nbfmm_solver* s, int enable) {
   return NBFMM_OK;
 }
 
+int nbfmm_set_small_front(nbfmm_solver* s, int enable) {
+  if (!s) return fail(NBFMM_ERR_INVALID_ARGUMENT, "null solver");
+  s->front_enabled = enable != 0;
+  drop_graph(s);
+  return NBFMM_OK;
+}
+
 int nbfmm_solver_flags(nbfmm_solver* s, unsigned* flags, int clear) {
   if (!s || !flags) return fail(NBFMM_ERR_INVALID_ARGUMENT, "null argument");
   NB_HIP(hipDeviceSynchronize());
@@ -609,7 +630,7 @@ int nbfmm_run(nbfmm_solver* s, int n, float tick, int steps, float* xcur, float*
     // eager: the launches of the replayed steps, one by one (with the warm
     // start on, every step but the last makes the next sort's first pass)
     CallIO io = call_io(xcur, w, acc, xcur, xpre);
-    const bool chain = s->warm_enabled && n > 0;
+    const bool chain = s->warm_enabled && n > 0 && !use_front(s, n);
     for (int k = 0; k < steps; ++k) {
       const int mode = chain && k + 1 < steps ? kP2PStepSort : kP2PStep;
       if ((rc = run_pipeline(s, n, tick, mode, io, st, chain && k > 0))) return rc;
@@ -636,7 +657,7 @@ int nbfmm_run(nbfmm_solver* s, int n, float tick, int steps, float* xcur, float*
   // step leaves nothing pending, and the next call sorts from the caller's
   // arrays as usual.
   CallIO io = call_io(xcur, w, acc, xcur, xpre);
-  const bool chain = s->warm_enabled;  // after the first step's sort, warm_n == n
+  const bool chain = s->warm_enabled && !use_front(s, n);  // after the first step's sort, warm_n == n
   if ((rc = run_pipeline(s, n, tick, chain && steps > 1 ? kP2PStepSort : kP2PStep, io, st))) return rc;
   if (--steps == 0) return NBFMM_OK;
   auto& g = s->gkey;

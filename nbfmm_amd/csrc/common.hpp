@@ -32,6 +32,24 @@ enum P2PMode : int {
   kP2PStepSort = 3, // as kP2PStep, and the first pass of the next step's warm-started sort
 };
 
+// Split of P2P runs: a run whose staged union holds more than S_min sources
+// is summed in K = min(kP2PMaxSeg, ceil(U / S_len)) segments of the union, by
+// whichever waves of the block are free, and the segment sums are added in
+// segment order.  K and the segment bounds depend on U and the system's
+// particle count alone, so every kernel configuration, the solver, the batch
+// and the allocation-free solve give the same bits.  Large systems
+// split only heavy runs (dense clusters, whose serial sum would be the
+// kernel's time); a system of at most kP2PSmallN particles has about n / 64
+// runs for the GPU's 1024 SIMDs, so its runs are split from
+// kP2PSmallSplitMin sources into segments of kP2PSmallSegLen (S_min, S_len;
+// kP2PSplitMin and kP2PSegLen otherwise): the block's idle waves share each
+// run and the chain of a wave is a few times shorter.
+constexpr int kP2PSplitMin = 2048;
+constexpr int kP2PSegLen = 1024;
+constexpr int kP2PSmallN = 1 << 16;
+constexpr int kP2PSmallSplitMin = 256;
+constexpr int kP2PSmallSegLen = 128;
+
 // Everything one system's kernels need: sizes, box and the device arrays.
 // Passed by value to every kernel.
 struct SysView {
@@ -107,13 +125,6 @@ constexpr uint32_t kMoverBit = 0x80000000u;  // tkey flag: the particle changed 
 constexpr int kWarmTouched = 1, kWarmCold = 2, kWarmColdReq = 3, kWarmInts = 4;
 constexpr int kMergeLdsMovers = 512;  // movers of one leaf the merge sorts in LDS
 
-// Split of heavy P2P runs: a run whose staged union holds more than
-// kP2PSplitMin sources is summed in K = min(kP2PMaxSeg, ceil(U / kP2PSegLen))
-// segments of the union, by whichever waves of the block are free, and the
-// segment sums are added in segment order.  K and the segment bounds depend
-// on U alone, so every kernel configuration gives the same bits.
-constexpr int kP2PSplitMin = 2048;
-constexpr int kP2PSegLen = 1024;
 constexpr int kP2PMaxSeg = 16;
 constexpr uint32_t kFlagP2PWait = 2u;      // NBFMM_FLAG_P2P_WAIT
 constexpr uint32_t kFlagP2PProtocol = 4u;  // NBFMM_FLAG_P2P_LOST

@@ -55,6 +55,10 @@ hipError_t launch_p2m(const SysView& v, hipStream_t s);
 hipError_t launch_upward(const SysView& v, hipStream_t s, bool with_top);
 // fused paths' M2L: cells with particles only, and the M2M levels above the tiles
 hipError_t launch_m2l_fused(const SysView& v, hipStream_t s);
+// small trees (small_front_fits): cold sort + P2M + every M2M level in one
+// block, the same arrays and bits as the sort + upward pass + fused M2L's top
+bool small_front_fits(int L, int n);
+hipError_t launch_small_front(const SysView& v, const CallIO& io, hipStream_t s);
 // forward: upward pass + fused M2L; else (adjoints) upward pass + the top M2M levels
 hipError_t launch_tree_batch(const BatchLaunch& b, hipStream_t s, bool forward);
 int upward_tile_blocks(int L, int B);  // adjoint upward tiles (0 when there is no M2M level)

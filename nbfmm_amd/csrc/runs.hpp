@@ -9,10 +9,11 @@
 // table (runs before each row) from head and its waves draw runs b, b + nblk,
 // ... through an LDS counter.
 //
-// A run whose union exceeds kP2PSplitMin (a cluster of dense leaves) would be
-// one long serial chain on one wave, so it is split into K segments of the
-// union: K = min(kP2PMaxSeg, ceil(U / kP2PSegLen)) and segments of S sources
-// (a multiple of 64), both functions of the union size U alone.  The wave
+// A run whose union exceeds the system's split threshold (a cluster of dense
+// leaves; in a small system, most runs) would be one long serial chain on one
+// wave, so it is split into K segments of the union (p2p_segments: K and
+// segments of S sources, a multiple of 64, both functions of the union size U
+// and the system's particle count alone).  The wave
 // that takes such a run publishes it in the block's slot; every wave of the
 // block that is free takes a segment (a ticket from an LDS counter,
 // generation and K in its high bits), sums it, waits until the segments
@@ -87,12 +88,15 @@ __device__ __forceinline__ void run_geom(const SysView& v, const int* s_runs, in
   }
 }
 
-// Segments of a union of `total` sources: 1 up to kP2PSplitMin, else
-// min(kP2PMaxSeg, ceil(total / kP2PSegLen)) segments of S sources (a
-// multiple of 64), the last one shorter.
-__device__ __forceinline__ int p2p_segments(int total) {
-  if (total <= kP2PSplitMin) return 1;
-  return min(kP2PMaxSeg, (total + kP2PSegLen - 1) / kP2PSegLen);
+// Segments of a union of `total` sources in a system of v.n particles: 1 up
+// to S_min, else min(kP2PMaxSeg, ceil(total / S_len)) segments of S sources
+// (a multiple of 64), the last one shorter; S_min, S_len are the small
+// system's rule when v.n <= kP2PSmallN.
+__device__ __forceinline__ int p2p_segments(const SysView& v, int total) {
+  const bool small = v.n <= kP2PSmallN;
+  if (total <= (small ? kP2PSmallSplitMin : kP2PSplitMin)) return 1;
+  const int len = small ? kP2PSmallSegLen : kP2PSegLen;
+  return min(kP2PMaxSeg, (total + len - 1) / len);
 }
 __device__ __forceinline__ int p2p_segment_len(int total, int K) { return (((total + K - 1) / K) + 63) & ~63; }
 
@@ -156,9 +160,9 @@ __device__ __forceinline__ uint32_t wave_load(uint32_t* w) {
 
 // The block's waves work through its runs.  seg(g, sa, sb, acc) sums this
 // lane's sources among union positions [sa, sb) of run g into acc[NACC]
-// (from zero); fin(g, acc) is the epilogue of run g (HK: g.hk is loaded).  The slot must have been
-// initialised and the block synchronised.  v.wait_polls bounds every wait
-// on another wave.
+// (from zero); fin(g, acc) is the epilogue of run g (HK: g.hk is loaded).
+// The slot must have been initialised and the block synchronised.
+// v.wait_polls bounds every wait on another wave.
 template <int NACC, bool HK, class Seg, class Fin>
 __device__ __forceinline__ void run_queue(const SysView& v, const int* s_runs, RunSlot<NACC>& sl, int blk, int nblk,
                                           int total, int lane, Seg&& seg, Fin&& fin) {
@@ -257,7 +261,7 @@ __device__ __forceinline__ void run_queue(const SysView& v, const int* s_runs, R
           continue;
         }
         run_geom<HK>(v, s_runs, j, lane, g);
-        K = p2p_segments(g.total);
+        K = p2p_segments(v, g.total);
         // a light run can publish nothing: it stops counting at once, so
         // that waves without work need not stay for its end
         if (K == 1 && lane == 0)

@@ -274,6 +274,199 @@ __global__ __launch_bounds__(kUpTile * kUpTile) void k_upward(Sys sys) {
   }
 }
 
+// Front of a small tree's step in one block (1 <= L, B <= kFrontMaxB,
+// n <= kFrontMaxN): the cold sort (keys, particles, head, row starts,
+// occupied leaves; out-of-box particles clamped and flagged as bin_key does)
+// and the multipoles of every level, as the sort, the upward pass and the
+// fused M2L's top levels write them, bit for bit (the same P2M and M2M
+// expressions on the same sorted particles).  At these sizes each of those
+// kernels is a few dependent memory round trips on an otherwise idle GPU;
+// here one block runs them back to back, its barriers in place of kernel
+// boundaries (predo.cu:26-135, p2m.cu:24-110, m2m.cu:22-54).
+//
+// Sort: stable counting sort by leaf.  Wave w owns the particles
+// [w P, w P + P) in input order.  Pass 1 counts them per leaf in the wave's
+// LDS histogram; the scan turns the histograms into each wave's first slot
+// in every leaf (leaf start + the counts of the waves before it); pass 2
+// walks the wave's particles again in order, 64 at a time, ranking equal
+// leaves inside the 64 with ballots, and scatters them.
+constexpr int kFrontThreads = 1024;
+constexpr int kFrontWaves = kFrontThreads / kWave;
+constexpr int kFrontMaxB = 32;
+constexpr int kFrontMaxC = kFrontMaxB * kFrontMaxB;
+// Measured crossover (10-step graph runs of a uniform disk, 32 x 32
+// leaves): the block is faster up to about 6000 particles (29.9 against
+// 36.0 us per step at 512, 36.8 against 40.2 at 4096) and slower beyond
+// (72 against 53 at 16384), where one CU's memory pipeline spends about 64
+// cycles on each wave access to 64 scattered addresses (the permutation's
+// stores, the per-leaf P2M reads) that the general kernels spread over the
+// whole GPU.
+constexpr int kFrontMaxN = 4096;
+constexpr int kFrontBatch = 4;  // 64-particle groups whose loads are issued together
+
+struct FrontLds {
+  union {
+    uint32_t cnt[kFrontWaves][kFrontMaxC];   // sort: per-wave leaf counts, then slots
+    float4 mp[(4 * kFrontMaxC) / 3 + 1];     // every level's multipoles (lvl_off layout)
+  } u;
+  int start[kFrontMaxC + 1];                 // head
+  int coop[kFrontMaxC];                      // leaves above kP2MCoop particles
+  int ncoop;
+  uint32_t sh[kFrontWaves];
+};
+
+__global__ __launch_bounds__(kFrontThreads) void k_small_front(SysView v, CallIO io) {
+#pragma clang fp contract(off)
+  __shared__ FrontLds S;
+  const int C = v.B * v.B;
+  const int n = v.n;
+  const int t = threadIdx.x;
+  const int lane = lane_id();
+  const int wave = t / kWave;
+  const float2* pos0 = io.pos0 + v.part_off;
+  const float* w0 = io.w0 + v.part_off;
+  constexpr int kSpan = kFrontWaves * kWave;
+  const int P = ((n + kSpan - 1) / kSpan) * kWave;
+  const int lo = min(wave * P, n), hi = min(lo + P, n);
+  if (t < C) {
+#pragma unroll
+    for (int w = 0; w < kFrontWaves; ++w) S.u.cnt[w][t] = 0;
+  }
+  if (t == 0) S.ncoop = 0;
+  __syncthreads();
+  // pass 1: leaf counts of the wave's particles
+  bool bad_any = false;
+  for (int i0 = lo; i0 < hi; i0 += kFrontBatch * kWave) {
+    float2 q[kFrontBatch];
+#pragma unroll
+    for (int k = 0; k < kFrontBatch; ++k) {
+      const int i = i0 + k * kWave + lane;
+      q[k] = i < hi ? pos0[i] : make_float2(0.0f, 0.0f);
+    }
+#pragma unroll
+    for (int k = 0; k < kFrontBatch; ++k) {
+      const int i = i0 + k * kWave + lane;
+      if (i < hi) {
+        bool bad;
+        const uint32_t key = bin_key(v, q[k], bad);
+        bad_any |= bad;
+        atomicAdd(&S.u.cnt[wave][key], 1u);
+      }
+    }
+  }
+  if (bad_any) atomicOr(v.flags, 1u);
+  __syncthreads();
+  // scan: the waves' slots in leaf t, head, row starts, occupied leaves
+  {
+    uint32_t tot = 0;
+    if (t < C) {
+#pragma unroll
+      for (int w = 0; w < kFrontWaves; ++w) {
+        const uint32_t k = S.u.cnt[w][t];
+        S.u.cnt[w][t] = tot;
+        tot += k;
+      }
+    }
+    uint32_t all, nocc;
+    const uint32_t e = block_exclusive_scan<kFrontThreads>(tot, S.sh, &all);
+    const uint32_t eo = block_exclusive_scan<kFrontThreads>(tot > 0 ? 1u : 0u, S.sh, &nocc);
+    if (t < C) {
+#pragma unroll
+      for (int w = 0; w < kFrontWaves; ++w) S.u.cnt[w][t] += e;
+      S.start[t] = (int)e;
+      v.head[t] = (int)e;
+      if ((t & (v.B - 1)) == 0) v.rowstart[t >> v.lgB] = (int)e;
+      if (tot > 0) v.occ[eo] = t;
+    }
+    if (t == 0) {
+      S.start[C] = n;
+      v.head[C] = n;
+      v.rowstart[v.B] = n;
+      *v.nocc = (int)nocc;
+    }
+  }
+  __syncthreads();
+  // pass 2: stable scatter
+  {
+    const uint64_t below = lanes_below();
+    for (int i0 = lo; i0 < hi; i0 += kFrontBatch * kWave) {
+      float2 q[kFrontBatch];
+      float m[kFrontBatch];
+#pragma unroll
+      for (int k = 0; k < kFrontBatch; ++k) {
+        const int i = i0 + k * kWave + lane;
+        q[k] = i < hi ? pos0[i] : make_float2(0.0f, 0.0f);
+        m[k] = i < hi ? w0[i] : 0.0f;
+      }
+#pragma unroll
+      for (int k = 0; k < kFrontBatch; ++k) {
+        const int i = i0 + k * kWave + lane;
+        const bool valid = i < hi;
+        bool bad;
+        const uint32_t key = valid ? bin_key(v, q[k], bad) : 0u;
+        const uint64_t mm = match_mask(key, valid, 2 * v.lgB);
+        const uint32_t r = (uint32_t)__popcll(mm & below);
+        uint32_t prior = 0;
+        if (valid) prior = S.u.cnt[wave][key];
+        wave_lds_order();
+        if (valid && r == 0) S.u.cnt[wave][key] = prior + (uint32_t)__popcll(mm);
+        wave_lds_order();
+        if (valid) {
+          const uint32_t p = prior + r;
+          v.key[p] = key;
+          v.part[p] = make_float4(q[k].x, q[k].y, m[k], __uint_as_float((uint32_t)i));
+        }
+      }
+    }
+  }
+  __syncthreads();
+  // P2M (the counts are dead: their LDS now holds the multipoles)
+  if (t < C) {
+    const int a = S.start[t], b = S.start[t + 1];
+    if (b - a > kP2MCoop) {
+      S.coop[atomicAdd(&S.ncoop, 1)] = t;
+    } else {
+      const float4 mt = b > a ? p2m_leaf_lane(v, a, b) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      S.u.mp[t] = mt;
+      v.mpole[t] = mt;
+    }
+  }
+  __syncthreads();
+  const int ncoop = S.ncoop;
+  for (int q = wave; q < ncoop; q += kFrontWaves) {
+    const int cq = __builtin_amdgcn_readfirstlane(S.coop[q]);
+    const int ha = __builtin_amdgcn_readfirstlane(S.start[cq]);
+    const int hb = __builtin_amdgcn_readfirstlane(S.start[cq + 1]);
+    const float4 mq = p2m_leaf_wave(v, ha, hb, lane);
+    if (lane == 0) {
+      S.u.mp[cq] = mq;
+      v.mpole[cq] = mq;
+    }
+  }
+  __syncthreads();
+  // M2M, levels 1..L-1
+  for (int l = 1; l < v.L; ++l) {
+    const int D = v.B >> l, Dc = 2 * D;
+    const float4* ch = S.u.mp + v.lvl_off(l - 1);
+    if (t < D * D) {
+      const int x = t % D, y = t / D;
+      const float4 mm = m2m_combine(ch[(2 * y) * Dc + 2 * x], ch[(2 * y) * Dc + 2 * x + 1],
+                                    ch[(2 * y + 1) * Dc + 2 * x], ch[(2 * y + 1) * Dc + 2 * x + 1]);
+      S.u.mp[v.lvl_off(l) + t] = mm;
+      v.mpole[v.lvl_off(l) + t] = mm;
+    }
+    __syncthreads();
+  }
+}
+
+bool small_front_fits(int L, int n) { return L >= 1 && (2 << L) <= kFrontMaxB && n > 0 && n <= kFrontMaxN; }
+
+hipError_t launch_small_front(const SysView& v, const CallIO& io, hipStream_t s) {
+  if (!small_front_fits(v.L, v.n)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_small_front, dim3(1), dim3(kFrontThreads), 0, s, v, io);
+  return hipGetLastError();
+}
+
 // The M2M levels above the tiles (levels lgT+1..L-1), one block.
 template <class Sys>
 __global__ __launch_bounds__(1024) void k_m2m_top(Sys sys) {

@@ -219,6 +219,11 @@ class Solver:
         results are identical either way."""
         check(lib().nbfmm_set_warm_start(self._h, int(bool(enable))))
 
+    def small_front(self, enable: bool = True) -> None:
+        """num_level 1..4 and N <= 4096: sort, P2M and M2M in one one-block
+        kernel (default on); results are identical either way."""
+        check(lib().nbfmm_set_small_front(self._h, int(bool(enable))))
+
     def set_p2p_wait_polls(self, polls: int) -> None:
         """Bound on every wait of a P2P wave for another wave of its block;
         past it the wave sums its run alone (FLAG_P2P_WAIT; same results,

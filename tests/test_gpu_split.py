@@ -1,5 +1,6 @@
-"""P2P runs whose source union exceeds 2048 (clusters of dense leaves) are
-summed in segments by several waves and combined in segment order.  The
+"""P2P runs whose source union exceeds the split threshold (2048 sources;
+256 in systems of at most 65536 particles, whose runs are too few to fill the
+GPU) are summed in segments by several waves and combined in segment order.  The
 forces must match the oracle, be the same bits for every kernel
 configuration (which decides how many waves help), equal graph replay and
 batches, and never raise the wait flag.  The adjoint P2P sums such runs
