@@ -96,15 +96,23 @@ int compute_index(int n, int B, const T* pos, T xmin, T ymin, T cw, T ch, int* i
 // their input order.
 // Pinned by: numpy argsort(kind="stable"), bincount/cumsum head and the
 // predo checks of check/core/solver/predo.cu:50-66.
+// given (finite differences with the cells held fixed, see orc_run_f64_cells):
+// the cell index pairs to use instead of computing them; record: where to
+// write the ones used.
 template <class T>
 int predo(int L, int n, const T* pos0, const T* w0, const T limits[4], T* pos, T* w, int* index,
-          int* perm, int* head) {
+          int* perm, int* head, const int* given = nullptr, int* record = nullptr) {
   const int B = 1 << (L + 1);
   const int C = B * B;
   const T cw = (limits[2] - limits[0]) / T(B);
   const T ch = (limits[3] - limits[1]) / T(B);
   std::vector<int> idx0(2 * (size_t)std::max(n, 1));
-  const int clamped = compute_index<T>(n, B, pos0, limits[0], limits[1], cw, ch, idx0.data());
+  int clamped = 0;
+  if (given)
+    std::copy(given, given + 2 * (size_t)n, idx0.begin());
+  else
+    clamped = compute_index<T>(n, B, pos0, limits[0], limits[1], cw, ch, idx0.data());
+  if (record) std::copy(idx0.begin(), idx0.begin() + 2 * (size_t)n, record);
   std::vector<int> count(C + 1, 0);
   for (int i = 0; i < n; ++i) count[idx0[2 * i] + idx0[2 * i + 1] * B]++;
   head[0] = 0;
@@ -317,7 +325,8 @@ int cube_levels(int L) { return L < 1 ? 1 : L; }
 // Pinned by: partition exactness, momentum conservation, the exact regime
 // (<=1 particle per cell at every level == direct sum) and L=0 == direct sum.
 template <class T>
-int solve(int L, int n, const T limits[4], T eps, const T* pos0, const T* w0, T* eff0) {
+int solve(int L, int n, const T limits[4], T eps, const T* pos0, const T* w0, T* eff0, const int* given = nullptr,
+          int* record = nullptr) {
   const int B = 1 << (L + 1);
   const size_t C = (size_t)B * B;
   const size_t nn = (size_t)std::max(n, 1);
@@ -325,7 +334,8 @@ int solve(int L, int n, const T limits[4], T eps, const T* pos0, const T* w0, T*
   std::vector<int> index(2 * nn), perm(nn), head(C + 1);
   const size_t cube = C * cube_levels(L);
   std::vector<T> cpos(2 * cube, T(0)), cw(cube, T(0)), ceff(2 * cube, T(0));
-  const int clamped = predo<T>(L, n, pos0, w0, limits, pos.data(), w.data(), index.data(), perm.data(), head.data());
+  const int clamped =
+      predo<T>(L, n, pos0, w0, limits, pos.data(), w.data(), index.data(), perm.data(), head.data(), given, record);
   p2p<T>(L, n, pos.data(), w.data(), index.data(), head.data(), eps, eff.data());
   p2m<T>(L, n, pos.data(), w.data(), head.data(), cpos.data(), cw.data());
   m2m<T>(L, cpos.data(), cw.data());
@@ -639,6 +649,24 @@ int orc_run_f64(int L, int n, const double limits[4], double eps, double tick, i
   int clamped = 0;
   for (int s = 0; s < steps; ++s) {
     clamped += solve<double>(L, n, limits, eps, xcur, w, acc);
+    update<double>(n, tick, xcur, xpre, acc);
+  }
+  return clamped;
+}
+
+// orc_run_f64 with the cells of every step recorded (record = 1: cells gets
+// steps x n index pairs) or replayed from cells (record = 0), for finite
+// differences of the trajectory with the cells held fixed, the function a
+// cells-fixed adjoint differentiates.  No reference counterpart.
+// Pinned by: recording gives orc_run_f64's bits, replaying the recorded cells
+// gives them again, and central differences of a replayed loss at h = 1e-3
+// and h = 1e-6 agree (the frozen-cell loss is smooth).
+int orc_run_f64_cells(int L, int n, const double limits[4], double eps, double tick, int steps, double* xcur,
+                      double* xpre, const double* w, double* acc, int* cells, int record) {
+  int clamped = 0;
+  for (int s = 0; s < steps; ++s) {
+    int* c = cells + (size_t)s * 2 * (size_t)n;
+    clamped += solve<double>(L, n, limits, eps, xcur, w, acc, record ? nullptr : c, record ? c : nullptr);
     update<double>(n, tick, xcur, xpre, acc);
   }
   return clamped;

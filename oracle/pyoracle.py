@@ -106,6 +106,9 @@ def lib():
         L.orc_run_f32.restype = c_int
         L.orc_run_f64.argtypes = [c_int, c_int, _f64p, c_double, c_double, c_int, _f64p, _f64p, _f64p, _f64p]
         L.orc_run_f64.restype = c_int
+        L.orc_run_f64_cells.argtypes = [c_int, c_int, _f64p, c_double, c_double, c_int, _f64p, _f64p, _f64p, _f64p,
+                                        _i32p, c_int]
+        L.orc_run_f64_cells.restype = c_int
         L.orc_direct_f64.argtypes = [c_int, _f32p, _f32p, c_double, c_int, ctypes.c_void_p, _f64p]
         L.orc_prune_f32.argtypes = [c_int, _f32p, _f32p, _f32p, _f32p]
         L.orc_prune_f32.restype = c_int
@@ -268,6 +271,22 @@ def run_f64(L, limits, eps, tick, steps, xcur, xpre, w):
     lib().orc_run_f64(L, xcur.shape[0], np.ascontiguousarray(limits, np.float64), float(eps), float(tick),
                       int(steps), xcur, xpre, np.ascontiguousarray(w, np.float64), acc)
     return xcur, xpre
+
+
+def run_f64_cells(L, limits, eps, tick, steps, xcur, xpre, w, cells=None):
+    """run_f64 with every step's cell index pairs recorded (cells None:
+    returned, shape [steps, n, 2]) or replayed from `cells` (the cells held
+    fixed): returns (xcur, xpre, cells)."""
+    xcur = np.ascontiguousarray(xcur, np.float64).copy()
+    xpre = np.ascontiguousarray(xpre, np.float64).copy()
+    n = xcur.shape[0]
+    acc = np.zeros_like(xcur)
+    record = cells is None
+    c = np.zeros((steps, n, 2), np.int32) if record else np.ascontiguousarray(cells, np.int32).copy()
+    assert c.shape == (steps, n, 2)
+    lib().orc_run_f64_cells(L, n, np.ascontiguousarray(limits, np.float64), float(eps), float(tick), int(steps),
+                            xcur, xpre, np.ascontiguousarray(w, np.float64), acc, c, int(record))
+    return xcur, xpre, c
 
 
 def direct_f64(pos, w, eps, targets=None):

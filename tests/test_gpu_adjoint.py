@@ -13,6 +13,9 @@ Pins:
     (sum_j m_j dloss/dm_j = sum_i g_i . a_i);
   * massless particles (the plain pair) beside massive ones (the pair with
     the target's gradient divided by its mass), against the same differences;
+  * the mass-inference configuration's own check: an N = 256 subset of its
+    disk, h = 1e-3 central differences of the fp64 oracle's trajectory loss
+    against the fp32 GPU gradients (positions, velocities, masses);
   * checkpointing: K = 1, K = 4 and K = steps give the same bits;
   * batches: system by system the same bits as separate solvers.
 """
@@ -384,3 +387,92 @@ def test_batch_trajectory_checkpoints_and_single():
             assert torch.equal(a[sl], t.grad), k
         s.close()
     b.close()
+
+
+@pytest.mark.parametrize("g_per_n,bar", [(1e4, 2e-3), (3e4, 1e-3)])
+def test_mass_inference_fd_check_256_subset(g_per_n, bar):
+    """The mass-inference configuration's finite-difference check as stated:
+    an N = 256 subset of its 65536-particle Gaussian disk (first 256
+    particles, their velocities; its box, 128 x 128 leaves, eps and dt), 50
+    Verlet steps checkpointed every 10, loss = mean squared distance of the
+    final positions to targets from a run with the masses perturbed by 5%
+    Gaussian noise; the GPU (fp32) gradients with respect to initial
+    positions (at fixed velocity), velocities and masses against central
+    differences with h = 1e-3 of the fp64 oracle's loss (32 particles'
+    positions and velocities, 128 particles' masses), per group.
+
+    G: the workload's G N = 1e4 (G = 1e4 / 256; at the full system's G the
+    subset's forces are 256 times weaker and the mass signal, rms 1.3e-5, is
+    at the fp32 trajectory's rounding, 8e-6), and 3e4.  The GPU gradient is
+    seeded with the fp32 trajectory's 2 (x - target) / N, whose rounding
+    against the signal (rms 3.4e-6 against 3.9e-3 at 1e4, 2.9e-6 against
+    1.2e-2 at 3e4) bounds the agreement at about 8.6e-4 and 2.5e-4: the bar
+    is 2e-3 at 1e4 and 1e-3 at 3e4 (measured: positions, velocities, masses
+    to 1.06e-3, 6.7e-4, 2.6e-4 at 1e4 and 2.4e-4, 1.1e-4, 7.2e-5 at 3e4).
+
+    The differences are taken with every step's leaf assignment held at the
+    base trajectory's (oracle run_f64_cells), the function the adjoint
+    differentiates: with free cells the loss jumps wherever a perturbed
+    trajectory moves a leaf crossing to another step (leaves are 0.033 wide
+    and particles move about 0.01 per step, so no 1e-3 shift of an initial
+    position keeps every crossing)."""
+    from nbfmm_amd import Solver
+    from nbfmm_amd.autograd import fmm_trajectory
+    from nbfmm_amd.workloads import FIT_G_PER_N, gaussian_disk_system
+    full = gaussian_disk_system(n=65536, steps=50)
+    sub = np.arange(256)
+    x0, p0, w = full.position[sub], full.position_pre[sub], full.weight[sub]
+    L, eps, dt, steps, K = full.num_level, full.eps, full.dt, 50, 10
+    assert FIT_G_PER_N == 1e4
+    tick = full.dt * np.sqrt(g_per_n / len(sub))
+    lim = np.array(full.limits, np.float32)
+    n = len(sub)
+    rng = np.random.default_rng(31)
+    w_pert = (w.astype(np.float64) * (1 + 0.05 * rng.normal(0, 1, n)))
+    target, _ = orc.run_f64(L, lim, eps, tick, steps, x0.astype(np.float64), p0.astype(np.float64), w_pert)
+    base = (x0.astype(np.float64), p0.astype(np.float64), w.astype(np.float64))
+    _, _, cells = orc.run_f64_cells(L, lim, eps, tick, steps, *base)
+
+    def loss64(x, p, m):
+        xs, _, _ = orc.run_f64_cells(L, lim, eps, tick, steps, x, p, m, cells)
+        return float(((xs - target) ** 2).sum(1).mean())
+
+    s = Solver(L, n, full.limits, eps)
+    X = dev(x0).requires_grad_(True)
+    P = dev(p0).requires_grad_(True)
+    M = dev(w).requires_grad_(True)
+    out = fmm_trajectory(s, tick, steps, K, [steps], X, P, M)
+    loss = ((out[0] - dev(target)) ** 2).sum(-1).mean()
+    loss.backward()
+    torch.cuda.synchronize()
+    gx, gp, gm = (t.grad.double().cpu().numpy() for t in (X, P, M))
+    s.close()
+    g_pos = gx + gp          # x0 and x_pre moved together: velocity held
+    g_vel = -dt * gp         # x_pre = x0 - v dt
+    l0 = loss64(*base)
+    assert abs(float(loss.detach()) - l0) <= 1e-3 * l0
+    h = 1e-3
+    groups = {"position": ([], []), "velocity": ([], []), "mass": ([], [])}
+    for i in rng.choice(n, 32, replace=False):
+        for d in range(2):
+            xp, pp, xm, pm = base[0].copy(), base[1].copy(), base[0].copy(), base[1].copy()
+            xp[i, d] += h; pp[i, d] += h
+            xm[i, d] -= h; pm[i, d] -= h
+            groups["position"][0].append(g_pos[i, d])
+            groups["position"][1].append((loss64(xp, pp, base[2]) - loss64(xm, pm, base[2])) / (2 * h))
+            pp, pm = base[1].copy(), base[1].copy()
+            pp[i, d] -= h * dt
+            pm[i, d] += h * dt
+            groups["velocity"][0].append(g_vel[i, d])
+            groups["velocity"][1].append((loss64(base[0], pp, base[2]) - loss64(base[0], pm, base[2])) / (2 * h))
+    for i in rng.choice(n, 128, replace=False):
+        mp, mm = base[2].copy(), base[2].copy()
+        mp[i] += h
+        mm[i] -= h
+        groups["mass"][0].append(gm[i])
+        groups["mass"][1].append((loss64(base[0], base[1], mp) - loss64(base[0], base[1], mm)) / (2 * h))
+    for name, (a, f) in groups.items():
+        f = np.array(f)
+        print(f"{name}: rel {rel(a, f):.3e} over {len(f)} parameters, |fd| max {np.abs(f).max():.3e}")
+        assert np.linalg.norm(f) > 0, name
+        assert rel(a, f) <= bar, (name, rel(a, f))

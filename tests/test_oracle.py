@@ -487,3 +487,46 @@ def test_display_is_a_union_of_stamps():
     assert np.array_equal(full[:W * H], acc)
     perm = rng.permutation(60)
     assert np.array_equal(orc.display(W, H, 1.0, lim, pos[perm], w[perm]), full)
+
+
+def test_run_f64_cells_record_replay_and_smoothness():
+    """Frozen-cell fp64 trajectories (for finite differences of a cells-fixed
+    adjoint): recording gives run_f64's bits, replaying the recorded cells
+    gives them again, and a perturbation that changes cells in the free run
+    is smooth in the frozen one (central differences at h = 1e-3 and 1e-6
+    agree), while the free run's differences jump."""
+    rng = np.random.default_rng(12)
+    n, L, steps = 200, 5, 20
+    lim = np.array([-1, -1, 1, 1], np.float32)
+    x = rng.uniform(-0.8, 0.8, (n, 2))
+    v = rng.normal(0, 30.0, (n, 2))          # about a third of a leaf per step
+    p = x - v * 1e-4
+    w = rng.uniform(0.5, 1.5, n)
+    tick = 1e-4 * np.sqrt(10.0)
+    xa, pa = orc.run_f64(L, lim, 1e-3, tick, steps, x, p, w)
+    xb, pb, cells = orc.run_f64_cells(L, lim, 1e-3, tick, steps, x, p, w)
+    assert np.array_equal(xa, xb) and np.array_equal(pa, pb)
+    xc, pc, _ = orc.run_f64_cells(L, lim, 1e-3, tick, steps, x, p, w, cells)
+    assert np.array_equal(xa, xc) and np.array_equal(pa, pc)
+    c = rng.normal(0, 1, (n, 2))
+
+    def loss(xx, frozen):
+        if frozen:
+            out, _, _ = orc.run_f64_cells(L, lim, 1e-3, tick, steps, xx, xx - v * 1e-4, w, cells)
+        else:
+            out, _ = orc.run_f64(L, lim, 1e-3, tick, steps, xx, xx - v * 1e-4, w)
+        return float((c * out).sum())
+
+    def fd(i, h, frozen):
+        xp, xm = x.copy(), x.copy()
+        xp[i, 0] += h
+        xm[i, 0] -= h
+        return (loss(xp, frozen) - loss(xm, frozen)) / (2 * h)
+
+    jumps = 0
+    for i in range(40):
+        a, b = fd(i, 1e-3, True), fd(i, 1e-6, True)
+        assert abs(a - b) <= 1e-4 * max(abs(b), 1.0), (i, a, b)  # O(h^2) truncation only
+        fa, fb = fd(i, 1e-3, False), fd(i, 1e-6, False)
+        jumps += abs(fa - fb) > 1e-4 * max(abs(fb), 1.0)
+    assert jumps > 0  # the free runs do change cells at h = 1e-3
