@@ -546,13 +546,19 @@ __device__ __host__ __forceinline__ int up_tile_lg(int B) {
   return lgT;
 }
 
-// One target's M2L sum over its whole 6 x 6 window, branch-free: a window
-// cell that is outside the grid, adjacent to the target or empty enters as a
-// massless source at the target itself, whose term is exactly +0 (d = 0,
-// 0 / eps = 0), so the sum has k_m2l's bits (its running sum is never -0)
-// while the 36 evaluations can overlap.  Rows are loaded one ahead.
+// One target's M2L sum over its 6 x 6 window in k_m2l's order (rows, then
+// columns), over the 27 cells not adjacent to the target.  The three window
+// rows that meet the target's 3 x 3 neighbourhood (rows 1-3 or 2-4 by the
+// target's row parity; the choice is the same for a whole wave on levels at
+// least 64 cells wide) evaluate only their three other cells: columns 0 and
+// 5 and, by the target's column parity, 4 or 1.  A remaining cell that is
+// outside the grid or empty enters as a massless source at the target
+// itself, whose term is exactly +0 (d = 0, 0 / eps = 0), so the sum has
+// k_m2l's bits (its running sum is never -0) while the evaluations of a row
+// can overlap.  Rows are loaded one ahead.
 __device__ __forceinline__ float2 m2l_window(const float4* mp, int D, int tx, int ty, float4 tc, float eps) {
   const int px = tx & ~1, py = ty & ~1;
+  const int mid = (tx & 1) ? 1 : 4;  // the non-adjacent column between 0 and 5 in a neighbouring row
   float ax = 0.0f, ay = 0.0f;
   float4 nxt[6];
   {
@@ -560,23 +566,29 @@ __device__ __forceinline__ float2 m2l_window(const float4* mp, int D, int tx, in
 #pragma clang loop unroll(full)
     for (int k = 0; k < 6; ++k) nxt[k] = mp[yc * D + min(max(px - 2 + k, 0), D - 1)];
   }
+  auto term = [&](const float4& sc, int x, bool row_in) {
+    const bool use = row_in && x >= 0 && x < D && sc.w != 0.0f;
+    kernel_accum_ieee(tc.x, tc.y, use ? sc.x : tc.x, use ? sc.y : tc.y, use ? sc.z : 0.0f, eps, ax, ay);
+  };
 #pragma clang loop unroll(full)
-  for (int wy = -2; wy < 4; ++wy) {
-    const int y = py + wy;
+  for (int r = 0; r < 6; ++r) {
+    const int y = py - 2 + r;
     const bool row_in = y >= 0 && y < D;
     float4 sc[6];
 #pragma clang loop unroll(full)
     for (int k = 0; k < 6; ++k) sc[k] = nxt[k];
-    if (wy + 1 < 4) {
+    if (r + 1 < 6) {
       const int yn = min(max(y + 1, 0), D - 1);
 #pragma clang loop unroll(full)
       for (int k = 0; k < 6; ++k) nxt[k] = mp[yn * D + min(max(px - 2 + k, 0), D - 1)];
     }
+    if ((unsigned)(r - 1 - (ty & 1)) < 3u) {
+      term(sc[0], px - 2, row_in);
+      term(mid == 1 ? sc[1] : sc[4], px - 2 + mid, row_in);
+      term(sc[5], px + 3, row_in);
+    } else {
 #pragma clang loop unroll(full)
-    for (int k = 0; k < 6; ++k) {
-      const int x = px - 2 + k;
-      const bool use = row_in && x >= 0 && x < D && (abs(x - tx) > 1 || abs(y - ty) > 1) && sc[k].w != 0.0f;
-      kernel_accum_ieee(tc.x, tc.y, use ? sc[k].x : tc.x, use ? sc[k].y : tc.y, use ? sc[k].z : 0.0f, eps, ax, ay);
+      for (int k = 0; k < 6; ++k) term(sc[k], px - 2 + k, row_in);
     }
   }
   return make_float2(ax, ay);
