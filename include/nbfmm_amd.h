@@ -44,8 +44,9 @@ extern "C" {
 #define NBFMM_FLAG_P2P_WAIT 2u   /* a P2P wave gave up a wait on another wave of its block (after
                                   * nbfmm_set_p2p_wait_polls polls): for the segments before its own in
                                   * a split run, or for the block's split-run slot.  That run was then
-                                  * summed by one wave in one pass: its results are right, but their bits
-                                  * may differ from the split sum's.  Not expected at the default limit. */
+                                  * summed by one wave, segment after segment: its results are the same
+                                  * bits as the split sum's, only later.  Not expected at the default
+                                  * limit. */
 #define NBFMM_FLAG_P2P_LOST 4u   /* a P2P wave found the split-run protocol broken (a lock held past every
                                   * bound) and dropped a segment: that run's results are missing.  Cannot
                                   * happen unless the protocol has a defect; reported rather than hung on. */
@@ -280,8 +281,8 @@ int nbfmm_set_p2p_config(nbfmm_solver* solver, int variant, int blocks_per_cu);
 int nbfmm_get_p2p_config(const nbfmm_solver* solver, int* variant, int* blocks_per_cu);
 /* Bound on every wait of a P2P wave for another wave of its block (each poll a
  * short sleep; default 2^20, >= 1).  After it the wave gives up the wait and
- * sums the run alone (NBFMM_FLAG_P2P_WAIT): results stay right, only their
- * bits may change.  Exists so that tests can force the give-up path from the
+ * sums the run alone, segment after segment (NBFMM_FLAG_P2P_WAIT): the same
+ * bits, only later.  Exists so that tests can force the give-up path from the
  * host; no reference counterpart. */
 int nbfmm_set_p2p_wait_polls(nbfmm_solver* solver, int polls);
 /* Times every configuration on this input (repeats launches each), keeps the

@@ -1269,6 +1269,7 @@ int nbfmm_batch_create(int S, const int* num_particle, const int* num_level, con
   for (int s = 0; s < S; ++s) {
     SysView& v = b->hv[s];
     v = SysView{};
+    v.split_small = 0;  // the other systems fill the GPU: small systems' runs are folded, not split
     v.n = num_particle[s];
     v.L = num_level[s];
     v.B = 1 << (v.L + 1);

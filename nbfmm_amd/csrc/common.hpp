@@ -32,18 +32,23 @@ enum P2PMode : int {
   kP2PStepSort = 3, // as kP2PStep, and the first pass of the next step's warm-started sort
 };
 
-// Split of P2P runs: a run whose staged union holds more than S_min sources
-// is summed in K = min(kP2PMaxSeg, ceil(U / S_len)) segments of the union, by
-// whichever waves of the block are free, and the segment sums are added in
-// segment order.  K and the segment bounds depend on U and the system's
-// particle count alone, so every kernel configuration, the solver, the batch
-// and the allocation-free solve give the same bits.  Large systems
-// split only heavy runs (dense clusters, whose serial sum would be the
-// kernel's time); a system of at most kP2PSmallN particles has about n / 64
-// runs for the GPU's 1024 SIMDs, so its runs are split from
-// kP2PSmallSplitMin sources into segments of kP2PSmallSegLen (S_min, S_len;
-// kP2PSplitMin and kP2PSegLen otherwise): the block's idle waves share each
-// run and the chain of a wave is a few times shorter.
+// Segments of P2P runs: a run whose staged union holds more than S_min
+// sources is summed in K = min(kP2PMaxSeg, ceil(U / S_len)) segments of the
+// union: each lane sums its sources of every segment from zero and adds the
+// segment sums in segment order.  K and the segment bounds depend on U and
+// the system's particle count alone.  Whether the segments are then shared
+// by the waves of the block (a split run) or summed one after another by one
+// wave in one pass (folded) changes no bits, so every kernel configuration,
+// the solver, the batch and the allocation-free solve give the same bits.
+// Large systems segment only heavy runs (dense clusters, whose serial sum
+// would be the kernel's time), and split them; a system of at most
+// kP2PSmallN particles has about n / 64 runs for the GPU's 1024 SIMDs, so its
+// runs are segmented from kP2PSmallSplitMin sources into segments of
+// kP2PSmallSegLen (S_min, S_len; kP2PSplitMin and kP2PSegLen otherwise) and,
+// when it runs alone (SysView::split_small), split: the block's idle waves
+// share each run and the chain of a wave is a few times shorter.  In a batch
+// the other systems fill the GPU, so a small system's runs are folded and
+// only runs above kP2PSplitMin sources are split.
 constexpr int kP2PSplitMin = 2048;
 constexpr int kP2PSegLen = 1024;
 constexpr int kP2PSmallN = 1 << 16;
@@ -63,6 +68,7 @@ struct SysView {
   float tick;       // Verlet step (dt*sqrt(G)), step mode only
   int part_off;     // first particle of this system in the caller's arrays
   int wait_polls = kP2PWaitPolls;  // P2P: bound on every wait for another wave of the block
+  int split_small = 1;             // P2P: split the segmented runs of a small system (0 in batches)
   // particle scratch (capacity max_n)
   uint32_t* tkey;   // row-sorted keys (row pass output)
   float4* tpart;    // row-sorted particles: x, y, m, original index (bits)

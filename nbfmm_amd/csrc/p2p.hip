@@ -183,6 +183,9 @@ __global__ __launch_bounds__(WAVES * kWave) void k_p2p(Sys sys, CallIO io) {
       [&](const RunGeom& g, int sa, int sb, float* acc) {
         seg_sum<STAGE, U>(v, src, g, lane, sa, sb, eps, acc[0], acc[1]);
       },
+      [&](const RunGeom& g, int sa, int sb, float* acc) {
+        seg_sum<STAGE, 2>(v, src, g, lane, sa, sb, eps, acc[0], acc[1]);
+      },
       [&](const RunGeom& g, const float* acc) { p2p_finish<MODE>(v, io, g, acc[0], acc[1]); });
 }
 

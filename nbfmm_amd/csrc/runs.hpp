@@ -9,11 +9,16 @@
 // table (runs before each row) from head and its waves draw runs b, b + nblk,
 // ... through an LDS counter.
 //
-// A run whose union exceeds the system's split threshold (a cluster of dense
-// leaves; in a small system, most runs) would be one long serial chain on one
-// wave, so it is split into K segments of the union (p2p_segments: K and
-// segments of S sources, a multiple of 64, both functions of the union size U
-// and the system's particle count alone).  The wave
+// A run whose union exceeds the system's segment threshold (a cluster of
+// dense leaves; in a small system, most runs) is summed in K segments of the
+// union (p2p_segments: K and segments of S sources, a multiple of 64, both
+// functions of the union size U and the system's particle count alone):
+// each lane's sum of every segment starts from zero and the segment sums are
+// added in segment order.  One wave can do this in one pass over the union
+// (folded at the segment bounds), or the segments can be shared by the waves
+// of the block (split); both give the same bits (the segment sums are added
+// in segment order either way).  A run that would be one
+// long serial chain on one wave is split (p2p_split).  The wave
 // that takes such a run publishes it in the block's slot; every wave of the
 // block that is free takes a segment (a ticket from an LDS counter,
 // generation and K in its high bits), sums it, waits until the segments
@@ -21,13 +26,13 @@
 // passes on; the last one runs the epilogue and frees the slot.  The wait
 // ends normally (tickets go out in order and segment 0 waits for nothing);
 // it is bounded all the same: after a poll limit the waiting wave abandons
-// the run, sums its whole union alone in one pass and runs the epilogue, and
-// the other waves of that run drop their segments (RunSlot below).  A wave
-// whose heavy run finds the slot taken keeps it pending, helps with the
-// published run and publishes its own once the slot is free (after a poll
-// limit it sums the whole union itself in one pass).  Either give-up raises
-// NBFMM_FLAG_P2P_WAIT: the results are right, but their bits may differ from
-// the split sum's.  A wave that finds the run
+// the run, sums its whole union alone in one folded pass and runs the
+// epilogue, and the other waves of that run drop their segments (RunSlot
+// below).  A wave whose heavy run finds the slot taken keeps it pending,
+// helps with the published run and publishes its own once the slot is free
+// (after a poll limit it sums the whole union itself in one folded pass).
+// Either give-up raises NBFMM_FLAG_P2P_WAIT, which reports the wait only:
+// the folded pass gives the split sum's bits.  A wave that finds the run
 // queue empty stays to help while another wave holds a run that may still be
 // published (the low bits of the queue word count runs from their take until
 // they are known light, published, or summed alone) or a split run is
@@ -99,6 +104,12 @@ __device__ __forceinline__ int p2p_segments(const SysView& v, int total) {
   return min(kP2PMaxSeg, (total + len - 1) / len);
 }
 __device__ __forceinline__ int p2p_segment_len(int total, int K) { return (((total + K - 1) / K) + 63) & ~63; }
+// Whether a run of K segments is split among the waves of its block (else a
+// K > 1 run is folded by one wave): a small system alone on the GPU splits
+// every segmented run, otherwise only runs above kP2PSplitMin sources.
+__device__ __forceinline__ bool p2p_split(const SysView& v, int total, int K) {
+  return K > 1 && (v.split_small || total > kP2PSplitMin);
+}
 
 // The block's run queue and its slot for a split run, in LDS; NACC running
 // sums per lane.
@@ -160,17 +171,20 @@ __device__ __forceinline__ uint32_t wave_load(uint32_t* w) {
 
 // The block's waves work through its runs.  seg(g, sa, sb, acc) sums this
 // lane's sources among union positions [sa, sb) of run g into acc[NACC]
-// (from zero); fin(g, acc) is the epilogue of run g (HK: g.hk is loaded).
+// (from zero); seg_lean is the same sum in the same order (its bits) with
+// fewer registers; fin(g, acc) is the epilogue of run g (HK: g.hk is
+// loaded).
 // The slot must have been initialised and the block synchronised.
 // v.wait_polls bounds every wait on another wave.
-template <int NACC, bool HK, class Seg, class Fin>
+template <int NACC, bool HK, class Seg, class SegLean, class Fin>
 __device__ __forceinline__ void run_queue(const SysView& v, const int* s_runs, RunSlot<NACC>& sl, int blk, int nblk,
-                                          int total, int lane, Seg&& seg, Fin&& fin) {
+                                          int total, int lane, Seg&& seg, SegLean&& seg_lean, Fin&& fin) {
   RunGeom g;
   const int limit = v.wait_polls;
   bool exhausted = false;  // this wave found the block's run queue empty
   int idle_polls = 0;
   int pend = -1, pend_k = 1, pend_polls = 0;  // a split run waiting for the slot (wave-uniform)
+  int redo = -1, redo_k = 1;  // a published run this wave abandoned: summed whole next, slot freed after
   // publishes split run jr of kr segments if the slot is free (lane 0's
   // view, made wave-uniform); the run stops counting as active once published
   auto try_publish = [&](int jr, int kr) -> bool {
@@ -195,12 +209,19 @@ __device__ __forceinline__ void run_queue(const SysView& v, const int* s_runs, R
   };
   for (;;) {
     // what this iteration sums: segment k0 of run j out of K (a ticket of
-    // the published run, generation gen), or all of run j in one pass
+    // the published run, generation gen), or all of run j
     int j = -1, K = 1, k0 = 0;
     uint32_t gen = 0;
-    bool ticket = false;
+    bool ticket = false, own_slot = false;
+    if (redo >= 0) {
+      // 0. the run this wave abandoned (g still holds its geometry)
+      j = redo;
+      K = redo_k;
+      redo = -1;
+      own_slot = true;
+    }
     // 1. a segment of the published split run, if any is left
-    {
+    if (!own_slot) {
       const uint32_t t0 = wave_load(&sl.ticket);
       if ((t0 & 0xffffu) < ((t0 >> 16) & 0x3fu)) {
         uint32_t t = 0;
@@ -218,10 +239,10 @@ __device__ __forceinline__ void run_queue(const SysView& v, const int* s_runs, R
         }
       }
     }
-    if (!ticket) {
+    if (!ticket && !own_slot) {
       if (pend >= 0) {
         // 2. this wave's split run waits for the slot, helping meanwhile;
-        // after the poll limit it is summed here alone, in one pass
+        // after the poll limit it is summed here alone, in one folded pass
         if (try_publish(pend, pend_k)) {
           pend = -1;
           continue;
@@ -262,11 +283,12 @@ __device__ __forceinline__ void run_queue(const SysView& v, const int* s_runs, R
         }
         run_geom<HK>(v, s_runs, j, lane, g);
         K = p2p_segments(v, g.total);
-        // a light run can publish nothing: it stops counting at once, so
-        // that waves without work need not stay for its end
-        if (K == 1 && lane == 0)
+        const bool split = p2p_split(v, g.total, K);
+        // a run summed here can publish nothing: it stops counting at once,
+        // so that waves without work need not stay for its end
+        if (!split && lane == 0)
           __hip_atomic_fetch_add(&sl.queue, 0xffffffffu, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        if (K > 1) {
+        if (split) {
           if (!try_publish(j, K)) {
             pend = j;
             pend_k = K;
@@ -276,20 +298,35 @@ __device__ __forceinline__ void run_queue(const SysView& v, const int* s_runs, R
         }
       }
     }
-    // (two call sites: one segment, or the whole union; the compiler keeps
-    // fewer registers live this way than with computed bounds)
+    // a ticket's one segment of S sources, or all K segments of the run
+    // (K = 1: S >= the union); each segment's sum from zero, added in
+    // segment order.  A whole run's first segment by seg, the others (of a
+    // folded run) by seg_lean: the same sum with fewer registers, since acc
+    // is live beside it.  (Separate call sites for a ticket and a whole run:
+    // the compiler keeps fewer registers live than with computed bounds.)
+    const int S = p2p_segment_len(g.total, K);
     float acc[NACC];
     if (ticket) {
-      const int S = p2p_segment_len(g.total, K);
       seg(g, k0 * S, min(k0 * S + S, g.total), acc);
     } else {
-      seg(g, 0, g.total, acc);
+      seg(g, 0, min(S, g.total), acc);
+      for (int k = 1; k < K; ++k) {
+        float part[NACC];
+        seg_lean(g, k * S, min(k * S + S, g.total), part);
+#pragma unroll
+        for (int c = 0; c < NACC; ++c) acc[c] = acc[c] + part[c];
+      }
     }
     if (!ticket) {
       fin(g, acc);
-      // a split run summed here alone (poll limit) counted until now
-      if (K > 1 && lane == 0)
+      if (own_slot) {
+        // the abandoned run is done: the slot is free
+        wave_lds_order();
+        if (lane == 0) __hip_atomic_store(&sl.busy, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      } else if (p2p_split(v, g.total, K) && lane == 0) {
+        // a split run summed here alone (poll limit) counted until now
         __hip_atomic_fetch_add(&sl.queue, 0xffffffffu, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      }
       continue;
     }
     // a ticket: lock the done word at count k0 (segments 0 .. k0-1 added),
@@ -316,18 +353,16 @@ __device__ __forceinline__ void run_queue(const SysView& v, const int* s_runs, R
     }
     if (outcome == 3) continue;
     if (outcome == 2) {
-      // this wave owns the run: no further tickets, the whole union here in
-      // one pass, its epilogue, then the slot is free
+      // this wave owns the run: no further tickets; next, the whole union
+      // here segment after segment, its epilogue, then the slot is free
       if (lane == 0) {
         atomicOr(v.flags, kFlagP2PWait);
         __hip_atomic_store(&sl.ticket, (gen << 22) | ((uint32_t)K << 16) | (uint32_t)K, __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_WORKGROUP);
       }
       wave_lds_order();
-      seg(g, 0, g.total, acc);
-      fin(g, acc);
-      wave_lds_order();
-      if (lane == 0) __hip_atomic_store(&sl.busy, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      redo = j;
+      redo_k = K;
       continue;
     }
     // locked: add this segment to the running sums

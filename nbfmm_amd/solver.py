@@ -226,8 +226,9 @@ class Solver:
 
     def set_p2p_wait_polls(self, polls: int) -> None:
         """Bound on every wait of a P2P wave for another wave of its block;
-        past it the wave sums its run alone (FLAG_P2P_WAIT; same results,
-        bits may differ).  Default 2^20; tests set 1 to force that path."""
+        past it the wave sums its run alone, segment after segment
+        (FLAG_P2P_WAIT; the same bits).  Default 2^20; tests set 1 to force
+        that path."""
         check(lib().nbfmm_set_p2p_wait_polls(self._h, int(polls)))
 
     # -- measurement ------------------------------------------------------

@@ -273,7 +273,9 @@ def test_forced_give_up_solve_matches_oracle(polls):
     segment give up on the segments before theirs, abandon the run and sum
     its whole union alone; waves with a heavy run that finds the slot taken
     sum it alone.  Every kernel still ends normally, each run's epilogue runs
-    once, and the forces match the oracle in every configuration."""
+    once, and a run summed alone is folded at its segment bounds, so the
+    forces equal the default solver's bit for bit (and the oracle's to
+    rounding) in every configuration."""
     from nbfmm_amd import FLAG_P2P_WAIT, Solver
     from nbfmm_amd.autotune import set_config, variants
     # about two heavy runs per block on the smallest grid, so that a heavy run
@@ -285,6 +287,13 @@ def test_forced_give_up_solve_matches_oracle(polls):
     s.set_p2p_wait_polls(polls)
     X, W = dev(x), dev(w)
     want, _ = orc.solve(L, x, w, lim, eps)
+    ref = Solver(L, n, lim, eps)
+    a_ref = torch.zeros_like(X)
+    ref.solve(n, X, W, a_ref)
+    torch.cuda.synchronize()
+    assert ref.flags() == 0
+    bits = a_ref.cpu().numpy()
+    ref.close()
     raised = 0
     for v in range(len(variants())):
         for bpc in (1, 16):
@@ -294,6 +303,7 @@ def test_forced_give_up_solve_matches_oracle(polls):
             torch.cuda.synchronize()
             got = a.cpu().numpy()
             assert np.linalg.norm(got - want) / np.linalg.norm(want) < 1e-5, (v, bpc)
+            np.testing.assert_array_equal(got, bits, err_msg=str((v, bpc)))
             f = s.flags(clear=True)
             assert not f & FLAG_LOST, (v, bpc, f)
             raised |= f
@@ -304,8 +314,8 @@ def test_forced_give_up_solve_matches_oracle(polls):
 def test_forced_give_up_steps_match_default():
     """Graph-replayed Verlet steps (the epilogue updates x and x_pre in place,
     so a run finished twice or not at all would show at once) with the wait
-    bound at 1 poll: the trajectory matches the default solver's to rounding
-    (bits may differ where a run was summed in one pass)."""
+    bound at 1 poll: the trajectory equals the default solver's bit for bit
+    (a run summed alone is folded at its segment bounds)."""
     from nbfmm_amd import FLAG_P2P_WAIT, Solver
     x, w, L, lim, eps = clustered(n=8000, seed=13)
     n = x.shape[0]
@@ -322,9 +332,7 @@ def test_forced_give_up_steps_match_default():
         forced.run(n, tick, 6, Xa, Pa, Wa, Aa, use_graph=True)
         ref.run(n, tick, 6, Xb, Pb, Wb, Ab, use_graph=True)
     st.synchronize()
-    scale = float(Ab.abs().max())
-    assert float((Aa - Ab).abs().max()) < 1e-4 * scale
-    assert float((Xa - Xb).abs().max()) < 1e-6 and float((Pa - Pb).abs().max()) < 1e-6
+    assert torch.equal(Aa, Ab) and torch.equal(Xa, Xb) and torch.equal(Pa, Pb)
     f = forced.flags()
     assert f & FLAG_P2P_WAIT and not f & FLAG_LOST
     assert ref.flags() == 0
@@ -350,5 +358,13 @@ def test_forced_give_up_in_a_batch():
         want, _ = orc.solve(L, x, w, lim, eps)
         got = A[off[k]:off[k + 1]].cpu().numpy()
         assert np.linalg.norm(got - want) / np.linalg.norm(want) < 1e-5, k
+        # and bit for bit the system's own solver at the default wait bound
+        from nbfmm_amd import Solver
+        s = Solver(L, x.shape[0], lim, eps)
+        a1 = torch.zeros_like(A[off[k]:off[k + 1]])
+        s.solve(x.shape[0], X[off[k]:off[k + 1]], W[off[k]:off[k + 1]], a1)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(got, a1.cpu().numpy(), err_msg=str(k))
+        s.close()
     assert not (b.flags() & FLAG_LOST).any()
     b.close()
