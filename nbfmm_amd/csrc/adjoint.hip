@@ -317,7 +317,8 @@ __global__ __launch_bounds__(kAdjP2PWaves * kWave) __attribute__((amdgpu_waves_p
   if (threadIdx.x == 0) s_next = 0;
   __syncthreads();
   const int total = s_runs[B];
-  for (int j = blk + block_take(&s_next) * nblk; j < total; j = blk + block_take(&s_next) * nblk) {
+  const int lblk = xcd_block(blk, nblk);
+  for (int j = lblk + block_take(&s_next) * nblk; j < total; j = lblk + block_take(&s_next) * nblk) {
     const int y = __builtin_amdgcn_readfirstlane(run_row_wave(s_runs, B, j, lane));
     const int T0 = v.head[y * B] + kWave * (j - s_runs[y]);
     const int nt = min(kWave, v.head[(y + 1) * B] - T0);

@@ -354,6 +354,22 @@ __device__ __forceinline__ void kernel_accum(float tx, float ty, float sx, float
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
 
+// The calling block's place among its system's nblk blocks in the run sweep
+// (block blk takes runs blk', blk' + nblk, ...).  Blocks go to the eight XCDs
+// round-robin by their global index, each XCD with its own L2; numbered so
+// that each XCD takes a contiguous eighth of every band of nblk runs,
+// neighbouring runs, which stage mostly the same sources, run on one XCD and
+// share its L2.  The global index is blockIdx.x = (the system's first block)
+// + blk; rotating blk by that first block's XCD gives each block its own XCD
+// in the low three bits, so the map is a bijection of [0, nblk) when nblk is
+// a multiple of 8 (otherwise blk itself).
+__device__ __forceinline__ int xcd_block(int blk, int nblk) {
+  if (nblk & 7) return blk;
+  int r = blk + (int)((blockIdx.x - (unsigned)blk) & 7u);
+  if (r >= nblk) r -= nblk;
+  return (r & 7) * (nblk >> 3) + (r >> 3);
+}
+
 // Next position in the calling block's share of a work list: lane 0 bumps
 // the block's LDS counter, the whole wave gets the value.
 __device__ __forceinline__ int block_take(int* s_next) {

@@ -7,7 +7,8 @@
 // source/nbfmm/core/solver/p2p.cu:105-118); those ranges lie inside the run's
 // union [head[r, x_first-1], head[r, x_last+2]).  A block rebuilds the run
 // table (runs before each row) from head and its waves draw runs b, b + nblk,
-// ... through an LDS counter.
+// ... through an LDS counter (b = xcd_block(block, nblk): the blocks of one
+// XCD take neighbouring runs).
 //
 // A run whose union exceeds the system's segment threshold (a cluster of
 // dense leaves; in a small system, most runs) is summed in K segments of the
@@ -181,6 +182,7 @@ __device__ __forceinline__ void run_queue(const SysView& v, const int* s_runs, R
                                           int total, int lane, Seg&& seg, SegLean&& seg_lean, Fin&& fin) {
   RunGeom g;
   const int limit = v.wait_polls;
+  const int lblk = xcd_block(blk, nblk);
   bool exhausted = false;  // this wave found the block's run queue empty
   int idle_polls = 0;
   int pend = -1, pend_k = 1, pend_polls = 0;  // a split run waiting for the slot (wave-uniform)
@@ -275,7 +277,7 @@ __device__ __forceinline__ void run_queue(const SysView& v, const int* s_runs, R
         uint32_t q = 0;
         if (lane == 0) q = __hip_atomic_fetch_add(&sl.queue, 17u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         wave_lds_order();
-        j = blk + (int)(__builtin_amdgcn_readfirstlane(q) >> 4) * nblk;
+        j = lblk + (int)(__builtin_amdgcn_readfirstlane(q) >> 4) * nblk;
         if (j >= total) {
           if (lane == 0) __hip_atomic_fetch_add(&sl.queue, 0xffffffffu, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
           exhausted = true;
