@@ -167,7 +167,9 @@ int nbfmm_prune(nbfmm_solver* solver, int num_star, float* d_position_cur, float
 
 /* num_steps calls of nbfmm_step (the same bits; with the warm start on, the
  * P2P of every step but the last also makes the first pass of the next
- * step's sort).  use_graph = 1 captures one step into a
+ * step's sort, and leaves d_acceleration alone, which no step reads: the
+ * call's last step writes it, so it ends with the last step's acceleration
+ * as after num_steps nbfmm_step calls).  use_graph = 1 captures one step into a
  * hipGraph on the first call and replays it.  A graph cannot be captured on
  * the NULL stream: with stream NULL the replays run on a stream the solver
  * makes on first use, ordered after the work already queued on the NULL

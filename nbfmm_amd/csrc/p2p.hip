@@ -95,8 +95,13 @@ __device__ __forceinline__ void seg_sum(const SysView& v, float4* src, const Run
 }
 
 // L2P, un-permute and the Verlet update of one valid target (step modes):
-// writes the effect, x_cur and x_pre in the caller's order and returns the
-// new position.
+// writes x_cur, x_pre and, with EFFECT, the effect in the caller's order;
+// returns the new position.  A step followed by another step of the same call
+// (kP2PStepSort) leaves the effect alone: nothing reads it before the next
+// step overwrites it, and only the call's last step's is seen.  (x_cur is
+// still written: a sort that switches to the cold passes bins the caller's
+// positions.)
+template <bool EFFECT>
 __device__ __forceinline__ float2 p2p_step_update(const SysView& v, const CallIO& io, const RunGeom& g, float ax,
                                                   float ay) {
 #pragma clang fp contract(off)
@@ -105,7 +110,7 @@ __device__ __forceinline__ float2 p2p_step_update(const SysView& v, const CallIO
   const float ex = ax + loc.x;
   const float ey = ay + loc.y;
   const uint32_t o = __float_as_uint(g.pt.w);
-  io.acc0[v.part_off + o] = make_float2(ex, ey);
+  if (EFFECT) io.acc0[v.part_off + o] = make_float2(ex, ey);
   // update.cu:39-41
   float2* xcur = io.xcur + v.part_off;
   float2* xpre = io.xpre + v.part_off;
@@ -125,14 +130,15 @@ __device__ __forceinline__ float2 p2p_step_update(const SysView& v, const CallIO
 // the leaf local (L2L folded in, L2P) and scatters to the original order
 // (postdo); kP2PStep also applies the Verlet update; kP2PStepSort then makes
 // the first pass of the next step's warm-started sort from the new position
-// (what k_warm_gather would do at the start of that step, same bits).
+// (what k_warm_gather would do at the start of that step, same bits), and
+// leaves the caller's effect array alone.
 template <int MODE>
 __device__ __forceinline__ void p2p_finish(const SysView& v, const CallIO& io, const RunGeom& g, float ax, float ay) {
 #pragma clang fp contract(off)
   if (MODE == kP2PStepSort) {
     // the whole wave takes part in the first pass's mover counts
     float2 nx = make_float2(0.0f, 0.0f);
-    if (g.valid) nx = p2p_step_update(v, io, g, ax, ay);
+    if (g.valid) nx = p2p_step_update<false>(v, io, g, ax, ay);
     if (warm_first_pass(v, g.valid, g.i, (uint32_t)(g.y * v.B + g.xt), g.hk, nx, g.pt.z, g.pt.w))
       atomicOr(v.flags, 1u);
     return;
@@ -148,7 +154,7 @@ __device__ __forceinline__ void p2p_finish(const SysView& v, const CallIO& io, c
     const uint32_t o = __float_as_uint(g.pt.w);
     io.acc0[v.part_off + o] = make_float2(ax + loc.x, ay + loc.y);
   } else {
-    p2p_step_update(v, io, g, ax, ay);
+    p2p_step_update<true>(v, io, g, ax, ay);
   }
 }
 
