@@ -91,14 +91,20 @@ class _BatchTrajectory(torch.autograd.Function):
         acc = torch.empty_like(x)
         ckpts = []
         seen = {}
-        for k in range(steps + 1):
+        k = 0
+        while True:
             if k % K == 0:
                 ckpts.append((x.clone(), p.clone()))
             if k in obs_steps:
                 seen[k] = x.clone()
             if k == steps:
                 break
-            batch.step(x, p, m, acc)
+            # the steps up to the next checkpoint or observation as one eager
+            # run (the same bits as that many batch.step calls, with the sorts
+            # chained; no graph, as x and p are new on every call)
+            nxt = min([steps, (k // K + 1) * K] + [o for o in obs_steps if o > k])
+            batch.run(nxt - k, x, p, m, acc, use_graph=False)
+            k = nxt
         ctx.batch, ctx.steps, ctx.K, ctx.obs_steps = batch, steps, K, list(obs_steps)
         ctx.ckpts = ckpts
         ctx.save_for_backward(m)

@@ -745,7 +745,11 @@ int nbfmm_trajectory_forward(nbfmm_solver* s, int n, float tick, int num_steps, 
       if (obs_steps[o] == k) NB_HIP(hipMemcpyAsync(obs_pos + (size_t)o * n * 2, xcur, sb, hipMemcpyDeviceToDevice, st));
     return NBFMM_OK;
   };
-  for (int k = 0; k <= num_steps; ++k) {
+  // the steps between two events (a checkpoint or an observation) as one
+  // eager run: the same bits as that many nbfmm_step calls, with the sorts
+  // chained (no graph: the caller's arrays, which key a run's graphs, are
+  // new on every call)
+  for (int k = 0;;) {
     if (k % K == 0) {
       float* c = ckpt + (size_t)(k / K) * 2 * n * 2;
       NB_HIP(hipMemcpyAsync(c, xcur, sb, hipMemcpyDeviceToDevice, st));
@@ -753,7 +757,11 @@ int nbfmm_trajectory_forward(nbfmm_solver* s, int n, float tick, int num_steps, 
     }
     if ((rc = observe(k))) return rc;
     if (k == num_steps) break;
-    if ((rc = nbfmm_step(s, n, tick, xcur, xpre, w, acc, stream))) return rc;
+    int next = std::min(num_steps, (k / K + 1) * K);
+    for (int o = 0; o < n_obs; ++o)
+      if (obs_steps[o] > k && obs_steps[o] < next) next = obs_steps[o];
+    if ((rc = nbfmm_run(s, n, tick, next - k, xcur, xpre, w, acc, 0, stream))) return rc;
+    k = next;
   }
   return NBFMM_OK;
 }
@@ -1491,8 +1499,14 @@ int nbfmm_batch_run(nbfmm_batch* b, int steps, float* xcur, float* xpre, const f
   hipStream_t st = as_stream(stream);
   int rc = NBFMM_OK;
   if (!use_graph) {
-    for (int k = 0; k < steps; ++k)
-      if ((rc = nbfmm_batch_step(b, xcur, xpre, w, acc, stream))) return rc;
+    // eager: the launches of the replayed steps, one by one (with the warm
+    // start on, every step but the last makes the next sort's first pass)
+    CallIO io = call_io(xcur, w, acc, xcur, xpre);
+    const bool chain = b->warm_enabled;
+    for (int k = 0; k < steps; ++k) {
+      const int mode = chain && k + 1 < steps ? kP2PStepSort : kP2PStep;
+      if ((rc = batch_pipeline(b, mode, io, st, chain && k > 0))) return rc;
+    }
     return NBFMM_OK;
   }
   if (steps == 0) return NBFMM_OK;

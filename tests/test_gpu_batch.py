@@ -79,6 +79,39 @@ def test_batch_run_graph_equals_single_steps(small_batch):
     b.close()
 
 
+def test_batch_runs_eager_and_graph_equal_steps(small_batch):
+    """A run of several steps, eager (every step but the last makes the next
+    sort's first pass, no graph) and replayed as graphs, equals as many
+    batch.step calls bit for bit, in positions and in the last step's
+    accelerations."""
+    from nbfmm_amd import Batch
+    bw = small_batch
+    x, p, w = bw.concat()
+    ticks = [s.tick for s in bw.systems]
+    args = ([s.n for s in bw.systems], [s.num_level for s in bw.systems], [s.limits for s in bw.systems],
+            [s.eps for s in bw.systems])
+    stream = torch.cuda.Stream()
+    out = []
+    for how in ("steps", "eager", "graph"):
+        b = Batch(*args, ticks=ticks)
+        X, P, W = dev(x), dev(p), dev(w)
+        A = torch.zeros_like(X)
+        with torch.cuda.stream(stream):
+            for n in (3, 1, 5):
+                if how == "steps":
+                    for _ in range(n):
+                        b.step(X, P, W, A)
+                else:
+                    b.run(n, X, P, W, A, use_graph=how == "graph")
+        stream.synchronize()
+        assert not b.flags().any(), how
+        out.append((X, P, A))
+        b.close()
+    for got in out[1:]:
+        for t_ref, t in zip(out[0], got):
+            assert torch.equal(t_ref, t)
+
+
 def test_batch_balance_keeps_results(small_batch):
     """Sharing the P2P grid by pair counts moves work between blocks only:
     solve gives the same bits before and after balance(), and a graph run
