@@ -1151,7 +1151,11 @@ std::vector<int> p2p_share(const nbfmm_batch* b, int waves, int total_blocks) {
     const SysView& v = b->hv[s];
     const int cap = std::max(cdiv(std::min(v.n, v.B * v.B), waves), cdiv(v.n, kWave) + v.B);  // as p2p_grid_for
     const int share = (int)((double)total_blocks * b->work[s] / b->total_work + 0.5);
-    blocks[s] = std::max(1, std::min(cap, share));
+    int nb = std::max(1, std::min(cap, share));
+    // a multiple of 8 from 8 blocks on, so that the system's blocks take
+    // their runs in XCD order (xcd_block)
+    if (nb >= 8) nb = (nb + 4) / 8 * 8;
+    blocks[s] = nb;
   }
   return blocks;
 }
