@@ -936,9 +936,10 @@ int nbfmm_autotune(nbfmm_solver* s, int n, const float* pos0, const float* w0, i
   hipEvent_t e0, e1;
   NB_HIP(hipEventCreate(&e0));
   NB_HIP(hipEventCreate(&e1));
-  // grids of 16, 32 and 64 waves per CU (8 per SIMD is the hardware's
-  // maximum residency; 64 queues a second round of blocks)
-  static const int wpcs[] = {16, 32, 64};
+  // grids of 16 to 64 waves per CU (8 per SIMD is the hardware's maximum
+  // residency; LDS holds three 8-wave blocks of the 384-source variants, so
+  // 24 is one round of them and 32 queues a second round)
+  static const int wpcs[] = {16, 24, 32, 48, 64};
   float best = 3.4e38f;
   int bv = s->p2p_variant, bb = s->p2p_bpc;
   hipError_t err = hipSuccess;

@@ -79,12 +79,13 @@ hipError_t launch_p2p(const SysView& v, int mode, const CallIO& io, int grid_blo
 hipError_t launch_p2p_batch(const BatchLaunch& b, int mode, const CallIO& io, int variant, hipStream_t s);
 int p2p_num_variants();
 // default P2P configurations (measured): a solver runs variant {384 staged,
-// unroll 4, 8 waves} on 4 blocks per CU; a batch spreads variant {384, 4, 4
-// waves} on 32 blocks per CU over its systems
+// unroll 4, 8 waves} on 3 blocks per CU (one round: LDS holds three such
+// blocks per CU, so every block starts at once); a batch spreads variant
+// {320, 4, 4 waves} on 48 blocks per CU over its systems
 constexpr int kDefaultP2PVariant = 5;
-constexpr int kDefaultP2PBlocksPerCU = 4;
-constexpr int kDefaultBatchP2PVariant = 0;
-constexpr int kDefaultBatchP2PBlocksPerCU = 32;
+constexpr int kDefaultP2PBlocksPerCU = 3;
+constexpr int kDefaultBatchP2PVariant = 6;
+constexpr int kDefaultBatchP2PBlocksPerCU = 48;
 void p2p_variant_info(int variant, int* stage, int* unroll, int* waves);
 hipError_t launch_l2p(const SysView& v, hipStream_t s);
 
