@@ -399,6 +399,13 @@ int nbfmm_batch_step_adjoint(nbfmm_batch* batch, const float* d_position, const 
 int nbfmm_batch_keep_sort(nbfmm_batch* batch, int slot, void* stream);
 int nbfmm_batch_step_adjoint_kept(nbfmm_batch* batch, int slot, float* d_lambda, float* d_pi, float* d_grad_weight,
                                   void* stream);
+/* num_steps (0..4097) steps as nbfmm_batch_run's eager steps, keeping the
+ * sort and tree of step k (of the positions it starts from) in slot k: the
+ * same bits and kept states as nbfmm_batch_step followed by
+ * nbfmm_batch_keep_sort(k), step by step, with the sorts chained as in a
+ * run.  No reference counterpart. */
+int nbfmm_batch_run_keep(nbfmm_batch* batch, int num_steps, float* d_position_cur, float* d_position_pre,
+                         const float* d_weight, float* d_acceleration, void* stream);
 
 #ifdef __cplusplus
 }

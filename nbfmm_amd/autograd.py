@@ -132,17 +132,14 @@ class _BatchTrajectory(torch.autograd.Function):
         for sg in range(nseg - 1, -1, -1):
             k0, k1 = sg * K, min(sg * K + K, steps)
             x, p = (t.clone() for t in ctx.ckpts[sg])
-            seg = []
-            for k in range(k0, k1):
-                seg.append(x.clone())
-                if k + 1 < k1:
-                    b.step(x, p, m, acc)
-                    b.keep_sort(k - k0)  # the sort and tree of x_k, for its adjoint
+            # steps k0 .. k1-2 again, keeping the sort and tree of each x_k for
+            # its adjoint; x ends at x_{k1-1}
+            b.run_keep(k1 - 1 - k0, x, p, m, acc)
             for k in range(k1 - 1, k0 - 1, -1):
                 if k + 1 < k1:
                     b.step_adjoint_kept(k - k0, lam, pi, gm)
                 else:
-                    b.step_adjoint(seg[k - k0], m, lam, pi, gm)
+                    b.step_adjoint(x, m, lam, pi, gm)
                 add_obs(k)
         return None, None, None, None, lam, pi, gm
 

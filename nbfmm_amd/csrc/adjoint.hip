@@ -471,6 +471,38 @@ hipError_t launch_axpy(int n, float a, const float* x, float* y, hipStream_t s) 
   return hipGetLastError();
 }
 
+// Device-to-device copies of up to kCopyRegions regions in one launch:
+// blockIdx.y picks the region, the x blocks stride over it 16 bytes at a
+// time when both ends are 16-byte aligned (4 bytes at a time otherwise).
+__global__ __launch_bounds__(256) void k_copy_regions(CopyRegions c) {
+  const int r = blockIdx.y;
+  const uint32_t* src = c.src[r];
+  uint32_t* dst = c.dst[r];
+  const size_t n = c.words[r];
+  const size_t first = (size_t)blockIdx.x * 256 + threadIdx.x, stride = (size_t)gridDim.x * 256;
+  size_t done = 0;
+  if (((reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(dst)) & 15) == 0) {
+    const size_t n4 = n / 4;
+    const uint4* s4 = reinterpret_cast<const uint4*>(src);
+    uint4* d4 = reinterpret_cast<uint4*>(dst);
+    for (size_t i = first; i < n4; i += stride) d4[i] = s4[i];
+    done = n4 * 4;
+  }
+  for (size_t i = done + first; i < n; i += stride) dst[i] = src[i];
+}
+
+hipError_t launch_copy_regions(const CopyRegions& c, int count, hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  if (count > kCopyRegions) return hipErrorInvalidValue;
+  size_t most = 0;
+  for (int r = 0; r < count; ++r) most = c.words[r] > most ? c.words[r] : most;
+  if (most == 0) return hipSuccess;
+  const size_t blocks = (most / 4 + 255) / 256;
+  hipLaunchKernelGGL(k_copy_regions, dim3((unsigned)(blocks < 2048 ? (blocks ? blocks : 1) : 2048), count),
+                     dim3(256), 0, s, c);
+  return hipGetLastError();
+}
+
 hipError_t launch_adjoint(const SysView& v, const CallIO& io, int mode, int grid_blocks, hipStream_t s) {
   if (v.n <= 0) return hipSuccess;
   const OneSys one{v};

@@ -37,6 +37,14 @@ int hip_fail(hipError_t e, const char* what) {
 
 int cdiv(int a, int b) { return (a + b - 1) / b; }
 
+// A sorted state kept for a trajectory's adjoint (solver or batch arrays).
+struct SortArrays {
+  float4* part;
+  uint32_t* key;
+  int *head, *occ, *nocc;
+  float4* mpole;
+};
+
 }  // namespace
 
 int nbfmm_amd::report_error(int code, const std::string& msg) { return fail(code, msg); }
@@ -137,12 +145,7 @@ struct nbfmm_solver {
   // sorted states a trajectory backward keeps for the steps it recomputes
   // (one per recomputed step of a segment; made by the first backward that
   // needs them, held until the solver is destroyed)
-  struct SortSlot {
-    float4* part;
-    uint32_t* key;
-    int *head, *occ, *nocc;
-    float4* mpole;
-  };
+  using SortSlot = SortArrays;
   std::vector<SortSlot> slots;
   int ncells_tree = 0;
   // warm-started sort: key/part/head hold a stable sort of warm_n particles
@@ -726,8 +729,8 @@ int nbfmm_trajectory_workspace(int n, int num_steps, int checkpoint_every, size_
   const size_t state = (size_t)n * 2 * sizeof(float);
   const size_t nck = (size_t)(num_steps / checkpoint_every) + 1;
   if (checkpoint_bytes) *checkpoint_bytes = nck * 2 * state;
-  // recompute buffer: positions of one segment, plus x, x_pre and acc scratch
-  if (work_bytes) *work_bytes = ((size_t)checkpoint_every + 3) * state;
+  // recompute state: x, x_pre and acceleration scratch
+  if (work_bytes) *work_bytes = 3 * state;
   return NBFMM_OK;
 }
 
@@ -768,6 +771,26 @@ int nbfmm_trajectory_forward(nbfmm_solver* s, int n, float tick, int num_steps, 
 
 namespace {
 
+// The copies that keep a sorted state (particles, keys, head, occupied
+// leaves and their count, multipoles: n, heads, occ, nocc and tree entries
+// of the arrays that start at v's pointers) in a slot, for one launch.
+CopyRegions sort_copy(const SortArrays& sl, const SysView& v, size_t n, size_t heads, size_t occ, size_t nocc,
+                      size_t tree) {
+  CopyRegions c{};
+  auto put = [&](int r, const void* src, void* dst, size_t bytes) {
+    c.src[r] = static_cast<const uint32_t*>(src);
+    c.dst[r] = static_cast<uint32_t*>(dst);
+    c.words[r] = bytes / sizeof(uint32_t);
+  };
+  put(0, v.part, sl.part, n * sizeof(float4));
+  put(1, v.key, sl.key, n * sizeof(uint32_t));
+  put(2, v.head, sl.head, heads * sizeof(int));
+  put(3, v.occ, sl.occ, occ * sizeof(int));
+  put(4, v.nocc, sl.nocc, nocc * sizeof(int));
+  put(5, v.mpole, sl.mpole, tree * sizeof(float4));
+  return c;
+}
+
 // At least `count` sorted-state slots, each for max_n particles.
 int ensure_slots(nbfmm_solver* s, int count) {
   const size_t N = (size_t)s->max_n, C = (size_t)s->B * s->B;
@@ -787,12 +810,8 @@ int ensure_slots(nbfmm_solver* s, int count) {
 int save_sort(nbfmm_solver* s, int n, const nbfmm_solver::SortSlot& sl, hipStream_t st) {
   const SysView& v = s->v;
   const size_t C = (size_t)s->B * s->B;
-  NB_HIP(hipMemcpyAsync(sl.part, v.part, (size_t)n * sizeof(float4), hipMemcpyDeviceToDevice, st));
-  NB_HIP(hipMemcpyAsync(sl.key, v.key, (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
-  NB_HIP(hipMemcpyAsync(sl.head, v.head, (C + 1) * sizeof(int), hipMemcpyDeviceToDevice, st));
-  NB_HIP(hipMemcpyAsync(sl.occ, v.occ, std::min(C, (size_t)n) * sizeof(int), hipMemcpyDeviceToDevice, st));
-  NB_HIP(hipMemcpyAsync(sl.nocc, v.nocc, sizeof(int), hipMemcpyDeviceToDevice, st));
-  NB_HIP(hipMemcpyAsync(sl.mpole, v.mpole, (size_t)s->ncells_tree * sizeof(float4), hipMemcpyDeviceToDevice, st));
+  NB_HIP(launch_copy_regions(sort_copy(sl, v, (size_t)n, C + 1, std::min(C, (size_t)n), 1, (size_t)s->ncells_tree),
+                             kCopyRegions, st));
   return NBFMM_OK;
 }
 
@@ -815,8 +834,7 @@ int nbfmm_trajectory_backward(nbfmm_solver* s, int n, float tick, int num_steps,
   const size_t sb = (size_t)n * 2 * sizeof(float);
   float* lam = grad_x0;
   float* pi = grad_xpre0;
-  float* seg = work;                         // K positions
-  float* xc = work + (size_t)K * n * 2;      // recompute state
+  float* xc = work;  // recompute state
   float* xp = xc + (size_t)n * 2;
   float* ac = xp + (size_t)n * 2;
   NB_HIP(launch_fill(2 * n, 0.0f, lam, st));
@@ -837,13 +855,16 @@ int nbfmm_trajectory_backward(nbfmm_solver* s, int n, float tick, int num_steps,
     const float* c = ckpt + (size_t)sg * 2 * n * 2;
     NB_HIP(hipMemcpyAsync(xc, c, sb, hipMemcpyDeviceToDevice, st));
     NB_HIP(hipMemcpyAsync(xp, c + (size_t)n * 2, sb, hipMemcpyDeviceToDevice, st));
-    for (int k = k0; k < k1; ++k) {
-      NB_HIP(hipMemcpyAsync(seg + (size_t)(k - k0) * n * 2, xc, sb, hipMemcpyDeviceToDevice, st));
-      if (k + 1 < k1) {
-        // the step sorts x_k and builds its tree: kept for the adjoint at x_k
-        if ((rc = nbfmm_step(s, n, tick, xc, xp, w, ac, stream))) return rc;
-        if (n > 0 && (rc = save_sort(s, n, s->slots[k - k0], st))) return rc;
-      }
+    // steps k0 .. k1-2 again, as nbfmm_run's eager steps (with the warm start
+    // on, each but the last makes the next sort's first pass: the same bits
+    // as separate steps); step k sorts x_k and builds its tree, kept for the
+    // adjoint at x_k.  xc ends at x_{k1-1}.
+    CallIO rio = call_io(xc, w, ac, xc, xp);
+    const bool chain = s->warm_enabled && n > 0 && !use_front(s, n);
+    for (int k = k0; k + 1 < k1; ++k) {
+      const int mode = chain && k + 2 < k1 ? kP2PStepSort : kP2PStep;
+      if ((rc = run_pipeline(s, n, tick, mode, rio, st, chain && k > k0))) return rc;
+      if (n > 0 && (rc = save_sort(s, n, s->slots[k - k0], st))) return rc;
     }
     for (int k = k1 - 1; k >= k0; --k) {
       if (k + 1 < k1 && n > 0) {
@@ -856,14 +877,14 @@ int nbfmm_trajectory_backward(nbfmm_solver* s, int n, float tick, int num_steps,
         v.occ = sl.occ;
         v.nocc = sl.nocc;
         v.mpole = sl.mpole;
-        CallIO io = call_io(seg + (size_t)(k - k0) * n * 2, w);
+        CallIO io = call_io(xc, w);  // (the adjoint reads the kept sort's positions)
         io.gin = (const float2*)lam;
         io.gpos = (float2*)lam;
         io.gpre = (float2*)pi;
         io.gw = grad_w;
         io.gscale = tick * tick;
         NB_HIP(launch_adjoint(v, io, 1, s->adj_grid, st));
-      } else if ((rc = nbfmm_step_adjoint(s, n, tick, seg + (size_t)(k - k0) * n * 2, w, lam, pi, grad_w, stream))) {
+      } else if ((rc = nbfmm_step_adjoint(s, n, tick, xc, w, lam, pi, grad_w, stream))) {
         return rc;
       }
       if ((rc = add_obs(k))) return rc;
@@ -1047,12 +1068,7 @@ struct nbfmm_batch {
   size_t n_all = 0, head_all = 0, cell_all = 0, tree_all = 0;  // sizes of the concatenated arrays
   // sorted states kept for the step adjoint (nbfmm_batch_keep_sort), each
   // laid out as the batch's own arrays
-  struct SortSlot {
-    float4* part;
-    uint32_t* key;
-    int *head, *occ, *nocc;
-    float4* mpole;
-  };
+  using SortSlot = SortArrays;
   std::vector<SortSlot> slots;
   std::vector<int> part_off;      // S+1
   std::vector<SysView> hv;        // host copies of the views
@@ -1584,10 +1600,12 @@ int nbfmm_batch_step_adjoint(nbfmm_batch* b, const float* x, const float* w, flo
   return NBFMM_OK;
 }
 
-int nbfmm_batch_keep_sort(nbfmm_batch* b, int slot, void* stream) {
-  if (!b) return fail(NBFMM_ERR_INVALID_ARGUMENT, "null batch");
-  if (slot < 0 || slot > 4096) return fail(NBFMM_ERR_INVALID_ARGUMENT, "slot in [0, 4096]");
-  if (!b->warm_valid) return fail(NBFMM_ERR_INVALID_ARGUMENT, "no sort to keep: step or solve first");
+namespace {
+
+constexpr int kMaxKeptSorts = 4096;  // slots 0..4096 of nbfmm_batch_keep_sort
+
+// The batch's current sort and tree into slot `slot` (allocated on first use).
+int batch_keep(nbfmm_batch* b, int slot, hipStream_t st) {
   while ((int)b->slots.size() <= slot) {
     nbfmm_batch::SortSlot sl{};
     int rc;
@@ -1597,15 +1615,37 @@ int nbfmm_batch_keep_sort(nbfmm_batch* b, int slot, void* stream) {
       return rc;
     b->slots.push_back(sl);
   }
-  hipStream_t st = as_stream(stream);
   const nbfmm_batch::SortSlot& sl = b->slots[slot];
   const SysView& v = b->hv[0];  // system 0's view: every array's base
-  NB_HIP(hipMemcpyAsync(sl.part, v.part, b->n_all * sizeof(float4), hipMemcpyDeviceToDevice, st));
-  NB_HIP(hipMemcpyAsync(sl.key, v.key, b->n_all * sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
-  NB_HIP(hipMemcpyAsync(sl.head, v.head, b->head_all * sizeof(int), hipMemcpyDeviceToDevice, st));
-  NB_HIP(hipMemcpyAsync(sl.occ, v.occ, b->cell_all * sizeof(int), hipMemcpyDeviceToDevice, st));
-  NB_HIP(hipMemcpyAsync(sl.nocc, v.nocc, (size_t)b->S * sizeof(int), hipMemcpyDeviceToDevice, st));
-  NB_HIP(hipMemcpyAsync(sl.mpole, v.mpole, b->tree_all * sizeof(float4), hipMemcpyDeviceToDevice, st));
+  NB_HIP(launch_copy_regions(sort_copy(sl, v, b->n_all, b->head_all, b->cell_all, (size_t)b->S, b->tree_all),
+                             kCopyRegions, st));
+  return NBFMM_OK;
+}
+
+}  // namespace
+
+int nbfmm_batch_keep_sort(nbfmm_batch* b, int slot, void* stream) {
+  if (!b) return fail(NBFMM_ERR_INVALID_ARGUMENT, "null batch");
+  if (slot < 0 || slot > kMaxKeptSorts) return fail(NBFMM_ERR_INVALID_ARGUMENT, "slot in [0, 4096]");
+  if (!b->warm_valid) return fail(NBFMM_ERR_INVALID_ARGUMENT, "no sort to keep: step or solve first");
+  return batch_keep(b, slot, as_stream(stream));
+}
+
+int nbfmm_batch_run_keep(nbfmm_batch* b, int steps, float* xcur, float* xpre, const float* w, float* acc,
+                         void* stream) {
+  if (!b || !xcur || !xpre || !w || !acc) return fail(NBFMM_ERR_INVALID_ARGUMENT, "null argument");
+  if (steps < 0 || steps > kMaxKeptSorts + 1) return fail(NBFMM_ERR_INVALID_ARGUMENT, "num_steps in [0, 4097]");
+  hipStream_t st = as_stream(stream);
+  // nbfmm_batch_run's eager steps, each kept once its sort and tree are built
+  // (the P2P of a step followed by another makes only the next sort's first
+  // pass, which leaves the kept arrays alone)
+  CallIO io = call_io(xcur, w, acc, xcur, xpre);
+  const bool chain = b->warm_enabled;
+  for (int k = 0; k < steps; ++k) {
+    const int mode = chain && k + 1 < steps ? kP2PStepSort : kP2PStep;
+    if (int rc = batch_pipeline(b, mode, io, st, chain && k > 0)) return rc;
+    if (int rc = batch_keep(b, k, st)) return rc;
+  }
   return NBFMM_OK;
 }
 

@@ -97,6 +97,15 @@ int adj_zero_blocks(const SysView& v);
 hipError_t launch_axpy(int n, float a, const float* x, float* y, hipStream_t s);
 // y = a over n floats (a kernel, so that callers may capture it in a graph)
 hipError_t launch_fill(int n, float a, float* y, hipStream_t s);
+// the first `count` (<= kCopyRegions) copies of dst[r] = src[r], words[r]
+// 32-bit words each, in one launch (regions must not overlap)
+constexpr int kCopyRegions = 6;
+struct CopyRegions {
+  const uint32_t* src[kCopyRegions];
+  uint32_t* dst[kCopyRegions];
+  size_t words[kCopyRegions];
+};
+hipError_t launch_copy_regions(const CopyRegions& c, int count, hipStream_t s);
 hipError_t launch_postdo(const SysView& v, float2* eff0, hipStream_t s);
 hipError_t launch_update(int n, float tick, float2* xcur, float2* xpre, const float2* acc, hipStream_t s);
 // particle buffers <-> reference layouts; which = NBFMM_BUF_* id

@@ -270,6 +270,40 @@ def test_trajectory_adjoint_matches_fd_and_checkpoint_invariance():
     s.close()
 
 
+def test_trajectory_checkpoints_chained_recompute():
+    """A system above the one-block front's size, whose recomputed steps run
+    chained as in a run (each step's P2P making the next sort's first pass)
+    with every step's sort kept for its adjoint: K = 1 (nothing recomputed),
+    3 and steps give the same bits, with particles changing leaves."""
+    from nbfmm_amd import Solver
+    from nbfmm_amd.autograd import fmm_trajectory
+    L, n, steps, tick = 5, 20000, 9, 2e-3
+    lim = (-1.0, -1.0, 1.0, 1.0)
+    rng = np.random.default_rng(41)
+    pos = rng.uniform(-0.9, 0.9, (n, 2))
+    pre = pos - rng.normal(0, 1.0, (n, 2)) * tick
+    w = rng.uniform(0.5, 1.5, n) / n
+    B = 1 << (L + 1)
+    leaf = lambda x: (np.floor((x[:, 0] + 1) * B / 2) + B * np.floor((x[:, 1] + 1) * B / 2)).astype(np.int64)
+    assert (leaf(pos) != leaf(2 * pos - pre)).sum() > 100  # movers from the first step on
+    obs = [4, steps]
+    c = dev(rng.normal(0, 1, (len(obs), n, 2)))
+    s = Solver(L, n, lim, 1e-3)
+    grads = {}
+    for K in (1, 3, steps):
+        x0, p0, m = (dev(a).requires_grad_(True) for a in (pos, pre, w))
+        out = fmm_trajectory(s, tick, steps, K, obs, x0, p0, m)
+        (out * c).sum().backward()
+        torch.cuda.synchronize()
+        grads[K] = (x0.grad.clone(), p0.grad.clone(), m.grad.clone())
+    assert int(s.flags()) == 0
+    for K in (3, steps):
+        for a, b in zip(grads[1], grads[K]):
+            assert torch.equal(a, b), K
+    assert all(bool(torch.isfinite(t).all()) for t in grads[1])
+    s.close()
+
+
 def test_forces_autograd_function():
     from nbfmm_amd import Solver
     from nbfmm_amd.autograd import fmm_forces
