@@ -94,15 +94,6 @@ __device__ __forceinline__ void adjoint_fold(float& gx, float& gy, float cx, flo
   gy = __builtin_fmaf(-3.0f, cy, gy);
 }
 
-// Level-0 subtree sums start at zero for every leaf.
-template <class Sys>
-__global__ __launch_bounds__(256) void k_adj_zero(Sys sys) {
-  int blk, nblk;
-  const SysView v = sys.get(blk, nblk);
-  for (int c = blk * blockDim.x + threadIdx.x; c < v.B * v.B; c += nblk * blockDim.x)
-    v.gup[c] = make_float2(0.0f, 0.0f);
-}
-
 // Upstream gradient into sorted order, scaled: one thread per particle.
 template <class Sys>
 __global__ __launch_bounds__(256) void k_adj_sort(Sys sys, CallIO io) {
@@ -116,64 +107,69 @@ __global__ __launch_bounds__(256) void k_adj_sort(Sys sys, CallIO io) {
   v.gsort[i] = make_float2(sc * g0.x, sc * g0.y);
 }
 
-// Sum of the sorted upstream gradient over each occupied leaf (L2P^T), in
-// sorted order.  As in P2M, the time is the chain of the heaviest leaf, so
-// each thread keeps the next kAdjLeafDepth values in flight while it adds
-// the current ones; loads are unconditional (indices clamped to the leaf's
-// last particle), only the adds are predicated.
+// Sum of the sorted upstream gradient over leaf c (L2P^T), in sorted order
+// (0 for an empty leaf).  As in P2M, the time is the chain of the heaviest
+// leaf, so the thread keeps the next kAdjLeafDepth values in flight while it
+// adds the current ones; loads are unconditional (indices clamped to the
+// leaf's last particle), only the adds are predicated.
 constexpr int kAdjLeafDepth = 16;
 
-template <class Sys>
-__global__ __launch_bounds__(256) void k_adj_leaf(Sys sys) {
+__device__ __forceinline__ float2 adj_leaf_sum(const SysView& v, int c) {
 #pragma clang fp contract(off)
-  int blk, nblk;
-  const SysView v = sys.get(blk, nblk);
-  // one thread per leaf, ranges read from head (empty leaves return)
-  const int c = blk * blockDim.x + threadIdx.x;
-  if (c >= v.B * v.B) return;
   const int a = v.head[c], b = v.head[c + 1];
-  if (b <= a) return;
-  const int last = b - 1;
-  float2 next[kAdjLeafDepth];
-#pragma unroll
-  for (int k = 0; k < kAdjLeafDepth; ++k) next[k] = v.gsort[min(a + k, last)];
   float sx = 0.0f, sy = 0.0f;
-  for (int s = a; s < b; s += kAdjLeafDepth) {
-    float2 cur[kAdjLeafDepth];
+  if (b > a) {
+    const int last = b - 1;
+    float2 next[kAdjLeafDepth];
 #pragma unroll
-    for (int k = 0; k < kAdjLeafDepth; ++k) cur[k] = next[k];
+    for (int k = 0; k < kAdjLeafDepth; ++k) next[k] = v.gsort[min(a + k, last)];
+    for (int s = a; s < b; s += kAdjLeafDepth) {
+      float2 cur[kAdjLeafDepth];
 #pragma unroll
-    for (int k = 0; k < kAdjLeafDepth; ++k) next[k] = v.gsort[min(s + kAdjLeafDepth + k, last)];
-    const int m = b - s;
+      for (int k = 0; k < kAdjLeafDepth; ++k) cur[k] = next[k];
 #pragma unroll
-    for (int k = 0; k < kAdjLeafDepth; ++k) {
-      if (k < m) {
-        sx += cur[k].x;
-        sy += cur[k].y;
+      for (int k = 0; k < kAdjLeafDepth; ++k) next[k] = v.gsort[min(s + kAdjLeafDepth + k, last)];
+      const int m = b - s;
+#pragma unroll
+      for (int k = 0; k < kAdjLeafDepth; ++k) {
+        if (k < m) {
+          sx += cur[k].x;
+          sy += cur[k].y;
+        }
       }
     }
   }
-  v.gup[c] = make_float2(sx, sy);
+  return make_float2(sx, sy);
 }
 
-constexpr int kAdjTile = 16;
+// Subtree sum of a parent from its four children (x0y0, x1y0, x0y1, x1y1).
+__device__ __forceinline__ float2 adj_up_combine(float2 a, float2 b, float2 c, float2 d) {
+#pragma clang fp contract(off)
+  return make_float2(a.x + b.x + c.x + d.x, a.y + b.y + c.y + d.y);
+}
 
-// Subtree sums of the upper levels: tiles of 16x16 leaves in LDS, then the
-// levels above the tiles in one block (same decomposition as the M2M pass).
+// Subtree sums of the gradient, levels 0..lgT (L2P^T and L2L^T): one block
+// per kUpTile x kUpTile tile of leaves (the upward pass's grid), one thread
+// per leaf summing its particles, then the tile's levels in LDS.  The levels
+// above the tiles are made by k_adj_m2l_fused.
 template <class Sys>
-__global__ __launch_bounds__(kAdjTile * kAdjTile) void k_adj_up_tile(Sys sys) {
-  __shared__ float2 lv[2][kAdjTile * kAdjTile];
+__global__ __launch_bounds__(kUpTile * kUpTile) void k_adj_up(Sys sys) {
+  __shared__ float2 lv[2][kUpTile * kUpTile];
   int blk, nblk;
   const SysView v = sys.get(blk, nblk);
   const int B = v.B;
-  const int T = B < kAdjTile ? B : kAdjTile;
+  const int T = B < kUpTile ? B : kUpTile;
   const int tiles = B / T;
   const int X0 = (blk % tiles) * T, Y0 = (blk / tiles) * T;
   const int t = threadIdx.x;
-  int lgT = 0;
-  while ((1 << lgT) < T) ++lgT;
-  if (t < T * T) lv[0][t] = v.gup[(Y0 + t / T) * B + X0 + t % T];
+  if (t < T * T) {
+    const int c = (Y0 + t / T) * B + X0 + t % T;
+    const float2 g = adj_leaf_sum(v, c);
+    v.gup[c] = g;
+    lv[0][t] = g;
+  }
   __syncthreads();
+  const int lgT = up_tile_lg(B);
   const int top = (v.L - 1) < lgT ? (v.L - 1) : lgT;
   int cur = 0;
   for (int l = 1; l <= top; ++l) {
@@ -181,9 +177,8 @@ __global__ __launch_bounds__(kAdjTile * kAdjTile) void k_adj_up_tile(Sys sys) {
     if (t < Dl * Dl) {
       const int x = t % Dl, y = t / Dl;
       const float2* ch = lv[cur];
-      const float2 a = ch[(2 * y) * Dc + 2 * x], b = ch[(2 * y) * Dc + 2 * x + 1];
-      const float2 c = ch[(2 * y + 1) * Dc + 2 * x], d = ch[(2 * y + 1) * Dc + 2 * x + 1];
-      const float2 m = make_float2(a.x + b.x + c.x + d.x, a.y + b.y + c.y + d.y);
+      const float2 m = adj_up_combine(ch[(2 * y) * Dc + 2 * x], ch[(2 * y) * Dc + 2 * x + 1],
+                                      ch[(2 * y + 1) * Dc + 2 * x], ch[(2 * y + 1) * Dc + 2 * x + 1]);
       lv[cur ^ 1][t] = m;
       v.gup[v.lvl_off(l) + ((Y0 >> l) + y) * (B >> l) + (X0 >> l) + x] = m;
     }
@@ -192,40 +187,12 @@ __global__ __launch_bounds__(kAdjTile * kAdjTile) void k_adj_up_tile(Sys sys) {
   }
 }
 
-template <class Sys>
-__global__ __launch_bounds__(1024) void k_adj_up_top(Sys sys) {
-  int blk, nblk;
-  const SysView v = sys.get(blk, nblk);
-  int first = 1;
-  while ((1 << (first - 1)) < (v.B < kAdjTile ? v.B : kAdjTile)) ++first;
-  for (int l = first; l < v.L; ++l) {
-    const int D = v.B >> l, Dc = D * 2;
-    const float2* ch = v.gup + v.lvl_off(l - 1);
-    for (int c = threadIdx.x; c < D * D; c += blockDim.x) {
-      const int x = c % D, y = c / D;
-      const float2 a = ch[(2 * y) * Dc + 2 * x], b = ch[(2 * y) * Dc + 2 * x + 1];
-      const float2 cc = ch[(2 * y + 1) * Dc + 2 * x], d = ch[(2 * y + 1) * Dc + 2 * x + 1];
-      v.gup[v.lvl_off(l) + c] = make_float2(a.x + b.x + cc.x + d.x, a.y + b.y + cc.y + d.y);
-    }
-    __syncthreads();
-  }
-}
-
-// M2L^T, all levels in one launch: gradient wrt each cell's COM and mass
-// from the M2L interactions it takes part in (as target and as source).
-// Consecutive threads take consecutive cells of a row; the six cells of a
-// window row are loaded together (clamped addresses, used only where the
+// M2L^T of cell c of level l: gradient wrt its COM and mass from the M2L
+// interactions it takes part in (as target and as source).  The six cells of
+// a window row are loaded together (clamped addresses, used only where the
 // window cell is in the box and not adjacent).
-template <class Sys>
-__global__ __launch_bounds__(256) void k_adj_m2l(Sys sys) {
-  int blk, nblk;
-  const SysView v = sys.get(blk, nblk);
-  const int t = blk * blockDim.x + threadIdx.x;
-  if (t >= v.lvl_off(v.L)) return;
-  int l = 0;
-  while (t >= v.lvl_off(l + 1)) ++l;
+__device__ __forceinline__ void adj_m2l_cell(const SysView& v, int l, int c) {
   const int D = v.B >> l;
-  const int c = t - v.lvl_off(l);
   const int tx = c % D, ty = c / D;
   const float4* mp = v.mpole + v.lvl_off(l);
   const float2* gu = v.gup + v.lvl_off(l);
@@ -256,7 +223,42 @@ __global__ __launch_bounds__(256) void k_adj_m2l(Sys sys) {
     }
   }
   adjoint_fold(gx, gy, cx, cy);
-  v.gmp[t] = make_float4(gx, gy, gm, 0.0f);
+  v.gmp[v.lvl_off(l) + c] = make_float4(gx, gy, gm, 0.0f);
+}
+
+// M2L^T, all levels in one launch (the fused M2L's grid): when the tree has
+// levels above the upward tiles, the system's block 0 first makes their
+// subtree sums, level by level from the tiles' top level, and then takes
+// their cells; the other blocks take the cells of levels 0..lgT, one thread
+// per cell, consecutive threads on consecutive cells of a row.
+template <class Sys>
+__global__ __launch_bounds__(256) void k_adj_m2l_fused(Sys sys) {
+  int blk, nblk;
+  const SysView v = sys.get(blk, nblk);
+  const int lgT = up_tile_lg(v.B);
+  const bool has_top = v.L - 1 > lgT;
+  if (has_top && blk == 0) {
+    for (int l = lgT + 1; l < v.L; ++l) {
+      const int D = v.B >> l, Dc = D * 2;
+      const float2* ch = v.gup + v.lvl_off(l - 1);
+      for (int c = threadIdx.x; c < D * D; c += blockDim.x) {
+        const int x = c % D, y = c / D;
+        v.gup[v.lvl_off(l) + c] = adj_up_combine(ch[(2 * y) * Dc + 2 * x], ch[(2 * y) * Dc + 2 * x + 1],
+                                                 ch[(2 * y + 1) * Dc + 2 * x], ch[(2 * y + 1) * Dc + 2 * x + 1]);
+      }
+      __syncthreads();  // (the block's own global writes, read by the next level)
+    }
+    for (int l = lgT + 1; l < v.L; ++l) {
+      const int D = v.B >> l;
+      for (int c = threadIdx.x; c < D * D; c += blockDim.x) adj_m2l_cell(v, l, c);
+    }
+    return;
+  }
+  const int t = (blk - (has_top ? 1 : 0)) * blockDim.x + threadIdx.x;
+  if (t >= v.lvl_off(has_top ? lgT + 1 : v.L)) return;
+  int l = 0;
+  while (t >= v.lvl_off(l + 1)) ++l;
+  adj_m2l_cell(v, l, t - v.lvl_off(l));
 }
 
 // Gradient wrt the leaf's (X, M): the M2M^T chain from the top level down,
@@ -508,13 +510,8 @@ hipError_t launch_adjoint(const SysView& v, const CallIO& io, int mode, int grid
   const OneSys one{v};
   hipLaunchKernelGGL(k_adj_sort<OneSys>, dim3(cdiv(v.n, 256)), dim3(256), 0, s, one, io);
   if (v.L > 0) {
-    hipLaunchKernelGGL(k_adj_zero<OneSys>, dim3(adj_zero_blocks(v)), dim3(256), 0, s, one);
-    // the P2M grid (one thread per leaf, blocks of 256)
-    hipLaunchKernelGGL(k_adj_leaf<OneSys>, dim3(p2m_leaf_blocks(v.B)), dim3(256), 0, s, one);
-    const int tb = upward_tile_blocks(v.L, v.B);
-    if (tb) hipLaunchKernelGGL(k_adj_up_tile<OneSys>, dim3(tb), dim3(kAdjTile * kAdjTile), 0, s, one);
-    if (upward_top_blocks(v.L, v.B)) hipLaunchKernelGGL(k_adj_up_top<OneSys>, dim3(1), dim3(1024), 0, s, one);
-    hipLaunchKernelGGL(k_adj_m2l<OneSys>, dim3(m2l_blocks(v)), dim3(256), 0, s, one);
+    hipLaunchKernelGGL(k_adj_up<OneSys>, dim3(up_blocks(v.L, v.B)), dim3(kUpTile * kUpTile), 0, s, one);
+    hipLaunchKernelGGL(k_adj_m2l_fused<OneSys>, dim3(m2l_fused_blocks(v)), dim3(256), 0, s, one);
   }
   if (mode == kAdjSolve)
     hipLaunchKernelGGL((k_adj_p2p<kAdjSolve, OneSys>), dim3(grid_blocks), dim3(kAdjP2PWaves * kWave), 0, s, one, io);
@@ -523,24 +520,11 @@ hipError_t launch_adjoint(const SysView& v, const CallIO& io, int mode, int grid
   return hipGetLastError();
 }
 
-int adj_zero_blocks(const SysView& v) {
-  if (v.L <= 0) return 0;
-  const int b = cdiv(v.B * v.B, 256);
-  return b < 1024 ? b : 1024;
-}
-
 hipError_t launch_adjoint_batch(const BatchLaunch& bl, const CallIO& io, int mode, hipStream_t s) {
   hipLaunchKernelGGL(k_adj_sort<ManySys>, dim3(bl.adj_sort.blocks), dim3(256), 0, s, bl.adj_sort.sys, io);
-  if (bl.adj_zero.blocks)
-    hipLaunchKernelGGL(k_adj_zero<ManySys>, dim3(bl.adj_zero.blocks), dim3(256), 0, s, bl.adj_zero.sys);
-  if (bl.p2m.blocks) hipLaunchKernelGGL(k_adj_leaf<ManySys>, dim3(bl.p2m.blocks), dim3(256), 0, s, bl.p2m.sys);
-  if (bl.m2m_tile.blocks)
-    hipLaunchKernelGGL(k_adj_up_tile<ManySys>, dim3(bl.m2m_tile.blocks), dim3(kAdjTile * kAdjTile), 0, s,
-                       bl.m2m_tile.sys);
-  if (bl.m2m_top.blocks)
-    hipLaunchKernelGGL(k_adj_up_top<ManySys>, dim3(bl.m2m_top.blocks), dim3(1024), 0, s, bl.m2m_top.sys);
-  if (bl.adj_m2l.blocks)
-    hipLaunchKernelGGL(k_adj_m2l<ManySys>, dim3(bl.adj_m2l.blocks), dim3(256), 0, s, bl.adj_m2l.sys);
+  if (bl.up.blocks) hipLaunchKernelGGL(k_adj_up<ManySys>, dim3(bl.up.blocks), dim3(kUpTile * kUpTile), 0, s, bl.up.sys);
+  if (bl.m2l.blocks)
+    hipLaunchKernelGGL(k_adj_m2l_fused<ManySys>, dim3(bl.m2l.blocks), dim3(256), 0, s, bl.m2l.sys);
   if (mode == kAdjSolve)
     hipLaunchKernelGGL((k_adj_p2p<kAdjSolve, ManySys>), dim3(bl.adj_p2p.blocks), dim3(kAdjP2PWaves * kWave), 0, s, bl.adj_p2p.sys,
                        io);

@@ -1437,31 +1437,24 @@ int nbfmm_batch_create(int S, const int* num_particle, const int* num_level, con
   hipError_t e = hipMemcpy(b->ddesc, b->hdesc.data(), S * sizeof(SysDesc), hipMemcpyHostToDevice);
   if (e != hipSuccess) { batch_free(b); delete b; return hip_fail(e, "hipMemcpy system descriptors"); }
   // launch maps
-  std::vector<int> tiles(S), rows(S), p2m(S), up(S), mtile(S), mtop(S), m2l(S), am2l(S), asort(S), azero(S),
-      wmerge(S), one(S, 1);
+  std::vector<int> tiles(S), rows(S), up(S), mtop(S), m2l(S), asort(S), wmerge(S), one(S, 1);
   for (int s = 0; s < S; ++s) {
     const SysView& v = b->hv[s];
     const int C = v.B * v.B;
     tiles[s] = cdiv(v.n, kSortTile);
     rows[s] = v.B;
-    p2m[s] = v.L > 0 ? p2m_leaf_blocks(v.B) : 0;  // also the adjoint leaf kernel's grid
     up[s] = up_blocks(v.L, v.B);
-    mtile[s] = upward_tile_blocks(v.L, v.B);
     mtop[s] = upward_top_blocks(v.L, v.B);
     m2l[s] = m2l_fused_blocks(v);
-    am2l[s] = m2l_blocks(v);
     asort[s] = cdiv(v.n, 256);
-    azero[s] = adj_zero_blocks(v);
     // one wave per 64 particles (touched leaves are about twice the movers;
     // the waves grid-stride over the list)
     wmerge[s] = cdiv(std::min(C, v.n / 64 + 64), 256 / kWave);
     b->bl.max_B = std::max(b->bl.max_B, v.B);
   }
   if ((rc = make_map(b, tiles, &b->bl.tiles)) || (rc = make_map(b, rows, &b->bl.rows)) ||
-      (rc = make_map(b, p2m, &b->bl.p2m)) || (rc = make_map(b, mtile, &b->bl.m2m_tile)) ||
-      (rc = make_map(b, up, &b->bl.up)) || (rc = make_map(b, am2l, &b->bl.adj_m2l)) ||
-      (rc = make_map(b, mtop, &b->bl.m2m_top)) || (rc = make_map(b, m2l, &b->bl.m2l)) ||
-      (rc = make_map(b, asort, &b->bl.adj_sort)) || (rc = make_map(b, azero, &b->bl.adj_zero)) ||
+      (rc = make_map(b, up, &b->bl.up)) || (rc = make_map(b, mtop, &b->bl.m2m_top)) ||
+      (rc = make_map(b, m2l, &b->bl.m2l)) || (rc = make_map(b, asort, &b->bl.adj_sort)) ||
       (rc = make_map(b, wmerge, &b->bl.warm_merge)) || (rc = make_map(b, one, &b->bl.one)) ||
       (rc = batch_alloc(b, &b->d_pairs, (size_t)S)) ||
       (rc = make_map(b, p2p_share(b, kAdjP2PWaves, cus * kAdjP2PBatchBlocksPerCU), &b->bl.adj_p2p)) ||
@@ -1656,7 +1649,7 @@ int nbfmm_batch_step_adjoint_kept(nbfmm_batch* b, int slot, float* lam, float* p
   const nbfmm_batch::SortSlot& sl = b->slots[slot];
   // the adjoint's launch maps with every system's sorted arrays in the slot
   BatchLaunch bl = b->bl;
-  for (KernelMap* km : {&bl.adj_sort, &bl.adj_zero, &bl.p2m, &bl.m2m_tile, &bl.m2m_top, &bl.adj_m2l, &bl.adj_p2p}) {
+  for (KernelMap* km : {&bl.adj_sort, &bl.up, &bl.m2l, &bl.adj_p2p}) {
     km->sys.base.part = sl.part;
     km->sys.base.key = sl.key;
     km->sys.base.head = sl.head;

@@ -121,6 +121,18 @@ struct SysView {
 // segments); segment sums, then the leaf's sum of them, both in order.
 constexpr int kP2MSeg = 16;
 
+// Upward passes (forward and adjoint): one block per kUpTile x kUpTile tile
+// of leaves makes levels 1..lgT of its tile in LDS; a single block makes the
+// levels above (block 0 of the fused M2L launch, or k_m2m_top).
+constexpr int kUpTile = 16;
+// log2 of the upward tiles' side (lgT).
+__device__ __host__ __forceinline__ int up_tile_lg(int B) {
+  const int T = B < kUpTile ? B : kUpTile;
+  int lgT = 0;
+  while ((1 << lgT) < T) ++lgT;
+  return lgT;
+}
+
 // Warm-started sort.
 constexpr uint32_t kMoverBit = 0x80000000u;  // tkey flag: the particle changed leaf
 // wstate words: [kWarmTouched] touched-leaf count; [kWarmColdReq] set by the

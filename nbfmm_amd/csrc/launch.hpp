@@ -20,16 +20,12 @@ struct KernelMap {
 struct BatchLaunch {
   KernelMap tiles;     // row pass: cdiv(n_s, kSortTile) per system
   KernelMap rows;      // row scan / column pass: B_s per system
-  KernelMap p2m;       // one thread per leaf: p2m_leaf_blocks(B_s) (the adjoint's leaf kernel)
-  KernelMap up;        // upward pass: one block per tile of leaves
-  KernelMap m2m_tile;  // adjoint upward tiles
+  KernelMap up;        // upward passes (forward and adjoint): one block per tile of leaves
   KernelMap m2m_top;   // M2M top levels
-  KernelMap m2l;       // fused M2L: m2l_fused_blocks
-  KernelMap adj_m2l;   // adjoint M2L: cdiv(tree cells, 256)
+  KernelMap m2l;       // fused M2L and adjoint M2L: m2l_fused_blocks
   KernelMap p2p;       // P2P: share of the P2P grid (blocks of the chosen variant)
   KernelMap adj_p2p;   // adjoint P2P: share of its grid (kAdjP2PWaves-wave blocks)
   KernelMap adj_sort;  // adjoint: upstream gradient to sorted order, cdiv(n_s, 256)
-  KernelMap adj_zero;  // adjoint: clear the level-0 subtree sums
   // warm-started sort: the particle kernels use adj_sort, the row kernels rows
   KernelMap warm_merge;  // one wave per touched leaf (grid-stride): its share of waves
   KernelMap one;       // one block per system
@@ -61,12 +57,10 @@ bool small_front_fits(int L, int n);
 hipError_t launch_small_front(const SysView& v, const CallIO& io, hipStream_t s);
 // forward: upward pass + fused M2L; else (adjoints) upward pass + the top M2M levels
 hipError_t launch_tree_batch(const BatchLaunch& b, hipStream_t s, bool forward);
-int upward_tile_blocks(int L, int B);  // adjoint upward tiles (0 when there is no M2M level)
 int up_blocks(int L, int B);           // the upward pass's tiles
 int upward_top_blocks(int L, int B);
 int m2l_blocks(const SysView& v);
 int m2l_fused_blocks(const SysView& v);
-int p2m_leaf_blocks(int B);       // one thread per leaf: all B * B leaves
 hipError_t launch_m2m(const SysView& v, hipStream_t s);
 hipError_t launch_m2l(const SysView& v, hipStream_t s);
 hipError_t launch_l2l(const SysView& v, hipStream_t s);
@@ -92,7 +86,6 @@ hipError_t launch_l2p(const SysView& v, hipStream_t s);
 // adjoint.hip; mode 0: solve adjoint, 1: Verlet step adjoint
 hipError_t launch_adjoint(const SysView& v, const CallIO& io, int mode, int grid_blocks, hipStream_t s);
 hipError_t launch_adjoint_batch(const BatchLaunch& b, const CallIO& io, int mode, hipStream_t s);
-int adj_zero_blocks(const SysView& v);
 // y += a * x over n floats
 hipError_t launch_axpy(int n, float a, const float* x, float* y, hipStream_t s);
 // y = a over n floats (a kernel, so that callers may capture it in a graph)

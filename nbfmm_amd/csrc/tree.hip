@@ -108,7 +108,6 @@ __device__ __forceinline__ float4 p2m_leaf(const SysView& v, int c) {
   return out;
 }
 
-constexpr int kUpTile = 16;  // leaves per side of one M2M tile block
 
 // P2M of the occupied leaves (the sort already cleared the empty ones), in
 // p2m_leaf's order.  The grid covers every leaf and reads its range from
@@ -122,7 +121,6 @@ constexpr int kUpTile = 16;  // leaves per side of one M2M tile block
 // listed leaves one each: lane t sums segment t and the wave adds the
 // segment sums in order.  Which wave takes a leaf does not change its bits.
 constexpr int kP2MCoop = 256;
-constexpr int kP2MThreads = 256;  // the adjoint's leaf kernel shares this grid (3 blocks per CU at 140 VGPRs)
 
 __device__ __forceinline__ void p2m_segment(const SysView& v, int s, int e, float& px, float& py, float& pw) {
 #pragma clang fp contract(off)
@@ -538,13 +536,6 @@ __global__ __launch_bounds__(256) void k_m2l(Sys sys) {
   v.local[t] = make_float2(ax, ay);
 }
 
-// log2 of the upward tiles' side: the upward pass makes levels 1..lgT.
-__device__ __host__ __forceinline__ int up_tile_lg(int B) {
-  const int T = B < kUpTile ? B : kUpTile;
-  int lgT = 0;
-  while ((1 << lgT) < T) ++lgT;
-  return lgT;
-}
 
 // One target's M2L sum over its 6 x 6 window in k_m2l's order (rows, then
 // columns), over the 27 cells not adjacent to the target.  The three window
@@ -734,12 +725,6 @@ hipError_t launch_p2m(const SysView& v, hipStream_t s) {
   return hipGetLastError();
 }
 
-int upward_tile_blocks(int L, int B) {
-  if (L <= 1) return 0;
-  const int T = B < kUpTile ? B : kUpTile;
-  return (B / T) * (B / T);
-}
-
 int up_blocks(int L, int B) {
   if (L <= 0) return 0;
   const int T = B < kUpTile ? B : kUpTile;
@@ -757,7 +742,6 @@ int m2l_fused_blocks(const SysView& v) {
   return cdiv(v.lvl_off(v.L), 256);
 }
 
-int p2m_leaf_blocks(int B) { return cdiv(B * B, kP2MThreads); }
 
 hipError_t launch_upward(const SysView& v, hipStream_t s, bool with_top) {
   if (v.L <= 0) return hipSuccess;
