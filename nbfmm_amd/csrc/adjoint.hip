@@ -188,14 +188,14 @@ __global__ __launch_bounds__(kUpTile * kUpTile) void k_adj_up(Sys sys) {
 }
 
 // M2L^T of cell c of level l: gradient wrt its COM and mass from the M2L
-// interactions it takes part in (as target and as source).  The six cells of
-// a window row are loaded together (clamped addresses, used only where the
+// interactions it takes part in (as target and as source); gu holds the
+// level's subtree sums (global memory or a copy in LDS).  The six cells of a
+// window row are loaded together (clamped addresses, used only where the
 // window cell is in the box and not adjacent).
-__device__ __forceinline__ void adj_m2l_cell(const SysView& v, int l, int c) {
+__device__ __forceinline__ void adj_m2l_cell(const SysView& v, int l, int c, const float2* gu) {
   const int D = v.B >> l;
   const int tx = c % D, ty = c / D;
   const float4* mp = v.mpole + v.lvl_off(l);
-  const float2* gu = v.gup + v.lvl_off(l);
   const float4 tc = mp[c];
   const float2 tg = gu[c];
   const int px = tx & ~1, py = ty & ~1;
@@ -227,38 +227,67 @@ __device__ __forceinline__ void adj_m2l_cell(const SysView& v, int l, int c) {
 }
 
 // M2L^T, all levels in one launch (the fused M2L's grid): when the tree has
-// levels above the upward tiles, the system's block 0 first makes their
-// subtree sums, level by level from the tiles' top level, and then takes
-// their cells; the other blocks take the cells of levels 0..lgT, one thread
-// per cell, consecutive threads on consecutive cells of a row.
+// levels above the upward tiles, each of the system's first kM2LTopBlocks
+// blocks makes their subtree sums in LDS, level by level from the tiles' top
+// level (block 0 writes them out), and takes its share of their cells; the
+// other blocks take the cells of levels 0..lgT, one thread per cell,
+// consecutive threads on consecutive cells of a row.
 template <class Sys>
 __global__ __launch_bounds__(256) void k_adj_m2l_fused(Sys sys) {
   int blk, nblk;
   const SysView v = sys.get(blk, nblk);
   const int lgT = up_tile_lg(v.B);
   const bool has_top = v.L - 1 > lgT;
-  if (has_top && blk == 0) {
-    for (int l = lgT + 1; l < v.L; ++l) {
-      const int D = v.B >> l, Dc = D * 2;
-      const float2* ch = v.gup + v.lvl_off(l - 1);
+  if (has_top && blk < kM2LTopBlocks) {
+    __shared__ float2 lv[kTopLds];  // level lgT + 1 at 0, each level above right after the one below
+    int D = v.B >> (lgT + 1);
+    {
+      const float2* ch = v.gup + v.lvl_off(lgT);
+      const int Dp = 2 * D;
       for (int c = threadIdx.x; c < D * D; c += blockDim.x) {
         const int x = c % D, y = c / D;
-        v.gup[v.lvl_off(l) + c] = adj_up_combine(ch[(2 * y) * Dc + 2 * x], ch[(2 * y) * Dc + 2 * x + 1],
-                                                 ch[(2 * y + 1) * Dc + 2 * x], ch[(2 * y + 1) * Dc + 2 * x + 1]);
+        const float2 m = adj_up_combine(ch[(2 * y) * Dp + 2 * x], ch[(2 * y) * Dp + 2 * x + 1],
+                                        ch[(2 * y + 1) * Dp + 2 * x], ch[(2 * y + 1) * Dp + 2 * x + 1]);
+        lv[c] = m;
+        if (blk == 0) v.gup[v.lvl_off(lgT + 1) + c] = m;
       }
-      __syncthreads();  // (the block's own global writes, read by the next level)
     }
-    for (int l = lgT + 1; l < v.L; ++l) {
-      const int D = v.B >> l;
-      for (int c = threadIdx.x; c < D * D; c += blockDim.x) adj_m2l_cell(v, l, c);
+    __syncthreads();
+    int po = 0;
+    for (int l = lgT + 2; l < v.L; ++l) {
+      const int Dp = D, o = po + Dp * Dp;
+      D >>= 1;
+      for (int c = threadIdx.x; c < D * D; c += blockDim.x) {
+        const int x = c % D, y = c / D;
+        const float2* ch = lv + po;
+        const float2 m = adj_up_combine(ch[(2 * y) * Dp + 2 * x], ch[(2 * y) * Dp + 2 * x + 1],
+                                        ch[(2 * y + 1) * Dp + 2 * x], ch[(2 * y + 1) * Dp + 2 * x + 1]);
+        lv[o + c] = m;
+        if (blk == 0) v.gup[v.lvl_off(l) + c] = m;
+      }
+      po = o;
+      __syncthreads();
+    }
+    // the top levels' cells, numbered across the levels: block b takes cells
+    // b * blockDim.x + threadIdx.x + k * kM2LTopBlocks * blockDim.x
+    const int ntop = v.lvl_off(v.L) - v.lvl_off(lgT + 1);
+    for (int f = blk * blockDim.x + threadIdx.x; f < ntop; f += kM2LTopBlocks * blockDim.x) {
+      int l = lgT + 1, off = 0;
+      D = v.B >> l;
+      while (f - off >= D * D) {
+        off += D * D;
+        ++l;
+        D >>= 1;
+      }
+      adj_m2l_cell(v, l, f - off, lv + off);
     }
     return;
   }
-  const int t = (blk - (has_top ? 1 : 0)) * blockDim.x + threadIdx.x;
+  const int t = (blk - (has_top ? kM2LTopBlocks : 0)) * blockDim.x + threadIdx.x;
   if (t >= v.lvl_off(has_top ? lgT + 1 : v.L)) return;
   int l = 0;
   while (t >= v.lvl_off(l + 1)) ++l;
-  adj_m2l_cell(v, l, t - v.lvl_off(l));
+  adj_m2l_cell(v, l, t - v.lvl_off(l), v.gup + v.lvl_off(l));
 }
 
 // Gradient wrt the leaf's (X, M): the M2M^T chain from the top level down,

@@ -125,6 +125,12 @@ constexpr int kP2MSeg = 16;
 // of leaves makes levels 1..lgT of its tile in LDS; a single block makes the
 // levels above (block 0 of the fused M2L launch, or k_m2m_top).
 constexpr int kUpTile = 16;
+// The levels above the tiles in a fused M2L launch (forward or adjoint):
+// each of the system's first kM2LTopBlocks blocks makes them in LDS (at most
+// kTopLds cells: levels lgT+1..7 at B = 512) and takes its share of their
+// cells.
+constexpr int kM2LTopBlocks = 2;
+constexpr int kTopLds = 256 + 64 + 16;
 // log2 of the upward tiles' side (lgT).
 __device__ __host__ __forceinline__ int up_tile_lg(int B) {
   const int T = B < kUpTile ? B : kUpTile;

@@ -585,15 +585,14 @@ __device__ __forceinline__ float2 m2l_window(const float4* mp, int D, int tx, in
   return make_float2(ax, ay);
 }
 
-constexpr int kTopLds = 256 + 64 + 16;  // levels lgT+1..L-1 at B = 512
-
 // M2L of the fused paths (solve, step, batches).  Cells without particles
 // get (0, 0) (the P2P reads the locals of occupied leaves and of their
 // ancestors only) and empty source cells are skipped: the other cells' bits
 // are k_m2l's.  When the tree has levels above the upward tiles, the
-// system's block 0 first makes them in LDS from the tiles' top level
-// (m2m_combine, written out as k_m2m_top would) and does their M2L; the
-// other blocks take levels 0..lgT as k_m2l does, one thread per cell.
+// system's first kM2LTopBlocks blocks each make them in LDS from the tiles'
+// top level (m2m_combine; block 0 writes them out as k_m2m_top would) and
+// share their cells' M2L; the other blocks take levels 0..lgT as k_m2l does,
+// one thread per cell.
 template <class Sys>
 __global__ __launch_bounds__(256) void k_m2l_fused(Sys sys) {
   int blk, nblk;
@@ -601,9 +600,10 @@ __global__ __launch_bounds__(256) void k_m2l_fused(Sys sys) {
   const int lgT = up_tile_lg(v.B);
   const bool has_top = v.L - 1 > lgT;
   const float eps = v.eps;
-  if (has_top && blk == 0) {
+  if (has_top && blk < kM2LTopBlocks) {
     // LDS: level lgT + 1 at 0 (made from level lgT in global memory), each
-    // level above right after the one below
+    // level above right after the one below; every top block makes them, and
+    // block 0 writes them out
     __shared__ float4 lv[kTopLds];
     int D = v.B >> (lgT + 1);
     {
@@ -614,7 +614,7 @@ __global__ __launch_bounds__(256) void k_m2l_fused(Sys sys) {
         const float4 m = m2m_combine(ch[(2 * y) * Dp + 2 * x], ch[(2 * y) * Dp + 2 * x + 1],
                                      ch[(2 * y + 1) * Dp + 2 * x], ch[(2 * y + 1) * Dp + 2 * x + 1]);
         lv[c] = m;
-        v.mpole[v.lvl_off(lgT + 1) + c] = m;
+        if (blk == 0) v.mpole[v.lvl_off(lgT + 1) + c] = m;
       }
     }
     __syncthreads();
@@ -628,24 +628,30 @@ __global__ __launch_bounds__(256) void k_m2l_fused(Sys sys) {
         const float4 m = m2m_combine(ch[(2 * y) * Dp + 2 * x], ch[(2 * y) * Dp + 2 * x + 1],
                                      ch[(2 * y + 1) * Dp + 2 * x], ch[(2 * y + 1) * Dp + 2 * x + 1]);
         lv[o + c] = m;
-        v.mpole[v.lvl_off(l) + c] = m;
+        if (blk == 0) v.mpole[v.lvl_off(l) + c] = m;
       }
       po = o;
       __syncthreads();
     }
-    po = 0;
-    D = v.B >> (lgT + 1);
-    for (int l = lgT + 1; l < v.L; ++l) {
-      const float4* mp = lv + po;
-      for (int c = threadIdx.x; c < D * D; c += blockDim.x)
-        v.local[v.lvl_off(l) + c] =
-            mp[c].w != 0.0f ? m2l_window(mp, D, c % D, c / D, mp[c], eps) : make_float2(0.0f, 0.0f);
-      po += D * D;
-      D >>= 1;
+    // M2L of the top levels' cells, numbered across the levels: block b takes
+    // cells b * blockDim.x + threadIdx.x + k * kM2LTopBlocks * blockDim.x
+    const int ntop = v.lvl_off(v.L) - v.lvl_off(lgT + 1);
+    for (int f = blk * blockDim.x + threadIdx.x; f < ntop; f += kM2LTopBlocks * blockDim.x) {
+      int l = lgT + 1, off = 0;
+      D = v.B >> l;
+      while (f - off >= D * D) {
+        off += D * D;
+        ++l;
+        D >>= 1;
+      }
+      const int c = f - off;
+      const float4* mp = lv + off;
+      v.local[v.lvl_off(l) + c] =
+          mp[c].w != 0.0f ? m2l_window(mp, D, c % D, c / D, mp[c], eps) : make_float2(0.0f, 0.0f);
     }
     return;
   }
-  const int t = (blk - (has_top ? 1 : 0)) * blockDim.x + threadIdx.x;
+  const int t = (blk - (has_top ? kM2LTopBlocks : 0)) * blockDim.x + threadIdx.x;
   if (t >= v.lvl_off(has_top ? lgT + 1 : v.L)) return;
   int l = 0;
   while (t >= v.lvl_off(l + 1)) ++l;
@@ -738,7 +744,7 @@ int m2l_blocks(const SysView& v) { return v.L > 0 ? cdiv(v.lvl_off(v.L), 256) : 
 int m2l_fused_blocks(const SysView& v) {
   if (v.L <= 0) return 0;
   const int lgT = up_tile_lg(v.B);
-  if (v.L - 1 > lgT) return 1 + cdiv(v.lvl_off(lgT + 1), 256);
+  if (v.L - 1 > lgT) return kM2LTopBlocks + cdiv(v.lvl_off(lgT + 1), 256);
   return cdiv(v.lvl_off(v.L), 256);
 }
 
