@@ -74,18 +74,28 @@ def test_graph_run_bitwise_equal_cold():
     cold.close()
 
 
-@pytest.mark.parametrize("jump", ["drift", "shuffle", "cluster", "crowd", "weights"])
+@pytest.mark.parametrize("jump", ["drift", "shuffle", "cluster", "crowd", "weights", "dense_few", "dense_many"])
 def test_solve_sequences(jump):
     """solve at x1, then at x2: small drift (warm placement), unrelated
     positions (most particles move), 8 groups of 500 particles into 8 leaves
     (leaves with 65..1024 movers: sorted in LDS), half the particles into one
-    leaf (thousands of movers), new masses only (no movers)."""
+    leaf (thousands of movers), new masses only (no movers), and a leaf of
+    3000 particles (its stayers walked in 47 chunks of 64, several in flight)
+    that gains 30 movers (ranked in registers) or 200 (sorted in LDS)."""
     from nbfmm_amd.workloads import disk
     wl = disk(n=20000)
     warm, cold = pair(wl)
     rng = np.random.default_rng(9)
     x1, m1 = wl.position, wl.weight
-    if jump == "drift":
+    if jump.startswith("dense"):
+        # the dense leaf at x1[0] in both states; movers come from elsewhere
+        x1 = x1.copy()
+        x1[1:3000] = x1[0] + 1e-6 * rng.random((2999, 2)).astype(np.float32)
+        nm = 30 if jump == "dense_few" else 200
+        x2 = x1.copy()
+        x2[5000:5000 + nm] = x1[0] + 1e-6 * rng.random((nm, 2)).astype(np.float32)
+        m2 = m1
+    elif jump == "drift":
         x2 = x1 + 2e-3 * rng.standard_normal(x1.shape).astype(np.float32)
         m2 = m1
     elif jump == "shuffle":
