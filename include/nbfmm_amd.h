@@ -328,10 +328,15 @@ int nbfmm_trajectory_forward(nbfmm_solver* solver, int num_particle, float tick,
 /* d_obs_grad: n_obs x N float2 = dloss/d(observed positions).  Writes the
  * gradients wrt the initial positions, initial x_pre and the weights.  The
  * step adjoint at a recomputed step uses the sort and tree that step built
- * (the same bits as sorting again): the solver keeps checkpoint_every - 1
- * such states, allocated by the first call that needs them (a synchronous
+ * (the same bits as sorting again).  The recompute of each segment runs on a
+ * stream of the library's own (ordered after the work queued on `stream`
+ * before the call) while `stream` runs the adjoint steps of the segment
+ * after it; all of the call's work is done once `stream` reaches the end of
+ * what the call queued.  The solver keeps two sets of checkpoint_every such
+ * states, allocated by the first call that needs them (a synchronous
  * allocation) and freed with the solver, each 20 B per particle of its
- * capacity plus its leaf grid and tree. */
+ * capacity plus its leaf grid and tree.  Concurrent calls on one solver
+ * race. */
 int nbfmm_trajectory_backward(nbfmm_solver* solver, int num_particle, float tick, int num_steps,
                               int checkpoint_every, const float* d_weight, const float* d_checkpoints, int n_obs,
                               const int* obs_steps, const float* d_obs_grad, float* d_grad_position0,

@@ -78,6 +78,31 @@ struct SideStream {
   }
 };
 
+// The trajectory backward's second stream: it recomputes the segment before
+// while the caller's stream runs the adjoint steps of the current one.
+// `start` orders it after the work queued on the caller's stream before the
+// call; `done[p]` / `kept[p]` mark the end of the adjoint steps / of the
+// recompute that last used (filled) kept-state set p.  Made on first use.
+struct RecomputeStream {
+  hipStream_t s = nullptr;
+  hipEvent_t start = nullptr, done[2] = {nullptr, nullptr}, kept[2] = {nullptr, nullptr};
+  hipError_t make() {
+    hipError_t e;
+    if (!s && (e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking)) != hipSuccess) return e;
+    for (hipEvent_t* ev : {&start, &done[0], &done[1], &kept[0], &kept[1]})
+      if (!*ev && (e = hipEventCreateWithFlags(ev, hipEventDisableTiming)) != hipSuccess) return e;
+    return hipSuccess;
+  }
+  void release() {
+    if (s) hipStreamDestroy(s);
+    for (hipEvent_t* ev : {&start, &done[0], &done[1], &kept[0], &kept[1]}) {
+      if (*ev) hipEventDestroy(*ev);
+      *ev = nullptr;
+    }
+    s = nullptr;
+  }
+};
+
 // Stages of the fused pipeline timed in profile mode: sort, upward pass
 // (P2M + M2M), M2L, P2P (with L2L, L2P, un-permute and the Verlet epilogue).
 constexpr int kProfStages = 4;
@@ -159,6 +184,7 @@ struct nbfmm_solver {
   hipGraphExec_t graph_exec[3] = {nullptr, nullptr, nullptr};
   hipGraph_t graph[3] = {nullptr, nullptr, nullptr};
   SideStream side;  // replays asked for on the null stream run here
+  RecomputeStream recompute;  // the trajectory backward's recompute
   struct {
     int n = -1;
     float tick = 0;
@@ -190,6 +216,7 @@ void free_all(nbfmm_solver* s) {
   }
   for (auto& e : s->ev) hipEventDestroy(e);
   s->side.release();
+  s->recompute.release();
   for (void* p : s->allocs) hipFree(p);
   s->allocs.clear();
 }
@@ -847,48 +874,72 @@ int nbfmm_trajectory_backward(nbfmm_solver* s, int n, float tick, int num_steps,
     return NBFMM_OK;
   };
   if ((rc = add_obs(num_steps))) return rc;
-  if ((rc = ensure_slots(s, K - 1))) return rc;
+  if (n == 0 || num_steps == 0) {
+    for (int k = num_steps - 1; k >= 0; --k)
+      if ((rc = add_obs(k))) return rc;
+    return NBFMM_OK;
+  }
+  // Two sets of K kept states: segment sg fills set sg % 2 on the recompute
+  // stream R (its steps k0 .. k1-2 again, as nbfmm_run's eager steps, each
+  // kept once its sort and tree are built, then the sort and tree of
+  // x_{k1-1}, as nbfmm_step_adjoint would build them) while the caller's
+  // stream runs the adjoint steps of segment sg + 1 from the other set.  The
+  // adjoint steps read only a kept state and their own scratch (the sorted
+  // gradient, the subtree sums, the cell gradients), which no forward kernel
+  // touches, and R alone uses the solver's sort and tree.
+  if ((rc = ensure_slots(s, 2 * K))) return rc;
+  RecomputeStream& rs = s->recompute;
+  NB_HIP(rs.make());
+  hipStream_t R = rs.s;
+  NB_HIP(hipEventRecord(rs.start, st));
+  NB_HIP(hipEventRecord(rs.done[0], st));
+  NB_HIP(hipEventRecord(rs.done[1], st));
+  NB_HIP(hipStreamWaitEvent(R, rs.start, 0));
+  const bool chain = s->warm_enabled && !use_front(s, n);
   const int nseg = (num_steps + K - 1) / K;
   for (int sg = nseg - 1; sg >= 0; --sg) {
+    const int p = sg & 1;
+    const nbfmm_solver::SortSlot* set = s->slots.data() + (size_t)p * K;
     const int k0 = sg * K;
     const int k1 = std::min(k0 + K, num_steps);
+    // R: the set is free once the adjoint steps of segment sg + 2 are done
+    NB_HIP(hipStreamWaitEvent(R, rs.done[p], 0));
     const float* c = ckpt + (size_t)sg * 2 * n * 2;
-    NB_HIP(hipMemcpyAsync(xc, c, sb, hipMemcpyDeviceToDevice, st));
-    NB_HIP(hipMemcpyAsync(xp, c + (size_t)n * 2, sb, hipMemcpyDeviceToDevice, st));
-    // steps k0 .. k1-2 again, as nbfmm_run's eager steps (with the warm start
-    // on, each but the last makes the next sort's first pass: the same bits
-    // as separate steps); step k sorts x_k and builds its tree, kept for the
-    // adjoint at x_k.  xc ends at x_{k1-1}.
+    NB_HIP(hipMemcpyAsync(xc, c, sb, hipMemcpyDeviceToDevice, R));
+    NB_HIP(hipMemcpyAsync(xp, c + (size_t)n * 2, sb, hipMemcpyDeviceToDevice, R));
+    // with the warm start on, every recomputed step makes the next sort's
+    // first pass (the last one's is x_{k1-1}'s): the same bits as separate steps
     CallIO rio = call_io(xc, w, ac, xc, xp);
-    const bool chain = s->warm_enabled && n > 0 && !use_front(s, n);
     for (int k = k0; k + 1 < k1; ++k) {
-      const int mode = chain && k + 2 < k1 ? kP2PStepSort : kP2PStep;
-      if ((rc = run_pipeline(s, n, tick, mode, rio, st, chain && k > k0))) return rc;
-      if (n > 0 && (rc = save_sort(s, n, s->slots[k - k0], st))) return rc;
+      if ((rc = run_pipeline(s, n, tick, chain ? kP2PStepSort : kP2PStep, rio, R, chain && k > k0))) return rc;
+      if ((rc = save_sort(s, n, set[k - k0], R))) return rc;
     }
+    const SysView v = view(s, n);
+    if ((rc = sort_stage(s, v, call_io(xc, w), R, chain && k1 - 1 > k0))) return rc;
+    NB_HIP(launch_upward(v, R, true));
+    if ((rc = save_sort(s, n, set[k1 - 1 - k0], R))) return rc;
+    NB_HIP(hipEventRecord(rs.kept[p], R));
+    // the caller's stream: steps k1-1 .. k0 back, each on its kept state
+    NB_HIP(hipStreamWaitEvent(st, rs.kept[p], 0));
     for (int k = k1 - 1; k >= k0; --k) {
-      if (k + 1 < k1 && n > 0) {
-        // the same sort and tree as nbfmm_step_adjoint would build at x_k
-        const nbfmm_solver::SortSlot& sl = s->slots[k - k0];
-        SysView v = view(s, n);
-        v.part = sl.part;
-        v.key = sl.key;
-        v.head = sl.head;
-        v.occ = sl.occ;
-        v.nocc = sl.nocc;
-        v.mpole = sl.mpole;
-        CallIO io = call_io(xc, w);  // (the adjoint reads the kept sort's positions)
-        io.gin = (const float2*)lam;
-        io.gpos = (float2*)lam;
-        io.gpre = (float2*)pi;
-        io.gw = grad_w;
-        io.gscale = tick * tick;
-        NB_HIP(launch_adjoint(v, io, 1, s->adj_grid, st));
-      } else if ((rc = nbfmm_step_adjoint(s, n, tick, xc, w, lam, pi, grad_w, stream))) {
-        return rc;
-      }
+      const nbfmm_solver::SortSlot& sl = set[k - k0];
+      SysView va = v;
+      va.part = sl.part;
+      va.key = sl.key;
+      va.head = sl.head;
+      va.occ = sl.occ;
+      va.nocc = sl.nocc;
+      va.mpole = sl.mpole;
+      CallIO io = call_io(nullptr, nullptr);  // (the adjoint reads the kept state's particles)
+      io.gin = (const float2*)lam;
+      io.gpos = (float2*)lam;
+      io.gpre = (float2*)pi;
+      io.gw = grad_w;
+      io.gscale = tick * tick;
+      NB_HIP(launch_adjoint(va, io, 1, s->adj_grid, st));
       if ((rc = add_obs(k))) return rc;
     }
+    NB_HIP(hipEventRecord(rs.done[p], st));
   }
   return NBFMM_OK;
 }
