@@ -404,13 +404,16 @@ int nbfmm_batch_step_adjoint(nbfmm_batch* batch, const float* d_position, const 
 int nbfmm_batch_keep_sort(nbfmm_batch* batch, int slot, void* stream);
 int nbfmm_batch_step_adjoint_kept(nbfmm_batch* batch, int slot, float* d_lambda, float* d_pi, float* d_grad_weight,
                                   void* stream);
-/* num_steps (0..4097) steps as nbfmm_batch_run's eager steps, keeping the
- * sort and tree of step k (of the positions it starts from) in slot k: the
- * same bits and kept states as nbfmm_batch_step followed by
- * nbfmm_batch_keep_sort(k), step by step, with the sorts chained as in a
- * run.  No reference counterpart. */
-int nbfmm_batch_run_keep(nbfmm_batch* batch, int num_steps, float* d_position_cur, float* d_position_pre,
-                         const float* d_weight, float* d_acceleration, void* stream);
+/* num_steps steps as nbfmm_batch_run's eager steps, keeping the sort and
+ * tree of step k (of the positions it starts from) in slot first_slot + k,
+ * then the sort and tree of the final positions in slot first_slot +
+ * num_steps (as nbfmm_batch_step_adjoint would build them there): the same
+ * bits and kept states as nbfmm_batch_step followed by
+ * nbfmm_batch_keep_sort, step by step, with the sorts chained as in a run
+ * (first_slot >= 0, first_slot + num_steps <= 4096).  d_acceleration is
+ * scratch.  No reference counterpart. */
+int nbfmm_batch_run_keep(nbfmm_batch* batch, int num_steps, int first_slot, float* d_position_cur,
+                         float* d_position_pre, const float* d_weight, float* d_acceleration, void* stream);
 
 #ifdef __cplusplus
 }

@@ -81,7 +81,7 @@ SIGNATURES = {
     "nbfmm_batch_step_adjoint": [P, P, P, P, P, P, P],
     "nbfmm_batch_keep_sort": [P, c_int, P],
     "nbfmm_batch_step_adjoint_kept": [P, c_int, P, P, P, P],
-    "nbfmm_batch_run_keep": [P, c_int, P, P, P, P, P],
+    "nbfmm_batch_run_keep": [P, c_int, c_int, P, P, P, P, P],
     "nbfmm_solve_adjoint": [P, c_int, P, P, P, P, P, P],
     "nbfmm_step_adjoint": [P, c_int, c_float, P, P, P, P, P, P],
     "nbfmm_trajectory_workspace": [c_int, c_int, c_int, ctypes.POINTER(c_size_t), ctypes.POINTER(c_size_t)],

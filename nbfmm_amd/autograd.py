@@ -120,7 +120,6 @@ class _BatchTrajectory(torch.autograd.Function):
         lam = torch.zeros((m.shape[0], 2), dtype=torch.float32, device=m.device)
         pi = torch.zeros_like(lam)
         gm = torch.zeros_like(m)
-        acc = torch.empty_like(lam)
 
         def add_obs(k):
             for o, ks in enumerate(ctx.obs_steps):
@@ -128,19 +127,34 @@ class _BatchTrajectory(torch.autograd.Function):
                     lam.add_(g[o])
 
         add_obs(steps)
+        # Segment sg is recomputed on a second stream into kept-state set
+        # sg % 2 (slots (sg % 2) * K ..): steps k0 .. k1-2 again, each kept for
+        # its adjoint, and the sort of x_{k1-1}; meanwhile this stream steps
+        # back through segment sg + 1 from the other set.  The adjoint steps
+        # read only kept states and their own scratch, the recompute only the
+        # batch's own sort.
+        main = torch.cuda.current_stream()
+        rec = torch.cuda.Stream(device=m.device)
+        rec.wait_stream(main)
+        done = [torch.cuda.Event(), torch.cuda.Event()]
+        kept = [torch.cuda.Event(), torch.cuda.Event()]
+        for e in done:
+            e.record(main)
         nseg = (steps + K - 1) // K
         for sg in range(nseg - 1, -1, -1):
             k0, k1 = sg * K, min(sg * K + K, steps)
-            x, p = (t.clone() for t in ctx.ckpts[sg])
-            # steps k0 .. k1-2 again, keeping the sort and tree of each x_k for
-            # its adjoint; x ends at x_{k1-1}
-            b.run_keep(k1 - 1 - k0, x, p, m, acc)
+            first = (sg % 2) * K
+            with torch.cuda.stream(rec):
+                rec.wait_event(done[sg % 2])  # the set's previous segment is stepped back
+                x, p = (t.clone() for t in ctx.ckpts[sg])
+                acc = torch.empty_like(x)
+                b.run_keep(k1 - 1 - k0, first, x, p, m, acc, stream=rec)
+                kept[sg % 2].record(rec)
+            main.wait_event(kept[sg % 2])
             for k in range(k1 - 1, k0 - 1, -1):
-                if k + 1 < k1:
-                    b.step_adjoint_kept(k - k0, lam, pi, gm)
-                else:
-                    b.step_adjoint(x, m, lam, pi, gm)
+                b.step_adjoint_kept(first + k - k0, lam, pi, gm, stream=main)
                 add_obs(k)
+            done[sg % 2].record(main)
         return None, None, None, None, lam, pi, gm
 
 

@@ -107,12 +107,13 @@ class Batch:
         """Keep the current sort and tree (of the last step's positions) in `slot`."""
         check(lib().nbfmm_batch_keep_sort(self._h, int(slot), _stream(stream)))
 
-    def run_keep(self, num_steps, position_cur, position_pre, weight, acceleration, stream=None):
+    def run_keep(self, num_steps, first_slot, position_cur, position_pre, weight, acceleration, stream=None):
         """num_steps eager steps, keeping the sort of step k (of the positions it
-        starts from) in slot k: step + keep_sort(k), step by step, with the sorts
-        chained as in a run (the same bits)."""
-        check(lib().nbfmm_batch_run_keep(self._h, int(num_steps), _ptr(position_cur), _ptr(position_pre),
-                                         _ptr(weight), _ptr(acceleration), _stream(stream)))
+        starts from) in slot first_slot + k and that of the final positions in
+        slot first_slot + num_steps: step + keep_sort, step by step, with the
+        sorts chained as in a run (the same bits); acceleration is scratch."""
+        check(lib().nbfmm_batch_run_keep(self._h, int(num_steps), int(first_slot), _ptr(position_cur),
+                                         _ptr(position_pre), _ptr(weight), _ptr(acceleration), _stream(stream)))
 
     def step_adjoint_kept(self, slot, lam, pi, grad_weight, stream=None):
         """step_adjoint at the positions whose sort `slot` kept, without sorting again."""

@@ -1677,22 +1677,26 @@ int nbfmm_batch_keep_sort(nbfmm_batch* b, int slot, void* stream) {
   return batch_keep(b, slot, as_stream(stream));
 }
 
-int nbfmm_batch_run_keep(nbfmm_batch* b, int steps, float* xcur, float* xpre, const float* w, float* acc,
-                         void* stream) {
+int nbfmm_batch_run_keep(nbfmm_batch* b, int steps, int first_slot, float* xcur, float* xpre, const float* w,
+                         float* acc, void* stream) {
   if (!b || !xcur || !xpre || !w || !acc) return fail(NBFMM_ERR_INVALID_ARGUMENT, "null argument");
-  if (steps < 0 || steps > kMaxKeptSorts + 1) return fail(NBFMM_ERR_INVALID_ARGUMENT, "num_steps in [0, 4097]");
+  if (steps < 0 || first_slot < 0 || first_slot + steps > kMaxKeptSorts)
+    return fail(NBFMM_ERR_INVALID_ARGUMENT, "kept slots first_slot .. first_slot + num_steps in [0, 4096]");
   hipStream_t st = as_stream(stream);
   // nbfmm_batch_run's eager steps, each kept once its sort and tree are built
-  // (the P2P of a step followed by another makes only the next sort's first
-  // pass, which leaves the kept arrays alone)
+  // (the P2P of a step makes only the next sort's first pass, which leaves
+  // the kept arrays alone; the last step's is the final positions')
   CallIO io = call_io(xcur, w, acc, xcur, xpre);
   const bool chain = b->warm_enabled;
   for (int k = 0; k < steps; ++k) {
-    const int mode = chain && k + 1 < steps ? kP2PStepSort : kP2PStep;
-    if (int rc = batch_pipeline(b, mode, io, st, chain && k > 0)) return rc;
-    if (int rc = batch_keep(b, k, st)) return rc;
+    if (int rc = batch_pipeline(b, chain ? kP2PStepSort : kP2PStep, io, st, chain && k > 0)) return rc;
+    if (int rc = batch_keep(b, first_slot + k, st)) return rc;
   }
-  return NBFMM_OK;
+  // the final positions: their sort and tree, as nbfmm_batch_step_adjoint builds them
+  if (int rc = batch_sort(b, io, st, chain && steps > 0)) return rc;
+  if (int rc = batch_balance_once(b, st)) return rc;
+  NB_HIP(launch_tree_batch(b->bl, st, false));
+  return batch_keep(b, first_slot + steps, st);
 }
 
 int nbfmm_batch_step_adjoint_kept(nbfmm_batch* b, int slot, float* lam, float* pi, float* grad_w, void* stream) {
