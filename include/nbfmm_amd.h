@@ -333,10 +333,10 @@ int nbfmm_trajectory_forward(nbfmm_solver* solver, int num_particle, float tick,
  * before the call) while `stream` runs the adjoint steps of the segment
  * after it; all of the call's work is done once `stream` reaches the end of
  * what the call queued.  The solver keeps two sets of checkpoint_every such
- * states, allocated by the first call that needs them (a synchronous
- * allocation) and freed with the solver, each 20 B per particle of its
- * capacity plus its leaf grid and tree.  Concurrent calls on one solver
- * race. */
+ * states (one when num_steps <= checkpoint_every), allocated by the first
+ * call that needs them (a synchronous allocation) and freed with the solver,
+ * each 20 B per particle of its capacity plus its leaf grid and tree.
+ * Concurrent calls on one solver race. */
 int nbfmm_trajectory_backward(nbfmm_solver* solver, int num_particle, float tick, int num_steps,
                               int checkpoint_every, const float* d_weight, const float* d_checkpoints, int n_obs,
                               const int* obs_steps, const float* d_obs_grad, float* d_grad_position0,

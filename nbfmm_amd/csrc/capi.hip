@@ -887,7 +887,9 @@ int nbfmm_trajectory_backward(nbfmm_solver* s, int n, float tick, int num_steps,
   // adjoint steps read only a kept state and their own scratch (the sorted
   // gradient, the subtree sums, the cell gradients), which no forward kernel
   // touches, and R alone uses the solver's sort and tree.
-  if ((rc = ensure_slots(s, 2 * K))) return rc;
+  const int nseg = (num_steps + K - 1) / K;
+  const int sets = nseg > 1 ? 2 : 1;  // one segment: nothing to overlap
+  if ((rc = ensure_slots(s, sets * K))) return rc;
   RecomputeStream& rs = s->recompute;
   NB_HIP(rs.make());
   hipStream_t R = rs.s;
@@ -896,9 +898,8 @@ int nbfmm_trajectory_backward(nbfmm_solver* s, int n, float tick, int num_steps,
   NB_HIP(hipEventRecord(rs.done[1], st));
   NB_HIP(hipStreamWaitEvent(R, rs.start, 0));
   const bool chain = s->warm_enabled && !use_front(s, n);
-  const int nseg = (num_steps + K - 1) / K;
   for (int sg = nseg - 1; sg >= 0; --sg) {
-    const int p = sg & 1;
+    const int p = sg % sets;
     const nbfmm_solver::SortSlot* set = s->slots.data() + (size_t)p * K;
     const int k0 = sg * K;
     const int k1 = std::min(k0 + K, num_steps);
