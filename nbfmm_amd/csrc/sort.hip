@@ -763,7 +763,12 @@ __global__ __launch_bounds__(NT) void k_warm_finish(Sys sys, CallIO io, int zero
     if (y == B - 1 && x == 0) {
       v.rowstart[B] = (int)tot;
       v.head[B * B] = (int)tot;
-      *v.nocc = (int)otot;
+      // with the cold switch on, the column pass (next kernel) counts the
+      // occupied leaves from zero: row 0's block clears the count in this
+      // kernel too (row_scan_row), so both blocks must store 0, or the
+      // column pass's list bases start past the count and its writes run
+      // off the end of occ
+      *v.nocc = cold ? 0 : (int)otot;
       v.wstate[kWarmTouched] = (int)ttot;
     }
   }

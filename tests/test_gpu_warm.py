@@ -34,6 +34,12 @@ def sorted_state(s, n):
 def assert_same_state(a, b, n):
     for x, y in zip(sorted_state(a, n), sorted_state(b, n)):
         assert torch.equal(x, y)
+    # the occupied-leaf count each sort keeps (reported by pair_count): a
+    # warm sort that switched to the cold passes once kept a stale count,
+    # and its occupied-leaf list then ran off the end of its array
+    for s in (a, b):
+        head = s.read_buffer("head").cpu().numpy()
+        assert s.pair_count(n)[1] == int((np.diff(head) > 0).sum())
 
 
 def test_steps_bitwise_equal_cold():
