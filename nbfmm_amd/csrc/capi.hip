@@ -442,6 +442,7 @@ int nbfmm_solver_create(int num_level, int max_num_particle, const float positio
 #undef ALLOC
   // P2P: enough waves to fill every CU
   s->cus = cus;
+  default_p2p_config(s->max_n, &s->p2p_variant, &s->p2p_bpc);
   s->p2p_grid = p2p_grid_for(s, s->p2p_variant, s->p2p_bpc);
   s->adj_grid = adj_grid_for(s);
   *out = s;

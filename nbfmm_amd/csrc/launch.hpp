@@ -74,12 +74,23 @@ hipError_t launch_p2p_batch(const BatchLaunch& b, int mode, const CallIO& io, in
 int p2p_num_variants();
 // default P2P configurations (measured): a solver runs variant {384 staged,
 // unroll 4, 8 waves} on 3 blocks per CU (one round: LDS holds three such
-// blocks per CU, so every block starts at once); a batch spreads variant
-// {320, 4, 4 waves} on 48 blocks per CU over its systems
+// blocks per CU, so every block starts at once), and one of at most
+// kP2PSmallN particles (whose runs are split among the waves of a block)
+// variant {384, 4, 4 waves} on 4 blocks per CU: 10-step runs of the
+// 16384-particle disk 52.9 -> 51.0 us a step, of the 65536-particle Gaussian
+// disk 76.5 -> 69.3 us (2^20 two-disk step: 213 against 230 us).  A batch
+// spreads variant {320, 4, 4 waves} on 48 blocks per CU over its systems.
 constexpr int kDefaultP2PVariant = 5;
 constexpr int kDefaultP2PBlocksPerCU = 3;
+constexpr int kDefaultSmallP2PVariant = 0;
+constexpr int kDefaultSmallP2PBlocksPerCU = 4;
 constexpr int kDefaultBatchP2PVariant = 6;
 constexpr int kDefaultBatchP2PBlocksPerCU = 48;
+inline void default_p2p_config(int max_n, int* variant, int* blocks_per_cu) {
+  const bool small = max_n <= kP2PSmallN;
+  *variant = small ? kDefaultSmallP2PVariant : kDefaultP2PVariant;
+  *blocks_per_cu = small ? kDefaultSmallP2PBlocksPerCU : kDefaultP2PBlocksPerCU;
+}
 void p2p_variant_info(int variant, int* stage, int* unroll, int* waves);
 hipError_t launch_l2p(const SysView& v, hipStream_t s);
 

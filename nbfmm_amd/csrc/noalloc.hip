@@ -146,13 +146,14 @@ extern "C" int nbfmm_solve_noalloc(int num_level, int num_particle, const float 
   if (e == hipSuccess) e = launch_upward(v, st, false);
   if (e == hipSuccess) e = launch_m2l_fused(v, st);
   if (e == hipSuccess) {
-    // the solver's P2P grid (p2p_grid_for) for this n
-    int stage = 0, unroll = 0, waves = 0;
-    p2p_variant_info(kDefaultP2PVariant, &stage, &unroll, &waves);
+    // the default P2P configuration and grid of a solver of this n (p2p_grid_for)
+    int variant = 0, bpc = 0, stage = 0, unroll = 0, waves = 0;
+    default_p2p_config(num_particle, &variant, &bpc);
+    p2p_variant_info(variant, &stage, &unroll, &waves);
     const int C = v.B * v.B;
     const int want = std::max(cdiv(std::min(C, num_particle), waves), cdiv(num_particle, kWave) + v.B);
-    const int grid = std::max(1, std::min(want, d.cus * kDefaultP2PBlocksPerCU));
-    e = launch_p2p(v, kP2PFinal, io, grid, kDefaultP2PVariant, st);
+    const int grid = std::max(1, std::min(want, d.cus * bpc));
+    e = launch_p2p(v, kP2PFinal, io, grid, variant, st);
   }
   if (e != hipSuccess) return report_error(NBFMM_ERR_HIP, std::string("nbfmm_solve_noalloc: ") + hipGetErrorString(e));
   return NBFMM_OK;
