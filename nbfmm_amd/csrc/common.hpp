@@ -20,9 +20,12 @@ constexpr int kScanChunk = kScanThreads * kScanItems;     // 2048 cells
 constexpr int kP2PStage = 256;        // adjoint P2P: sources staged in LDS per wave and pass
 constexpr int kAdjP2PWaves = 8;       // adjoint P2P: waves per block (48 KiB of LDS)
 constexpr int kAdjP2PBlocksPerCU = 3; // adjoint P2P: resident blocks per CU (LDS bound)
-// Batched adjoint P2P: several rounds of residency, so that blocks ending
-// early are replaced while others still work through heavy runs.
-constexpr int kAdjP2PBatchBlocksPerCU = 16;
+// Batched adjoint P2P: a few rounds of residency, so that blocks ending
+// early are replaced while others still work through heavy runs.  Measured
+// on the 64-collision trajectory backward, whose adjoint steps share the GPU
+// with the recompute of the segment before: 4, 6, 8, 12, 16 and 32 blocks
+// per CU gave 538, 516, 497-507, 523, 519 and 567 ms.
+constexpr int kAdjP2PBatchBlocksPerCU = 8;
 
 // P2P output modes.
 enum P2PMode : int {
