@@ -199,6 +199,11 @@ int main(int argc, char const* argv[]) {
 
   nbfmm_solver* solver = nullptr;
   if (!ok(nbfmm_solver_create(fmm_level, n_alloc, lim, eps, &solver), "nbfmm_solver_create")) return 1;
+  // P2P configuration timed on the initial stars (every configuration gives
+  // the same bits): the fixed default for small systems splits each leaf's
+  // run over 4 waves, which loses to 8-wave blocks when a few leaves hold
+  // thousands of stars, as a centred disk's do.
+  if (n > 0 && !ok(nbfmm_autotune(solver, n, x, w, 3, nullptr, nullptr, nullptr, st), "nbfmm_autotune")) return 1;
   nbfmm_y4m* video = nullptr;
   if (!ok(nbfmm_y4m_open(result_y4m, width, height, fps, &video), "nbfmm_y4m_open")) return 1;
 
