@@ -795,6 +795,80 @@ __global__ __launch_bounds__(NT) void k_warm_finish(Sys sys, CallIO io, int zero
   }
 }
 
+// Small systems (at most kWarmOneThreads leaves and kWarmOneMaxN particles):
+// k_warm_rows and k_warm_finish as one block, a thread per leaf in
+// row-major order, so that the scans over all leaves need no second launch.
+// It writes what the two kernels leave for the passes after them: the same
+// head, rowstart, movers' group offsets, occupied and touched leaf lists,
+// counts and cleared counters (the row-local offsets and row totals that only
+// pass from one of them to the other are not kept).  With the cold switch on
+// it also makes the cold row histograms and row scans, every tile and row in
+// turn (at most kWarmOneMaxN / kSortTile tiles).  The 16384-particle disk's
+// step: 49.5 -> 47.3 us.  (With several consecutive leaves per thread, on
+// 128 x 128 leaves, one block's scattered loads took longer than the two
+// kernels: the Gaussian disk's 50 steps 3.67 -> 6.27 ms.)
+constexpr int kWarmOneThreads = 1024;
+constexpr int kWarmOneMaxN = 65536;
+
+__global__ __launch_bounds__(kWarmOneThreads) void k_warm_rows_finish_one(SysView v, CallIO io, int zero_empty) {
+  __shared__ uint32_t sh[kWarmOneThreads / kWave];
+  __shared__ uint32_t s_hist[512];
+  __shared__ uint32_t s_carry;
+  const int B = v.B, C = B * B;
+  const int c = (int)threadIdx.x;
+  const bool in = c < C;
+  const int cold = v.wstate[kWarmColdReq];
+  // k_warm_rows: new count; a leaf that movers enter or leave is touched and
+  // keeps its old slot range
+  int cnt = 0, cin = 0;
+  bool touched = false;
+  if (in) {
+    const int ci = v.wcin[c], co = v.wcout[c];
+    const int h0 = v.head[c], h1 = v.head[c + 1];
+    touched = (ci | co) != 0;
+    if (touched) {
+      v.wcin[c] = 0;
+      v.wcout[c] = 0;
+      v.wtouch[c] = 1;
+      v.wtold[c] = make_int2(h0, h1 - h0);
+    }
+    cnt = h1 - h0 - co + ci;
+    cin = ci;
+  }
+  // k_warm_finish: exclusive scans over all leaves (their barriers also
+  // order every read of the old head above before the writes below)
+  uint32_t tot, itot, otot, ttot;
+  const uint32_t e = block_exclusive_scan<kWarmOneThreads>((uint32_t)cnt, sh, &tot);
+  const uint32_t ei = block_exclusive_scan<kWarmOneThreads>((uint32_t)cin, sh, &itot);
+  const uint32_t eo = block_exclusive_scan<kWarmOneThreads>(cnt > 0 ? 1u : 0u, sh, &otot);
+  const uint32_t et = block_exclusive_scan<kWarmOneThreads>(touched ? 1u : 0u, sh, &ttot);
+  if (in) {
+    v.head[c] = (int)e;
+    v.winoff[c] = (int)ei;
+    if ((c & (B - 1)) == 0) v.rowstart[c >> v.lgB] = (int)e;
+    if (cnt > 0)
+      v.occ[eo] = c;
+    else if (zero_empty)
+      v.mpole[c] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (touched) v.wtlist[et] = c;
+  }
+  if (c == 0) {
+    v.rowstart[B] = (int)tot;
+    v.head[C] = (int)tot;
+    // with the cold switch on, the column pass counts the occupied leaves
+    // from zero (row_scan_row below clears the count as well)
+    *v.nocc = cold ? 0 : (int)otot;
+    v.wstate[kWarmTouched] = (int)ttot;
+    v.wstate[kWarmCold] = cold;
+    v.wstate[kWarmColdReq] = 0;  // read by every thread before the scans
+  }
+  if (cold) {
+    const int ntiles = (v.n + kSortTile - 1) / kSortTile;
+    for (int t = 0; t < ntiles; ++t) row_up_tile<kWarmOneThreads>(v, io, t, ntiles, s_hist);
+    for (int d = 0; d < B; ++d) row_scan_row<kWarmOneThreads>(v, d, sh, &s_carry);
+  }
+}
+
 // Stayers of untouched leaves go to head[leaf] + their old rank; movers are
 // put into their new leaf's group (slots of the group in any order).
 template <class Sys>
@@ -1076,8 +1150,12 @@ hipError_t launch_predo_warm(const SysView& v, const CallIO& io, int zero_empty,
   const OneSys one{v};
   const int nb = cdiv(v.n, 256);
   if (!gathered) hipLaunchKernelGGL(k_warm_gather<OneSys>, dim3(nb), dim3(256), 0, s, one, io);
-  launch_warm_rows(row_threads(v.B), dim3(v.B), one, io, s);
-  launch_warm_finish(row_threads(v.B), dim3(v.B), one, io, zero_empty, 0, s);
+  if (v.B * v.B <= kWarmOneThreads && v.n <= kWarmOneMaxN) {
+    hipLaunchKernelGGL(k_warm_rows_finish_one, dim3(1), dim3(kWarmOneThreads), 0, s, v, io, zero_empty);
+  } else {
+    launch_warm_rows(row_threads(v.B), dim3(v.B), one, io, s);
+    launch_warm_finish(row_threads(v.B), dim3(v.B), one, io, zero_empty, 0, s);
+  }
   hipLaunchKernelGGL(k_warm_place<OneSys>, dim3(nb), dim3(256), 0, s, one, io);
   hipLaunchKernelGGL(k_warm_merge<OneSys>, dim3(cdiv(std::min(kWarmMergeWaves, v.B * v.B), 256 / kWave)),
                      dim3(256), 0, s, one);

@@ -80,16 +80,22 @@ def test_graph_run_bitwise_equal_cold():
     cold.close()
 
 
-@pytest.mark.parametrize("jump", ["drift", "shuffle", "cluster", "crowd", "weights", "dense_few", "dense_many"])
+@pytest.mark.parametrize("jump", ["drift", "shuffle", "cluster", "crowd", "crowd_fine", "weights", "dense_few",
+                                  "dense_many"])
 def test_solve_sequences(jump):
     """solve at x1, then at x2: small drift (warm placement), unrelated
     positions (most particles move), 8 groups of 500 particles into 8 leaves
     (leaves with 65..1024 movers: sorted in LDS), half the particles into one
     leaf (thousands of movers), new masses only (no movers), and a leaf of
     3000 particles (its stayers walked in 47 chunks of 64, several in flight)
-    that gains 30 movers (ranked in registers) or 200 (sorted in LDS)."""
+    that gains 30 movers (ranked in registers) or 200 (sorted in LDS).  The
+    32 x 32 leaves take the one-block row passes; crowd_fine repeats the crowd
+    on 256 x 256 leaves, where the row passes are a block per row."""
+    import dataclasses
     from nbfmm_amd.workloads import disk
     wl = disk(n=20000)
+    if jump == "crowd_fine":
+        wl = dataclasses.replace(wl, num_level=7)
     warm, cold = pair(wl)
     rng = np.random.default_rng(9)
     x1, m1 = wl.position, wl.weight
@@ -112,7 +118,7 @@ def test_solve_sequences(jump):
         for k in range(8):
             x2[500 * k: 500 * (k + 1)] = x1[1000 + 1234 * k] + 1e-6 * rng.random((500, 2)).astype(np.float32)
         m2 = m1
-    elif jump == "crowd":
+    elif jump.startswith("crowd"):
         x2 = x1.copy()
         x2[: wl.n // 2] = x1[0] + 1e-6 * rng.random((wl.n // 2, 2)).astype(np.float32)
         m2 = m1
