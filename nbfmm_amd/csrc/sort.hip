@@ -802,8 +802,8 @@ __global__ __launch_bounds__(NT) void k_warm_finish(Sys sys, CallIO io, int zero
 // head, rowstart, movers' group offsets, occupied and touched leaf lists,
 // counts and cleared counters (the row-local offsets and row totals that only
 // pass from one of them to the other are not kept).  With the cold switch on
-// it also makes the cold row histograms and row scans, every tile and row in
-// turn (at most kWarmOneMaxN / kSortTile tiles).  The 16384-particle disk's
+// it also makes the cold row histograms and row scans (at most
+// kWarmOneMaxN / kSortTile = 32 tiles of 32 rows).  The 16384-particle disk's
 // step: 49.5 -> 47.3 us.  (With several consecutive leaves per thread, on
 // 128 x 128 leaves, one block's scattered loads took longer than the two
 // kernels: the Gaussian disk's 50 steps 3.67 -> 6.27 ms.)
@@ -812,8 +812,7 @@ constexpr int kWarmOneMaxN = 65536;
 
 __global__ __launch_bounds__(kWarmOneThreads) void k_warm_rows_finish_one(SysView v, CallIO io, int zero_empty) {
   __shared__ uint32_t sh[kWarmOneThreads / kWave];
-  __shared__ uint32_t s_hist[512];
-  __shared__ uint32_t s_carry;
+  __shared__ uint32_t s_hist[kWarmOneThreads];  // cold switch: [tile][row] counts
   const int B = v.B, C = B * B;
   const int c = (int)threadIdx.x;
   const bool in = c < C;
@@ -863,9 +862,30 @@ __global__ __launch_bounds__(kWarmOneThreads) void k_warm_rows_finish_one(SysVie
     v.wstate[kWarmColdReq] = 0;  // read by every thread before the scans
   }
   if (cold) {
+    // the cold row pass's histograms (k_row_up) of every tile at once, at
+    // most 32 tiles x 32 rows of counts in LDS, then its scans (k_row_scan):
+    // one wave per row takes the row's tile counts to exclusive offsets
+    // (hist) and its total (dtotal); the occupied count is already 0
     const int ntiles = (v.n + kSortTile - 1) / kSortTile;
-    for (int t = 0; t < ntiles; ++t) row_up_tile<kWarmOneThreads>(v, io, t, ntiles, s_hist);
-    for (int d = 0; d < B; ++d) row_scan_row<kWarmOneThreads>(v, d, sh, &s_carry);
+    for (int k = c; k < ntiles * B; k += kWarmOneThreads) s_hist[k] = 0;
+    __syncthreads();
+    const float2* pos0 = io.pos0 + v.part_off;
+    bool any_bad = false;
+    for (int i = c; i < v.n; i += kWarmOneThreads) {
+      bool bad;
+      const uint32_t key = bin_key(v, pos0[i], bad);
+      any_bad |= bad;
+      atomicAdd(&s_hist[(i / kSortTile) * B + (int)(key >> v.lgB)], 1u);
+    }
+    if (any_bad) atomicOr(v.flags, 1u);
+    __syncthreads();
+    const int lane = lane_id();
+    for (int d = c / kWave; d < B; d += kWarmOneThreads / kWave) {
+      const uint32_t k = lane < ntiles ? s_hist[lane * B + d] : 0u;
+      const uint32_t inc = wave_inclusive_scan(k);
+      if (lane < ntiles) v.hist[(size_t)d * ntiles + lane] = inc - k;
+      if (lane == kWave - 1) v.dtotal[d] = inc;
+    }
   }
 }
 
