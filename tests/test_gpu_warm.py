@@ -80,8 +80,8 @@ def test_graph_run_bitwise_equal_cold():
     cold.close()
 
 
-@pytest.mark.parametrize("jump", ["drift", "shuffle", "cluster", "crowd", "crowd_fine", "weights", "dense_few",
-                                  "dense_many"])
+@pytest.mark.parametrize("jump", ["drift", "shuffle", "cluster", "crowd", "crowd_fine", "crowd_coarse", "weights",
+                                  "dense_few", "dense_many"])
 def test_solve_sequences(jump):
     """solve at x1, then at x2: small drift (warm placement), unrelated
     positions (most particles move), 8 groups of 500 particles into 8 leaves
@@ -90,12 +90,16 @@ def test_solve_sequences(jump):
     3000 particles (its stayers walked in 47 chunks of 64, several in flight)
     that gains 30 movers (ranked in registers) or 200 (sorted in LDS).  The
     32 x 32 leaves take the one-block row passes; crowd_fine repeats the crowd
-    on 256 x 256 leaves, where the row passes are a block per row."""
+    on 256 x 256 leaves, where the row passes are a block per row, and
+    crowd_coarse on 8 x 8 leaves (most of the one block's threads idle; the
+    way back from the crowd merges about 150 movers into each leaf)."""
     import dataclasses
     from nbfmm_amd.workloads import disk
     wl = disk(n=20000)
     if jump == "crowd_fine":
         wl = dataclasses.replace(wl, num_level=7)
+    elif jump == "crowd_coarse":
+        wl = dataclasses.replace(wl, num_level=2)
     warm, cold = pair(wl)
     rng = np.random.default_rng(9)
     x1, m1 = wl.position, wl.weight
