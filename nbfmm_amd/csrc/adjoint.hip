@@ -158,7 +158,7 @@ __global__ __launch_bounds__(kUpTile * kUpTile) void k_adj_up(Sys sys) {
   int blk, nblk;
   const SysView v = sys.get(blk, nblk);
   const int B = v.B;
-  const int T = B < kUpTile ? B : kUpTile;
+  const int T = up_tile_dim(B);
   const int tiles = B / T;
   const int X0 = (blk % tiles) * T, Y0 = (blk / tiles) * T;
   const int t = threadIdx.x;

@@ -124,10 +124,14 @@ struct SysView {
 // segments); segment sums, then the leaf's sum of them, both in order.
 constexpr int kP2MSeg = 16;
 
-// Upward passes (forward and adjoint): one block per kUpTile x kUpTile tile
-// of leaves makes levels 1..lgT of its tile in LDS; a single block makes the
-// levels above (block 0 of the fused M2L launch, or k_m2m_top).
+// Upward passes (forward and adjoint): one block of kUpTile x kUpTile
+// threads per T x T tile of leaves (up_tile_dim) makes levels 1..lgT of its
+// tile in LDS; a single block makes the levels above (block 0 of the fused
+// M2L launch, or k_m2m_top).  Grids of at most kUpTileSmallMaxB leaves a side
+// take 8 x 8 tiles, so that their leaves spread over four times as many CUs.
 constexpr int kUpTile = 16;
+constexpr int kUpTileSmall = 8;
+constexpr int kUpTileSmallMaxB = 128;
 // The levels above the tiles in a fused M2L launch (forward or adjoint):
 // each of the system's first kM2LTopBlocks blocks makes them in LDS (at most
 // kTopLds cells: levels lgT+1..7 at B = 512) and takes its share of their
@@ -135,8 +139,12 @@ constexpr int kUpTile = 16;
 constexpr int kM2LTopBlocks = 2;
 constexpr int kTopLds = 256 + 64 + 16;
 // log2 of the upward tiles' side (lgT).
+__device__ __host__ __forceinline__ int up_tile_dim(int B) {
+  const int T = B <= kUpTileSmallMaxB ? kUpTileSmall : kUpTile;
+  return B < T ? B : T;
+}
 __device__ __host__ __forceinline__ int up_tile_lg(int B) {
-  const int T = B < kUpTile ? B : kUpTile;
+  const int T = up_tile_dim(B);
   int lgT = 0;
   while ((1 << lgT) < T) ++lgT;
   return lgT;

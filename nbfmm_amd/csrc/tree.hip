@@ -217,7 +217,7 @@ __global__ __launch_bounds__(kUpTile * kUpTile) void k_upward(Sys sys) {
   int blk, nblk;
   const SysView v = sys.get(blk, nblk);
   const int B = v.B;
-  const int T = B < kUpTile ? B : kUpTile;
+  const int T = up_tile_dim(B);
   const int tiles = B / T;
   const int X0 = (blk % tiles) * T, Y0 = (blk / tiles) * T;
   const int t = threadIdx.x;
@@ -471,7 +471,7 @@ __global__ __launch_bounds__(1024) void k_m2m_top(Sys sys) {
   int blk, nblk;
   const SysView v = sys.get(blk, nblk);
   int first = 1;
-  while ((1 << (first - 1)) < (v.B < kUpTile ? v.B : kUpTile)) ++first;
+  while ((1 << (first - 1)) < up_tile_dim(v.B)) ++first;
   for (int l = first; l < v.L; ++l) {
     const int D = v.B >> l, Dc = D * 2;
     const float4* ch = v.mpole + v.lvl_off(l - 1);
@@ -733,7 +733,7 @@ hipError_t launch_p2m(const SysView& v, hipStream_t s) {
 
 int up_blocks(int L, int B) {
   if (L <= 0) return 0;
-  const int T = B < kUpTile ? B : kUpTile;
+  const int T = up_tile_dim(B);
   return (B / T) * (B / T);
 }
 
