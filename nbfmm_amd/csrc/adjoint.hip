@@ -158,7 +158,7 @@ __global__ __launch_bounds__(kUpTile * kUpTile) void k_adj_up(Sys sys) {
   int blk, nblk;
   const SysView v = sys.get(blk, nblk);
   const int B = v.B;
-  const int T = up_tile_dim(B);
+  const int T = adj_tile_dim(B);
   const int tiles = B / T;
   const int X0 = (blk % tiles) * T, Y0 = (blk / tiles) * T;
   const int t = threadIdx.x;
@@ -169,7 +169,7 @@ __global__ __launch_bounds__(kUpTile * kUpTile) void k_adj_up(Sys sys) {
     lv[0][t] = g;
   }
   __syncthreads();
-  const int lgT = up_tile_lg(B);
+  const int lgT = adj_tile_lg(B);
   const int top = (v.L - 1) < lgT ? (v.L - 1) : lgT;
   int cur = 0;
   for (int l = 1; l <= top; ++l) {
@@ -236,7 +236,7 @@ template <class Sys>
 __global__ __launch_bounds__(256) void k_adj_m2l_fused(Sys sys) {
   int blk, nblk;
   const SysView v = sys.get(blk, nblk);
-  const int lgT = up_tile_lg(v.B);
+  const int lgT = adj_tile_lg(v.B);
   const bool has_top = v.L - 1 > lgT;
   if (has_top && blk < kM2LTopBlocks) {
     __shared__ float2 lv[kTopLds];  // level lgT + 1 at 0, each level above right after the one below
@@ -534,13 +534,26 @@ hipError_t launch_copy_regions(const CopyRegions& c, int count, hipStream_t s) {
   return hipGetLastError();
 }
 
+int adj_up_blocks(int L, int B) {
+  if (L <= 0) return 0;
+  const int T = adj_tile_dim(B);
+  return (B / T) * (B / T);
+}
+
+int adj_m2l_fused_blocks(const SysView& v) {
+  if (v.L <= 0) return 0;
+  const int lgT = adj_tile_lg(v.B);
+  if (v.L - 1 > lgT) return kM2LTopBlocks + cdiv(v.lvl_off(lgT + 1), 256);
+  return cdiv(v.lvl_off(v.L), 256);
+}
+
 hipError_t launch_adjoint(const SysView& v, const CallIO& io, int mode, int grid_blocks, hipStream_t s) {
   if (v.n <= 0) return hipSuccess;
   const OneSys one{v};
   hipLaunchKernelGGL(k_adj_sort<OneSys>, dim3(cdiv(v.n, 256)), dim3(256), 0, s, one, io);
   if (v.L > 0) {
-    hipLaunchKernelGGL(k_adj_up<OneSys>, dim3(up_blocks(v.L, v.B)), dim3(kUpTile * kUpTile), 0, s, one);
-    hipLaunchKernelGGL(k_adj_m2l_fused<OneSys>, dim3(m2l_fused_blocks(v)), dim3(256), 0, s, one);
+    hipLaunchKernelGGL(k_adj_up<OneSys>, dim3(adj_up_blocks(v.L, v.B)), dim3(kUpTile * kUpTile), 0, s, one);
+    hipLaunchKernelGGL(k_adj_m2l_fused<OneSys>, dim3(adj_m2l_fused_blocks(v)), dim3(256), 0, s, one);
   }
   if (mode == kAdjSolve)
     hipLaunchKernelGGL((k_adj_p2p<kAdjSolve, OneSys>), dim3(grid_blocks), dim3(kAdjP2PWaves * kWave), 0, s, one, io);
@@ -551,9 +564,10 @@ hipError_t launch_adjoint(const SysView& v, const CallIO& io, int mode, int grid
 
 hipError_t launch_adjoint_batch(const BatchLaunch& bl, const CallIO& io, int mode, hipStream_t s) {
   hipLaunchKernelGGL(k_adj_sort<ManySys>, dim3(bl.adj_sort.blocks), dim3(256), 0, s, bl.adj_sort.sys, io);
-  if (bl.up.blocks) hipLaunchKernelGGL(k_adj_up<ManySys>, dim3(bl.up.blocks), dim3(kUpTile * kUpTile), 0, s, bl.up.sys);
-  if (bl.m2l.blocks)
-    hipLaunchKernelGGL(k_adj_m2l_fused<ManySys>, dim3(bl.m2l.blocks), dim3(256), 0, s, bl.m2l.sys);
+  if (bl.adj_up.blocks)
+    hipLaunchKernelGGL(k_adj_up<ManySys>, dim3(bl.adj_up.blocks), dim3(kUpTile * kUpTile), 0, s, bl.adj_up.sys);
+  if (bl.adj_m2l.blocks)
+    hipLaunchKernelGGL(k_adj_m2l_fused<ManySys>, dim3(bl.adj_m2l.blocks), dim3(256), 0, s, bl.adj_m2l.sys);
   if (mode == kAdjSolve)
     hipLaunchKernelGGL((k_adj_p2p<kAdjSolve, ManySys>), dim3(bl.adj_p2p.blocks), dim3(kAdjP2PWaves * kWave), 0, s, bl.adj_p2p.sys,
                        io);

@@ -1490,7 +1490,7 @@ int nbfmm_batch_create(int S, const int* num_particle, const int* num_level, con
   hipError_t e = hipMemcpy(b->ddesc, b->hdesc.data(), S * sizeof(SysDesc), hipMemcpyHostToDevice);
   if (e != hipSuccess) { batch_free(b); delete b; return hip_fail(e, "hipMemcpy system descriptors"); }
   // launch maps
-  std::vector<int> tiles(S), rows(S), up(S), mtop(S), m2l(S), asort(S), wmerge(S), one(S, 1);
+  std::vector<int> tiles(S), rows(S), up(S), mtop(S), m2l(S), aup(S), am2l(S), asort(S), wmerge(S), one(S, 1);
   for (int s = 0; s < S; ++s) {
     const SysView& v = b->hv[s];
     const int C = v.B * v.B;
@@ -1499,6 +1499,8 @@ int nbfmm_batch_create(int S, const int* num_particle, const int* num_level, con
     up[s] = up_blocks(v.L, v.B);
     mtop[s] = upward_top_blocks(v.L, v.B);
     m2l[s] = m2l_fused_blocks(v);
+    aup[s] = adj_up_blocks(v.L, v.B);
+    am2l[s] = adj_m2l_fused_blocks(v);
     asort[s] = cdiv(v.n, 256);
     // one wave per 256 particles (touched leaves are about twice the movers;
     // the waves grid-stride over the list).  Measured on the 256-system batch:
@@ -1509,7 +1511,8 @@ int nbfmm_batch_create(int S, const int* num_particle, const int* num_level, con
   }
   if ((rc = make_map(b, tiles, &b->bl.tiles)) || (rc = make_map(b, rows, &b->bl.rows)) ||
       (rc = make_map(b, up, &b->bl.up)) || (rc = make_map(b, mtop, &b->bl.m2m_top)) ||
-      (rc = make_map(b, m2l, &b->bl.m2l)) || (rc = make_map(b, asort, &b->bl.adj_sort)) ||
+      (rc = make_map(b, m2l, &b->bl.m2l)) || (rc = make_map(b, aup, &b->bl.adj_up)) ||
+      (rc = make_map(b, am2l, &b->bl.adj_m2l)) || (rc = make_map(b, asort, &b->bl.adj_sort)) ||
       (rc = make_map(b, wmerge, &b->bl.warm_merge)) || (rc = make_map(b, one, &b->bl.one)) ||
       (rc = batch_alloc(b, &b->d_pairs, (size_t)S)) ||
       (rc = make_map(b, p2p_share(b, kAdjP2PWaves, cus * kAdjP2PBatchBlocksPerCU), &b->bl.adj_p2p)) ||
@@ -1708,7 +1711,7 @@ int nbfmm_batch_step_adjoint_kept(nbfmm_batch* b, int slot, float* lam, float* p
   const nbfmm_batch::SortSlot& sl = b->slots[slot];
   // the adjoint's launch maps with every system's sorted arrays in the slot
   BatchLaunch bl = b->bl;
-  for (KernelMap* km : {&bl.adj_sort, &bl.up, &bl.m2l, &bl.adj_p2p}) {
+  for (KernelMap* km : {&bl.adj_sort, &bl.adj_up, &bl.adj_m2l, &bl.adj_p2p}) {
     km->sys.base.part = sl.part;
     km->sys.base.key = sl.key;
     km->sys.base.head = sl.head;

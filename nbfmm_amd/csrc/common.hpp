@@ -149,6 +149,16 @@ __device__ __host__ __forceinline__ int up_tile_lg(int B) {
   while ((1 << lgT) < T) ++lgT;
   return lgT;
 }
+// The adjoint's upward and M2L^T passes keep kUpTile x kUpTile tiles at every
+// size: there a lane sums a whole leaf's gradient, and 8 x 8 tiles left three
+// of a block's four waves idle (the 64-collision backward 496 -> 502 ms).
+__device__ __host__ __forceinline__ int adj_tile_dim(int B) { return B < kUpTile ? B : kUpTile; }
+__device__ __host__ __forceinline__ int adj_tile_lg(int B) {
+  const int T = adj_tile_dim(B);
+  int lgT = 0;
+  while ((1 << lgT) < T) ++lgT;
+  return lgT;
+}
 
 // Warm-started sort.
 constexpr uint32_t kMoverBit = 0x80000000u;  // tkey flag: the particle changed leaf

@@ -20,9 +20,11 @@ struct KernelMap {
 struct BatchLaunch {
   KernelMap tiles;     // row pass: cdiv(n_s, kSortTile) per system
   KernelMap rows;      // row scan / column pass: B_s per system
-  KernelMap up;        // upward passes (forward and adjoint): one block per tile of leaves
+  KernelMap up;        // forward upward pass: one block per tile of leaves (up_blocks)
   KernelMap m2m_top;   // M2M top levels
-  KernelMap m2l;       // fused M2L and adjoint M2L: m2l_fused_blocks
+  KernelMap m2l;       // fused M2L: m2l_fused_blocks
+  KernelMap adj_up;    // adjoint upward pass: one block per adjoint tile (adj_up_blocks)
+  KernelMap adj_m2l;   // adjoint M2L: adj_m2l_fused_blocks
   KernelMap p2p;       // P2P: share of the P2P grid (blocks of the chosen variant)
   KernelMap adj_p2p;   // adjoint P2P: share of its grid (kAdjP2PWaves-wave blocks)
   KernelMap adj_sort;  // adjoint: upstream gradient to sorted order, cdiv(n_s, 256)
@@ -95,6 +97,8 @@ void p2p_variant_info(int variant, int* stage, int* unroll, int* waves);
 hipError_t launch_l2p(const SysView& v, hipStream_t s);
 
 // adjoint.hip; mode 0: solve adjoint, 1: Verlet step adjoint
+int adj_up_blocks(int L, int B);  // the adjoint's upward tiles
+int adj_m2l_fused_blocks(const SysView& v);
 hipError_t launch_adjoint(const SysView& v, const CallIO& io, int mode, int grid_blocks, hipStream_t s);
 hipError_t launch_adjoint_batch(const BatchLaunch& b, const CallIO& io, int mode, hipStream_t s);
 // y += a * x over n floats
