@@ -80,8 +80,8 @@ def test_graph_run_bitwise_equal_cold():
     cold.close()
 
 
-@pytest.mark.parametrize("jump", ["drift", "shuffle", "cluster", "crowd", "crowd_fine", "crowd_coarse", "weights",
-                                  "dense_few", "dense_many"])
+@pytest.mark.parametrize("jump", ["drift", "drift_many", "shuffle", "cluster", "crowd", "crowd_fine", "crowd_coarse",
+                                  "weights", "dense_few", "dense_many"])
 def test_solve_sequences(jump):
     """solve at x1, then at x2: small drift (warm placement), unrelated
     positions (most particles move), 8 groups of 500 particles into 8 leaves
@@ -92,10 +92,12 @@ def test_solve_sequences(jump):
     32 x 32 leaves take the one-block row passes; crowd_fine repeats the crowd
     on 256 x 256 leaves, where the row passes are a block per row, and
     crowd_coarse on 8 x 8 leaves (most of the one block's threads idle; the
-    way back from the crowd merges about 150 movers into each leaf)."""
+    way back from the crowd merges about 150 movers into each leaf);
+    drift_many drifts 70000 particles on 32 x 32 leaves, past the one-block
+    row passes' particle limit."""
     import dataclasses
     from nbfmm_amd.workloads import disk
-    wl = disk(n=20000)
+    wl = disk(n=70000 if jump == "drift_many" else 20000)
     if jump == "crowd_fine":
         wl = dataclasses.replace(wl, num_level=7)
     elif jump == "crowd_coarse":
@@ -111,7 +113,7 @@ def test_solve_sequences(jump):
         x2 = x1.copy()
         x2[5000:5000 + nm] = x1[0] + 1e-6 * rng.random((nm, 2)).astype(np.float32)
         m2 = m1
-    elif jump == "drift":
+    elif jump.startswith("drift"):
         x2 = x1 + 2e-3 * rng.standard_normal(x1.shape).astype(np.float32)
         m2 = m1
     elif jump == "shuffle":
