@@ -1509,6 +1509,8 @@ int nbfmm_batch_create(int S, const int* num_particle, const int* num_level, con
     wmerge[s] = cdiv(std::min(C, v.n / 256 + 64), 256 / kWave);
     b->bl.max_B = std::max(b->bl.max_B, v.B);
   }
+  b->bl.warm_one = true;
+  for (int s = 0; s < S; ++s) b->bl.warm_one = b->bl.warm_one && warm_rows_one_fits(b->hv[s].B, b->hv[s].n);
   if ((rc = make_map(b, tiles, &b->bl.tiles)) || (rc = make_map(b, rows, &b->bl.rows)) ||
       (rc = make_map(b, up, &b->bl.up)) || (rc = make_map(b, mtop, &b->bl.m2m_top)) ||
       (rc = make_map(b, m2l, &b->bl.m2l)) || (rc = make_map(b, aup, &b->bl.adj_up)) ||

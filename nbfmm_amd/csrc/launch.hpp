@@ -32,6 +32,7 @@ struct BatchLaunch {
   KernelMap warm_merge;  // one wave per touched leaf (grid-stride): its share of waves
   KernelMap one;       // one block per system
   int max_B = 0;       // widest leaf row among the systems
+  bool warm_one = false;  // every system within warm_rows_one_fits: the warm row passes as one launch
 };
 
 // sort.hip
@@ -44,6 +45,9 @@ hipError_t launch_occ_from_head(const SysView& v, hipStream_t s);
 // gathered: the previous step's P2P (kP2PStepSort) made the first pass
 hipError_t launch_predo_warm(const SysView& v, const CallIO& io, int zero_empty, hipStream_t s, bool gathered = false);
 hipError_t launch_predo_warm_batch(const BatchLaunch& b, const CallIO& io, hipStream_t s, bool gathered = false);
+// whether a system of B x B leaves and n particles fits the one-block warm
+// row passes (k_warm_rows_finish_one)
+bool warm_rows_one_fits(int B, int n);
 
 // tree.hip
 hipError_t launch_p2m(const SysView& v, hipStream_t s);

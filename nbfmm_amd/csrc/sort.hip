@@ -795,83 +795,106 @@ __global__ __launch_bounds__(NT) void k_warm_finish(Sys sys, CallIO io, int zero
   }
 }
 
-// Small systems (at most kWarmOneThreads leaves and kWarmOneMaxN particles):
-// k_warm_rows and k_warm_finish as one block, a thread per leaf in
-// row-major order, so that the scans over all leaves need no second launch.
-// It writes what the two kernels leave for the passes after them: the same
-// head, rowstart, movers' group offsets, occupied and touched leaf lists,
-// counts and cleared counters (the row-local offsets and row totals that only
-// pass from one of them to the other are not kept).  With the cold switch on
-// it also makes the cold row histograms and row scans (at most
-// kWarmOneMaxN / kSortTile = 32 tiles of 32 rows).  The 16384-particle disk's
-// step: 49.5 -> 47.3 us.  (With several consecutive leaves per thread, on
-// 128 x 128 leaves, one block's scattered loads took longer than the two
-// kernels: the Gaussian disk's 50 steps 3.67 -> 6.27 ms.)
+// Small systems: k_warm_rows and k_warm_finish (and a batch's
+// k_warm_rowscan) as one block per system, each thread taking `per`
+// consecutive leaves in row-major order, so that the scans over all leaves
+// need no second launch.  It writes what those kernels leave for the passes
+// after them: the same head, rowstart, movers' group offsets, occupied and
+// touched leaf lists, counts and cleared counters (the row-local offsets and
+// row totals that only pass between them are not kept).  With the cold switch
+// on it also makes the cold row histograms, every tile's at once in LDS, and
+// the row scans, a row per wave.  Bounds: at most kWarmOneMaxPer leaves per
+// thread (64 x 64 leaves) and kWarmOneMaxTiles tiles of kSortTile particles.
+// A solver takes it up to 32 x 32 leaves and 65536 particles (one leaf per
+// thread: the 16384-particle disk's step 49.5 -> 47.3 us; with 16 consecutive
+// leaves per thread on 128 x 128 leaves one block's scattered loads took
+// longer than the two kernels, the Gaussian disk's 50 steps 3.67 -> 6.27 ms),
+// a batch whenever every system is within the bounds.
 constexpr int kWarmOneThreads = 1024;
-constexpr int kWarmOneMaxN = 65536;
+constexpr int kWarmOneMaxPer = 4;
+constexpr int kWarmOneMaxTiles = 64;
+constexpr int kWarmOneMaxN = 65536;  // a solver's limit (a batch's: kWarmOneMaxTiles * kSortTile)
 
-__global__ __launch_bounds__(kWarmOneThreads) void k_warm_rows_finish_one(SysView v, CallIO io, int zero_empty) {
+template <class Sys>
+__global__ __launch_bounds__(kWarmOneThreads) void k_warm_rows_finish_one(Sys sys, CallIO io, int zero_empty) {
   __shared__ uint32_t sh[kWarmOneThreads / kWave];
-  __shared__ uint32_t s_hist[kWarmOneThreads];  // cold switch: [tile][row] counts
+  __shared__ uint32_t s_hist[kWarmOneMaxTiles * kWave];  // cold switch: [tile][row] counts, B <= 64
+  int blk, nblk;
+  const SysView v = sys.get(blk, nblk);
   const int B = v.B, C = B * B;
-  const int c = (int)threadIdx.x;
-  const bool in = c < C;
+  const int per = C > kWarmOneThreads ? C / kWarmOneThreads : 1;
+  const int c0 = (int)threadIdx.x * per;
   const int cold = v.wstate[kWarmColdReq];
-  // k_warm_rows: new count; a leaf that movers enter or leave is touched and
-  // keeps its old slot range
-  int cnt = 0, cin = 0;
-  bool touched = false;
-  if (in) {
-    const int ci = v.wcin[c], co = v.wcout[c];
-    const int h0 = v.head[c], h1 = v.head[c + 1];
-    touched = (ci | co) != 0;
-    if (touched) {
-      v.wcin[c] = 0;
-      v.wcout[c] = 0;
-      v.wtouch[c] = 1;
-      v.wtold[c] = make_int2(h0, h1 - h0);
+  // k_warm_rows: new counts; a leaf that movers enter or leave is touched
+  // and keeps its old slot range
+  int cnt[kWarmOneMaxPer], cin[kWarmOneMaxPer];
+  uint32_t tmask = 0, scnt = 0, scin = 0, socc = 0, stch = 0;
+#pragma unroll
+  for (int j = 0; j < kWarmOneMaxPer; ++j) {
+    cnt[j] = 0;
+    cin[j] = 0;
+    const int c = c0 + j;
+    if (j < per && c < C) {
+      const int ci = v.wcin[c], co = v.wcout[c];
+      const int h0 = v.head[c], h1 = v.head[c + 1];
+      if ((ci | co) != 0) {
+        v.wcin[c] = 0;
+        v.wcout[c] = 0;
+        v.wtouch[c] = 1;
+        v.wtold[c] = make_int2(h0, h1 - h0);
+        tmask |= 1u << j;
+        ++stch;
+      }
+      cnt[j] = h1 - h0 - co + ci;
+      cin[j] = ci;
+      scnt += (uint32_t)cnt[j];
+      scin += (uint32_t)ci;
+      socc += cnt[j] > 0 ? 1u : 0u;
     }
-    cnt = h1 - h0 - co + ci;
-    cin = ci;
   }
   // k_warm_finish: exclusive scans over all leaves (their barriers also
   // order every read of the old head above before the writes below)
   uint32_t tot, itot, otot, ttot;
-  const uint32_t e = block_exclusive_scan<kWarmOneThreads>((uint32_t)cnt, sh, &tot);
-  const uint32_t ei = block_exclusive_scan<kWarmOneThreads>((uint32_t)cin, sh, &itot);
-  const uint32_t eo = block_exclusive_scan<kWarmOneThreads>(cnt > 0 ? 1u : 0u, sh, &otot);
-  const uint32_t et = block_exclusive_scan<kWarmOneThreads>(touched ? 1u : 0u, sh, &ttot);
-  if (in) {
-    v.head[c] = (int)e;
-    v.winoff[c] = (int)ei;
-    if ((c & (B - 1)) == 0) v.rowstart[c >> v.lgB] = (int)e;
-    if (cnt > 0)
-      v.occ[eo] = c;
-    else if (zero_empty)
-      v.mpole[c] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if (touched) v.wtlist[et] = c;
+  uint32_t e = block_exclusive_scan<kWarmOneThreads>(scnt, sh, &tot);
+  uint32_t ei = block_exclusive_scan<kWarmOneThreads>(scin, sh, &itot);
+  uint32_t eo = block_exclusive_scan<kWarmOneThreads>(socc, sh, &otot);
+  uint32_t et = block_exclusive_scan<kWarmOneThreads>(stch, sh, &ttot);
+#pragma unroll
+  for (int j = 0; j < kWarmOneMaxPer; ++j) {
+    const int c = c0 + j;
+    if (j < per && c < C) {
+      v.head[c] = (int)e;
+      v.winoff[c] = (int)ei;
+      if ((c & (B - 1)) == 0) v.rowstart[c >> v.lgB] = (int)e;
+      if (cnt[j] > 0)
+        v.occ[eo++] = c;
+      else if (zero_empty)
+        v.mpole[c] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      if ((tmask >> j) & 1u) v.wtlist[et++] = c;
+      e += (uint32_t)cnt[j];
+      ei += (uint32_t)cin[j];
+    }
   }
-  if (c == 0) {
+  if (threadIdx.x == 0) {
     v.rowstart[B] = (int)tot;
     v.head[C] = (int)tot;
     // with the cold switch on, the column pass counts the occupied leaves
-    // from zero (row_scan_row below clears the count as well)
+    // from zero
     *v.nocc = cold ? 0 : (int)otot;
     v.wstate[kWarmTouched] = (int)ttot;
     v.wstate[kWarmCold] = cold;
     v.wstate[kWarmColdReq] = 0;  // read by every thread before the scans
   }
   if (cold) {
-    // the cold row pass's histograms (k_row_up) of every tile at once, at
-    // most 32 tiles x 32 rows of counts in LDS, then its scans (k_row_scan):
-    // one wave per row takes the row's tile counts to exclusive offsets
-    // (hist) and its total (dtotal); the occupied count is already 0
+    // the cold row pass's histograms (k_row_up) of every tile at once, then
+    // its scans (k_row_scan): one wave per row takes the row's tile counts to
+    // exclusive offsets (hist) and its total (dtotal)
     const int ntiles = (v.n + kSortTile - 1) / kSortTile;
-    for (int k = c; k < ntiles * B; k += kWarmOneThreads) s_hist[k] = 0;
+    for (int k = (int)threadIdx.x; k < ntiles * B; k += kWarmOneThreads) s_hist[k] = 0;
     __syncthreads();
     const float2* pos0 = io.pos0 + v.part_off;
     bool any_bad = false;
-    for (int i = c; i < v.n; i += kWarmOneThreads) {
+    for (int i = (int)threadIdx.x; i < v.n; i += kWarmOneThreads) {
       bool bad;
       const uint32_t key = bin_key(v, pos0[i], bad);
       any_bad |= bad;
@@ -880,13 +903,17 @@ __global__ __launch_bounds__(kWarmOneThreads) void k_warm_rows_finish_one(SysVie
     if (any_bad) atomicOr(v.flags, 1u);
     __syncthreads();
     const int lane = lane_id();
-    for (int d = c / kWave; d < B; d += kWarmOneThreads / kWave) {
+    for (int d = (int)threadIdx.x / kWave; d < B; d += kWarmOneThreads / kWave) {
       const uint32_t k = lane < ntiles ? s_hist[lane * B + d] : 0u;
       const uint32_t inc = wave_inclusive_scan(k);
       if (lane < ntiles) v.hist[(size_t)d * ntiles + lane] = inc - k;
       if (lane == kWave - 1) v.dtotal[d] = inc;
     }
   }
+}
+
+bool warm_rows_one_fits(int B, int n) {
+  return B * B <= kWarmOneThreads * kWarmOneMaxPer && B <= kWave && n <= kWarmOneMaxTiles * kSortTile;
 }
 
 // Stayers of untouched leaves go to head[leaf] + their old rank; movers are
@@ -1170,8 +1197,8 @@ hipError_t launch_predo_warm(const SysView& v, const CallIO& io, int zero_empty,
   const OneSys one{v};
   const int nb = cdiv(v.n, 256);
   if (!gathered) hipLaunchKernelGGL(k_warm_gather<OneSys>, dim3(nb), dim3(256), 0, s, one, io);
-  if (v.B * v.B <= kWarmOneThreads && v.n <= kWarmOneMaxN) {
-    hipLaunchKernelGGL(k_warm_rows_finish_one, dim3(1), dim3(kWarmOneThreads), 0, s, v, io, zero_empty);
+  if (v.B * v.B <= kWarmOneThreads && v.n <= kWarmOneMaxN && warm_rows_one_fits(v.B, v.n)) {
+    hipLaunchKernelGGL(k_warm_rows_finish_one<OneSys>, dim3(1), dim3(kWarmOneThreads), 0, s, one, io, zero_empty);
   } else {
     launch_warm_rows(row_threads(v.B), dim3(v.B), one, io, s);
     launch_warm_finish(row_threads(v.B), dim3(v.B), one, io, zero_empty, 0, s);
@@ -1185,10 +1212,15 @@ hipError_t launch_predo_warm(const SysView& v, const CallIO& io, int zero_empty,
 hipError_t launch_predo_warm_batch(const BatchLaunch& b, const CallIO& io, hipStream_t s, bool gathered) {
   if (!gathered)
     hipLaunchKernelGGL(k_warm_gather<ManySys>, dim3(b.adj_sort.blocks), dim3(256), 0, s, b.adj_sort.sys, io);
-  const int nt = row_threads(b.max_B);
-  launch_warm_rows(nt, dim3(b.rows.blocks), b.rows.sys, io, s);
-  launch_warm_rowscan(nt, dim3(b.one.blocks), b.one.sys, s);
-  launch_warm_finish(nt, dim3(b.rows.blocks), b.rows.sys, io, 1, 1, s);
+  if (b.warm_one) {
+    hipLaunchKernelGGL(k_warm_rows_finish_one<ManySys>, dim3(b.one.blocks), dim3(kWarmOneThreads), 0, s, b.one.sys, io,
+                       1);
+  } else {
+    const int nt = row_threads(b.max_B);
+    launch_warm_rows(nt, dim3(b.rows.blocks), b.rows.sys, io, s);
+    launch_warm_rowscan(nt, dim3(b.one.blocks), b.one.sys, s);
+    launch_warm_finish(nt, dim3(b.rows.blocks), b.rows.sys, io, 1, 1, s);
+  }
   hipLaunchKernelGGL(k_warm_place<ManySys>, dim3(b.adj_sort.blocks), dim3(256), 0, s, b.adj_sort.sys, io);
   hipLaunchKernelGGL(k_warm_merge<ManySys>, dim3(b.warm_merge.blocks), dim3(256), 0, s, b.warm_merge.sys);
   return hipGetLastError();
